@@ -1,0 +1,260 @@
+"""Benchmark of the MI355X Magnus path.  Prints ONE JSON line.
+
+Default workload (one GPU): the SpaceCurve batch -- 4096 independent 3-vector
+trajectories f(t) = sum_{m=1..4} a_m sin(2 pi m t + phi_m), a_m ~ N(0,1)/m,
+phi_m ~ U[0, 2 pi), S = 16385 samples on [0, 1], Simpson, compute_sc with
+n = 6, op = 'many' recording the carry, then compute_sc_vjp with a N(0,1)
+(6, 3) cotangent per trajectory.  One step = forward + VJP of the whole batch
+in batched launches; inputs are resident in HBM before timing starts.
+
+  python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
+
+Other workloads: dense-d32 (256 x d=32, S=4097, Boole, n=4, sum, record + VJP),
+plain-d4 (one d=4 instance, S=1025, Boole, n=4, sum, forward only).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import math
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT / "magnus-dp_amd"))
+
+MI355X_HBM_GBS = 8000.0
+MI355X_FP64_TFLOPS = 78.6
+
+
+def _sc_inputs(torch, B, S, modes, device, seed=2):
+    """Seeded Fourier-form trajectories, evaluated on the device."""
+    import numpy as np
+
+    m = np.arange(1, modes + 1, dtype=np.float64)
+    amp = np.empty((B, modes, 3))
+    phi = np.empty((B, modes, 3))
+    for b in range(B):
+        rng = np.random.default_rng([seed, b])
+        amp[b] = rng.standard_normal((modes, 3)) / m[:, None]
+        phi[b] = rng.uniform(0.0, 2 * np.pi, size=(modes, 3))
+    t = torch.from_numpy(np.linspace(0.0, 1.0, S)).to(device)
+    amp_t = torch.from_numpy(amp).to(device)
+    phi_t = torch.from_numpy(phi).to(device)
+    out = torch.zeros((B, S, 3), dtype=torch.float64, device=device)
+    for k in range(modes):
+        arg = (2 * math.pi * (k + 1)) * t[None, :, None] + phi_t[:, k, None, :]
+        out += amp_t[:, k, None, :] * torch.sin(arg)
+    cot = np.stack([np.random.default_rng([seed, b, 1]).standard_normal((6, 3)) for b in range(B)])
+    return out, torch.from_numpy(cot).to(device)
+
+
+def _dense_inputs(torch, B, S, d, modes, device, seed):
+    import numpy as np
+
+    coeffs = np.stack([np.random.default_rng([seed, b]).standard_normal((2 * modes + 1, d, d)) / np.sqrt(d)
+                       for b in range(B)])
+    t = torch.from_numpy(np.linspace(0.0, 1.0, S)).to(device)
+    c = torch.from_numpy(coeffs).to(device)
+    out = c[:, 0, None].expand(B, S, d, d).clone()
+    for k in range(1, modes + 1):
+        out += torch.sin(2 * math.pi * k * t)[None, :, None, None] * c[:, 2 * k - 1, None]
+        out += torch.cos(2 * math.pi * k * t)[None, :, None, None] * c[:, 2 * k, None]
+    cot = np.stack([np.random.default_rng([seed, b, 1]).standard_normal((d, d)) for b in range(B)])
+    return out, torch.from_numpy(cot).to(device)
+
+
+def _counts(Q, K, S, m, m_in, P, n_out, op_one=False):
+    """Algorithmic flops and bytes per solve (FMA = 2 flops, 8 B per value)."""
+    F_fwd = Q * K * S * (P + 8 * m) + (0 if op_one else 2 * m * S)
+    F_vjp = Q * K * S * (2 * P + 10 * m) + Q * S * m + 2 * m * S
+    B_fwd = 8 * (S * m_in + n_out * m_in)
+    B_carry = 8 * Q * K * S * m
+    B_vjp = 8 * (2 * S * m_in + n_out * m_in)
+    return F_fwd, F_vjp, B_fwd, B_carry, B_vjp
+
+
+def _cpu_baseline_sc(a_host, cot_host, n, S, budget_s):
+    """Oracle (C restatement of the reference CPU path) on host cores: the faster of
+    (1) instances in sequence with the reference Auto GL-lane rule, VJP serial, and
+    (2) one instance per core.  Time-bounded sample of the same trajectories."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    cores = min(cores, 16)
+    Q, K = (n + 3) // 2, n - 1
+    results = {}
+    for mode in ("reference-faithful", "all-cores"):
+        done, t0 = 0, time.perf_counter()
+        chunk = 1 if mode == "reference-faithful" else cores
+        while time.perf_counter() - t0 < budget_s / 2 and done + chunk <= a_host.shape[0]:
+            a = np.ascontiguousarray(a_host[done:done + chunk])
+            c = np.ascontiguousarray(cot_host[done:done + chunk])
+            carry = np.zeros((chunk, Q, K, S, 2, 2))
+            if mode == "reference-faithful":
+                lanes = O.auto_lanes(n, S, cores)
+                O.sc_forward_batch(n, a, 0.0, 1.0, op="many", integrator="Simpson", lanes=lanes, threads=1,
+                                   carry=carry)
+                O.sc_vjp_batch(n, a, c, 0.0, 1.0, op="many", integrator="Simpson", threads=1, carry=carry)
+            else:
+                O.sc_forward_batch(n, a, 0.0, 1.0, op="many", integrator="Simpson", lanes=1, threads=cores,
+                                   carry=carry)
+                O.sc_vjp_batch(n, a, c, 0.0, 1.0, op="many", integrator="Simpson", threads=cores, carry=carry)
+            done += chunk
+        el = time.perf_counter() - t0
+        results[mode] = (done / el if el > 0 else 0.0, done, el)
+    best = max(results, key=lambda k: results[k][0])
+    v, done, el = results[best]
+    return {
+        "value": v,
+        "unit": "solves/s",
+        "cores": cores if best == "all-cores" else min(cores, Q),
+        "kind": "port",
+        "sample": f"{done} of the batch's trajectories (full S={S}, forward+VJP) in {el:.1f} s, mode={best}; "
+                  f"other mode {[(k, round(r[0], 2)) for k, r in results.items() if k != best]}",
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="spacecurve-batch", choices=["spacecurve-batch", "dense-d32", "plain-d4"])
+    ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-budget", type=float, default=20.0)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
+              file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from magnus import capi, torch_ops
+
+    dev = torch.device("cuda:0")
+    wl = args.workload
+    if wl == "spacecurve-batch":
+        B, S, n, modes = args.batch or 4096, 16385, 6, 4
+        a, cot = _sc_inputs(torch, B, S, modes, dev)
+
+        def step():
+            out, carry = torch_ops.sc_forward(n, a, 0.0, 1.0, op="many", integrator="Simpson", record_vjp=True)
+            dA = torch_ops.sc_vjp(n, a, cot, carry, 0.0, 1.0, op="many", integrator="Simpson")
+            return out, dA
+
+        Q, K, m, m_in, P, n_out = (n + 3) // 2, n - 1, 4, 3, 20, n
+        config = {"workload": "spacecurve-batch", "instances": B, "samples": S, "n": n, "integrator": "Simpson",
+                  "op": "many", "vector": "su(2) 3-vector", "fourier_modes": modes, "cotangent": "N(0,1) (6,3)",
+                  "carry": "recorded"}
+        do_vjp = True
+    elif wl == "dense-d32":
+        B, S, d, n, modes = args.batch or 256, 4097, 32, 4, 5
+        a, cot = _dense_inputs(torch, B, S, d, modes, dev, seed=3)
+
+        def step():
+            out, carry = torch_ops.forward(n, a, 0.0, 1.0, op="sum", integrator="Boole", record_vjp=True)
+            dA = torch_ops.vjp(n, a, cot, carry, 0.0, 1.0, op="sum", integrator="Boole")
+            return out, dA
+
+        Q, K, m, m_in, P, n_out = (n + 3) // 2, n - 1, d * d, d * d, 2 * d ** 3, 1
+        config = {"workload": "dense-d32", "instances": B, "samples": S, "dim": d, "n": n, "integrator": "Boole",
+                  "op": "sum", "carry": "recorded"}
+        do_vjp = True
+    else:
+        B, S, d, n, modes = 1, 1025, 4, 4, 3
+        a, cot = _dense_inputs(torch, B, S, d, modes, dev, seed=1)
+
+        def step():
+            return torch_ops.forward(n, a, 0.0, 1.0, op="sum", matrix_backend="Manual", integrator="Boole"), None
+
+        Q, K, m, m_in, P, n_out = (n + 3) // 2, n - 1, d * d, d * d, 2 * d ** 3, 1
+        config = {"workload": "plain-d4", "instances": 1, "samples": S, "dim": d, "n": n, "integrator": "Boole",
+                  "op": "sum", "backend": "Manual"}
+        do_vjp = False
+
+    print(f"[bench] workload={wl} B={B} S={S} n={n}: warmup {args.warmup}", file=sys.stderr, flush=True)
+    for _ in range(args.warmup):
+        step()
+    torch.cuda.synchronize()
+    kernels = capi.timing_kernels() if hasattr(capi, "timing_kernels") else ["ep_forward", "ep_vjp"]
+    capi.timing_reset()
+    capi.timing_enable(True)
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    torch.cuda.synchronize()
+    start.record()
+    for i in range(args.steps):
+        step()
+    stop.record()
+    torch.cuda.synchronize()
+    capi.timing_enable(False)
+    ms_total = start.elapsed_time(stop)
+    ms_step = ms_total / args.steps
+    per_kernel = {}
+    for name in ("ep_forward", "ep_vjp", "ep_forward_recompute"):
+        cnt, tot = capi.timing_read(name)
+        if cnt:
+            per_kernel[name] = {"launches": cnt, "avg_ms": tot / cnt}
+    print(f"[bench] timed {args.steps} steps: {ms_step:.3f} ms/step; kernels {per_kernel}", file=sys.stderr,
+          flush=True)
+
+    F_fwd, F_vjp, B_fwd, B_carry, B_vjp = _counts(Q, K, S, m, m_in, P, n_out)
+    kernel_bytes = {"ep_forward": B * (B_fwd + B_carry), "ep_vjp": B * (B_vjp + B_carry)}
+    kernel_flops = {"ep_forward": B * F_fwd, "ep_vjp": B * F_vjp}
+    dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"]) if per_kernel else None
+    roofline = None
+    if dom in kernel_bytes:
+        dur_s = per_kernel[dom]["avg_ms"] / 1e3
+        bytes_ach = kernel_bytes[dom] / dur_s / 1e9
+        flops_ach = kernel_flops[dom] / dur_s / 1e12
+        t_bytes = kernel_bytes[dom] / (MI355X_HBM_GBS * 1e9)
+        t_flops = kernel_flops[dom] / (MI355X_FP64_TFLOPS * 1e12)
+        if t_bytes >= t_flops:
+            roofline = {"bound": "hbm", "achieved": round(bytes_ach, 2), "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                        "frac": round(bytes_ach / MI355X_HBM_GBS, 4), "traffic": None}
+        else:
+            roofline = {"bound": "fp64", "achieved": round(flops_ach, 3), "peak": MI355X_FP64_TFLOPS,
+                        "unit": "TFLOP/s", "frac": round(flops_ach / MI355X_FP64_TFLOPS, 4), "traffic": None}
+        roofline["kernel"] = dom
+        roofline["kernel_avg_ms"] = round(per_kernel[dom]["avg_ms"], 4)
+        roofline["algorithmic_bytes_per_launch"] = kernel_bytes[dom]
+        roofline["algorithmic_flops_per_launch"] = kernel_flops[dom]
+
+    cpu = None
+    if not args.no_cpu and wl == "spacecurve-batch":
+        k = min(B, 1024)
+        cpu = _cpu_baseline_sc(a[:k].cpu().numpy(), cot[:k].cpu().numpy(), n, S, args.cpu_budget)
+
+    value = B / (ms_step / 1e3)
+    line = {
+        "metric": "Magnus fwd+VJP solves/s" if do_vjp else "Magnus fwd solves/s",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms_step, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Fourier-form A(t), evaluated on the device)",
+        "config": config,
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+        "kernels": per_kernel,
+        "kernel_family": capi.last_kernel(),
+    }
+    print(json.dumps(line))
+
+
+if __name__ == "__main__":
+    main()

@@ -1,0 +1,159 @@
+/*
+ * magnus_capi.h -- C ABI of the MI355X (gfx950) Magnus hot path.
+ *
+ * Library: magnus-dp_amd/magnus/libmagnus_hip.so.  All compute entry points
+ * take DEVICE pointers (HIP device memory of the current device) and an
+ * optional hipStream_t passed as `void*` (NULL = the default stream).  Work is
+ * enqueued on that stream; the call returns after enqueueing (asynchronous),
+ * except that argument errors are reported synchronously.  FP64 only.
+ *
+ * The entry points replace the reference's composer boundary:
+ *   magnus_forward_f64     <- detail::run_raw_impl              (src/composer/backends.hpp:182-220)
+ *   magnus_vjp_f64         <- detail::run_vjp_raw_impl          (src/composer/backends.hpp:222-251)
+ *   magnus_sc_forward_f64  <- detail::run_spacecurve_raw_impl   (src/composer/backends.hpp:253-301)
+ *   magnus_sc_vjp_f64      <- detail::run_spacecurve_vjp_raw_impl (src/composer/backends.hpp:303-357)
+ * with one addition: `count` independent instances of the same problem,
+ * packed contiguously, are solved by one call (the reference solves one).
+ * count == 1 is exactly the reference call.
+ *
+ * Layouts (row-major, sample-major, as in the reference):
+ *   matrix input     in    [count][samples][dim][dim]
+ *   SpaceCurve input in    [count][samples][3]
+ *   output (one/sum) out   [count][dim][dim]          SC: [count][3]
+ *   output (many)    out   [count][n][dim][dim]       SC: [count][n][3]
+ *   carry            carry [count][Q][n-1][samples][dim][dim]   Q = (n+3)/2
+ *                          (SC: dim = 2, i.e. 4 scalars per sample)
+ *                          src/graddata.hpp:27-41, src/composer/backends.hpp:104-106
+ *   cotangent        same shape as the output
+ *   dA               same shape as the input
+ *
+ * Error contract: a nonzero status is returned and magnus_last_error() (a
+ * thread-local string) holds the reference's message text for the same
+ * condition, e.g. "invalid dispatch option (based on params)".
+ */
+#ifndef MAGNUS_CAPI_H
+#define MAGNUS_CAPI_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* status codes */
+#define MAGNUS_OK 0
+#define MAGNUS_EINVAL 1  /* std::invalid_argument in the reference -> ValueError */
+#define MAGNUS_ERANGE 2  /* std::out_of_range -> IndexError                      */
+#define MAGNUS_ENOMEM 3  /* device allocation failed                             */
+#define MAGNUS_EDEVICE 4 /* HIP runtime / launch error                           */
+
+/* kernel ops, src/composer/dispatch.hpp:285-293 */
+#define MAGNUS_OP_ONE 0
+#define MAGNUS_OP_MANY 1
+#define MAGNUS_OP_SUM 2
+
+/* "Auto" for every registry */
+#define MAGNUS_AUTO (-1)
+
+/* matrix backends, src/linalg/backends.hpp:63-68 */
+#define MAGNUS_MAT_FIXED2 0
+#define MAGNUS_MAT_MANUAL 1
+#define MAGNUS_MAT_BLAS 2
+#define MAGNUS_MAT_SPACECURVE 3
+
+/* integrators, src/integration/backends.hpp:53-58 */
+#define MAGNUS_INT_BOOLE 0
+#define MAGNUS_INT_SIMPSON 1
+#define MAGNUS_INT_TRAPEZOID 2
+#define MAGNUS_INT_RIEMANN 3
+
+/* GL lane executors, src/util/gausslegendre_backends.hpp:55-59 (validated only:
+ * on the device the GL nodes are a grid dimension) */
+#define MAGNUS_GL_THREADPOOL 0
+#define MAGNUS_GL_OPENMP 1
+#define MAGNUS_GL_SERIAL 2
+
+/* Problem description; mirrors ParamsBase + the resolved dispatch indices
+ * (src/composer/dispatch.hpp:30-36, src/composer/backends.cpp:65-102). */
+typedef struct magnus_problem {
+    uint64_t n;       /* Magnus order, >= 1                       */
+    uint64_t dim;     /* matrix dimension (ignored by the SC calls) */
+    uint64_t samples; /* uniform samples on [t0, tf], >= 2          */
+    double t0;
+    double tf;
+    int32_t op;             /* MAGNUS_OP_*                          */
+    int32_t matrix_backend; /* MAGNUS_MAT_* or MAGNUS_AUTO          */
+    int32_t integrator;     /* MAGNUS_INT_* or MAGNUS_AUTO          */
+    int32_t gl_backend;     /* MAGNUS_GL_*  or MAGNUS_AUTO          */
+} magnus_problem;
+
+/* ---- name resolution: type_list::resolve (dispatch.hpp:119-129) and
+ *      op_from_str (dispatch.hpp:295-301).  "Auto" -> MAGNUS_AUTO. ---- */
+int magnus_resolve_op(const char* name, int32_t* out);
+int magnus_resolve_matrix_backend(const char* name, int32_t* out);
+int magnus_resolve_integrator(const char* name, int32_t* out);
+int magnus_resolve_gl_backend(const char* name, int32_t* out);
+
+/* dispatch names in registry order, NULL past the end (pybind.cpp:38-55) */
+const char* magnus_matrix_backend_name(int32_t idx);
+const char* magnus_integrator_name(int32_t idx);
+const char* magnus_gl_backend_name(int32_t idx);
+const char* magnus_op_name(int32_t idx);
+const char* magnus_numeric_backend_name(int32_t idx);
+
+/* ---- Gauss-Legendre table: pybind.cpp:57-59 (max_order) and
+ *      pybind.cpp:102-116 + util/gausslegendre.cpp:74-111 (replace).
+ *      Host arrays, flattened by order: order o occupies o entries. ---- */
+uint64_t magnus_max_order(void);
+int magnus_replace_gl_table(uint64_t max_order, const double* weights, const double* nodes);
+int magnus_gl_order(uint64_t order, double* weights, double* nodes);
+
+/* ---- sizes (scalars per instance) ---- */
+uint64_t magnus_carry_elems(const magnus_problem* p);    /* Q*(n-1)*S*dim*dim   */
+uint64_t magnus_output_elems(const magnus_problem* p);   /* n_out*dim*dim       */
+uint64_t magnus_sc_carry_elems(const magnus_problem* p); /* Q*(n-1)*S*4         */
+uint64_t magnus_sc_output_elems(const magnus_problem* p);/* n_out*3             */
+
+/* ---- validation only (no device work): the checks and messages the compute
+ *      calls apply before touching memory.  spacecurve != 0 validates the
+ *      SC entry points; forward != 0 also validates gl_backend. ---- */
+int magnus_validate(const magnus_problem* p, int spacecurve, int forward);
+
+/* ---- compute: device pointers; carry may be NULL ---- */
+
+/* Forward.  carry != NULL records the VJP carry (record_vjp=True). */
+int magnus_forward_f64(const magnus_problem* p, uint64_t count, const double* in, double* out,
+                       double* carry, void* stream);
+
+/* Reverse sweep.  carry == NULL recomputes the forward internally
+ * (VJP::none, src/grad.hpp:14-56). */
+int magnus_vjp_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot,
+                   const double* carry, double* dA, void* stream);
+
+/* SpaceCurve (su(2)) variants: (S,3) input padded to 4-vectors on the fly. */
+int magnus_sc_forward_f64(const magnus_problem* p, uint64_t count, const double* in3,
+                          double* out3, double* carry, void* stream);
+int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3,
+                      const double* cot3, const double* carry, double* dA3, void* stream);
+
+/* ---- kernel timing (diagnostics).  While enabled, every main kernel launch
+ *      is bracketed by a HIP event pair recorded on its launch stream.
+ *      magnus_timing_read waits for the recorded events and returns, for the
+ *      kernel family `name` (e.g. "ep_forward", "ep_vjp"), the number of
+ *      launches and their total duration in milliseconds since the last reset. ---- */
+int magnus_timing_enable(int on);
+int magnus_timing_read(const char* name, uint64_t* launches, double* total_ms);
+void magnus_timing_reset(void);
+
+/* thread-local message of the last failing call on this thread */
+const char* magnus_last_error(void);
+
+/* name of the kernel family the last successful call on this thread used
+ * (diagnostics: "ep", "tile", "mfma", ...) */
+const char* magnus_last_kernel(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* MAGNUS_CAPI_H */

@@ -1,0 +1,578 @@
+// capi.cpp -- host side of the C ABI (include/magnus_capi.h).
+//
+// Mirrors the reference composer boundary: name resolution and "Auto" rules
+// (composer/dispatch.hpp:107-185, linalg/backends.hpp:11-68,
+// integration/backends.hpp:53-58, util/gausslegendre_backends.hpp:11-59),
+// shape and order validation with the reference's message texts
+// (composer/backends.hpp:104-180, composer/backends.cpp:26-61), and the
+// process-global, replaceable Gauss-Legendre table (util/gausslegendre.cpp).
+// Device work is enqueued on the caller's stream; temporary device buffers
+// are stream-ordered allocations freed on the same stream.
+#include "magnus_capi.h"
+
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <limits>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "kernels.hpp"
+
+namespace {
+
+#include "gl_table_data.inc"
+
+thread_local std::string g_err;
+thread_local const char* g_kernel = "";
+
+int fail(int code, const std::string& msg) {
+    g_err = msg;
+    return code;
+}
+
+// ---------------------------------------------------------------------------
+// registries
+// ---------------------------------------------------------------------------
+const char* const kMatrixNames[] = {"Fixed2", "Manual", "Blas", "SpaceCurve"};
+const char* const kIntegratorNames[] = {"Boole", "Simpson", "Trapezoid", "Riemann"};
+const char* const kGLNames[] = {"threadpool", "openmp", "serial"};
+const char* const kOpNames[] = {"one", "many", "sum"};
+const char* const kNumericNames[] = {"float", "double", "std::complex<float>", "std::complex<double>"};
+
+constexpr const char* kResolveError = "Could not resolve dispatch name option";
+constexpr const char* kInvalidOption = "invalid dispatch option (based on params)";
+constexpr const char* kOpError =
+    "could not deduce kernel op from str in Magnus::Dispatch::op_from_str(std::string_view)";
+constexpr const char* kDivisibility =
+    "sample interval count violates integrator divisibility (a power of 2 is recommneded; i.e., total "
+    "samples is 2^k + 1 for some k > 4)";
+constexpr const char* kOrderError =
+    "requested magnus order exceeds GL table. Consider providing a larger table.";
+
+int resolve(const char* name, const char* const* names, int count, bool auto_enabled, int32_t* out) {
+    if (!name || !out) return fail(MAGNUS_EINVAL, "null argument");
+    if (auto_enabled && std::strcmp(name, "Auto") == 0) {
+        *out = MAGNUS_AUTO;
+        return MAGNUS_OK;
+    }
+    for (int i = 0; i < count; ++i)
+        if (std::strcmp(name, names[i]) == 0) {
+            *out = i;
+            return MAGNUS_OK;
+        }
+    return fail(MAGNUS_EINVAL, kResolveError);
+}
+
+// ---------------------------------------------------------------------------
+// Gauss-Legendre table
+// ---------------------------------------------------------------------------
+struct GLTable {
+    uint64_t max_order;
+    std::vector<double> w, x;  // flattened by order, order o at o(o-1)/2
+};
+
+std::mutex g_gl_mu;
+std::shared_ptr<const GLTable> g_gl;
+
+std::shared_ptr<const GLTable> gl_get() {
+    std::lock_guard<std::mutex> lock(g_gl_mu);
+    if (!g_gl) {
+        auto t = std::make_shared<GLTable>();
+        t->max_order = kGLDefaultMaxOrder;
+        const size_t total = kGLDefaultMaxOrder * (kGLDefaultMaxOrder + 1) / 2;
+        t->w.assign(kGLDefaultWeights, kGLDefaultWeights + total);
+        t->x.assign(kGLDefaultNodes, kGLDefaultNodes + total);
+        g_gl = t;
+    }
+    return g_gl;
+}
+
+size_t gl_offset(uint64_t order) { return (size_t)(order - 1) * order / 2; }
+
+// ---------------------------------------------------------------------------
+// validation
+// ---------------------------------------------------------------------------
+struct Plan {
+    uint64_t n, S, d, m, m_in, m_cot, Q, Qj, n_out;
+    int op, mat, integ;
+    bool sc, pad;
+    double dt;
+    std::shared_ptr<const GLTable> gl;
+};
+
+int divisibility(int integ) {  // boole.hpp:18, simpson.hpp:18, trapezoid.hpp:18, default.hpp:15
+    switch (integ) {
+        case MAGNUS_INT_BOOLE: return 4;
+        case MAGNUS_INT_SIMPSON: return 2;
+        case MAGNUS_INT_TRAPEZOID: return 2;
+        default: return 1;
+    }
+}
+
+// matrix_input_shape (composer/backends.hpp:108-122) + dispatch_plan
+// (composer/backends.cpp:26-61) + GL backend dispatch (backends.cpp:65-93)
+int make_plan(const magnus_problem* p, bool spacecurve, bool forward, Plan* out) {
+    if (!p) return fail(MAGNUS_EINVAL, "problem must not be null");
+    if (p->n == 0) return fail(MAGNUS_EINVAL, "n must be at least 1");
+    if (p->samples < 2) return fail(MAGNUS_EINVAL, "data must contain at least two samples");
+    const uint64_t dim = spacecurve ? 2 : p->dim;
+    if (dim == 0) return fail(MAGNUS_EINVAL, "data must contain square matrices");
+    if (dim > std::numeric_limits<uint64_t>::max() / dim) return fail(MAGNUS_EINVAL, "matrix dimension is too large");
+    if (dim * dim > std::numeric_limits<uint64_t>::max() / p->samples)
+        return fail(MAGNUS_EINVAL, "input shape is too large");
+    if (p->op < 0 || p->op > 2) return fail(MAGNUS_EINVAL, "invalid kernel operation");
+
+    auto gl = gl_get();
+    if (p->n > 1) {
+        const uint64_t order = (p->n + 3) / 2;
+        if (order > gl->max_order) return fail(MAGNUS_EINVAL, kOrderError);
+    }
+
+    // matrix backend: Auto priority Fixed2 (d==2) -> Manual (d<64) -> Blas
+    int mat = spacecurve ? MAGNUS_MAT_SPACECURVE : p->matrix_backend;
+    if (mat == MAGNUS_AUTO) {
+        mat = (dim == 2) ? MAGNUS_MAT_FIXED2 : (dim < 64 ? MAGNUS_MAT_MANUAL : MAGNUS_MAT_BLAS);
+    } else if (mat < 0 || mat > 3) {
+        return fail(MAGNUS_ERANGE, "dispatch index out of range");
+    } else if ((mat == MAGNUS_MAT_FIXED2 || mat == MAGNUS_MAT_SPACECURVE) && dim != 2) {
+        return fail(MAGNUS_EINVAL, kInvalidOption);
+    }
+
+    int integ = p->integrator;
+    if (integ == MAGNUS_AUTO) integ = MAGNUS_INT_BOOLE;
+    else if (integ < 0 || integ > 3) return fail(MAGNUS_ERANGE, "dispatch index out of range");
+
+    if ((p->samples - 1) % (uint64_t)divisibility(integ) != 0) return fail(MAGNUS_EINVAL, kDivisibility);
+
+    if (forward) {  // GL lane executor: validated only (the GL nodes are a grid dimension)
+        const int glb = p->gl_backend;
+        if (glb == MAGNUS_GL_THREADPOOL) return fail(MAGNUS_EINVAL, kInvalidOption);  // no XLA pool here
+        if (glb != MAGNUS_AUTO && (glb < 0 || glb > 2)) return fail(MAGNUS_ERANGE, "dispatch index out of range");
+    }
+
+    const uint64_t m = dim * dim;
+    if (m > 16384) return fail(MAGNUS_EINVAL, "matrix dimension exceeds the device kernel limit (dim <= 128)");
+
+    out->n = p->n;
+    out->S = p->samples;
+    out->d = dim;
+    out->m = m;
+    out->sc = (mat == MAGNUS_MAT_SPACECURVE);
+    out->pad = spacecurve;
+    out->m_in = spacecurve ? 3 : m;
+    out->m_cot = spacecurve ? 3 : m;
+    out->Q = (p->n + 3) / 2;
+    out->Qj = (p->n > 1) ? out->Q : 1;
+    out->n_out = (p->op == MAGNUS_OP_MANY) ? p->n : 1;
+    out->op = p->op;
+    out->mat = mat;
+    out->integ = integ;
+    out->dt = (p->tf - p->t0) / (double)(p->samples - 1);  // magnus.hpp:32
+    out->gl = gl;
+    if (out->Q > (uint64_t)magnus::kArgMaxQ && p->n > 1)
+        return fail(MAGNUS_EINVAL, kOrderError);
+    return MAGNUS_OK;
+}
+
+// stream-ordered scratch allocation, freed on scope exit on the same stream
+struct DeviceScratch {
+    hipStream_t stream;
+    std::vector<void*> ptrs;
+    explicit DeviceScratch(hipStream_t s) : stream(s) {}
+    ~DeviceScratch() {
+        for (void* p : ptrs) (void)hipFreeAsync(p, stream);
+    }
+    template <class T>
+    T* alloc(size_t count) {
+        if (count == 0) return nullptr;
+        void* p = nullptr;
+        if (hipMallocAsync(&p, count * sizeof(T), stream) != hipSuccess) {
+            (void)hipGetLastError();
+            return nullptr;
+        }
+        ptrs.push_back(p);
+        return static_cast<T*>(p);
+    }
+};
+
+struct EpShape {
+    int tpj, jpw, block, ept;
+};
+
+EpShape ep_shape(uint64_t m) {
+    EpShape s{};
+    if (m <= 256) {
+        s.block = 256;
+        s.tpj = (int)m;
+        s.jpw = 256 / (int)m;
+        s.ept = 1;
+    } else if (m <= 1024) {
+        s.block = 256;
+        s.tpj = 256;
+        s.jpw = 1;
+        s.ept = (m <= 512) ? 2 : 4;
+    } else {
+        s.block = 1024;
+        s.tpj = 1024;
+        s.jpw = 1;
+        const uint64_t e = (m + 1023) / 1024;
+        s.ept = e <= 2 ? 2 : (e <= 4 ? 4 : (e <= 8 ? 8 : 16));
+    }
+    return s;
+}
+
+void fill_gl(const Plan& pl, double* w, double* x) {
+    for (int i = 0; i < magnus::kArgMaxQ; ++i) {
+        w[i] = 0.0;
+        x[i] = 0.0;
+    }
+    if (pl.n < 2) return;
+    const size_t off = gl_offset(pl.Q);
+    for (uint64_t i = 0; i < pl.Q; ++i) {
+        w[i] = pl.gl->w[off + i];
+        x[i] = pl.gl->x[off + i];
+    }
+}
+
+// ---------------------------------------------------------------------------
+// kernel timing
+// ---------------------------------------------------------------------------
+struct TimedLaunch {
+    std::string name;
+    hipEvent_t start, stop;
+};
+struct TimingTotals {
+    uint64_t launches = 0;
+    double ms = 0.0;
+};
+std::mutex g_timing_mu;
+bool g_timing_on = false;
+std::vector<TimedLaunch> g_pending;
+std::vector<std::pair<std::string, TimingTotals>> g_totals;
+
+struct KernelTimer {
+    hipEvent_t start = nullptr, stop = nullptr;
+    hipStream_t stream;
+    const char* name;
+    KernelTimer(const char* n, hipStream_t s) : stream(s), name(n) {
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        if (!g_timing_on) return;
+        if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
+            start = stop = nullptr;
+            return;
+        }
+        (void)hipEventRecord(start, stream);
+    }
+    void done() {
+        if (!start) return;
+        (void)hipEventRecord(stop, stream);
+        std::lock_guard<std::mutex> lock(g_timing_mu);
+        g_pending.push_back({name, start, stop});
+        start = stop = nullptr;
+    }
+};
+
+void timing_drain_locked() {
+    for (auto& t : g_pending) {
+        float ms = 0.f;
+        if (hipEventSynchronize(t.stop) == hipSuccess && hipEventElapsedTime(&ms, t.start, t.stop) == hipSuccess) {
+            auto it = g_totals.begin();
+            for (; it != g_totals.end(); ++it)
+                if (it->first == t.name) break;
+            if (it == g_totals.end()) {
+                g_totals.push_back({t.name, TimingTotals{}});
+                it = g_totals.end() - 1;
+            }
+            it->second.launches += 1;
+            it->second.ms += ms;
+        }
+        (void)hipEventDestroy(t.start);
+        (void)hipEventDestroy(t.stop);
+    }
+    g_pending.clear();
+}
+
+int device_error(hipError_t e, const char* where) {
+    return fail(MAGNUS_EDEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// forward: record into `carry` when non-null; outputs=false runs VJP::none
+int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, double* carry, bool outputs,
+                hipStream_t stream) {
+    DeviceScratch scratch(stream);
+    const long long B = (long long)count;
+    const long long jobs = B * (long long)pl.Qj;
+    const uint64_t S = pl.S, m = pl.m;
+    double* partial = outputs ? scratch.alloc<double>((size_t)jobs * pl.n_out * m) : nullptr;
+    double* omega1 = outputs ? scratch.alloc<double>((size_t)B * m) : nullptr;
+    double* pingpong = nullptr;
+    if (!carry && pl.n >= 2) pingpong = scratch.alloc<double>((size_t)jobs * 2 * S * m);
+    if ((outputs && (!partial || !omega1)) || (!carry && pl.n >= 2 && !pingpong))
+        return fail(MAGNUS_ENOMEM, "device allocation failed");
+
+    magnus::EpFwdArgs a{};
+    a.in = in;
+    a.carry = carry;
+    a.pingpong = pingpong;
+    a.partial = partial;
+    a.omega1 = omega1;
+    a.S = (long long)S;
+    a.jobs = jobs;
+    a.n = (int)pl.n;
+    a.d = (int)pl.d;
+    a.m = (int)m;
+    a.m_in = (int)pl.m_in;
+    a.Q = (int)pl.Q;
+    a.Qj = (int)pl.Qj;
+    a.n_out = (int)pl.n_out;
+    a.op = pl.op;
+    a.sc = pl.sc ? (pl.pad ? 1 : 2) : 0;
+    a.outputs = outputs ? 1 : 0;
+    a.dt = pl.dt;
+    fill_gl(pl, a.w, a.x);
+    const EpShape sh = ep_shape(m);
+    a.tpj = sh.tpj;
+    a.jpw = sh.jpw;
+    const long long grid = (jobs + sh.jpw - 1) / sh.jpw;
+    KernelTimer tf(outputs ? "ep_forward" : "ep_forward_recompute", stream);
+    hipError_t e = magnus::launch_ep_forward(pl.integ, a, sh.block, grid, sh.ept, stream);
+    tf.done();
+    if (e != hipSuccess) return device_error(e, "forward kernel launch");
+    if (outputs) {
+        magnus::FwdReduceArgs r{};
+        r.partial = partial;
+        r.omega1 = omega1;
+        r.out = out;
+        r.B = B;
+        r.n = (int)pl.n;
+        r.m = (int)m;
+        r.m_out = pl.pad ? 3 : (int)m;
+        r.n_out = (int)pl.n_out;
+        r.Qj = (int)pl.Qj;
+        r.op = pl.op;
+        r.sc = pl.pad ? 1 : 0;
+        e = magnus::launch_fwd_reduce(r, stream);
+        if (e != hipSuccess) return device_error(e, "reduce kernel launch");
+    }
+    g_kernel = "ep";
+    return MAGNUS_OK;
+}
+
+int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, const double* carry,
+            double* dA, hipStream_t stream) {
+    DeviceScratch scratch(stream);
+    const long long B = (long long)count;
+    const uint64_t S = pl.S, m = pl.m;
+    long long jobs = 0;
+    double* dAjob = nullptr;
+    if (pl.n >= 2) {
+        jobs = B * (long long)pl.Q;
+        if (!carry) {  // VJP::none recompute (grad.hpp:95-110)
+            double* tmp = scratch.alloc<double>((size_t)B * pl.Q * (pl.n - 1) * S * m);
+            if (!tmp) return fail(MAGNUS_ENOMEM, "device allocation failed");
+            int rc = run_forward(pl, count, in, nullptr, tmp, false, stream);
+            if (rc) return rc;
+            carry = tmp;
+        }
+        double* gbuf = scratch.alloc<double>((size_t)jobs * 2 * S * m);
+        dAjob = scratch.alloc<double>((size_t)jobs * S * m);
+        if (!gbuf || !dAjob) return fail(MAGNUS_ENOMEM, "device allocation failed");
+        magnus::EpVjpArgs a{};
+        a.in = in;
+        a.cot = cot;
+        a.carry = carry;
+        a.gbuf = gbuf;
+        a.dAjob = dAjob;
+        a.S = (long long)S;
+        a.jobs = jobs;
+        a.n = (int)pl.n;
+        a.d = (int)pl.d;
+        a.m = (int)m;
+        a.m_in = (int)pl.m_in;
+        a.m_cot = (int)pl.m_cot;
+        a.Q = (int)pl.Q;
+        a.n_out = (int)pl.n_out;
+        a.op = pl.op;
+        a.sc = pl.sc ? (pl.pad ? 1 : 2) : 0;
+        a.bt_fused = (pl.mat == MAGNUS_MAT_FIXED2 || pl.mat == MAGNUS_MAT_SPACECURVE) ? 1 : 0;
+        a.dt = pl.dt;
+        fill_gl(pl, a.w, a.x);
+        const EpShape sh = ep_shape(m);
+        a.tpj = sh.tpj;
+        a.jpw = sh.jpw;
+        const long long grid = (jobs + sh.jpw - 1) / sh.jpw;
+        KernelTimer tv("ep_vjp", stream);
+        hipError_t e = magnus::launch_ep_vjp(pl.integ, a, sh.block, grid, sh.ept, stream);
+        tv.done();
+        if (e != hipSuccess) return device_error(e, "vjp kernel launch");
+    }
+    magnus::VjpReduceArgs r{};
+    r.dAjob = dAjob;
+    r.cot = cot;
+    r.dA = dA;
+    r.B = B;
+    r.S = (long long)S;
+    r.n = (int)pl.n;
+    r.m = (int)m;
+    r.m_out = pl.pad ? 3 : (int)m;
+    r.m_cot = (int)pl.m_cot;
+    r.n_out = (int)pl.n_out;
+    r.Qj = pl.n >= 2 ? (int)pl.Q : 0;
+    r.op = pl.op;
+    r.sc = pl.pad ? 1 : 0;
+    r.dt = pl.dt;
+    hipError_t e = magnus::launch_vjp_reduce(pl.integ, r, stream);
+    if (e != hipSuccess) return device_error(e, "vjp reduce launch");
+    g_kernel = "ep";
+    return MAGNUS_OK;
+}
+
+uint64_t carry_elems(const magnus_problem* p, uint64_t dim) {
+    if (!p || p->n < 2) return 0;
+    return ((p->n + 3) / 2) * (p->n - 1) * p->samples * dim * dim;
+}
+
+}  // namespace
+
+extern "C" {
+
+int magnus_resolve_op(const char* name, int32_t* out) {
+    if (!name || !out) return fail(MAGNUS_EINVAL, "null argument");
+    static const char* const upper[] = {"ONE", "MANY", "SUM"};
+    for (int i = 0; i < 3; ++i)
+        if (std::strcmp(name, kOpNames[i]) == 0 || std::strcmp(name, upper[i]) == 0) {
+            *out = i;
+            return MAGNUS_OK;
+        }
+    return fail(MAGNUS_EINVAL, kOpError);
+}
+int magnus_resolve_matrix_backend(const char* name, int32_t* out) { return resolve(name, kMatrixNames, 4, true, out); }
+int magnus_resolve_integrator(const char* name, int32_t* out) { return resolve(name, kIntegratorNames, 4, true, out); }
+int magnus_resolve_gl_backend(const char* name, int32_t* out) { return resolve(name, kGLNames, 3, true, out); }
+
+const char* magnus_matrix_backend_name(int32_t i) { return (i >= 0 && i < 4) ? kMatrixNames[i] : nullptr; }
+const char* magnus_integrator_name(int32_t i) { return (i >= 0 && i < 4) ? kIntegratorNames[i] : nullptr; }
+const char* magnus_gl_backend_name(int32_t i) { return (i >= 0 && i < 3) ? kGLNames[i] : nullptr; }
+const char* magnus_op_name(int32_t i) { return (i >= 0 && i < 3) ? kOpNames[i] : nullptr; }
+const char* magnus_numeric_backend_name(int32_t i) { return (i >= 0 && i < 4) ? kNumericNames[i] : nullptr; }
+
+uint64_t magnus_max_order(void) { return gl_get()->max_order; }
+
+int magnus_replace_gl_table(uint64_t max_order, const double* weights, const double* nodes) {
+    // OwningTable constructor checks (util/gausslegendre.cpp:74-99)
+    if (max_order == 0) return fail(MAGNUS_EINVAL, "OwningTable requires max_order >= 1");
+    if (!weights || !nodes) return fail(MAGNUS_EINVAL, "GL data must not be null");
+    auto t = std::make_shared<GLTable>();
+    const size_t total = (size_t)max_order * (max_order + 1) / 2;
+    t->max_order = max_order;
+    t->w.assign(weights, weights + total);
+    t->x.assign(nodes, nodes + total);
+    std::lock_guard<std::mutex> lock(g_gl_mu);
+    g_gl = t;
+    return MAGNUS_OK;
+}
+
+int magnus_gl_order(uint64_t order, double* weights, double* nodes) {
+    auto gl = gl_get();
+    if (order == 0 || order > gl->max_order)
+        return fail(MAGNUS_ERANGE, "Gauss-Legendre order is outside the available table");
+    const size_t off = gl_offset(order);
+    for (uint64_t i = 0; i < order; ++i) {
+        if (weights) weights[i] = gl->w[off + i];
+        if (nodes) nodes[i] = gl->x[off + i];
+    }
+    return MAGNUS_OK;
+}
+
+int magnus_validate(const magnus_problem* p, int spacecurve, int forward) {
+    Plan pl;
+    return make_plan(p, spacecurve != 0, forward != 0, &pl);
+}
+
+uint64_t magnus_carry_elems(const magnus_problem* p) { return p ? carry_elems(p, p->dim) : 0; }
+uint64_t magnus_sc_carry_elems(const magnus_problem* p) { return carry_elems(p, 2); }
+uint64_t magnus_output_elems(const magnus_problem* p) {
+    if (!p) return 0;
+    return (p->op == MAGNUS_OP_MANY ? p->n : 1) * p->dim * p->dim;
+}
+uint64_t magnus_sc_output_elems(const magnus_problem* p) {
+    if (!p) return 0;
+    return (p->op == MAGNUS_OP_MANY ? p->n : 1) * 3;
+}
+
+int magnus_forward_f64(const magnus_problem* p, uint64_t count, const double* in, double* out, double* carry,
+                       void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, true, &pl)) return rc;
+    if (count == 0) return MAGNUS_OK;
+    if (!in) return fail(MAGNUS_EINVAL, "input buffer must not be null");
+    if (!out) return fail(MAGNUS_EINVAL, "output buffer must not be null");
+    return run_forward(pl, count, in, out, pl.n >= 2 ? carry : nullptr, true, (hipStream_t)stream);
+}
+
+int magnus_vjp_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot,
+                   const double* carry, double* dA, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, false, &pl)) return rc;
+    if (count == 0) return MAGNUS_OK;
+    if (!in) return fail(MAGNUS_EINVAL, "input buffer must not be null");
+    if (!dA) return fail(MAGNUS_EINVAL, "output buffer must not be null");
+    if (!cot) return fail(MAGNUS_EINVAL, "cotangent buffer must not be null");
+    return run_vjp(pl, count, in, cot, carry, dA, (hipStream_t)stream);
+}
+
+int magnus_sc_forward_f64(const magnus_problem* p, uint64_t count, const double* in3, double* out3,
+                          double* carry, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, true, true, &pl)) return rc;
+    if (count == 0) return MAGNUS_OK;
+    if (!in3) return fail(MAGNUS_EINVAL, "input buffer must not be null");
+    if (!out3) return fail(MAGNUS_EINVAL, "output buffer must not be null");
+    return run_forward(pl, count, in3, out3, pl.n >= 2 ? carry : nullptr, true, (hipStream_t)stream);
+}
+
+int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3, const double* cot3,
+                      const double* carry, double* dA3, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, true, false, &pl)) return rc;
+    if (count == 0) return MAGNUS_OK;
+    if (!in3) return fail(MAGNUS_EINVAL, "input buffer must not be null");
+    if (!dA3) return fail(MAGNUS_EINVAL, "output buffer must not be null");
+    if (!cot3) return fail(MAGNUS_EINVAL, "cotangent buffer must not be null");
+    return run_vjp(pl, count, in3, cot3, carry, dA3, (hipStream_t)stream);
+}
+
+int magnus_timing_enable(int on) {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    g_timing_on = on != 0;
+    return MAGNUS_OK;
+}
+
+int magnus_timing_read(const char* name, uint64_t* launches, double* total_ms) {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    timing_drain_locked();
+    uint64_t n = 0;
+    double ms = 0.0;
+    for (auto& t : g_totals)
+        if (name && t.first == name) {
+            n = t.second.launches;
+            ms = t.second.ms;
+        }
+    if (launches) *launches = n;
+    if (total_ms) *total_ms = ms;
+    return MAGNUS_OK;
+}
+
+void magnus_timing_reset(void) {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    timing_drain_locked();
+    g_totals.clear();
+}
+
+const char* magnus_last_error(void) { return g_err.c_str(); }
+const char* magnus_last_kernel(void) { return g_kernel; }
+
+}  // extern "C"

@@ -1,0 +1,157 @@
+"""Device-tensor front end: batched Magnus on torch tensors resident on the GPU.
+
+The tensors' device memory is handed to the C ABI directly (no host staging)
+and work is enqueued on torch's current HIP stream.  `MagnusFunction` /
+`SpaceCurveFunction` are torch.autograd.Functions whose backward is the HIP
+reverse sweep, the counterpart of the reference's JAX custom_vjp
+(src/api/magnus/jax.py:209-342): the forward residual is the input plus,
+when recorded, the carry.
+
+Shapes (B = batch of independent instances):
+  matrix:     A (B, S, d, d) -> out (B, d, d) or (B, n, d, d)
+  SpaceCurve: a (B, S, 3)    -> out (B, 3)    or (B, n, 3)
+"""
+from __future__ import annotations
+
+import ctypes
+
+import torch
+
+from . import capi
+
+
+def _stream() -> ctypes.c_void_p:
+    return ctypes.c_void_p(torch.cuda.current_stream().cuda_stream)
+
+
+def _ptr(t):
+    return ctypes.c_void_p(t.data_ptr()) if t is not None else None
+
+
+def _check_input(A: torch.Tensor, rank: int, what: str) -> None:
+    if not A.is_cuda:
+        raise ValueError(f"{what} must be a GPU tensor (the MI355X build has no CPU path)")
+    if A.dtype != torch.float64:
+        raise TypeError(f"{what} must be float64")
+    if A.dim() != rank:
+        raise ValueError(f"{what} must have {rank} dimensions")
+
+
+def carry_shape(n: int, samples: int, dim: int):
+    return ((n + 3) // 2, n - 1, samples, dim, dim)
+
+
+def forward(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
+            matrix_backend: str = "Auto", integrator: str = "Auto", record_vjp: bool = False):
+    """Batched matrix Magnus: A (B, S, d, d) float64 on the GPU."""
+    _check_input(A, 4, "A")
+    A = A.contiguous()
+    B, S, d, d2 = A.shape
+    if d != d2:
+        raise ValueError("data must contain square matrices")
+    p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    out = torch.empty((B, n, d, d) if op == "many" else (B, d, d), dtype=A.dtype, device=A.device)
+    carry = torch.empty((B,) + carry_shape(n, S, d), dtype=A.dtype, device=A.device) if (record_vjp and n > 1) else None
+    capi.check(capi.lib().magnus_forward_f64(ctypes.byref(p), B, _ptr(A), _ptr(out), _ptr(carry), _stream()))
+    if record_vjp:
+        if carry is None:
+            carry = torch.empty((B,) + carry_shape(n, S, d), dtype=A.dtype, device=A.device)
+        return out, carry
+    return out
+
+
+def vjp(n: int, A: torch.Tensor, cot: torch.Tensor, carry: torch.Tensor | None = None, t0: float = 0.0,
+        tf: float = 1.0, *, op: str = "sum", matrix_backend: str = "Auto", integrator: str = "Auto"):
+    _check_input(A, 4, "A")
+    A = A.contiguous()
+    cot = cot.contiguous().to(torch.float64)
+    B, S, d, _ = A.shape
+    p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    dA = torch.empty_like(A)
+    c = carry.contiguous() if (carry is not None and n > 1) else None
+    capi.check(capi.lib().magnus_vjp_f64(ctypes.byref(p), B, _ptr(A), _ptr(cot), _ptr(c), _ptr(dA), _stream()))
+    return dA
+
+
+def sc_forward(n: int, a: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
+               integrator: str = "Auto", record_vjp: bool = False):
+    """Batched SpaceCurve Magnus: a (B, S, 3) float64 on the GPU."""
+    _check_input(a, 3, "a")
+    a = a.contiguous()
+    B, S, w = a.shape
+    if w != 3:
+        raise ValueError("data must have shape (samples, 3)")
+    p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
+    out = torch.empty((B, n, 3) if op == "many" else (B, 3), dtype=a.dtype, device=a.device)
+    carry = torch.empty((B,) + carry_shape(n, S, 2), dtype=a.dtype, device=a.device) if (record_vjp and n > 1) else None
+    capi.check(capi.lib().magnus_sc_forward_f64(ctypes.byref(p), B, _ptr(a), _ptr(out), _ptr(carry), _stream()))
+    if record_vjp:
+        if carry is None:
+            carry = torch.empty((B,) + carry_shape(n, S, 2), dtype=a.dtype, device=a.device)
+        return out, carry
+    return out
+
+
+def sc_vjp(n: int, a: torch.Tensor, cot: torch.Tensor, carry: torch.Tensor | None = None, t0: float = 0.0,
+           tf: float = 1.0, *, op: str = "sum", integrator: str = "Auto"):
+    _check_input(a, 3, "a")
+    a = a.contiguous()
+    cot = cot.contiguous().to(torch.float64)
+    B, S, _ = a.shape
+    p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
+    da = torch.empty_like(a)
+    c = carry.contiguous() if (carry is not None and n > 1) else None
+    capi.check(capi.lib().magnus_sc_vjp_f64(ctypes.byref(p), B, _ptr(a), _ptr(cot), _ptr(c), _ptr(da), _stream()))
+    return da
+
+
+class MagnusFunction(torch.autograd.Function):
+    """out = Magnus(A); backward = HIP reverse sweep.  record_vjp keeps the carry
+    as the residual; otherwise the backward recomputes it (VJP::none)."""
+
+    @staticmethod
+    def forward(ctx, A, n, t0, tf, op, matrix_backend, integrator, record_vjp):
+        if record_vjp:
+            out, carry = forward(n, A, t0, tf, op=op, matrix_backend=matrix_backend, integrator=integrator,
+                                 record_vjp=True)
+        else:
+            out, carry = forward(n, A, t0, tf, op=op, matrix_backend=matrix_backend, integrator=integrator), None
+        ctx.save_for_backward(A, carry if carry is not None else torch.empty(0, device=A.device))
+        ctx.cfg = (n, t0, tf, op, matrix_backend, integrator, carry is not None)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        A, carry = ctx.saved_tensors
+        n, t0, tf, op, mb, integ, has_carry = ctx.cfg
+        dA = vjp(n, A, g, carry if has_carry else None, t0, tf, op=op, matrix_backend=mb, integrator=integ)
+        return dA, None, None, None, None, None, None, None
+
+
+class SpaceCurveFunction(torch.autograd.Function):
+    @staticmethod
+    def forward(ctx, a, n, t0, tf, op, integrator, record_vjp):
+        if record_vjp:
+            out, carry = sc_forward(n, a, t0, tf, op=op, integrator=integrator, record_vjp=True)
+        else:
+            out, carry = sc_forward(n, a, t0, tf, op=op, integrator=integrator), None
+        ctx.save_for_backward(a, carry if carry is not None else torch.empty(0, device=a.device))
+        ctx.cfg = (n, t0, tf, op, integrator, carry is not None)
+        return out
+
+    @staticmethod
+    def backward(ctx, g):
+        a, carry = ctx.saved_tensors
+        n, t0, tf, op, integ, has_carry = ctx.cfg
+        da = sc_vjp(n, a, g, carry if has_carry else None, t0, tf, op=op, integrator=integ)
+        return da, None, None, None, None, None, None
+
+
+def magnus(A, n, t0=0.0, tf=1.0, *, op="sum", matrix_backend="Auto", integrator="Auto", record_vjp=True):
+    """Differentiable batched Magnus on GPU tensors (A: (B, S, d, d))."""
+    return MagnusFunction.apply(A, n, t0, tf, op, matrix_backend, integrator, record_vjp)
+
+
+def magnus_sc(a, n, t0=0.0, tf=1.0, *, op="sum", integrator="Auto", record_vjp=True):
+    """Differentiable batched SpaceCurve Magnus on GPU tensors (a: (B, S, 3))."""
+    return SpaceCurveFunction.apply(a, n, t0, tf, op, integrator, record_vjp)

@@ -1,0 +1,143 @@
+"""CPU-side checks of the product library: it loads, exports every function
+include/magnus_capi.h declares, and applies the reference's validation rules
+and message texts before any device work (no GPU needed)."""
+from __future__ import annotations
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from magnus import capi
+from helpers import matrix_values, spacecurve_values
+
+
+def test_library_exports_every_header_function():
+    L = capi.lib()
+    names = capi.header_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+
+
+def test_core_module_loads_native_library():
+    import magnus
+    from magnus import _core
+
+    assert "libmagnus_hip" in open("/proc/self/maps").read()
+    assert magnus.max_order() == 64
+    assert magnus.ops() == ("one", "many", "sum")
+    assert magnus.matrix_backends() == ("Fixed2", "Manual", "Blas", "SpaceCurve", "Auto")
+    assert magnus.integrators() == ("Boole", "Simpson", "Trapezoid", "Riemann", "Auto")
+    assert magnus.gl_backends() == ("threadpool", "openmp", "serial", "Auto")
+    assert "double" in magnus.numeric_backends() and "Auto" not in magnus.numeric_backends()
+    assert callable(_core.compute)
+
+
+def test_public_api_matches_reference_surface():
+    import magnus
+
+    assert set(magnus.__all__) == {
+        "max_order", "numeric_backends", "matrix_backends", "integrators", "gl_backends", "ops",
+        "replace_gl_table", "compute", "compute_sc", "compute_vjp", "compute_sc_vjp",
+    }
+    for removed in ["one", "many", "sum", "one_sc", "many_sc", "sum_sc"]:
+        assert not hasattr(magnus, removed)
+
+
+def _validate(**kw):
+    base = dict(n=4, dim=2, samples=9, t0=0.0, tf=1.0, op="sum", matrix_backend="Auto", integrator="Auto",
+                gl_backend="Auto")
+    base.update(kw)
+    sc = base.pop("sc", False)
+    fwd = base.pop("forward", True)
+    p = capi.problem(**base)
+    rc = capi.lib().magnus_validate(ctypes.byref(p), int(sc), int(fwd))
+    return rc, capi.lib().magnus_last_error().decode()
+
+
+def test_validation_rules_and_messages():
+    assert _validate()[0] == capi.OK
+    assert _validate(n=0) == (capi.EINVAL, "n must be at least 1")
+    assert _validate(samples=1) == (capi.EINVAL, "data must contain at least two samples")
+    assert _validate(dim=3, matrix_backend="Fixed2")[1] == "invalid dispatch option (based on params)"
+    assert _validate(dim=3, matrix_backend="SpaceCurve")[1] == "invalid dispatch option (based on params)"
+    assert "divisibility" in _validate(samples=7, integrator="Boole")[1]
+    assert _validate(samples=7, integrator="Simpson")[0] == capi.OK
+    assert "divisibility" in _validate(samples=8, integrator="Simpson")[1]
+    assert _validate(samples=8, integrator="Riemann")[0] == capi.OK
+    assert _validate(gl_backend="threadpool")[1] == "invalid dispatch option (based on params)"
+    assert _validate(gl_backend="threadpool", forward=False)[0] == capi.OK
+    rc, msg = _validate(n=127)  # Q = (n + 3) // 2 = 65 > max_order 64
+    assert rc == capi.EINVAL and msg.startswith("requested magnus order exceeds GL table")
+    assert _validate(n=126)[0] == capi.OK
+    with pytest.raises(ValueError, match="Could not resolve dispatch name option"):
+        capi.problem(4, 2, 9, 0.0, 1.0, matrix_backend="Missing")
+    with pytest.raises(ValueError, match="deduce kernel op"):
+        capi.problem(4, 2, 9, 0.0, 1.0, op="missing")
+
+
+def test_sizes():
+    p = capi.problem(4, 3, 9, 0.0, 1.0, op="many")
+    assert capi.lib().magnus_carry_elems(ctypes.byref(p)) == 3 * 3 * 9 * 9
+    assert capi.lib().magnus_output_elems(ctypes.byref(p)) == 4 * 9
+    assert capi.lib().magnus_sc_carry_elems(ctypes.byref(p)) == 3 * 3 * 9 * 4
+    assert capi.lib().magnus_sc_output_elems(ctypes.byref(p)) == 4 * 3
+    p1 = capi.problem(1, 3, 9, 0.0, 1.0)
+    assert capi.lib().magnus_carry_elems(ctypes.byref(p1)) == 0
+
+
+def test_default_gl_table_is_numpy_leggauss_bitwise():
+    L = capi.lib()
+    for order in (1, 2, 3, 5, 17, 64):
+        w = (ctypes.c_double * order)()
+        x = (ctypes.c_double * order)()
+        capi.check(L.magnus_gl_order(order, w, x))
+        xr, wr = np.polynomial.legendre.leggauss(order)
+        np.testing.assert_array_equal(np.array(x[:]), 0.5 * (xr + 1.0))
+        np.testing.assert_array_equal(np.array(w[:]), 0.5 * wr)
+
+
+def test_error_contract_through_public_api_without_device():
+    """These all fail during validation, before any device allocation."""
+    import magnus
+
+    def zeros3(t):
+        return np.zeros((np.asarray(t).size, 3, 3))
+
+    with pytest.raises(ValueError, match="invalid dispatch option"):
+        magnus.compute(1, zeros3, 0.0, 1.0, 9, op="one", matrix_backend="Fixed2")
+    with pytest.raises(ValueError, match="dispatch name"):
+        magnus.compute(1, zeros3, 0.0, 1.0, 9, op="one", matrix_backend="Missing")
+    with pytest.raises(ValueError, match="dispatch name"):
+        magnus.compute(1, zeros3, 0.0, 1.0, 9, op="one", integrator="Missing")
+    with pytest.raises(ValueError, match="dispatch name"):
+        magnus.compute(1, zeros3, 0.0, 1.0, 9, op="one", gl_backend="Missing")
+    with pytest.raises(ValueError, match="deduce kernel op"):
+        magnus.compute(1, zeros3, 0.0, 1.0, 9, op="missing")
+    with pytest.raises(ValueError, match="at least two samples"):
+        magnus.compute(1, matrix_values, 0.0, 1.0, 1)
+    with pytest.raises(ValueError, match="at least two samples"):
+        magnus.compute_sc(1, spacecurve_values, 0.0, 1.0, 1)
+    with pytest.raises(ValueError, match="n must be at least 1"):
+        magnus.compute(0, matrix_values, 0.0, 1.0, 9)
+    with pytest.raises(ValueError, match="n must be at least 1"):
+        magnus.compute_sc(0, spacecurve_values, 0.0, 1.0, 9)
+    with pytest.raises(TypeError, match="supports dtypes"):
+        magnus.compute(1, lambda t: matrix_values(t).astype(np.int64), 0.0, 1.0, 9)
+    with pytest.raises(TypeError, match="supports dtypes"):
+        magnus.compute_sc(1, lambda t: spacecurve_values(t).astype(np.int64), 0.0, 1.0, 9)
+    with pytest.raises(ValueError, match="shape"):
+        magnus.compute_sc(1, lambda t: np.zeros((np.asarray(t).size, 4)), 0.0, 1.0, 9)
+    for bad in (np.zeros((9, 2)), np.zeros((8, 2, 2)), np.zeros((9, 2, 3))):
+        with pytest.raises(ValueError):
+            magnus.compute(1, lambda t, b=bad: b, 0.0, 1.0, 9)
+    for bad in (np.zeros((9, 2)), np.zeros((8, 3)), np.zeros((9, 3, 1))):
+        with pytest.raises(ValueError):
+            magnus.compute_sc(1, lambda t, b=bad: b, 0.0, 1.0, 9)
+    with pytest.raises(ValueError, match="divisibility"):
+        magnus.compute(2, matrix_values, 0.0, 1.0, 7, integrator="Boole")
+    with pytest.raises(ValueError, match="one/sum output must have shape"):
+        magnus.compute_vjp(2, matrix_values, np.zeros((3, 3)), 0.0, 1.0, 9)
+    with pytest.raises(ValueError, match="VJP data has an invalid shape"):
+        magnus.compute_vjp(2, matrix_values, np.zeros((2, 2)), 0.0, 1.0, 9, vjp_data=np.zeros((3, 1, 9, 2, 2)))

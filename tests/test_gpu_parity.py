@@ -1,0 +1,301 @@
+"""GPU parity: the HIP path (through the reference-shaped `magnus` API and the
+batched C ABI) against the CPU oracle on the same seeded inputs.
+
+Bar (north star): relative Frobenius error <= 1e-12 over the whole output
+tensor, for every output (forward, carry, VJP).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import (
+    dense_matrix_values,
+    fourier_matrices,
+    fourier_spacecurve,
+    gradient_values,
+    matrix_values,
+    rel_fro,
+    spacecurve_values,
+)
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+INTEGRATORS = ["Boole", "Simpson", "Trapezoid", "Riemann"]
+OPS = ["one", "many", "sum"]
+
+
+@pytest.fixture(scope="module", autouse=True)
+def numpy_gl_table():
+    """The product embeds numpy's leggauss table (as the reference does); give the
+    oracle the same table so both run on identical nodes and weights."""
+    total = 64 * 65 // 2
+    w = np.empty(total)
+    x = np.empty(total)
+    off = 0
+    for order in range(1, 65):
+        xr, wr = np.polynomial.legendre.leggauss(order)
+        x[off:off + order] = (xr + 1) / 2
+        w[off:off + order] = wr / 2
+        off += order
+    O.gl_replace(64, w, x)
+    yield
+    O.gl_reset()
+
+
+def _magnus():
+    import magnus
+
+    return magnus
+
+
+def _core():
+    from magnus import _core
+
+    return _core
+
+
+def _samples_for(integrator, base):
+    return base if integrator != "Boole" else base - (base - 1) % 4
+
+
+# ---------------------------------------------------------------- forward ---
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [1, 2, 4, 7])
+@pytest.mark.parametrize("backend,dim", [("Fixed2", 2), ("Manual", 3), ("Blas", 3), ("Manual", 5)])
+def test_forward_and_carry_match_oracle(integrator, op, n, backend, dim):
+    S = 33
+    data = dense_matrix_values(S, dim) + 0.3 * np.sin(np.arange(S * dim * dim)).reshape(S, dim, dim)
+    core = _core()
+    got, carry = core.compute(n, data, 0.0, 1.3, op=op, matrix_backend=backend, integrator=integrator,
+                              record_vjp=True)
+    ref, ref_carry = O.forward(n, data, 0.0, 1.3, op=op, matrix_backend=backend, integrator=integrator,
+                               record_vjp=True)
+    assert got.shape == ref.shape and carry.shape == ref_carry.shape
+    assert rel_fro(got, ref) <= TOL
+    if n > 1:
+        assert rel_fro(carry, ref_carry) <= TOL
+    plain = core.compute(n, data, 0.0, 1.3, op=op, matrix_backend=backend, integrator=integrator)
+    assert rel_fro(plain, ref) <= TOL
+
+
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [1, 3, 6])
+def test_spacecurve_forward_and_carry_match_oracle(integrator, op, n):
+    S = 65
+    data = fourier_spacecurve(np.random.default_rng(n), S)
+    core = _core()
+    got, carry = core.compute_sc(n, data, 0.0, 1.0, op=op, integrator=integrator, record_vjp=True)
+    ref, ref_carry = O.sc_forward(n, data, 0.0, 1.0, op=op, integrator=integrator, record_vjp=True)
+    assert got.shape == ref.shape
+    assert rel_fro(got, ref) <= TOL
+    if n > 1:
+        assert rel_fro(carry, ref_carry) <= TOL
+
+
+# -------------------------------------------------------------------- VJP ---
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [1, 2, 5])
+@pytest.mark.parametrize("backend,dim", [("Fixed2", 2), ("Manual", 3), ("Blas", 4)])
+def test_vjp_matches_oracle(integrator, op, n, backend, dim):
+    S = 17
+    data = dense_matrix_values(S, dim)
+    core = _core()
+    out, carry = core.compute(n, data, 0.0, 1.0, op=op, matrix_backend=backend, integrator=integrator,
+                              record_vjp=True)
+    cot = gradient_values(out.shape, (0.2, 1.1))
+    ref = O.vjp(n, data, cot, op=op, matrix_backend=backend, integrator=integrator)
+    with_carry = core.compute_vjp(n, data, cot, vjp_data=carry, op=op, matrix_backend=backend, integrator=integrator)
+    recompute = core.compute_vjp(n, data, cot, op=op, matrix_backend=backend, integrator=integrator)
+    assert rel_fro(with_carry, ref) <= TOL
+    assert rel_fro(recompute, ref) <= TOL
+
+
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [1, 2, 6])
+def test_spacecurve_vjp_matches_oracle(integrator, op, n):
+    S = 33
+    data = spacecurve_values(np.linspace(0, 1, S))
+    core = _core()
+    out, carry = core.compute_sc(n, data, 0.0, 1.0, op=op, integrator=integrator, record_vjp=True)
+    cot = gradient_values(out.shape, (0.2, 1.1))
+    ref = O.sc_vjp(n, data, cot, op=op, integrator=integrator)
+    got = core.compute_sc_vjp(n, data, cot, vjp_data=carry, op=op, integrator=integrator)
+    got2 = core.compute_sc_vjp(n, data, cot, op=op, integrator=integrator)
+    assert rel_fro(got, ref) <= TOL
+    assert rel_fro(got2, ref) <= TOL
+
+
+# ------------------------------------------------------- reference tests ---
+# (tests/test_python_api.py of the reference, through the same public API)
+def test_matrix_ops_and_shapes():
+    magnus = _magnus()
+    one = magnus.compute(4, matrix_values, 0.0, 1.0, 9, op="one", integrator="Boole")
+    many = magnus.compute(4, matrix_values, 0.0, 1.0, 9, op="many", integrator="Boole")
+    total = magnus.compute(4, matrix_values, 0.0, 1.0, 9, op="sum", integrator="Boole")
+    assert one.shape == (2, 2) and many.shape == (4, 2, 2) and total.shape == (2, 2)
+    np.testing.assert_allclose(one, many[3], rtol=1e-11, atol=1e-12)
+    np.testing.assert_allclose(total, np.sum(many, axis=0), rtol=1e-11, atol=1e-12)
+
+
+def test_gl_backend_names_do_not_change_results():
+    magnus = _magnus()
+    a = magnus.compute(4, matrix_values, 0.0, 1.0, 33, integrator="Boole", gl_backend="serial")
+    b = magnus.compute(4, matrix_values, 0.0, 1.0, 33, integrator="Boole", gl_backend="Auto")
+    np.testing.assert_array_equal(a, b)
+    c, cc = magnus.compute(8, matrix_values, 0.0, 1.0, 1025, integrator="Boole", gl_backend="openmp", record_vjp=True)
+    d, dc = magnus.compute(8, matrix_values, 0.0, 1.0, 1025, integrator="Boole", gl_backend="Auto", record_vjp=True)
+    np.testing.assert_array_equal(c, d)
+    np.testing.assert_array_equal(cc, dc)
+
+
+def test_pointwise_sampling_matches_vectorized():
+    magnus = _magnus()
+    seen = []
+
+    def pointwise(t):
+        assert isinstance(t, float)
+        seen.append(t)
+        return matrix_values(np.asarray([t]))[0]
+
+    v = magnus.compute(2, matrix_values, 0.0, 2.0, 9, matrix_backend="Fixed2", integrator="Trapezoid")
+    p = magnus.compute(2, pointwise, 0.0, 2.0, 9, vectorized=False, matrix_backend="Fixed2", integrator="Trapezoid")
+    np.testing.assert_array_equal(seen, np.linspace(0.0, 2.0, 9))
+    np.testing.assert_allclose(p, v, rtol=1e-11, atol=1e-12)
+
+
+def test_carry_shapes():
+    magnus = _magnus()
+    out, carry = magnus.compute(4, matrix_values, 0.0, 1.0, 9, record_vjp=True)
+    assert out.shape == (2, 2) and carry.shape == (3, 3, 9, 2, 2)
+    out, carry = magnus.compute_sc(4, spacecurve_values, 0.0, 1.0, 9, record_vjp=True)
+    assert out.shape == (3,) and carry.shape == (3, 3, 9, 2, 2)
+
+
+def _fd(data, n, op, backend, integrator, sc=False, eps=1e-6):
+    magnus = _magnus()
+    f = lambda _: data  # noqa: E731
+    direction = gradient_values(data.shape, (-0.3, 0.7))
+    if sc:
+        out, carry = magnus.compute_sc(n, f, 0.0, 1.0, data.shape[0], op=op, integrator=integrator, record_vjp=True)
+        cot = gradient_values(out.shape, (0.2, 1.1))
+        g = magnus.compute_sc_vjp(n, f, cot, 0.0, 1.0, data.shape[0], op=op, integrator=integrator, vjp_data=carry)
+        plus = magnus.compute_sc(n, lambda _: data + eps * direction, 0.0, 1.0, data.shape[0], op=op,
+                                 integrator=integrator)
+        minus = magnus.compute_sc(n, lambda _: data - eps * direction, 0.0, 1.0, data.shape[0], op=op,
+                                  integrator=integrator)
+    else:
+        kw = dict(op=op, matrix_backend=backend, integrator=integrator)
+        out, carry = magnus.compute(n, f, 0.0, 1.0, data.shape[0], record_vjp=True, **kw)
+        cot = gradient_values(out.shape, (0.2, 1.1))
+        g = magnus.compute_vjp(n, f, cot, 0.0, 1.0, data.shape[0], vjp_data=carry, **kw)
+        plus = magnus.compute(n, lambda _: data + eps * direction, 0.0, 1.0, data.shape[0], **kw)
+        minus = magnus.compute(n, lambda _: data - eps * direction, 0.0, 1.0, data.shape[0], **kw)
+    expected = np.sum((plus - minus) * cot) / (2 * eps)
+    np.testing.assert_allclose(np.sum(g * direction), expected, rtol=1e-7, atol=1e-9)
+
+
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+def test_vjp_finite_differences(op, integrator):
+    _fd(matrix_values(np.linspace(0, 1, 9)), 4, op, "Fixed2", integrator)
+    _fd(spacecurve_values(np.linspace(0, 1, 9)), 4, op, None, integrator, sc=True)
+
+
+def test_vjp_recompute_with_nested_workspace_is_finite():
+    magnus = _magnus()
+    data = np.zeros((9, 16, 16))
+    g = magnus.compute_vjp(2, lambda _: data, np.ones((16, 16)), 0.0, 1.0, 9, matrix_backend="Manual",
+                           integrator="Boole")
+    assert g.shape == data.shape and np.isfinite(g).all()
+
+
+def test_float32_is_computed_in_float64_and_returned_as_float32():
+    magnus = _magnus()
+    out = magnus.compute(3, matrix_values, 0.0, 1.0, 9, dtype=np.float32)
+    assert out.dtype == np.float32
+    ref = O.forward(3, matrix_values(np.linspace(0, 1, 9)).astype(np.float32).astype(np.float64), 0.0, 1.0)
+    np.testing.assert_allclose(out, ref.astype(np.float32), rtol=1e-6)
+
+
+# -------------------------------------------- workload-sized parity cases ---
+def test_config_plain_small_d4_full():
+    """d=4 Fourier A(t) (3 modes), S=1025, Boole, n=4, sum, Manual."""
+    data = fourier_matrices(np.random.default_rng([1, 0]), 1025, 4, modes=3)
+    got = _core().compute(4, data, 0.0, 1.0, op="sum", matrix_backend="Manual", integrator="Boole")
+    ref = O.forward(4, data, 0.0, 1.0, op="sum", matrix_backend="Manual", integrator="Boole")
+    assert rel_fro(got, ref) <= TOL
+
+
+def test_config_spacecurve_batch_subset_full_length():
+    """SpaceCurve trajectories at full length S=16385, Simpson, n=6, many, plus VJP,
+    through the batched device API (4 trajectories of the 4096-trajectory batch)."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, n = 4, 16385, 6
+    a = np.stack([fourier_spacecurve(np.random.default_rng([2, b]), S) for b in range(B)])
+    cot = np.stack([np.random.default_rng([2, b, 1]).standard_normal((n, 3)) for b in range(B)])
+    at = torch.from_numpy(a).cuda()
+    out, carry = torch_ops.sc_forward(n, at, 0.0, 1.0, op="many", integrator="Simpson", record_vjp=True)
+    dA = torch_ops.sc_vjp(n, at, torch.from_numpy(cot).cuda(), carry, 0.0, 1.0, op="many", integrator="Simpson")
+    out, dA = out.cpu().numpy(), dA.cpu().numpy()
+    for b in range(B):
+        ref = O.sc_forward(n, a[b], 0.0, 1.0, op="many", integrator="Simpson")
+        ref_g = O.sc_vjp(n, a[b], cot[b], 0.0, 1.0, op="many", integrator="Simpson")
+        assert rel_fro(out[b], ref) <= TOL
+        assert rel_fro(dA[b], ref_g) <= TOL
+
+
+def test_config_dense_d32_instance_full():
+    """d=32 Fourier A(t) (5 modes), S=4097, Boole, n=4, sum, record + VJP (one instance)."""
+    import torch
+
+    from magnus import torch_ops
+
+    S, d, n = 4097, 32, 4
+    A = fourier_matrices(np.random.default_rng([3, 0]), S, d, modes=5)
+    cot = np.random.default_rng([3, 0, 1]).standard_normal((d, d))
+    At = torch.from_numpy(A)[None].cuda()
+    out, carry = torch_ops.forward(n, At, 0.0, 1.0, op="sum", integrator="Boole", record_vjp=True)
+    dA = torch_ops.vjp(n, At, torch.from_numpy(cot)[None].cuda(), carry, 0.0, 1.0, op="sum", integrator="Boole")
+    ref, ref_carry = O.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole", record_vjp=True)
+    ref_g = O.vjp(n, A, cot, 0.0, 1.0, op="sum", integrator="Boole", vjp_data=ref_carry)
+    assert rel_fro(out[0].cpu().numpy(), ref) <= TOL
+    assert rel_fro(carry[0].cpu().numpy(), ref_carry) <= TOL
+    assert rel_fro(dA[0].cpu().numpy(), ref_g) <= TOL
+
+
+def test_batched_matrix_api_matches_per_instance():
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, d, n = 5, 129, 8, 5
+    A = np.stack([fourier_matrices(np.random.default_rng([5, b]), S, d, modes=3) for b in range(B)])
+    out = torch_ops.forward(n, torch.from_numpy(A).cuda(), 0.0, 1.0, op="many").cpu().numpy()
+    for b in range(B):
+        assert rel_fro(out[b], O.forward(n, A[b], 0.0, 1.0, op="many")) <= TOL
+
+
+def test_autograd_function_matches_vjp():
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, d, n = 2, 33, 3, 4
+    A = torch.from_numpy(np.stack([dense_matrix_values(S, d) * (1 + 0.1 * b) for b in range(B)])).cuda()
+    A.requires_grad_(True)
+    out = torch_ops.magnus(A, n, 0.0, 1.0, op="many", record_vjp=True)
+    w = torch.from_numpy(gradient_values(tuple(out.shape), (0.2, 1.1))).cuda()
+    (out * w).sum().backward()
+    for b in range(B):
+        ref = O.vjp(n, A[b].detach().cpu().numpy(), w[b].cpu().numpy(), op="many")
+        assert rel_fro(A.grad[b].cpu().numpy(), ref) <= TOL

@@ -3,8 +3,8 @@
 Default workload (one GPU): the SpaceCurve batch -- 4096 independent 3-vector
 trajectories f(t) = sum_{m=1..4} a_m sin(2 pi m t + phi_m), a_m ~ N(0,1)/m,
 phi_m ~ U[0, 2 pi), S = 16385 samples on [0, 1], Simpson, compute_sc with
-n = 6, op = 'many' recording the carry, then compute_sc_vjp with a N(0,1)
-(6, 3) cotangent per trajectory.  One step = forward + VJP of the whole batch
+n = 6, op = 'many', then compute_sc_vjp (no stored carry: the VJP re-derives
+its forward quantities) with a N(0,1) (6, 3) cotangent per trajectory.  One step = forward + VJP of the whole batch
 in batched launches; inputs are resident in HBM before timing starts.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
@@ -74,6 +74,33 @@ def _counts(Q, K, S, m, m_in, P, n_out, op_one=False):
     B_carry = 8 * Q * K * S * m
     B_vjp = 8 * (2 * S * m_in + n_out * m_in)
     return F_fwd, F_vjp, B_fwd, B_carry, B_vjp
+
+
+KERNELS = ("ep_forward", "ep_vjp", "ep_forward_recompute", "wb_fwd_pass1", "wb_fwd_combine", "wb_fwd_pass1_vjp",
+           "wb_fwd_combine_vjp", "wb_fwd_pass2", "wb_rev_pass1", "wb_rev_combine", "wb_rev_pass2")
+
+
+def _wb_counts(K, S, P, m, m_in):
+    """Algorithmic flops / HBM bytes per solve of each single-pass kernel
+    (K = n-1 basis streams; per sample: one product (P flops) per stream, a
+    block prefix step (~8m flops forward, ~6m flops transposed), sum/axpy
+    terms (2m flops); bytes = compulsory input/output and stream traffic)."""
+    f1 = S * (K * (P + 8 * m) + 2 * (K + 1) * m)
+    flops = {
+        "wb_fwd_pass1": f1,
+        "wb_fwd_pass1_vjp": f1,
+        "wb_fwd_pass2": S * K * (P + 6 * m),
+        "wb_rev_pass1": S * ((2 * K - 1) * P + (2 * K - 2) * 6 * m + 2 * K * m),
+        "wb_rev_pass2": S * (2 * K * P + 8 * m * K),
+    }
+    by = {
+        "wb_fwd_pass1": 8 * S * m_in,
+        "wb_fwd_pass1_vjp": 8 * S * m_in,
+        "wb_fwd_pass2": 8 * S * (m_in + K * m),
+        "wb_rev_pass1": 8 * S * m_in,
+        "wb_rev_pass2": 8 * S * (2 * m_in + K * m),
+    }
+    return flops, by
 
 
 def _cpu_baseline_sc(a_host, cot_host, n, S, budget_s):
@@ -147,14 +174,16 @@ def main():
         a, cot = _sc_inputs(torch, B, S, modes, dev)
 
         def step():
-            out, carry = torch_ops.sc_forward(n, a, 0.0, 1.0, op="many", integrator="Simpson", record_vjp=True)
-            dA = torch_ops.sc_vjp(n, a, cot, carry, 0.0, 1.0, op="many", integrator="Simpson")
+            # compute_sc (op='many') then compute_sc_vjp without a stored carry: the
+            # VJP re-derives its forward quantities (the reference's VJP::none path)
+            out = torch_ops.sc_forward(n, a, 0.0, 1.0, op="many", integrator="Simpson")
+            dA = torch_ops.sc_vjp(n, a, cot, None, 0.0, 1.0, op="many", integrator="Simpson")
             return out, dA
 
         Q, K, m, m_in, P, n_out = (n + 3) // 2, n - 1, 4, 3, 20, n
         config = {"workload": "spacecurve-batch", "instances": B, "samples": S, "n": n, "integrator": "Simpson",
                   "op": "many", "vector": "su(2) 3-vector", "fourier_modes": modes, "cotangent": "N(0,1) (6,3)",
-                  "carry": "recorded"}
+                  "carry": "none (VJP recomputes its forward quantities)"}
         do_vjp = True
     elif wl == "dense-d32":
         B, S, d, n, modes = args.batch or 256, 4097, 32, 4, 5
@@ -185,7 +214,6 @@ def main():
     for _ in range(args.warmup):
         step()
     torch.cuda.synchronize()
-    kernels = capi.timing_kernels() if hasattr(capi, "timing_kernels") else ["ep_forward", "ep_vjp"]
     capi.timing_reset()
     capi.timing_enable(True)
     start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -199,7 +227,7 @@ def main():
     ms_total = start.elapsed_time(stop)
     ms_step = ms_total / args.steps
     per_kernel = {}
-    for name in ("ep_forward", "ep_vjp", "ep_forward_recompute"):
+    for name in KERNELS:
         cnt, tot = capi.timing_read(name)
         if cnt:
             per_kernel[name] = {"launches": cnt, "avg_ms": tot / cnt}
@@ -209,6 +237,10 @@ def main():
     F_fwd, F_vjp, B_fwd, B_carry, B_vjp = _counts(Q, K, S, m, m_in, P, n_out)
     kernel_bytes = {"ep_forward": B * (B_fwd + B_carry), "ep_vjp": B * (B_vjp + B_carry)}
     kernel_flops = {"ep_forward": B * F_fwd, "ep_vjp": B * F_vjp}
+    wbf, wbb = _wb_counts(K, S, P, m, m_in)
+    for k in wbf:
+        kernel_flops[k] = B * wbf[k]
+        kernel_bytes[k] = B * wbb[k]
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"]) if per_kernel else None
     roofline = None
     if dom in kernel_bytes:

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "kernels_wb.hpp"
 
 namespace {
 
@@ -275,6 +276,27 @@ struct KernelTimer {
     }
 };
 
+}  // namespace
+
+namespace magnus {
+void* timer_begin(const char* name, hipStream_t s) {
+    auto* t = new KernelTimer(name, s);
+    if (!t->start) {
+        delete t;
+        return nullptr;
+    }
+    return t;
+}
+void timer_end(void* h) {
+    if (!h) return;
+    auto* t = static_cast<KernelTimer*>(h);
+    t->done();
+    delete t;
+}
+}  // namespace magnus
+
+namespace {
+
 void timing_drain_locked() {
     for (auto& t : g_pending) {
         float ms = 0.f;
@@ -297,6 +319,64 @@ void timing_drain_locked() {
 
 int device_error(hipError_t e, const char* where) {
     return fail(MAGNUS_EDEVICE, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+// Single-pass W-basis path: su(2) SpaceCurve input, block integrators,
+// 2 <= n <= kWbMaxK + 1, no reference-layout carry requested or supplied.
+bool use_wb(const Plan& pl, bool has_carry) {
+    if (!pl.pad || has_carry) return false;
+    if (pl.integ == MAGNUS_INT_TRAPEZOID) return false;  // reference reconstruction quirk: EP path
+    return pl.n >= 2 && pl.n - 1 <= (uint64_t)magnus::kWbMaxK;
+}
+
+int wb_chunk_len(const Plan& pl, uint64_t count) {
+    // aim for ~512k chunk-threads, chunks of 16..256 samples, multiple of 4
+    const double target = 524288.0;
+    double lc = (double)(pl.S - 1) * (double)count / target;
+    int LC = (int)lc;
+    if (LC < 16) LC = 16;
+    if (LC > 256) LC = 256;
+    LC = (LC + 3) / 4 * 4;
+    return LC;
+}
+
+int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, double* out, double* dA,
+           hipStream_t stream) {
+    DeviceScratch scratch(stream);
+    const bool vjp = dA != nullptr;
+    magnus::WbArgs a{};
+    a.B = (long long)count;
+    a.S = (long long)pl.S;
+    a.LC = wb_chunk_len(pl, count);
+    a.C = (int)((pl.S - 1 + a.LC - 1) / a.LC);
+    a.K = (int)pl.n - 1;
+    a.n_out = (int)pl.n_out;
+    a.op = pl.op;
+    a.Q = (int)pl.Q;
+    a.dt = pl.dt;
+    fill_gl(pl, a.w, a.x);
+    const size_t BC = (size_t)count * a.C, K = (size_t)a.K;
+    a.in = in;
+    a.cot = cot;
+    a.out = out;
+    a.dA = dA;
+    a.fsum = scratch.alloc<double>(BC * (2 * K + 1) * 4);
+    if (!a.fsum) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    if (vjp) {
+        a.bases = scratch.alloc<double>(BC * K * 4);
+        a.what = scratch.alloc<double>((size_t)count * K * 4);
+        a.src = scratch.alloc<double>((size_t)count * (2 * K + 1) * 4);
+        a.wbuf = scratch.alloc<double>((size_t)count * K * a.LC * a.C * 4);
+        a.w0 = scratch.alloc<double>((size_t)count * K * 4);
+        a.rsum = scratch.alloc<double>(BC * (2 * K - 1) * 4);
+        a.uin = scratch.alloc<double>(BC * K * 4);
+        if (!a.bases || !a.what || !a.src || !a.wbuf || !a.w0 || !a.rsum || !a.uin)
+            return fail(MAGNUS_ENOMEM, "device allocation failed");
+    }
+    hipError_t e = magnus::launch_wb(pl.integ, a, vjp, stream);
+    if (e != hipSuccess) return device_error(e, "wb kernel launch");
+    g_kernel = "wb";
+    return MAGNUS_OK;
 }
 
 // forward: record into `carry` when non-null; outputs=false runs VJP::none
@@ -531,7 +611,9 @@ int magnus_sc_forward_f64(const magnus_problem* p, uint64_t count, const double*
     if (count == 0) return MAGNUS_OK;
     if (!in3) return fail(MAGNUS_EINVAL, "input buffer must not be null");
     if (!out3) return fail(MAGNUS_EINVAL, "output buffer must not be null");
-    return run_forward(pl, count, in3, out3, pl.n >= 2 ? carry : nullptr, true, (hipStream_t)stream);
+    double* c = pl.n >= 2 ? carry : nullptr;
+    if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, nullptr, out3, nullptr, (hipStream_t)stream);
+    return run_forward(pl, count, in3, out3, c, true, (hipStream_t)stream);
 }
 
 int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3, const double* cot3,
@@ -542,7 +624,9 @@ int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3
     if (!in3) return fail(MAGNUS_EINVAL, "input buffer must not be null");
     if (!dA3) return fail(MAGNUS_EINVAL, "output buffer must not be null");
     if (!cot3) return fail(MAGNUS_EINVAL, "cotangent buffer must not be null");
-    return run_vjp(pl, count, in3, cot3, carry, dA3, (hipStream_t)stream);
+    const double* c = pl.n >= 2 ? carry : nullptr;
+    if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, cot3, nullptr, dA3, (hipStream_t)stream);
+    return run_vjp(pl, count, in3, cot3, c, dA3, (hipStream_t)stream);
 }
 
 int magnus_timing_enable(int on) {

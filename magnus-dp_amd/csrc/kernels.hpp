@@ -60,6 +60,12 @@ struct VjpReduceArgs {
     double dt;
 };
 
+// Per-kernel timing hooks (implemented by the C-ABI layer).  When timing is
+// enabled, timer_begin records a start event on `s` and returns a handle that
+// timer_end closes; both are no-ops (nullptr) otherwise.
+void* timer_begin(const char* name, hipStream_t s);
+void timer_end(void* handle);
+
 hipError_t launch_ep_forward(int integ, const EpFwdArgs& a, int block, long long grid, int ept, hipStream_t s);
 hipError_t launch_ep_vjp(int integ, const EpVjpArgs& a, int block, long long grid, int ept, hipStream_t s);
 hipError_t launch_fwd_reduce(const FwdReduceArgs& a, hipStream_t s);

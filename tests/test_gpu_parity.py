@@ -244,14 +244,20 @@ def test_config_spacecurve_batch_subset_full_length():
     a = np.stack([fourier_spacecurve(np.random.default_rng([2, b]), S) for b in range(B)])
     cot = np.stack([np.random.default_rng([2, b, 1]).standard_normal((n, 3)) for b in range(B)])
     at = torch.from_numpy(a).cuda()
-    out, carry = torch_ops.sc_forward(n, at, 0.0, 1.0, op="many", integrator="Simpson", record_vjp=True)
-    dA = torch_ops.sc_vjp(n, at, torch.from_numpy(cot).cuda(), carry, 0.0, 1.0, op="many", integrator="Simpson")
-    out, dA = out.cpu().numpy(), dA.cpu().numpy()
+    ct = torch.from_numpy(cot).cuda()
+    out = torch_ops.sc_forward(n, at, 0.0, 1.0, op="many", integrator="Simpson")
+    dA = torch_ops.sc_vjp(n, at, ct, None, 0.0, 1.0, op="many", integrator="Simpson")
+    out2, carry = torch_ops.sc_forward(n, at[:1], 0.0, 1.0, op="many", integrator="Simpson", record_vjp=True)
+    dA2 = torch_ops.sc_vjp(n, at[:1], ct[:1], carry, 0.0, 1.0, op="many", integrator="Simpson")
+    out, dA, out2, dA2 = out.cpu().numpy(), dA.cpu().numpy(), out2.cpu().numpy(), dA2.cpu().numpy()
     for b in range(B):
         ref = O.sc_forward(n, a[b], 0.0, 1.0, op="many", integrator="Simpson")
         ref_g = O.sc_vjp(n, a[b], cot[b], 0.0, 1.0, op="many", integrator="Simpson")
         assert rel_fro(out[b], ref) <= TOL
         assert rel_fro(dA[b], ref_g) <= TOL
+        if b == 0:  # recorded-carry path (entry-parallel kernels)
+            assert rel_fro(out2[0], ref) <= TOL
+            assert rel_fro(dA2[0], ref_g) <= TOL
 
 
 def test_config_dense_d32_instance_full():
@@ -271,6 +277,33 @@ def test_config_dense_d32_instance_full():
     assert rel_fro(out[0].cpu().numpy(), ref) <= TOL
     assert rel_fro(carry[0].cpu().numpy(), ref_carry) <= TOL
     assert rel_fro(dA[0].cpu().numpy(), ref_g) <= TOL
+
+
+@pytest.mark.parametrize("integrator", ["Boole", "Simpson", "Riemann"])
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [2, 3, 5, 8, 9])
+def test_spacecurve_single_pass_batched_matches_oracle(integrator, op, n):
+    """Batched SpaceCurve forward + VJP without a carry (single-pass kernels for
+    n <= 8, entry-parallel kernels for n = 9), multi-chunk sample ranges."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    B, S = 3, 1025
+    a = np.stack([fourier_spacecurve(np.random.default_rng([7, b, n]), S) for b in range(B)])
+    out_shape = (n, 3) if op == "many" else (3,)
+    cot = np.stack([np.random.default_rng([7, b, n, 1]).standard_normal(out_shape) for b in range(B)])
+    at = torch.from_numpy(a).cuda()
+    out = torch_ops.sc_forward(n, at, 0.0, 1.0, op=op, integrator=integrator)
+    kernel = capi.last_kernel()
+    dA = torch_ops.sc_vjp(n, at, torch.from_numpy(cot).cuda(), None, 0.0, 1.0, op=op, integrator=integrator)
+    assert kernel == ("wb" if n <= 8 else "ep") and capi.last_kernel() == kernel
+    out, dA = out.cpu().numpy(), dA.cpu().numpy()
+    for b in range(B):
+        ref = O.sc_forward(n, a[b], 0.0, 1.0, op=op, integrator=integrator)
+        ref_g = O.sc_vjp(n, a[b], cot[b], 0.0, 1.0, op=op, integrator=integrator)
+        assert rel_fro(out[b], ref) <= TOL
+        assert rel_fro(dA[b], ref_g) <= TOL
 
 
 def test_batched_matrix_api_matches_per_instance():

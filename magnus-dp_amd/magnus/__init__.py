@@ -13,6 +13,10 @@ from typing import Callable
 
 import numpy as np
 
+from ._runtime import bind_runtime
+
+bind_runtime()
+
 try:
     from ._core import _replace_gl_table
     from ._core import compute as _compute_sampled

@@ -41,6 +41,9 @@ def lib() -> ctypes.CDLL:
     if _lib is None:
         if not LIB_PATH.exists():
             raise ImportError(f"{LIB_PATH} is not built; run `python magnus-dp_amd/build.py`")
+        from ._runtime import bind_runtime
+
+        bind_runtime()
         L = ctypes.CDLL(str(LIB_PATH))
         P = ctypes.POINTER(Problem)
         vp = ctypes.c_void_p

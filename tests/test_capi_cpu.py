@@ -141,3 +141,21 @@ def test_error_contract_through_public_api_without_device():
         magnus.compute_vjp(2, matrix_values, np.zeros((3, 3)), 0.0, 1.0, 9)
     with pytest.raises(ValueError, match="VJP data has an invalid shape"):
         magnus.compute_vjp(2, matrix_values, np.zeros((2, 2)), 0.0, 1.0, 9, vjp_data=np.zeros((3, 1, 9, 2, 2)))
+
+
+def test_one_hip_runtime_per_process():
+    """Importing the package maps exactly one HIP and one HSA runtime, so device
+    pointers and streams from torch and from the library are interchangeable."""
+    import subprocess
+    import sys
+
+    code = (
+        "import sys; sys.path.insert(0, 'magnus-dp_amd'); import magnus; from magnus import capi; capi.lib();"
+        "m = open('/proc/self/maps').read().splitlines();"
+        "print(len({l.split()[-1] for l in m if 'libamdhip64' in l}), len({l.split()[-1] for l in m if 'libhsa-runtime64' in l}))"
+    )
+    from pathlib import Path
+
+    out = subprocess.run([sys.executable, "-c", code], capture_output=True, text=True, check=True,
+                         cwd=Path(__file__).resolve().parents[1]).stdout.split()
+    assert out == ["1", "1"], out

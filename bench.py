@@ -76,30 +76,30 @@ def _counts(Q, K, S, m, m_in, P, n_out, op_one=False):
     return F_fwd, F_vjp, B_fwd, B_carry, B_vjp
 
 
-KERNELS = ("ep_forward", "ep_vjp", "ep_forward_recompute", "wb_fwd_pass1", "wb_fwd_combine", "wb_fwd_pass1_vjp",
-           "wb_fwd_combine_vjp", "wb_fwd_pass2", "wb_rev_pass1", "wb_rev_combine", "wb_rev_pass2")
+KERNELS = ("ep_forward", "ep_vjp", "ep_forward_recompute", "wb_fwd_pass1", "wb_combine", "wb_fwd_pass1_vjp",
+           "wb_combine_vjp", "wb_rev", "wb_fixup")
+
+# su(2) products on the single-pass path: pure sample x 4-vector = 4 mul + 8 FMA
+# (20 flops), 4-vector x 4-vector = 4 mul + 12 FMA (28 flops).
+SC_AMUL_FLOPS = 20
+SC_QMUL_FLOPS = 28
 
 
-def _wb_counts(K, S, P, m, m_in):
-    """Algorithmic flops / HBM bytes per solve of each single-pass kernel
-    (K = n-1 basis streams; per sample: one product (P flops) per stream, a
-    block prefix step (~8m flops forward, ~6m flops transposed), sum/axpy
-    terms (2m flops); bytes = compulsory input/output and stream traffic)."""
-    f1 = S * (K * (P + 8 * m) + 2 * (K + 1) * m)
-    flops = {
-        "wb_fwd_pass1": f1,
-        "wb_fwd_pass1_vjp": f1,
-        "wb_fwd_pass2": S * K * (P + 6 * m),
-        "wb_rev_pass1": S * ((2 * K - 1) * P + (2 * K - 2) * 6 * m + 2 * K * m),
-        "wb_rev_pass2": S * (2 * K * P + 8 * m * K),
-    }
-    by = {
-        "wb_fwd_pass1": 8 * S * m_in,
-        "wb_fwd_pass1_vjp": 8 * S * m_in,
-        "wb_fwd_pass2": 8 * S * (m_in + K * m),
-        "wb_rev_pass1": 8 * S * m_in,
-        "wb_rev_pass2": 8 * S * (2 * m_in + K * m),
-    }
+def _wb_counts(K, S):
+    """Algorithmic flops / HBM bytes per solve of each single-pass kernel.
+
+    Chunk pass: K+1 local prefix levels (a block prefix step is 16 flops per
+    sample) and K sample products.  Reverse pass, per level and sample: the
+    backward rebuild of the local stream (one product, 16 flops of block
+    rule), the dA accumulation (one 4-vector product and an add), the
+    adjoint product with the sample and the transposed block rule with the
+    suffix sums (18 flops).  Bytes: the samples read (and dA written) once;
+    the per-chunk summaries are a few hundred bytes per chunk and left out."""
+    m_in = 3
+    f1 = S * (K * SC_AMUL_FLOPS + (K + 1) * 16)
+    fr = S * (K * (2 * SC_AMUL_FLOPS + SC_QMUL_FLOPS + 4 + 16 + 18) + 8)
+    flops = {"wb_fwd_pass1": f1, "wb_fwd_pass1_vjp": f1, "wb_rev": fr}
+    by = {"wb_fwd_pass1": 8 * S * m_in, "wb_fwd_pass1_vjp": 8 * S * m_in, "wb_rev": 8 * S * 2 * m_in}
     return flops, by
 
 
@@ -237,7 +237,7 @@ def main():
     F_fwd, F_vjp, B_fwd, B_carry, B_vjp = _counts(Q, K, S, m, m_in, P, n_out)
     kernel_bytes = {"ep_forward": B * (B_fwd + B_carry), "ep_vjp": B * (B_vjp + B_carry)}
     kernel_flops = {"ep_forward": B * F_fwd, "ep_vjp": B * F_vjp}
-    wbf, wbb = _wb_counts(K, S, P, m, m_in)
+    wbf, wbb = _wb_counts(K, S)
     for k in wbf:
         kernel_flops[k] = B * wbf[k]
         kernel_bytes[k] = B * wbb[k]

@@ -363,15 +363,10 @@ int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, 
     a.fsum = scratch.alloc<double>(BC * (2 * K + 1) * 4);
     if (!a.fsum) return fail(MAGNUS_ENOMEM, "device allocation failed");
     if (vjp) {
+        a.src = scratch.alloc<double>(BC * (2 * K + 1) * 4);
         a.bases = scratch.alloc<double>(BC * K * 4);
-        a.what = scratch.alloc<double>((size_t)count * K * 4);
-        a.src = scratch.alloc<double>((size_t)count * (2 * K + 1) * 4);
-        a.wbuf = scratch.alloc<double>((size_t)count * K * a.LC * a.C * 4);
-        a.w0 = scratch.alloc<double>((size_t)count * K * 4);
-        a.rsum = scratch.alloc<double>(BC * (2 * K - 1) * 4);
-        a.uin = scratch.alloc<double>(BC * K * 4);
-        if (!a.bases || !a.what || !a.src || !a.wbuf || !a.w0 || !a.rsum || !a.uin)
-            return fail(MAGNUS_ENOMEM, "device allocation failed");
+        a.edge = scratch.alloc<double>(BC * 6);
+        if (!a.src || !a.bases || !a.edge) return fail(MAGNUS_ENOMEM, "device allocation failed");
     }
     hipError_t e = magnus::launch_wb(pl.integ, a, vjp, stream);
     if (e != hipSuccess) return device_error(e, "wb kernel launch");

@@ -10,15 +10,36 @@
 // every level's prefix is  P^(j) = W_j + s_q sum_{l<j} W_l * T_{j-l}  and
 //      Omega_j = sum_q w_q ( Z_{j-1} + s_q sum_{l<j-1} Z_l * T_{j-1-l} ),
 //      T_j     = W_j[last] + s_q sum_{l<j} W_l[last] * T_{j-l},
-// with Z_l = Sum(A * W_l) (Z_0 = Sum(A)).  The samples are therefore swept
-// ONCE, in parallel chunks: a chunk computes its streams from zero bases, a
-// per-instance combine composes the chunks (W_j[i] = V_j[i] +
-// sum_{l<j} V_{j-l}[i] * base_l + base_j) and evaluates the small per-node
-// algebra.  The VJP is the exact adjoint of this graph: one reverse sweep
-// with the same chunk decomposition.  Products use the su(2) algebra of
-// linalg/spacecurve.hpp:8-18 extended associatively (a0 b0 and a0 b terms),
-// which coincides with the reference product whenever the left factor is a
-// padded input sample (a0 = 0).
+// with Z_l = Sum(A * W_l) (Z_0 = Sum(A)).
+//
+// Chunks.  The samples of an instance are cut into chunks (s, e].  A chunk's
+// local streams V_j (zero at s) give its transfer series and sum series
+//      V_c(x) = I + sum_{j=1..K} V_j[e] x^j,     Z_c(x) = sum_{l=0..K} ZV_l x^l,
+// ZV_l = sum_{i in (s,e]} c_i A_i V_l[i].  In the ring of quaternion-valued
+// polynomials truncated at degree K, the streams entering chunk c are
+// W~_c = V_{c-1} ... V_0, and the instance totals are one associative
+// reduction of the pairs (V_c, Z_c) under
+//      (V_b, Z_b) o (V_a, Z_a) = (V_b V_a,  Z_b V_a + Z_a):
+// the V-part of the total holds What_j = W_j[last], its Z-part holds Z_l.
+// The chunk pass computes ZV_l from one extra prefix level: the prefix
+// weights at e equal the sum weights c_i inside the chunk, so
+// ZV_l = V_{l+1}[e] + (c_e - alpha_end) A_e V_l[e]  (minus the start-role
+// weight of A_s for l = 0 in chunks after the first).
+//
+// VJP.  The gradient is the adjoint of  chunk summaries -> pair reduction ->
+// GL algebra.  The combine kernel runs the GL adjoint, an exclusive prefix of
+// the V series and an exclusive suffix of the pairs, and gives every chunk the
+// cotangents of its summaries.  The reverse chunk pass is then the local
+// adjoint of one chunk: it rebuilds the chunk's local streams backwards from
+// V_j[e] (no per-sample storage), and folds the sum cotangents c_i ZV_bar
+// into the suffix sums of the prefix transposes (the prefix weights at e are
+// the c_i).  A sample on a chunk boundary belongs to two chunks; the fixup
+// kernel adds the two halves.
+//
+// Products use the su(2) algebra of linalg/spacecurve.hpp:8-18 extended
+// associatively (a0 b0 and a0 b terms), which coincides with the reference
+// product whenever the left factor is a padded input sample (a0 = 0); the
+// identity is (-1, 0, 0, 0).
 #include "kernels.hpp"
 #include "kernels_wb.hpp"
 #include "rules.hpp"
@@ -37,8 +58,15 @@ __device__ __forceinline__ V4 vadd(const V4& a, const V4& b) {
     return V4{{a.x[0] + b.x[0], a.x[1] + b.x[1], a.x[2] + b.x[2], a.x[3] + b.x[3]}};
 }
 
+__device__ __forceinline__ V4 vneg(const V4& a) { return V4{{-a.x[0], -a.x[1], -a.x[2], -a.x[3]}}; }
+
+__device__ __forceinline__ V4 vscale(double s, const V4& a) {
+    return V4{{a.x[0] * s, a.x[1] * s, a.x[2] * s, a.x[3] * s}};
+}
+
 __device__ __forceinline__ V4 vaxpy(const V4& acc, double s, const V4& b) {  // acc + s*b
-    return V4{{acc.x[0] + b.x[0] * s, acc.x[1] + b.x[1] * s, acc.x[2] + b.x[2] * s, acc.x[3] + b.x[3] * s}};
+    return V4{{fma(s, b.x[0], acc.x[0]), fma(s, b.x[1], acc.x[1]), fma(s, b.x[2], acc.x[2]),
+               fma(s, b.x[3], acc.x[3])}};
 }
 
 // full associative product (reference formula when a0 == 0)
@@ -63,6 +91,8 @@ __device__ __forceinline__ V4 amul(const double* a, const V4& b) {
 
 __device__ __forceinline__ V4 qconj(const V4& a) { return V4{{a.x[0], -a.x[1], -a.x[2], -a.x[3]}}; }
 
+__device__ __forceinline__ V4 pure(const double* a) { return V4{{0.0, a[0], a[1], a[2]}}; }
+
 __device__ __forceinline__ V4 ld4(const double* p) { return V4{{p[0], p[1], p[2], p[3]}}; }
 __device__ __forceinline__ void st4(double* p, const V4& v) {
     p[0] = v.x[0];
@@ -78,9 +108,15 @@ __device__ __forceinline__ void lda(const double* A3, long long i, double* a) {
 }
 
 // ---------------------------------------------------------------------------
-// block rules.  Block [s, s+B]: P_{s+r} = base + sum_t alpha_{r,t} f_{s+t}
-// (boole.hpp:37-72, simpson.hpp:24-58, default.hpp:23-26), evaluated in the
-// reference's operation order per component.
+// block rules.  Block [s0, s0+B]: P_{s0+r} = P_{s0} + sum_t alpha_{r,t} f_{s0+t}
+// (boole.hpp:37-72, simpson.hpp:24-58, default.hpp:23-26).
+//   fwd:      P at s0+1..s0+B from P_{s0} and f_{s0..s0+B}
+//   back:     P_{s0} from P_{s0+B} and f_{s0..s0+B}
+//   interior: P at s0+1..s0+B-1 from P_{s0}
+//   rev:      transpose: f-adjoints at s0..s0+B-1 from the P-adjoints g at
+//             s0+1..s0+B-1 and U, the sum of the P-adjoints from s0+B on
+//             (the start-role term at s0 included, its end role excluded)
+//   end_role / start_role: alpha_{B,B} / alpha_{B,0}
 // ---------------------------------------------------------------------------
 template <int I>
 struct BlockRule;
@@ -88,108 +124,76 @@ struct BlockRule;
 template <>
 struct BlockRule<BOOLE> {
     static constexpr int B = 4;
-    __device__ static void fwd(const V4& f0, const V4* f, const V4& base, double dt, V4* P) {
+    __device__ static double tail(const V4* f, int c) {  // 7 f0 + 32 f1 + 12 f2 + 32 f3 + 7 f4
+        return fma(7.0, f[0].x[c] + f[4].x[c], fma(32.0, f[1].x[c] + f[3].x[c], 12.0 * f[2].x[c]));
+    }
+    __device__ static void interior(const V4* f, const V4& v0, double dt, V4* P) {
         for (int c = 0; c < 4; ++c) {
-            double p;
-            p = f[0].x[c] * (dt / 2);
-            p += f0.x[c] * (dt / 2);
-            p += base.x[c];
-            P[0].x[c] = p;
-            p = f[1].x[c] * (dt / 3);
-            p += f0.x[c] * (dt / 3);
-            p += f[0].x[c] * (4 * dt / 3);
-            p += base.x[c];
-            P[1].x[c] = p;
-            p = f[2].x[c] * (3 * dt / 8);
-            p += f0.x[c] * (3 * dt / 8);
-            p += f[0].x[c] * (9 * dt / 8);
-            p += f[1].x[c] * (9 * dt / 8);
-            p += base.x[c];
-            P[2].x[c] = p;
-            p = f[3].x[c] * (14 * dt / 45);
-            p += f0.x[c] * (14 * dt / 45);
-            p += f[0].x[c] * (64 * dt / 45);
-            p += f[1].x[c] * (24 * dt / 45);
-            p += f[2].x[c] * (64 * dt / 45);
-            p += base.x[c];
-            P[3].x[c] = p;
+            const double b = v0.x[c];
+            P[0].x[c] = fma(dt / 2, f[0].x[c] + f[1].x[c], b);
+            P[1].x[c] = fma(dt / 3, fma(4.0, f[1].x[c], f[0].x[c] + f[2].x[c]), b);
+            P[2].x[c] = fma(3 * dt / 8, fma(3.0, f[1].x[c] + f[2].x[c], f[0].x[c] + f[3].x[c]), b);
         }
     }
-    // transpose (boole.hpp:108-151): g[0..B-2] = interior adjoints g1..g3,
-    // gB = total adjoint of the block end.  xb[t] (t = 0..B-1) are the
-    // contributions to the f-adjoints at s+t (t = 0 is the pending term).
-    __device__ static void rev(const V4* g, const V4& gB, double dt, V4* xb) {
+    __device__ static void fwd(const V4* f, const V4& base, double dt, V4* P) {
+        interior(f, base, dt, P);
+        for (int c = 0; c < 4; ++c) P[3].x[c] = fma(2 * dt / 45, tail(f, c), base.x[c]);
+    }
+    __device__ static V4 back(const V4* f, const V4& end, double dt) {
+        V4 v;
+        for (int c = 0; c < 4; ++c) v.x[c] = fma(-2 * dt / 45, tail(f, c), end.x[c]);
+        return v;
+    }
+    __device__ static void rev(const V4* g, const V4& U, double dt, V4* xb) {
         for (int c = 0; c < 4; ++c) {
-            const double g1 = g[0].x[c], g2 = g[1].x[c], g3 = g[2].x[c], g4 = gB.x[c];
-            double v;
-            v = g1 * (dt / 2);
-            v += g2 * (dt / 3);
-            v += g3 * (3 * dt / 8);
-            v += g4 * (14 * dt / 45);
-            xb[0].x[c] = v;
-            v = g1 * (dt / 2);
-            v += g2 * (4 * dt / 3);
-            v += g3 * (9 * dt / 8);
-            v += g4 * (64 * dt / 45);
-            xb[1].x[c] = v;
-            v = g2 * (dt / 3);
-            v += g3 * (9 * dt / 8);
-            v += g4 * (24 * dt / 45);
-            xb[2].x[c] = v;
-            v = g3 * (3 * dt / 8);
-            v += g4 * (64 * dt / 45);
-            xb[3].x[c] = v;
+            const double g1 = g[0].x[c], g2 = g[1].x[c], g3 = g[2].x[c], u = U.x[c];
+            const double h1 = g1 * (dt / 2);
+            xb[0].x[c] = fma(14 * dt / 45, u, fma(3 * dt / 8, g3, fma(dt / 3, g2, h1)));
+            xb[1].x[c] = fma(64 * dt / 45, u, fma(9 * dt / 8, g3, fma(4 * dt / 3, g2, h1)));
+            xb[2].x[c] = fma(24 * dt / 45, u, fma(9 * dt / 8, g3, g2 * (dt / 3)));
+            xb[3].x[c] = fma(64 * dt / 45, u, g3 * (3 * dt / 8));
         }
     }
     __device__ static double end_role(double dt) { return 14 * dt / 45; }
+    __device__ static double start_role(double dt) { return 14 * dt / 45; }
 };
 
 template <>
 struct BlockRule<SIMPSON> {
     static constexpr int B = 2;
-    __device__ static void fwd(const V4& f0, const V4* f, const V4& base, double dt, V4* P) {
-        const double half_dt = dt / 2, third_dt = dt / 3;
-        for (int c = 0; c < 4; ++c) {
-            double p;
-            p = f[0].x[c] * half_dt;
-            p += f0.x[c] * half_dt;
-            p += base.x[c];
-            P[0].x[c] = p;
-            p = f[1].x[c] * third_dt;
-            p += f0.x[c] * third_dt;
-            p += f[0].x[c] * (4 * third_dt);
-            p += base.x[c];
-            P[1].x[c] = p;
-        }
+    __device__ static void interior(const V4* f, const V4& v0, double dt, V4* P) {
+        for (int c = 0; c < 4; ++c) P[0].x[c] = fma(dt / 2, f[0].x[c] + f[1].x[c], v0.x[c]);
     }
-    // simpson.hpp:77-123
-    __device__ static void rev(const V4* g, const V4& gB, double dt, V4* xb) {
-        const double half_dt = dt / 2, third_dt = dt / 3;
+    __device__ static void fwd(const V4* f, const V4& base, double dt, V4* P) {
+        interior(f, base, dt, P);
+        for (int c = 0; c < 4; ++c)
+            P[1].x[c] = fma(dt / 3, fma(4.0, f[1].x[c], f[0].x[c] + f[2].x[c]), base.x[c]);
+    }
+    __device__ static V4 back(const V4* f, const V4& end, double dt) {
+        V4 v;
+        for (int c = 0; c < 4; ++c) v.x[c] = fma(-dt / 3, fma(4.0, f[1].x[c], f[0].x[c] + f[2].x[c]), end.x[c]);
+        return v;
+    }
+    __device__ static void rev(const V4* g, const V4& U, double dt, V4* xb) {
         for (int c = 0; c < 4; ++c) {
-            const double g1 = g[0].x[c], g2 = gB.x[c];
-            double v = g1 * half_dt;
-            v += g2 * third_dt;
-            xb[0].x[c] = v;
-            v = g1 * half_dt;
-            v += g2 * (4 * third_dt);
-            xb[1].x[c] = v;
+            const double h1 = g[0].x[c] * (dt / 2);
+            xb[0].x[c] = fma(dt / 3, U.x[c], h1);
+            xb[1].x[c] = fma(4 * dt / 3, U.x[c], h1);
         }
     }
     __device__ static double end_role(double dt) { return dt / 3; }
+    __device__ static double start_role(double dt) { return dt / 3; }
 };
 
 template <>
 struct BlockRule<RIEMANN> {
     static constexpr int B = 1;
-    __device__ static void fwd(const V4&, const V4* f, const V4& base, double dt, V4* P) {
-        for (int c = 0; c < 4; ++c) {
-            double p = f[0].x[c] * dt;
-            p += base.x[c];
-            P[0].x[c] = p;
-        }
-    }
+    __device__ static void interior(const V4*, const V4&, double, V4*) {}
+    __device__ static void fwd(const V4* f, const V4& base, double dt, V4* P) { P[0] = vaxpy(base, dt, f[1]); }
+    __device__ static V4 back(const V4* f, const V4& end, double dt) { return vaxpy(end, -dt, f[1]); }
     __device__ static void rev(const V4*, const V4&, double, V4* xb) { xb[0] = vzero(); }
     __device__ static double end_role(double dt) { return dt; }
+    __device__ static double start_role(double) { return 0.0; }
 };
 
 __device__ __forceinline__ long long chunk_begin(const WbArgs& a, int c) { return (long long)c * a.LC; }
@@ -199,13 +203,14 @@ __device__ __forceinline__ long long chunk_end(const WbArgs& a, int c) {
 }
 
 // ---------------------------------------------------------------------------
-// forward pass 1: per chunk (s, e], streams from zero bases.
-// summary layout per (b, c): [Vhat_1..Vhat_K, ZV_0..ZV_K] (4 doubles each)
+// chunk pass: local streams V_1..V_{K+1} over (s, e] from zero bases.
+// summary per (b, c): [Vhat_1..Vhat_K, ZV_0..ZV_K] (4 doubles each)
 // ---------------------------------------------------------------------------
 template <int I, int K>
 __global__ void __launch_bounds__(256) wb_fwd_pass1(WbArgs a) {
     using R = BlockRule<I>;
     constexpr int B = R::B;
+    constexpr int L = K + 1;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.B * a.C) return;
     const long long b = t / a.C;
@@ -215,230 +220,367 @@ __global__ void __launch_bounds__(256) wb_fwd_pass1(WbArgs a) {
     const double* A3 = a.in + b * S * 3;
     const long long s = chunk_begin(a, c), e = chunk_end(a, c);
 
-    V4 base[K], f0[K], ZV[K + 1];
+    V4 base[L], f0[L];  // level l: stream V_{l+1}; f0[l] = its integrand at the block start
 #pragma unroll
-    for (int k = 0; k < K; ++k) {
-        base[k] = vzero();
-        f0[k] = vzero();
+    for (int l = 0; l < L; ++l) {
+        base[l] = vzero();
+        f0[l] = vzero();
     }
+    double as[3];
+    lda(A3, s, as);
+    f0[0] = pure(as);
+    if (I == RIEMANN && c == 0) {  // P_0 = dt f_0 is a step of its own
+        V4 X = f0[0];
 #pragma unroll
-    for (int k = 0; k <= K; ++k) ZV[k] = vzero();
-    {
-        double a0[3];
-        lda(A3, s, a0);
-        f0[0] = V4{{0.0, a0[0], a0[1], a0[2]}};
-        if (c == 0) {
-            const double w0 = sum_vjp_coef<I>(0, S, dt);
-            ZV[0] = vaxpy(ZV[0], w0, f0[0]);
-            if (I == RIEMANN) {  // P_0 = dt f_0 is a step of its own
-                V4 X = f0[0];
-#pragma unroll
-                for (int k = 0; k < K; ++k) {
-                    V4 P0[1];
-                    R::fwd(vzero(), &X, base[k], dt, P0);
-                    base[k] = P0[0];
-                    X = amul(a0, P0[0]);
-                    ZV[k + 1] = vaxpy(ZV[k + 1], w0, X);
-                }
-            }
+        for (int l = 0; l < L; ++l) {
+            base[l] = vscale(dt, X);
+            X = amul(as, base[l]);
         }
     }
+    double nx[B][3];
+#pragma unroll
+    for (int r = 0; r < B; ++r) lda(A3, s + 1 + r, nx[r]);
     for (long long s0 = s; s0 < e; s0 += B) {
         double ab[B][3];
-        V4 X[B];
-        double wgt[B];
 #pragma unroll
         for (int r = 0; r < B; ++r) {
-            lda(A3, s0 + 1 + r, ab[r]);
-            X[r] = V4{{0.0, ab[r][0], ab[r][1], ab[r][2]}};
-            wgt[r] = sum_vjp_coef<I>(s0 + 1 + r, S, dt);
-            ZV[0] = vaxpy(ZV[0], wgt[r], X[r]);
+            ab[r][0] = nx[r][0];
+            ab[r][1] = nx[r][1];
+            ab[r][2] = nx[r][2];
         }
+        if (s0 + B < e) {  // next block's samples, in flight during this block
 #pragma unroll
-        for (int k = 0; k < K; ++k) {
+            for (int r = 0; r < B; ++r) lda(A3, s0 + B + 1 + r, nx[r]);
+        }
+        V4 f[B + 1];
+#pragma unroll
+        for (int r = 0; r < B; ++r) f[r + 1] = pure(ab[r]);
+#pragma unroll
+        for (int l = 0; l < L; ++l) {
+            f[0] = f0[l];
             V4 P[B];
-            R::fwd(f0[k], X, base[k], dt, P);
-            f0[k] = X[B - 1];
-            base[k] = P[B - 1];
+            R::fwd(f, base[l], dt, P);
+            f0[l] = f[B];
+            base[l] = P[B - 1];
+            if (l + 1 < L) {
 #pragma unroll
-            for (int r = 0; r < B; ++r) {
-                X[r] = amul(ab[r], P[r]);
-                ZV[k + 1] = vaxpy(ZV[k + 1], wgt[r], X[r]);
+                for (int r = 0; r < B; ++r) f[r + 1] = amul(ab[r], P[r]);
             }
         }
     }
+    // f0[l] = A_e V_l[e] (V_0 = identity), base[l] = V_{l+1}[e]
+    const double cfix = (I == RIEMANN || e == S - 1) ? 0.0 : R::end_role(dt);
     double* out = a.fsum + t * (long long)(2 * K + 1) * 4;
 #pragma unroll
     for (int k = 0; k < K; ++k) st4(out + k * 4, base[k]);
+    V4 z0 = vaxpy(base[0], cfix, f0[0]);
+    if (c > 0) z0 = vaxpy(z0, -R::start_role(dt), pure(as));
+    st4(out + K * 4, z0);
 #pragma unroll
-    for (int k = 0; k <= K; ++k) st4(out + (K + k) * 4, ZV[k]);
+    for (int k = 1; k <= K; ++k) st4(out + (K + k) * 4, vaxpy(base[k], cfix, f0[k]));
 }
 
 // ---------------------------------------------------------------------------
-// per-instance combine: chunk composition, GL algebra, outputs; in VJP mode
-// also the adjoint of the GL algebra (sources Zbar_0..K, What_bar_1..K).
+// truncated series.  USer: I + sum_{j=1..K} v[j-1] x^j;  FSer: sum_{j=0..K} v[j] x^j
 // ---------------------------------------------------------------------------
 template <int K>
-__global__ void __launch_bounds__(64) wb_fwd_combine(WbArgs a) {
+struct USer {
+    V4 v[K];
+};
+template <int K>
+struct FSer {
+    V4 v[K + 1];
+};
+template <int K>
+struct Pair {
+    USer<K> V;
+    FSer<K> Z;
+};
+
+template <int K>
+__device__ __forceinline__ USer<K> umul(const USer<K>& a, const USer<K>& b) {
+    USer<K> h;
+#pragma unroll
+    for (int m = 1; m <= K; ++m) {
+        V4 acc = vadd(a.v[m - 1], b.v[m - 1]);
+#pragma unroll
+        for (int j = 1; j < m; ++j) acc = vadd(acc, qmul(a.v[j - 1], b.v[m - j - 1]));
+        h.v[m - 1] = acc;
+    }
+    return h;
+}
+
+template <int K>
+__device__ __forceinline__ Pair<K> compose(const Pair<K>& later, const Pair<K>& earlier) {
+    Pair<K> r;
+    r.V = umul(later.V, earlier.V);
+#pragma unroll
+    for (int m = 0; m <= K; ++m) {
+        V4 acc = vadd(later.Z.v[m], earlier.Z.v[m]);
+#pragma unroll
+        for (int j = 0; j < m; ++j) acc = vadd(acc, qmul(later.Z.v[j], earlier.V.v[m - j - 1]));
+        r.Z.v[m] = acc;
+    }
+    return r;
+}
+
+template <int K>
+__device__ __forceinline__ Pair<K> pair_identity() {
+    Pair<K> p;
+#pragma unroll
+    for (int j = 0; j < K; ++j) p.V.v[j] = vzero();
+#pragma unroll
+    for (int j = 0; j <= K; ++j) p.Z.v[j] = vzero();
+    return p;
+}
+
+template <int K>
+__device__ __forceinline__ Pair<K> load_pair(const double* p) {
+    Pair<K> r;
+#pragma unroll
+    for (int j = 0; j < K; ++j) r.V.v[j] = ld4(p + j * 4);
+#pragma unroll
+    for (int j = 0; j <= K; ++j) r.Z.v[j] = ld4(p + (K + j) * 4);
+    return r;
+}
+
+__device__ __forceinline__ V4 shfl4(const V4& v, int src) {
+    return V4{{__shfl(v.x[0], src, 64), __shfl(v.x[1], src, 64), __shfl(v.x[2], src, 64), __shfl(v.x[3], src, 64)}};
+}
+__device__ __forceinline__ V4 shfl_down4(const V4& v, int d) {
+    return V4{{__shfl_down(v.x[0], d, 64), __shfl_down(v.x[1], d, 64), __shfl_down(v.x[2], d, 64),
+               __shfl_down(v.x[3], d, 64)}};
+}
+__device__ __forceinline__ V4 shfl_up4(const V4& v, int d) {
+    return V4{{__shfl_up(v.x[0], d, 64), __shfl_up(v.x[1], d, 64), __shfl_up(v.x[2], d, 64),
+               __shfl_up(v.x[3], d, 64)}};
+}
+
+template <int K>
+__device__ __forceinline__ Pair<K> shfl_down_pair(const Pair<K>& p, int d) {
+    Pair<K> r;
+#pragma unroll
+    for (int j = 0; j < K; ++j) r.V.v[j] = shfl_down4(p.V.v[j], d);
+#pragma unroll
+    for (int j = 0; j <= K; ++j) r.Z.v[j] = shfl_down4(p.Z.v[j], d);
+    return r;
+}
+
+// ---------------------------------------------------------------------------
+// combine: one wavefront per instance, each lane owns m consecutive chunks.
+// Suffix scan of the pairs over the lanes -> totals (lane 0) -> GL algebra
+// (lane q evaluates node q) -> outputs.  VJP: GL adjoint, prefix scan of the
+// V series, then per chunk the cotangents of its summary:
+//   Wbar_{c+1} = adj_right(Y_{c+1}; Zbar) + adj_right(S_{c+1}; What_bar)
+//   Vhat_bar_c = adj_left(W~_c; Wbar_{c+1}),   ZV_bar_c = adj_left(W~_c; Zbar)
+// with (S_{c+1}, Y_{c+1}) the composition of the chunks after c and
+// W~_c the product of the chunks before c.
+// ---------------------------------------------------------------------------
+template <int K, bool VJP>
+__global__ void __launch_bounds__(256) wb_combine(WbArgs a) {
     constexpr int n = K + 1;
-    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.B) return;
-    V4 W[K + 1], Z[K + 1];  // W[1..K] running bases; Z[0..K]
-#pragma unroll
-    for (int k = 0; k <= K; ++k) {
-        W[k] = vzero();
-        Z[k] = vzero();
+    constexpr int stride = (2 * K + 1) * 4;
+    const int lane = (int)(threadIdx.x & 63);
+    const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
+    if (b >= a.B) return;  // uniform over the wavefront
+    const int C = a.C;
+    const int m = (C + 63) / 64;
+    const int nl = (C + m - 1) / m;
+    const int c0 = lane * m;
+    const int c1 = c0 + m < C ? c0 + m : C;
+    const bool act = lane < nl;
+    const double* fs = a.fsum + b * (long long)C * stride;
+
+    Pair<K> pr = pair_identity<K>();
+    if (act) {
+        pr = load_pair<K>(fs + (long long)c0 * stride);
+        for (int c = c0 + 1; c < c1; ++c) pr = compose<K>(load_pair<K>(fs + (long long)c * stride), pr);
     }
-    for (int c = 0; c < a.C; ++c) {
-        const double* sm = a.fsum + (b * a.C + c) * (long long)(2 * K + 1) * 4;
-        if (a.bases) {
-            double* bs = a.bases + (b * a.C + c) * (long long)K * 4;
-#pragma unroll
-            for (int j = 1; j <= K; ++j) st4(bs + (j - 1) * 4, W[j]);
-        }
-        V4 ZV[K + 1], Vh[K + 1];
-#pragma unroll
-        for (int k = 0; k <= K; ++k) ZV[k] = ld4(sm + (K + k) * 4);
-#pragma unroll
-        for (int k = 1; k <= K; ++k) Vh[k] = ld4(sm + (k - 1) * 4);
-        Z[0] = vadd(Z[0], ZV[0]);
-#pragma unroll
-        for (int l = 1; l <= K; ++l) {
-            V4 acc = vadd(ZV[l], qmul(ZV[0], W[l]));
-#pragma unroll
-            for (int lp = 1; lp < l; ++lp) acc = vadd(acc, qmul(ZV[l - lp], W[lp]));
-            Z[l] = vadd(Z[l], acc);
-        }
-        V4 NW[K + 1];
-#pragma unroll
-        for (int j = 1; j <= K; ++j) {
-            V4 acc = vadd(Vh[j], W[j]);
-#pragma unroll
-            for (int l = 1; l < j; ++l) acc = vadd(acc, qmul(Vh[j - l], W[l]));
-            NW[j] = acc;
-        }
-#pragma unroll
-        for (int j = 1; j <= K; ++j) W[j] = NW[j];
+    Pair<K> suf = pr;
+    for (int d = 1; d < nl; d <<= 1) {
+        const Pair<K> o = shfl_down_pair<K>(suf, d);
+        if (lane + d < nl) suf = compose<K>(o, suf);
     }
-    // GL algebra (per node): T_j = What_j + s sum_{l<j} What_l T_{j-l};
-    // Omega_j += w (Z_{j-1} + s sum_{l<=j-2} Z_l T_{j-1-l})
-    V4 Om[n];
-    Om[0] = Z[0];
+    // instance totals: Z_0..K and What_1..K
+    V4 Z[K + 1], Wh[K];
 #pragma unroll
-    for (int j = 1; j < n; ++j) Om[j] = vzero();
+    for (int j = 0; j <= K; ++j) Z[j] = shfl4(suf.Z.v[j], 0);
+#pragma unroll
+    for (int j = 0; j < K; ++j) Wh[j] = shfl4(suf.V.v[j], 0);
+
     const int Q = a.Q;
-    for (int q = 0; q < Q; ++q) {
-        const double s = a.x[q] - 1.0, w = a.w[q];
-        V4 T[K + 1];
-        T[0] = vzero();
-#pragma unroll
-        for (int j = 1; j <= K; ++j) {
-            V4 acc = W[j];
-#pragma unroll
-            for (int l = 1; l < j; ++l) acc = vaxpy(acc, s, qmul(W[l], T[j - l]));
-            T[j] = acc;
+    double sq = 0.0, wq = 0.0;
+    for (int q = 0; q < Q; ++q)
+        if (lane == q) {
+            sq = a.x[q] - 1.0;
+            wq = a.w[q];
         }
+    // GL algebra for node `lane`: T_j = What_j + s sum_{l<j} What_l T_{j-l};
+    // Omega_j += w (Z_{j-1} + s sum_{l<=j-2} Z_l T_{j-1-l})
+    V4 T[K + 1];
+    T[0] = vzero();
 #pragma unroll
-        for (int j = 2; j <= n; ++j) {
-            V4 acc = Z[j - 1];
+    for (int j = 1; j <= K; ++j) {
+        V4 acc = Wh[j - 1];
 #pragma unroll
-            for (int l = 0; l <= j - 2; ++l) acc = vaxpy(acc, s, qmul(Z[l], T[j - 1 - l]));
-            Om[j - 1] = vaxpy(Om[j - 1], w, acc);
-        }
+        for (int l = 1; l < j; ++l) acc = vaxpy(acc, sq, qmul(Wh[l - 1], T[j - l]));
+        T[j] = acc;
     }
-    if (a.out) {  // outputs: components 1..3
+    V4 part[K];  // Omega_2..Omega_n of this node
+#pragma unroll
+    for (int j = 2; j <= n; ++j) {
+        V4 acc = Z[j - 1];
+#pragma unroll
+        for (int l = 0; l <= j - 2; ++l) acc = vaxpy(acc, sq, qmul(Z[l], T[j - 1 - l]));
+        part[j - 2] = (lane < Q) ? vscale(wq, acc) : vzero();
+    }
+    V4 Om[K];
+#pragma unroll
+    for (int j = 0; j < K; ++j) Om[j] = vzero();
+    for (int q = 0; q < Q; ++q) {
+#pragma unroll
+        for (int j = 0; j < K; ++j) Om[j] = vadd(Om[j], shfl4(part[j], q));
+    }
+    if (a.out && lane == 0) {  // outputs: components 1..3
         double* o = a.out + b * (long long)a.n_out * 3;
         if (a.op == OP_MANY) {
+            for (int cc = 0; cc < 3; ++cc) o[cc] = Z[0].x[cc + 1];
 #pragma unroll
-            for (int j = 0; j < n; ++j)
-                for (int cc = 0; cc < 3; ++cc) o[j * 3 + cc] = Om[j].x[cc + 1];
+            for (int j = 1; j < n; ++j)
+                for (int cc = 0; cc < 3; ++cc) o[j * 3 + cc] = Om[j - 1].x[cc + 1];
         } else if (a.op == OP_SUM) {
-            V4 acc = Om[0];
+            V4 acc = Z[0];
 #pragma unroll
-            for (int j = 1; j < n; ++j) acc = vadd(acc, Om[j]);
+            for (int j = 0; j < K; ++j) acc = vadd(acc, Om[j]);
             for (int cc = 0; cc < 3; ++cc) o[cc] = acc.x[cc + 1];
         } else {
-            for (int cc = 0; cc < 3; ++cc) o[cc] = Om[n - 1].x[cc + 1];
+            for (int cc = 0; cc < 3; ++cc) o[cc] = Om[K - 1].x[cc + 1];
         }
     }
-    if (a.what) {
+    if constexpr (VJP) {
+        // ---- adjoint of the GL algebra, node `lane` ----
+        const double* cot = a.cot + b * (long long)a.n_out * 3;
+        V4 Ob[n];
 #pragma unroll
-        for (int j = 1; j <= K; ++j) st4(a.what + (b * K + (j - 1)) * 4, W[j]);
-    }
-    if (!a.src) return;
-    // ---- adjoint of the GL algebra ----
-    const double* cot = a.cot + b * (long long)a.n_out * 3;
-    V4 Ob[n];
+        for (int j = 0; j < n; ++j) {
+            Ob[j] = vzero();
+            const double* row = nullptr;
+            if (a.op == OP_MANY) row = cot + j * 3;
+            else if (a.op == OP_SUM || j == n - 1) row = cot;
+            if (row) Ob[j] = pure(row);
+        }
+        V4 Zp[K + 1], Wp[K], Tb[K + 1];
 #pragma unroll
-    for (int j = 0; j < n; ++j) {
-        Ob[j] = vzero();
-        const double* row = nullptr;
-        if (a.op == OP_MANY) row = cot + j * 3;
-        else if (a.op == OP_SUM || j == n - 1) row = cot;
-        if (row) Ob[j] = V4{{0.0, row[0], row[1], row[2]}};
-    }
-    V4 Zb[n], Wb[K + 1];
-#pragma unroll
-    for (int j = 0; j < n; ++j) Zb[j] = vzero();
-#pragma unroll
-    for (int j = 0; j <= K; ++j) Wb[j] = vzero();
-    Zb[0] = Ob[0];
-    for (int q = 0; q < Q; ++q) {
-        const double s = a.x[q] - 1.0, w = a.w[q];
-        V4 T[K + 1], Tb[K + 1];
-        T[0] = vzero();
-#pragma unroll
-        for (int j = 1; j <= K; ++j) {
-            V4 acc = W[j];
-#pragma unroll
-            for (int l = 1; l < j; ++l) acc = vaxpy(acc, s, qmul(W[l], T[j - l]));
-            T[j] = acc;
+        for (int j = 0; j <= K; ++j) {
+            Zp[j] = vzero();
             Tb[j] = vzero();
         }
-        Tb[0] = vzero();
 #pragma unroll
-        for (int j = 2; j <= n; ++j) {
-            V4 g = Ob[j - 1];
-            g.x[0] *= w;
-            g.x[1] *= w;
-            g.x[2] *= w;
-            g.x[3] *= w;
-            Zb[j - 1] = vadd(Zb[j - 1], g);
+        for (int j = 0; j < K; ++j) Wp[j] = vzero();
+        if (lane < Q) {
 #pragma unroll
-            for (int l = 0; l <= j - 2; ++l) {
-                Zb[l] = vaxpy(Zb[l], s, qmul(g, qconj(T[j - 1 - l])));
-                Tb[j - 1 - l] = vaxpy(Tb[j - 1 - l], s, qmul(qconj(Z[l]), g));
+            for (int j = 2; j <= n; ++j) {
+                const V4 g = vscale(wq, Ob[j - 1]);
+                Zp[j - 1] = vadd(Zp[j - 1], g);
+#pragma unroll
+                for (int l = 0; l <= j - 2; ++l) {
+                    Zp[l] = vaxpy(Zp[l], sq, qmul(g, qconj(T[j - 1 - l])));
+                    Tb[j - 1 - l] = vaxpy(Tb[j - 1 - l], sq, qmul(qconj(Z[l]), g));
+                }
+            }
+#pragma unroll
+            for (int j = K; j >= 1; --j) {
+                Wp[j - 1] = vadd(Wp[j - 1], Tb[j]);
+#pragma unroll
+                for (int l = 1; l < j; ++l) {
+                    Wp[l - 1] = vaxpy(Wp[l - 1], sq, qmul(Tb[j], qconj(T[j - l])));
+                    Tb[j - l] = vaxpy(Tb[j - l], sq, qmul(qconj(Wh[l - 1]), Tb[j]));
+                }
             }
         }
+        V4 Zbar[K + 1], Wbar[K];
+        Zbar[0] = Ob[0];
 #pragma unroll
-        for (int j = K; j >= 1; --j) {
-            Wb[j] = vadd(Wb[j], Tb[j]);
+        for (int j = 1; j <= K; ++j) Zbar[j] = vzero();
 #pragma unroll
-            for (int l = 1; l < j; ++l) {
-                Wb[l] = vaxpy(Wb[l], s, qmul(Tb[j], qconj(T[j - l])));
-                Tb[j - l] = vaxpy(Tb[j - l], s, qmul(qconj(W[l]), Tb[j]));
+        for (int j = 0; j < K; ++j) Wbar[j] = vzero();
+        for (int q = 0; q < Q; ++q) {
+#pragma unroll
+            for (int j = 0; j <= K; ++j) Zbar[j] = vadd(Zbar[j], shfl4(Zp[j], q));
+#pragma unroll
+            for (int j = 0; j < K; ++j) Wbar[j] = vadd(Wbar[j], shfl4(Wp[j], q));
+        }
+        // exclusive suffix (chunks after this lane) and exclusive prefix of V
+        Pair<K> sf = shfl_down_pair<K>(suf, 1);
+        if (lane + 1 >= nl) sf = pair_identity<K>();
+        USer<K> pre = pr.V;
+        for (int d = 1; d < nl; d <<= 1) {
+            USer<K> o;
+#pragma unroll
+            for (int j = 0; j < K; ++j) o.v[j] = shfl_up4(pre.v[j], d);
+            if (lane >= d) pre = umul<K>(pre, o);
+        }
+        USer<K> W;
+#pragma unroll
+        for (int j = 0; j < K; ++j) W.v[j] = shfl_up4(pre.v[j], 1);
+        if (lane == 0) {
+#pragma unroll
+            for (int j = 0; j < K; ++j) W.v[j] = vzero();
+        }
+        if (!act) return;
+        double* bs = a.bases + b * (long long)C * K * 4;
+        for (int c = c0; c < c1; ++c) {  // W~_c for the lane's chunks
+#pragma unroll
+            for (int j = 0; j < K; ++j) st4(bs + ((long long)c * K + j) * 4, W.v[j]);
+            if (c + 1 < c1) {
+                USer<K> v;
+#pragma unroll
+                for (int j = 0; j < K; ++j) v.v[j] = ld4(fs + (long long)c * stride + j * 4);
+                W = umul<K>(v, W);
             }
+        }
+        double* so = a.src + b * (long long)C * stride;
+        for (int c = c1 - 1; c >= c0; --c) {
+            // Wbar_{c+1}, coefficients 1..K
+            V4 wb[K];
+#pragma unroll
+            for (int l = 1; l <= K; ++l) {
+                V4 acc = Wbar[l - 1];
+#pragma unroll
+                for (int mm = l + 1; mm <= K; ++mm) acc = vadd(acc, qmul(qconj(sf.V.v[mm - l - 1]), Wbar[mm - 1]));
+#pragma unroll
+                for (int mm = l; mm <= K; ++mm) acc = vadd(acc, qmul(qconj(sf.Z.v[mm - l]), Zbar[mm]));
+                wb[l - 1] = acc;
+            }
+#pragma unroll
+            for (int j = 0; j < K; ++j) W.v[j] = ld4(bs + ((long long)c * K + j) * 4);
+            double* o = so + (long long)c * stride;
+#pragma unroll
+            for (int j = 1; j <= K; ++j) {
+                V4 acc = wb[j - 1];
+#pragma unroll
+                for (int mm = j + 1; mm <= K; ++mm) acc = vadd(acc, qmul(wb[mm - 1], qconj(W.v[mm - j - 1])));
+                st4(o + (j - 1) * 4, acc);
+            }
+#pragma unroll
+            for (int j = 0; j <= K; ++j) {
+                V4 acc = Zbar[j];
+#pragma unroll
+                for (int mm = j + 1; mm <= K; ++mm) acc = vadd(acc, qmul(Zbar[mm], qconj(W.v[mm - j - 1])));
+                st4(o + (K + j) * 4, acc);
+            }
+            if (c > c0) sf = compose<K>(sf, load_pair<K>(fs + (long long)c * stride));
         }
     }
-    double* src = a.src + b * (long long)(2 * K + 1) * 4;  // [Zbar_0..K, Wbar_1..K]
-#pragma unroll
-    for (int j = 0; j <= K; ++j) st4(src + j * 4, Zb[j]);
-#pragma unroll
-    for (int j = 1; j <= K; ++j) st4(src + (K + j) * 4, Wb[j]);
-}
-
-// stream buffer index: W_j at sample p of instance b, interleaved by chunk
-__device__ __forceinline__ long long wix(const WbArgs& a, long long b, int j, long long p) {
-    const long long c = (p - 1) / a.LC;  // owning forward chunk (p >= 1)
-    const long long r = p - 1 - c * a.LC;
-    return (((b * a.K + j) * a.LC + r) * a.C + c) * 4;
 }
 
 // ---------------------------------------------------------------------------
-// forward pass 2: W streams from the true chunk bases (needed by the VJP).
+// reverse chunk pass: local adjoint of chunk (s, e] given its summary
+// cotangents; local streams rebuilt backwards from Vhat.  dA is written for
+// s < i < e; the values at e and at s go to `edge` for the fixup.
 // ---------------------------------------------------------------------------
 template <int I, int K>
-__global__ void __launch_bounds__(256) wb_fwd_pass2(WbArgs a) {
+__global__ void __launch_bounds__(256) wb_rev(WbArgs a) {
     using R = BlockRule<I>;
     constexpr int B = R::B;
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -447,306 +589,152 @@ __global__ void __launch_bounds__(256) wb_fwd_pass2(WbArgs a) {
     const int c = (int)(t - b * a.C);
     const long long S = a.S;
     const double dt = a.dt;
+    const double er = R::end_role(dt);
     const double* A3 = a.in + b * S * 3;
     const long long s = chunk_begin(a, c), e = chunk_end(a, c);
-    V4 base[K], f0[K];
-    double as[3];
-    lda(A3, s, as);
-    const double* bs = a.bases + t * (long long)K * 4;
+    const double* sp = a.src + t * (long long)(2 * K + 1) * 4;
+    const double* fp = a.fsum + t * (long long)(2 * K + 1) * 4;
+
+    V4 Vend[K], U[K];
 #pragma unroll
-    for (int k = 0; k < K; ++k) base[k] = ld4(bs + k * 4);
-    f0[0] = V4{{0.0, as[0], as[1], as[2]}};
+    for (int k = 0; k < K; ++k) Vend[k] = ld4(fp + k * 4);
+    double aend[3];
+    lda(A3, e, aend);
+    const V4 zK = ld4(sp + 2 * K * 4);  // ZV_bar_K
+    {
+        // the chunk end e: Xbar_{k+1} known, U_k = Gbar_k + Vhat_bar_k + ZV_bar_{k-1}
+        // (the last term is the folded sum cotangent), Xbar_k = er U_k + (c_e - er) ZV_bar_{k-1}
+        const double ce = sum_vjp_coef<I>(e, S, dt);
+        V4 Xu = vscale(ce, zK);
+        V4 d = vzero();
 #pragma unroll
-    for (int k = 1; k < K; ++k) f0[k] = amul(as, base[k - 1]);
-    if (c == 0) {
-        if (I == RIEMANN) {
-            V4 X = f0[0];
-#pragma unroll
-            for (int k = 0; k < K; ++k) {
-                V4 P0[1];
-                R::fwd(vzero(), &X, base[k], dt, P0);
-                base[k] = P0[0];
-                X = amul(as, P0[0]);
-                st4(a.w0 + (b * K + k) * 4, P0[0]);
-            }
-        } else {
-#pragma unroll
-            for (int k = 0; k < K; ++k) st4(a.w0 + (b * K + k) * 4, vzero());
+        for (int k = K; k >= 1; --k) {
+            const V4 zk = ld4(sp + (K + k - 1) * 4);
+            d = vadd(d, qmul(Xu, qconj(Vend[k - 1])));
+            const V4 uv = vadd(vadd(vneg(amul(aend, Xu)), ld4(sp + (k - 1) * 4)), zk);
+            U[k - 1] = uv;
+            Xu = vaxpy(vscale(er, uv), ce - er, zk);
         }
+        d = vadd(d, Xu);
+        double* ed = a.edge + t * 6;
+        ed[0] = d.x[1];
+        ed[1] = d.x[2];
+        ed[2] = d.x[3];
     }
-    for (long long s0 = s; s0 < e; s0 += B) {
+    const bool riem0 = (I == RIEMANN && c == 0);
+    const long long first = (I == RIEMANN && c > 0) ? s + 1 : s;  // Riemann: s has no role
+    double* dAo = a.dA + b * S * 3;
+    V4 pend = vzero();
+    double nx[B][3];
+    if (e - B >= first) {
+#pragma unroll
+        for (int r = 0; r < B; ++r) lda(A3, e - B + r, nx[r]);
+    }
+    for (long long s0 = e - B; s0 >= first; s0 -= B) {
         double ab[B][3];
-        V4 X[B];
 #pragma unroll
         for (int r = 0; r < B; ++r) {
-            lda(A3, s0 + 1 + r, ab[r]);
-            X[r] = V4{{0.0, ab[r][0], ab[r][1], ab[r][2]}};
+            ab[r][0] = nx[r][0];
+            ab[r][1] = nx[r][1];
+            ab[r][2] = nx[r][2];
         }
+        if (s0 - B >= first) {  // previous block's samples, in flight during this block
+#pragma unroll
+            for (int r = 0; r < B; ++r) lda(A3, s0 - B + r, nx[r]);
+        }
+        // rebuild V_1..V_K at s0..s0+B-1 (ascending levels)
+        const bool zero_start = (s0 == s) && !riem0;
+        V4 Vl[K][B];
 #pragma unroll
         for (int k = 0; k < K; ++k) {
-            V4 P[B];
-            R::fwd(f0[k], X, base[k], dt, P);
-            f0[k] = X[B - 1];
-            base[k] = P[B - 1];
+            V4 f[B + 1];
+            if (k == 0) {
 #pragma unroll
-            for (int r = 0; r < B; ++r) {
-                st4(a.wbuf + wix(a, b, k, s0 + 1 + r), P[r]);
-                X[r] = amul(ab[r], P[r]);
+                for (int r = 0; r < B; ++r) f[r] = pure(ab[r]);
+                f[B] = pure(aend);
+            } else {
+#pragma unroll
+                for (int r = 0; r < B; ++r) f[r] = amul(ab[r], Vl[k - 1][r]);
+                f[B] = amul(aend, Vend[k - 1]);
             }
-        }
-    }
-}
-
-__device__ __forceinline__ V4 ldw(const WbArgs& a, long long b, int j, long long p) {
-    if (p == 0) return ld4(a.w0 + (b * a.K + j) * 4);
-    return ld4(a.wbuf + wix(a, b, j, p));
-}
-
-// ---------------------------------------------------------------------------
-// reverse sweep of one chunk [s, e) given Uend (the suffix adjoint sums at e).
-// MODE 0: zero incoming; accumulate the chunk summary (actual streams and the
-//         unit-response streams R_0..R_{K-2}).
-// MODE 1: true incoming; produce dA at every sample of the chunk.
-// Adjoint graph: Xbar_n = c Zbar_{n-1};  G_j = conj(A) Xbar_{j+1};
-// Xbar_j = c Zbar_{j-1} + PrefixVJP(G_j);  dA = Xbar_1 + sum_j Xbar_{j+1} conj(W_j).
-// ---------------------------------------------------------------------------
-template <int I, int K, int MODE>
-__device__ void wb_rev_chunk(const WbArgs& a, long long b, int c, V4* U) {
-    using R = BlockRule<I>;
-    constexpr int B = R::B;
-    const long long S = a.S;
-    const double dt = a.dt;
-    const double* A3 = a.in + b * S * 3;
-    const double* src = a.src + b * (long long)(2 * K + 1) * 4;
-    V4 Zb[K + 1];
-#pragma unroll
-    for (int j = 0; j <= K; ++j) Zb[j] = ld4(src + j * 4);
-    const long long s = chunk_begin(a, c), e = chunk_end(a, c);
-    // unit-response streams (MODE 0): RU[k] = Uend of R_k's PrefixVJP (k = 1..K-2
-    // are scans; R_0 = c has no scan)
-    V4 RU[K > 1 ? K : 1], SR[K > 1 ? K - 1 : 1], SW[K];
-    if (MODE == 0) {
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            RU[k] = vzero();
-            SW[k] = vzero();
+            const V4 v0 = zero_start ? vzero() : R::back(f, Vend[k], dt);
+            Vl[k][0] = v0;
+            R::interior(f, v0, dt, &Vl[k][0] + 1);
         }
 #pragma unroll
-        for (int k = 0; k < K - 1; ++k) SR[k] = vzero();
-    }
-    for (long long s0 = e - B; s0 >= s; s0 -= B) {
-        // positions s0 + t, t = 0..B-1
-        double ab[B][3];
-        double wgt[B];
+        for (int k = 0; k < K; ++k) Vend[k] = Vl[k][0];
+        // reverse sweep of the block, top level down
+        V4 Xr[B], dAb[B];
 #pragma unroll
         for (int r = 0; r < B; ++r) {
-            lda(A3, s0 + r, ab[r]);
-            wgt[r] = sum_vjp_coef<I>(s0 + r, S, dt);
-        }
-        // level n: Xbar_n = c Zbar_{n-1}
-        V4 Xu[B];  // Xbar_{j+1} at the block positions (complete)
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            Xu[r] = Zb[K];
-            Xu[r].x[0] *= wgt[r];
-            Xu[r].x[1] *= wgt[r];
-            Xu[r].x[2] *= wgt[r];
-            Xu[r].x[3] *= wgt[r];
-        }
-        V4 dA[B];
-        if (MODE == 1) {
-#pragma unroll
-            for (int r = 0; r < B; ++r) dA[r] = vzero();
-        }
-        // unit-response chain values at the block positions (MODE 0)
-        V4 Ru[B];
-        if (MODE == 0) {
-#pragma unroll
-            for (int r = 0; r < B; ++r) Ru[r] = V4{{-wgt[r], 0.0, 0.0, 0.0}};  // R_0 = c * identity
+            Xr[r] = vscale(sum_vjp_coef<I>(s0 + r, S, dt), zK);
+            dAb[r] = vzero();
         }
 #pragma unroll
-        for (int j = K; j >= 1; --j) {
-            // dA += Xbar_{j+1} conj(W_j)
-            if (MODE == 1) {
-#pragma unroll
-                for (int r = 0; r < B; ++r) dA[r] = vadd(dA[r], qmul(Xu[r], qconj(ldw(a, b, j - 1, s0 + r))));
-            }
-            // G_j = conj(A) Xbar_{j+1} = -(A Xbar_{j+1})
+        for (int k = K - 1; k >= 0; --k) {
             V4 G[B];
 #pragma unroll
             for (int r = 0; r < B; ++r) {
-                V4 g = amul(ab[r], Xu[r]);
-                G[r] = V4{{-g.x[0], -g.x[1], -g.x[2], -g.x[3]}};
+                dAb[r] = vadd(dAb[r], qmul(Xr[r], qconj(Vl[k][r])));
+                G[r] = vneg(amul(ab[r], Xr[r]));
             }
-            if (MODE == 0) {
-#pragma unroll
-                for (int r = 0; r < B; ++r) SW[j - 1] = vadd(SW[j - 1], G[r]);
-            }
-            // PrefixVJP block step for level j
             V4 xb[B];
-            if (I == RIEMANN) {
-                U[j - 1] = vadd(U[j - 1], G[0]);
-                xb[0] = U[j - 1];
-                xb[0].x[0] *= dt;
-                xb[0].x[1] *= dt;
-                xb[0].x[2] *= dt;
-                xb[0].x[3] *= dt;
+            R::rev(G + 1, U[k], dt, xb);
+            V4 us = U[k];
+#pragma unroll
+            for (int r = B - 1; r >= 0; --r) us = vadd(us, G[r]);
+            U[k] = us;
+            if (s0 > s || riem0) xb[0] = vaxpy(xb[0], er, us);
+#pragma unroll
+            for (int r = 0; r < B; ++r) Xr[r] = xb[r];
+        }
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            const V4 v = vadd(dAb[r], Xr[r]);  // Xr holds Xbar_1
+            if (s0 + r == s) {
+                pend = v;
             } else {
-                R::rev(G + 1, U[j - 1], dt, xb);
-                V4 Us = U[j - 1];
-#pragma unroll
-                for (int r = B - 1; r >= 0; --r) Us = vadd(Us, G[r]);
-                U[j - 1] = Us;  // U_j(s0)
-                if (s0 > 0) xb[0] = vaxpy(xb[0], R::end_role(dt), Us);
-            }
-            // Xbar_j = c Zbar_{j-1} + prefix-VJP contributions
-#pragma unroll
-            for (int r = 0; r < B; ++r) Xu[r] = vaxpy(xb[r], wgt[r], Zb[j - 1]);
-            if (MODE == 0 && j >= 2) {
-                // unit responses: R_{k} chain runs alongside, k = K - j ... handled below
+                dAo[(s0 + r) * 3 + 0] = v.x[1];
+                dAo[(s0 + r) * 3 + 1] = v.x[2];
+                dAo[(s0 + r) * 3 + 2] = v.x[3];
             }
         }
-        if (MODE == 0 && K >= 2) {
-            // R_0 = c (identity); R_k = PrefixVJP_local0(conj(A) R_{k-1}), k = 1..K-2;
-            // SR_k = sum conj(A) R_k, k = 0..K-2
-#pragma unroll
-            for (int k = 0; k <= K - 2; ++k) {
-                V4 G[B];
-#pragma unroll
-                for (int r = 0; r < B; ++r) {
-                    V4 g = amul(ab[r], Ru[r]);
-                    G[r] = V4{{-g.x[0], -g.x[1], -g.x[2], -g.x[3]}};
-                    SR[k] = vadd(SR[k], G[r]);
-                }
-                if (k == K - 2) break;
-                V4 xb[B];
-                if (I == RIEMANN) {
-                    RU[k] = vadd(RU[k], G[0]);
-                    xb[0] = RU[k];
-                    xb[0].x[0] *= dt;
-                    xb[0].x[1] *= dt;
-                    xb[0].x[2] *= dt;
-                    xb[0].x[3] *= dt;
-                } else {
-                    R::rev(G + 1, RU[k], dt, xb);
-                    V4 Us = RU[k];
-#pragma unroll
-                    for (int r = B - 1; r >= 0; --r) Us = vadd(Us, G[r]);
-                    RU[k] = Us;
-                    if (s0 > 0) xb[0] = vaxpy(xb[0], R::end_role(dt), Us);
-                }
-#pragma unroll
-                for (int r = 0; r < B; ++r) Ru[r] = xb[r];
-            }
-        }
-        if (MODE == 1) {
-            double* out = a.dA + b * S * 3;
-#pragma unroll
-            for (int r = 0; r < B; ++r) {
-                const V4 v = vadd(dA[r], Xu[r]);  // Xu holds Xbar_1
-                out[(s0 + r) * 3 + 0] = v.x[1];
-                out[(s0 + r) * 3 + 1] = v.x[2];
-                out[(s0 + r) * 3 + 2] = v.x[3];
-            }
-        }
+        aend[0] = ab[0][0];
+        aend[1] = ab[0][1];
+        aend[2] = ab[0][2];
     }
-    if (MODE == 0) {
-        double* out = a.rsum + (b * a.C + c) * (long long)(2 * K - 1) * 4;  // [SW_1..K, SR_0..K-2]
-#pragma unroll
-        for (int k = 0; k < K; ++k) st4(out + k * 4, SW[k]);
-#pragma unroll
-        for (int k = 0; k < K - 1; ++k) st4(out + (K + k) * 4, SR[k]);
-    }
+    // the start role of A_s saw the folded ZV_bar_0; in chunks after the first
+    // that sample's sum weight belongs to the previous chunk
+    if (c > 0 && I != RIEMANN) pend = vaxpy(pend, -R::start_role(dt), ld4(sp + K * 4));
+    double* ed = a.edge + t * 6;
+    ed[3] = pend.x[1];
+    ed[4] = pend.x[2];
+    ed[5] = pend.x[3];
 }
 
-template <int I, int K>
-__global__ void __launch_bounds__(256) wb_rev_pass1(WbArgs a) {
+// dA at the chunk boundaries: value at e from chunk c plus the start value of chunk c+1
+__global__ void __launch_bounds__(256) wb_fixup(WbArgs a) {
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= a.B * a.C) return;
     const long long b = t / a.C;
     const int c = (int)(t - b * a.C);
-    V4 U[K];
-#pragma unroll
-    for (int k = 0; k < K; ++k) U[k] = vzero();
-    wb_rev_chunk<I, K, 0>(a, b, c, U);
-}
-
-template <int I, int K>
-__global__ void __launch_bounds__(256) wb_rev_pass2(WbArgs a) {
-    const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.B * a.C) return;
-    const long long b = t / a.C;
-    const int c = (int)(t - b * a.C);
-    V4 U[K];
-    const double* ui = a.uin + t * (long long)K * 4;
-#pragma unroll
-    for (int k = 0; k < K; ++k) U[k] = ld4(ui + k * 4);
-    wb_rev_chunk<I, K, 1>(a, b, c, U);
-}
-
-// ---------------------------------------------------------------------------
-// reverse combine per instance: the last sample (S-1) as a step of its own,
-// then the incoming suffix sums of every chunk, last to first.
-// ---------------------------------------------------------------------------
-template <int I, int K>
-__global__ void __launch_bounds__(64) wb_rev_combine(WbArgs a) {
-    using R = BlockRule<I>;
-    const long long b = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (b >= a.B) return;
-    const long long S = a.S;
-    const double dt = a.dt;
-    const long long L = S - 1;
-    const double* src = a.src + b * (long long)(2 * K + 1) * 4;
-    V4 Zb[K + 1], Wbh[K + 1];
-#pragma unroll
-    for (int j = 0; j <= K; ++j) Zb[j] = ld4(src + j * 4);
-#pragma unroll
-    for (int j = 1; j <= K; ++j) Wbh[j] = ld4(src + (K + j) * 4);
-    double aL[3];
-    lda(a.in + b * S * 3, L, aL);
-    const double wL = sum_vjp_coef<I>(L, S, dt);
-    V4 U[K];
-    // top-down at the last sample: Xbar_n = c Zbar_{n-1}; G_j = conj(A) Xbar_{j+1} + What_bar_j;
-    // U_j = G_j; Xbar_j = c Zbar_{j-1} + end_role * U_j
-    V4 Xu = Zb[K];
-    Xu.x[0] *= wL;
-    Xu.x[1] *= wL;
-    Xu.x[2] *= wL;
-    Xu.x[3] *= wL;
-    V4 dA = vzero();
-#pragma unroll
-    for (int j = K; j >= 1; --j) {
-        const V4 Wj = ld4(a.what + (b * K + (j - 1)) * 4);
-        dA = vadd(dA, qmul(Xu, qconj(Wj)));
-        V4 g = amul(aL, Xu);
-        V4 G = V4{{-g.x[0], -g.x[1], -g.x[2], -g.x[3]}};
-        G = vadd(G, Wbh[j]);
-        U[j - 1] = G;
-        V4 x = vzero();
-        x = vaxpy(x, wL, Zb[j - 1]);
-        x = vaxpy(x, R::end_role(dt), G);
-        Xu = x;
+    const long long e = chunk_end(a, c);
+    const double* ed = a.edge + t * 6;
+    double v0 = ed[0], v1 = ed[1], v2 = ed[2];
+    if (c + 1 < a.C) {
+        v0 += ed[9];
+        v1 += ed[10];
+        v2 += ed[11];
     }
-    dA = vadd(dA, Xu);
-    double* out = a.dA + b * S * 3 + L * 3;
-    out[0] = dA.x[1];
-    out[1] = dA.x[2];
-    out[2] = dA.x[3];
-    for (int c = a.C - 1; c >= 0; --c) {
-        double* ui = a.uin + (b * a.C + c) * (long long)K * 4;
-#pragma unroll
-        for (int k = 0; k < K; ++k) st4(ui + k * 4, U[k]);
-        const double* sm = a.rsum + (b * a.C + c) * (long long)(2 * K - 1) * 4;
-        V4 NU[K];
-#pragma unroll
-        for (int j = 1; j <= K; ++j) {
-            V4 acc = vadd(U[j - 1], ld4(sm + (j - 1) * 4));
-#pragma unroll
-            for (int l = j + 1; l <= K; ++l) acc = vadd(acc, qmul(ld4(sm + (K + l - j - 1) * 4), U[l - 1]));
-            NU[j - 1] = acc;
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) U[k] = NU[k];
+    double* o = a.dA + (b * a.S + e) * 3;
+    o[0] = v0;
+    o[1] = v1;
+    o[2] = v2;
+    if (c == 0) {
+        double* z = a.dA + b * a.S * 3;
+        z[0] = ed[3];
+        z[1] = ed[4];
+        z[2] = ed[5];
     }
 }
 
@@ -755,25 +743,24 @@ hipError_t launch_wb_k(const WbArgs& a, bool vjp, hipStream_t st) {
     const long long threads = a.B * a.C;
     const int blk = 256;
     const long long grid = (threads + blk - 1) / blk;
-    const long long cgrid = (a.B + 63) / 64;
+    const long long cgrid = (a.B + 3) / 4;  // one wavefront per instance
     void* h = timer_begin(vjp ? "wb_fwd_pass1_vjp" : "wb_fwd_pass1", st);
     hipLaunchKernelGGL((wb_fwd_pass1<I, K>), dim3(grid), dim3(blk), 0, st, a);
     timer_end(h);
-    h = timer_begin(vjp ? "wb_fwd_combine_vjp" : "wb_fwd_combine", st);
-    hipLaunchKernelGGL((wb_fwd_combine<K>), dim3(cgrid), dim3(64), 0, st, a);
+    if (!vjp) {
+        h = timer_begin("wb_combine", st);
+        hipLaunchKernelGGL((wb_combine<K, false>), dim3(cgrid), dim3(256), 0, st, a);
+        timer_end(h);
+        return hipGetLastError();
+    }
+    h = timer_begin("wb_combine_vjp", st);
+    hipLaunchKernelGGL((wb_combine<K, true>), dim3(cgrid), dim3(256), 0, st, a);
     timer_end(h);
-    if (!vjp) return hipGetLastError();
-    h = timer_begin("wb_fwd_pass2", st);
-    hipLaunchKernelGGL((wb_fwd_pass2<I, K>), dim3(grid), dim3(blk), 0, st, a);
+    h = timer_begin("wb_rev", st);
+    hipLaunchKernelGGL((wb_rev<I, K>), dim3(grid), dim3(blk), 0, st, a);
     timer_end(h);
-    h = timer_begin("wb_rev_pass1", st);
-    hipLaunchKernelGGL((wb_rev_pass1<I, K>), dim3(grid), dim3(blk), 0, st, a);
-    timer_end(h);
-    h = timer_begin("wb_rev_combine", st);
-    hipLaunchKernelGGL((wb_rev_combine<I, K>), dim3(cgrid), dim3(64), 0, st, a);
-    timer_end(h);
-    h = timer_begin("wb_rev_pass2", st);
-    hipLaunchKernelGGL((wb_rev_pass2<I, K>), dim3(grid), dim3(blk), 0, st, a);
+    h = timer_begin("wb_fixup", st);
+    hipLaunchKernelGGL(wb_fixup, dim3(grid), dim3(blk), 0, st, a);
     timer_end(h);
     return hipGetLastError();
 }

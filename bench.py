@@ -156,6 +156,9 @@ def main():
     ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
+    ap.add_argument("--copy-mib", type=int, default=0,
+                    help="only run a plain device copy (y = 1.0 * x) of this many MiB, three times: the "
+                         "reference reading for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
@@ -168,6 +171,14 @@ def main():
     from magnus import capi, torch_ops
 
     dev = torch.device("cuda:0")
+    if args.copy_mib:
+        x = torch.ones(args.copy_mib * 131072, dtype=torch.float64, device=dev)
+        y = torch.empty_like(x)
+        for _ in range(3):
+            torch.mul(x, 1.0, out=y)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}))
+        return
     wl = args.workload
     if wl == "spacecurve-batch":
         B, S, n, modes = args.batch or 4096, 16385, 6, 4

@@ -79,6 +79,16 @@ __device__ __forceinline__ V4 qmul(const V4& a, const V4& b) {
     return c;
 }
 
+// acc + a * b with every term fused
+__device__ __forceinline__ V4 qmul_acc(const V4& acc, const V4& a, const V4& b) {
+    V4 c;
+    c.x[0] = fma(-a.x[0], b.x[0], fma(a.x[3], b.x[3], fma(a.x[2], b.x[2], fma(a.x[1], b.x[1], acc.x[0]))));
+    c.x[1] = fma(-a.x[0], b.x[1], fma(-a.x[3], b.x[2], fma(a.x[2], b.x[3], fma(-b.x[0], a.x[1], acc.x[1]))));
+    c.x[2] = fma(-a.x[0], b.x[2], fma(-a.x[1], b.x[3], fma(a.x[3], b.x[1], fma(-b.x[0], a.x[2], acc.x[2]))));
+    c.x[3] = fma(-a.x[0], b.x[3], fma(-a.x[2], b.x[1], fma(a.x[1], b.x[2], fma(-b.x[0], a.x[3], acc.x[3]))));
+    return c;
+}
+
 // a * b for a pure vector a = (0, a1, a2, a3): SpaceCurve::sample_update's product
 __device__ __forceinline__ V4 amul(const double* a, const V4& b) {
     V4 c;
@@ -109,13 +119,14 @@ __device__ __forceinline__ void lda(const double* A3, long long i, double* a) {
 
 // ---------------------------------------------------------------------------
 // block rules.  Block [s0, s0+B]: P_{s0+r} = P_{s0} + sum_t alpha_{r,t} f_{s0+t}
-// (boole.hpp:37-72, simpson.hpp:24-58, default.hpp:23-26).
-//   fwd:      P at s0+1..s0+B from P_{s0} and f_{s0..s0+B}
-//   back:     P_{s0} from P_{s0+B} and f_{s0..s0+B}
-//   interior: P at s0+1..s0+B-1 from P_{s0}
-//   rev:      transpose: f-adjoints at s0..s0+B-1 from the P-adjoints g at
-//             s0+1..s0+B-1 and U, the sum of the P-adjoints from s0+B on
-//             (the start-role term at s0 included, its end role excluded)
+// (boole.hpp:37-72, simpson.hpp:24-58, default.hpp:23-26).  fs is the
+// integrand at s0, fn[0..B-1] the integrand at s0+1..s0+B.
+//   step:     v (= P_{s0}) becomes P_{s0+B}; Pi[0..B-2] = P at s0+1..s0+B-1
+//   back:     v (= P_{s0+B}) becomes P_{s0}
+//   interior: Pi[0..B-2] = P at s0+1..s0+B-1 from v = P_{s0}
+//   rev:      transpose: f-adjoints xb[0..B-1] at s0..s0+B-1 from the
+//             P-adjoints g[0..B-2] at s0+1..s0+B-1 and U, the sum of the
+//             P-adjoints from s0+B on (start role at s0 included, end role not)
 //   end_role / start_role: alpha_{B,B} / alpha_{B,0}
 // ---------------------------------------------------------------------------
 template <int I>
@@ -124,25 +135,23 @@ struct BlockRule;
 template <>
 struct BlockRule<BOOLE> {
     static constexpr int B = 4;
-    __device__ static double tail(const V4* f, int c) {  // 7 f0 + 32 f1 + 12 f2 + 32 f3 + 7 f4
-        return fma(7.0, f[0].x[c] + f[4].x[c], fma(32.0, f[1].x[c] + f[3].x[c], 12.0 * f[2].x[c]));
+    __device__ static double tail(const V4& fs, const V4* fn, int c) {  // 7 f0 + 32 f1 + 12 f2 + 32 f3 + 7 f4
+        return fma(7.0, fs.x[c] + fn[3].x[c], fma(32.0, fn[0].x[c] + fn[2].x[c], 12.0 * fn[1].x[c]));
     }
-    __device__ static void interior(const V4* f, const V4& v0, double dt, V4* P) {
+    __device__ static void interior(const V4& fs, const V4* fn, const V4& v, double dt, V4* Pi) {
         for (int c = 0; c < 4; ++c) {
-            const double b = v0.x[c];
-            P[0].x[c] = fma(dt / 2, f[0].x[c] + f[1].x[c], b);
-            P[1].x[c] = fma(dt / 3, fma(4.0, f[1].x[c], f[0].x[c] + f[2].x[c]), b);
-            P[2].x[c] = fma(3 * dt / 8, fma(3.0, f[1].x[c] + f[2].x[c], f[0].x[c] + f[3].x[c]), b);
+            const double b = v.x[c];
+            Pi[0].x[c] = fma(dt / 2, fs.x[c] + fn[0].x[c], b);
+            Pi[1].x[c] = fma(dt / 3, fma(4.0, fn[0].x[c], fs.x[c] + fn[1].x[c]), b);
+            Pi[2].x[c] = fma(3 * dt / 8, fma(3.0, fn[0].x[c] + fn[1].x[c], fs.x[c] + fn[2].x[c]), b);
         }
     }
-    __device__ static void fwd(const V4* f, const V4& base, double dt, V4* P) {
-        interior(f, base, dt, P);
-        for (int c = 0; c < 4; ++c) P[3].x[c] = fma(2 * dt / 45, tail(f, c), base.x[c]);
+    __device__ static void step(const V4& fs, const V4* fn, V4& v, double dt, V4* Pi) {
+        interior(fs, fn, v, dt, Pi);
+        for (int c = 0; c < 4; ++c) v.x[c] = fma(2 * dt / 45, tail(fs, fn, c), v.x[c]);
     }
-    __device__ static V4 back(const V4* f, const V4& end, double dt) {
-        V4 v;
-        for (int c = 0; c < 4; ++c) v.x[c] = fma(-2 * dt / 45, tail(f, c), end.x[c]);
-        return v;
+    __device__ static void back(const V4& fs, const V4* fn, V4& v, double dt) {
+        for (int c = 0; c < 4; ++c) v.x[c] = fma(-2 * dt / 45, tail(fs, fn, c), v.x[c]);
     }
     __device__ static void rev(const V4* g, const V4& U, double dt, V4* xb) {
         for (int c = 0; c < 4; ++c) {
@@ -156,23 +165,24 @@ struct BlockRule<BOOLE> {
     }
     __device__ static double end_role(double dt) { return 14 * dt / 45; }
     __device__ static double start_role(double dt) { return 14 * dt / 45; }
+    // composite sum weight at offset r of a block that does not start at 0 (boole.hpp:76-106)
+    __device__ static double weight(int r, double dt) {
+        return r == 0 ? 28 * dt / 45 : (r == 2 ? 24 * dt / 45 : 64 * dt / 45);
+    }
 };
 
 template <>
 struct BlockRule<SIMPSON> {
     static constexpr int B = 2;
-    __device__ static void interior(const V4* f, const V4& v0, double dt, V4* P) {
-        for (int c = 0; c < 4; ++c) P[0].x[c] = fma(dt / 2, f[0].x[c] + f[1].x[c], v0.x[c]);
+    __device__ static void interior(const V4& fs, const V4* fn, const V4& v, double dt, V4* Pi) {
+        for (int c = 0; c < 4; ++c) Pi[0].x[c] = fma(dt / 2, fs.x[c] + fn[0].x[c], v.x[c]);
     }
-    __device__ static void fwd(const V4* f, const V4& base, double dt, V4* P) {
-        interior(f, base, dt, P);
-        for (int c = 0; c < 4; ++c)
-            P[1].x[c] = fma(dt / 3, fma(4.0, f[1].x[c], f[0].x[c] + f[2].x[c]), base.x[c]);
+    __device__ static void step(const V4& fs, const V4* fn, V4& v, double dt, V4* Pi) {
+        interior(fs, fn, v, dt, Pi);
+        for (int c = 0; c < 4; ++c) v.x[c] = fma(dt / 3, fma(4.0, fn[0].x[c], fs.x[c] + fn[1].x[c]), v.x[c]);
     }
-    __device__ static V4 back(const V4* f, const V4& end, double dt) {
-        V4 v;
-        for (int c = 0; c < 4; ++c) v.x[c] = fma(-dt / 3, fma(4.0, f[1].x[c], f[0].x[c] + f[2].x[c]), end.x[c]);
-        return v;
+    __device__ static void back(const V4& fs, const V4* fn, V4& v, double dt) {
+        for (int c = 0; c < 4; ++c) v.x[c] = fma(-dt / 3, fma(4.0, fn[0].x[c], fs.x[c] + fn[1].x[c]), v.x[c]);
     }
     __device__ static void rev(const V4* g, const V4& U, double dt, V4* xb) {
         for (int c = 0; c < 4; ++c) {
@@ -183,17 +193,19 @@ struct BlockRule<SIMPSON> {
     }
     __device__ static double end_role(double dt) { return dt / 3; }
     __device__ static double start_role(double dt) { return dt / 3; }
+    __device__ static double weight(int r, double dt) { return r == 0 ? 2 * dt / 3 : 4 * dt / 3; }  // simpson.hpp:60-75
 };
 
 template <>
 struct BlockRule<RIEMANN> {
     static constexpr int B = 1;
-    __device__ static void interior(const V4*, const V4&, double, V4*) {}
-    __device__ static void fwd(const V4* f, const V4& base, double dt, V4* P) { P[0] = vaxpy(base, dt, f[1]); }
-    __device__ static V4 back(const V4* f, const V4& end, double dt) { return vaxpy(end, -dt, f[1]); }
+    __device__ static void interior(const V4&, const V4*, const V4&, double, V4*) {}
+    __device__ static void step(const V4&, const V4* fn, V4& v, double dt, V4*) { v = vaxpy(v, dt, fn[0]); }
+    __device__ static void back(const V4&, const V4* fn, V4& v, double dt) { v = vaxpy(v, -dt, fn[0]); }
     __device__ static void rev(const V4*, const V4&, double, V4* xb) { xb[0] = vzero(); }
     __device__ static double end_role(double dt) { return dt; }
     __device__ static double start_role(double) { return 0.0; }
+    __device__ static double weight(int, double dt) { return dt; }  // default.hpp:28-30
 };
 
 __device__ __forceinline__ long long chunk_begin(const WbArgs& a, int c) { return (long long)c * a.LC; }
@@ -240,6 +252,7 @@ __global__ void __launch_bounds__(256) wb_fwd_pass1(WbArgs a) {
     double nx[B][3];
 #pragma unroll
     for (int r = 0; r < B; ++r) lda(A3, s + 1 + r, nx[r]);
+#pragma unroll 2
     for (long long s0 = s; s0 < e; s0 += B) {
         double ab[B][3];
 #pragma unroll
@@ -252,19 +265,18 @@ __global__ void __launch_bounds__(256) wb_fwd_pass1(WbArgs a) {
 #pragma unroll
             for (int r = 0; r < B; ++r) lda(A3, s0 + B + 1 + r, nx[r]);
         }
-        V4 f[B + 1];
+        V4 fn[B];  // integrand of the current level at s0+1..s0+B
 #pragma unroll
-        for (int r = 0; r < B; ++r) f[r + 1] = pure(ab[r]);
+        for (int r = 0; r < B; ++r) fn[r] = pure(ab[r]);
 #pragma unroll
         for (int l = 0; l < L; ++l) {
-            f[0] = f0[l];
-            V4 P[B];
-            R::fwd(f, base[l], dt, P);
-            f0[l] = f[B];
-            base[l] = P[B - 1];
+            V4 Pi[B > 1 ? B - 1 : 1];
+            R::step(f0[l], fn, base[l], dt, Pi);
+            f0[l] = fn[B - 1];
             if (l + 1 < L) {
 #pragma unroll
-                for (int r = 0; r < B; ++r) f[r + 1] = amul(ab[r], P[r]);
+                for (int r = 0; r + 1 < B; ++r) fn[r] = amul(ab[r], Pi[r]);
+                fn[B - 1] = amul(ab[B - 1], base[l]);
             }
         }
     }
@@ -576,24 +588,169 @@ __global__ void __launch_bounds__(256) wb_combine(WbArgs a) {
 
 // ---------------------------------------------------------------------------
 // reverse chunk pass: local adjoint of chunk (s, e] given its summary
-// cotangents; local streams rebuilt backwards from Vhat.  dA is written for
-// s < i < e; the values at e and at s go to `edge` for the fixup.
+// cotangents; local streams rebuilt backwards from Vhat.  The values at e and
+// at s go to `edge` for the fixup; dA for s < i < e is staged per wavefront in
+// LDS, 8 samples (192 bytes) per lane, and written by the whole wavefront with
+// stores that cover full 64-byte segments (one lane's 48-byte block writes
+// would straddle segments and be written twice).
 // ---------------------------------------------------------------------------
-template <int I, int K>
-__global__ void __launch_bounds__(256) wb_rev(WbArgs a) {
+constexpr int kRevGroup = 8;                          // samples per lane and flush
+constexpr int kRevRow = kRevGroup * 3;                // doubles per lane row
+constexpr int kRevWaves = 4;                          // wavefronts per workgroup
+
+struct RevDest {
+    long long off;  // element offset in dA of slot 0 of the lane's current group
+    int lim;        // slots q <= lim lie at or below the lane's chunk start s
+    int pad;
+};
+
+struct RevStage {
+    double row[kRevWaves][64 * kRevRow];
+    RevDest dst[kRevWaves][64];
+};
+
+// One block [s0, s0+B] of the reverse chunk pass.  On entry Vend[k] holds
+// V_{k+1} at s0+B and U[k] the suffix sum of level k+1 from s0+B on; on exit
+// they refer to s0.  FIRST: the block starting at the chunk start s, whose
+// value at s (r = 0) is the chunk's start half and goes to `pend`; there the
+// local streams vanish at s and s is not the end of a block of this chunk
+// (except under Riemann, where s = 0 is a step of its own).
+// dA at s0+r goes to slot (s0 + r - e) mod 8 of the lane's staging row.
+template <int I, int K, bool FIRST>
+__device__ __forceinline__ void rev_block(long long s0, long long e, const double (*ab)[3], double* aend, V4* Vend,
+                                          V4* U, const V4& zK, double dt, double* row, V4& pend) {
     using R = BlockRule<I>;
     constexpr int B = R::B;
+    constexpr int BI = B > 1 ? B - 1 : 1;
+    constexpr bool zero_start = FIRST && I != RIEMANN;  // local streams vanish at s (no P_0 step)
+    constexpr bool end_role0 = !FIRST || I == RIEMANN;  // s0 is also the end of the block before
+    // rebuild V_1..V_K at s0..s0+B-1, levels ascending: Vend[k] becomes V_{k+1}[s0]
+    // in place, Vi[k][r-1] holds V_{k+1}[s0+r]
+    V4 Vi[K][BI];
+    V4 fB = pure(aend);  // integrand of the level at s0+B
+#pragma unroll
+    for (int k = 0; k < K; ++k) {
+        V4 fs, fn[B];
+        if (k == 0) {
+            fs = pure(ab[0]);
+#pragma unroll
+            for (int r = 1; r < B; ++r) fn[r - 1] = pure(ab[r]);
+        } else {
+            fs = amul(ab[0], Vend[k - 1]);
+#pragma unroll
+            for (int r = 1; r < B; ++r) fn[r - 1] = amul(ab[r], Vi[k - 1][r - 1]);
+        }
+        fn[B - 1] = fB;
+        if (k + 1 < K) fB = amul(aend, Vend[k]);
+        if (zero_start) Vend[k] = vzero();
+        else R::back(fs, fn, Vend[k], dt);
+        R::interior(fs, fn, Vend[k], dt, Vi[k]);
+    }
+    // reverse sweep of the block, top level down.  (The sum weight at s in a
+    // FIRST block never reaches the output: there V = 0 and the end role is off.)
+    V4 Xr[B], dAb[B];
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        Xr[r] = vscale(R::weight(r, dt), zK);
+        dAb[r] = vzero();
+    }
+#pragma unroll
+    for (int k = K - 1; k >= 0; --k) {
+        V4 G[B];
+#pragma unroll
+        for (int r = 0; r < B; ++r) {
+            if (!(zero_start && r == 0)) dAb[r] = qmul_acc(dAb[r], Xr[r], qconj(r == 0 ? Vend[k] : Vi[k][r - 1]));
+            G[r] = vneg(amul(ab[r], Xr[r]));
+        }
+        V4 xb[B];
+        R::rev(G + 1, U[k], dt, xb);
+        V4 us = U[k];
+#pragma unroll
+        for (int r = B - 1; r >= 0; --r) us = vadd(us, G[r]);
+        U[k] = us;
+        if (end_role0) xb[0] = vaxpy(xb[0], R::end_role(dt), us);
+#pragma unroll
+        for (int r = 0; r < B; ++r) Xr[r] = xb[r];
+    }
+    const int q0 = (int)((s0 - e) & (kRevGroup - 1));
+#pragma unroll
+    for (int r = 0; r < B; ++r) {
+        const V4 v = vadd(dAb[r], Xr[r]);  // Xr holds Xbar_1
+        if (FIRST && r == 0) {
+            pend = v;
+        } else {
+            double* w = row + (q0 + r) * 3;
+            w[0] = v.x[1];
+            w[1] = v.x[2];
+            w[2] = v.x[3];
+        }
+    }
+    aend[0] = ab[0][0];
+    aend[1] = ab[0][1];
+    aend[2] = ab[0][2];
+}
+
+// Write the staged group of every lane o of the wavefront: slot q of lane o
+// goes to dA[off_o + 3 q + comp] when q > lim_o (the sample lies above the
+// chunk start).  Piece idx = 64 r + lane is double k = idx mod 24 of owner lane
+// o = idx / 24, so the 64 pieces of one store cover whole 64-byte segments;
+// pieces r and r + 3 have the same k and owners 8 lanes apart.
+__device__ __forceinline__ void rev_flush(const double* rows, const RevDest* dst, double* dA, int lane) {
+    asm volatile("" : "+v"(lane));  // keep the per-piece index math inside the flush
+    int ob[3], kk[3], qq[3];
+#pragma unroll
+    for (int m = 0; m < 3; ++m) {
+        const int idx = m * 64 + lane;
+        ob[m] = idx / kRevRow;
+        kk[m] = idx - ob[m] * kRevRow;
+        qq[m] = kk[m] / 3;
+    }
+#pragma unroll
+    for (int r0 = 0; r0 < kRevRow; r0 += 8) {  // 8 pieces: all LDS reads first, then the stores
+        RevDest d[8];
+        double v[8];
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int r = r0 + i;
+            d[i] = dst[ob[r % 3] + 8 * (r / 3)];
+            v[i] = rows[r * 64 + lane];
+        }
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int m = (r0 + i) % 3;
+            if (qq[m] > d[i].lim) dA[d[i].off + kk[m]] = v[i];
+        }
+    }
+}
+
+// publish this lane's current group before a flush
+__device__ __forceinline__ void rev_publish(RevStage& st, int wv, int lane, long long pe, int len, int grp) {
+    const int top = kRevGroup * (grp + 1);
+    st.dst[wv][lane] = RevDest{pe - 3 * top, top - len, 0};
+}
+
+template <int I, int K>
+__global__ void __launch_bounds__(256, (I == BOOLE || (I == SIMPSON && K >= 6)) ? 1 : 2) wb_rev(WbArgs a) {
+    using R = BlockRule<I>;
+    constexpr int B = R::B;
+    static_assert(kRevGroup % B == 0, "a block must not straddle two staging groups");
+    __shared__ RevStage st;
+    const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.B * a.C) return;
-    const long long b = t / a.C;
-    const int c = (int)(t - b * a.C);
+    const bool valid = t < a.B * a.C;  // every lane stays: the flushes are cooperative
+    const long long b = valid ? t / a.C : 0;
+    const int c = valid ? (int)(t - b * a.C) : 0;
     const long long S = a.S;
     const double dt = a.dt;
     const double er = R::end_role(dt);
     const double* A3 = a.in + b * S * 3;
     const long long s = chunk_begin(a, c), e = chunk_end(a, c);
-    const double* sp = a.src + t * (long long)(2 * K + 1) * 4;
-    const double* fp = a.fsum + t * (long long)(2 * K + 1) * 4;
+    const long long tt = valid ? t : 0;
+    const double* sp = a.src + tt * (long long)(2 * K + 1) * 4;
+    const double* fp = a.fsum + tt * (long long)(2 * K + 1) * 4;
+    double* row = st.row[wv] + lane * kRevRow;
+    const long long pe = (b * S + e) * 3;  // dA offset of the chunk end
+    const int len = valid ? (int)(e - s) : 0;
 
     V4 Vend[K], U[K];
 #pragma unroll
@@ -610,106 +767,71 @@ __global__ void __launch_bounds__(256) wb_rev(WbArgs a) {
 #pragma unroll
         for (int k = K; k >= 1; --k) {
             const V4 zk = ld4(sp + (K + k - 1) * 4);
-            d = vadd(d, qmul(Xu, qconj(Vend[k - 1])));
+            d = qmul_acc(d, Xu, qconj(Vend[k - 1]));
             const V4 uv = vadd(vadd(vneg(amul(aend, Xu)), ld4(sp + (k - 1) * 4)), zk);
             U[k - 1] = uv;
             Xu = vaxpy(vscale(er, uv), ce - er, zk);
         }
         d = vadd(d, Xu);
-        double* ed = a.edge + t * 6;
-        ed[0] = d.x[1];
-        ed[1] = d.x[2];
-        ed[2] = d.x[3];
+        if (valid) {
+            double* ed = a.edge + t * 6;
+            ed[0] = d.x[1];
+            ed[1] = d.x[2];
+            ed[2] = d.x[3];
+        }
     }
-    const bool riem0 = (I == RIEMANN && c == 0);
-    const long long first = (I == RIEMANN && c > 0) ? s + 1 : s;  // Riemann: s has no role
-    double* dAo = a.dA + b * S * 3;
+    // blocks s0 = e-B, e-2B, ... above s in a wavefront-uniform loop, then the
+    // block starting at s, unless the chunk's start sample has no role
+    // (Riemann after chunk 0)
+    const bool has_first = valid && (I != RIEMANN || c == 0);
+    const int nb = valid ? (int)((e - s) / B) - (has_first ? 0 : 1) : 0;  // blocks in all
+    const int nbn = has_first ? nb - 1 : nb;                              // blocks above s
+    int nbw = nbn;
+    for (int off = 32; off > 0; off >>= 1) nbw = max(nbw, __shfl_xor(nbw, off, 64));
     V4 pend = vzero();
     double nx[B][3];
-    if (e - B >= first) {
+    if (nb > 0) {
 #pragma unroll
         for (int r = 0; r < B; ++r) lda(A3, e - B + r, nx[r]);
     }
-    for (long long s0 = e - B; s0 >= first; s0 -= B) {
-        double ab[B][3];
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            ab[r][0] = nx[r][0];
-            ab[r][1] = nx[r][1];
-            ab[r][2] = nx[r][2];
-        }
-        if (s0 - B >= first) {  // previous block's samples, in flight during this block
-#pragma unroll
-            for (int r = 0; r < B; ++r) lda(A3, s0 - B + r, nx[r]);
-        }
-        // rebuild V_1..V_K at s0..s0+B-1 (ascending levels)
-        const bool zero_start = (s0 == s) && !riem0;
-        V4 Vl[K][B];
-#pragma unroll
-        for (int k = 0; k < K; ++k) {
-            V4 f[B + 1];
-            if (k == 0) {
-#pragma unroll
-                for (int r = 0; r < B; ++r) f[r] = pure(ab[r]);
-                f[B] = pure(aend);
-            } else {
-#pragma unroll
-                for (int r = 0; r < B; ++r) f[r] = amul(ab[r], Vl[k - 1][r]);
-                f[B] = amul(aend, Vend[k - 1]);
-            }
-            const V4 v0 = zero_start ? vzero() : R::back(f, Vend[k], dt);
-            Vl[k][0] = v0;
-            R::interior(f, v0, dt, &Vl[k][0] + 1);
-        }
-#pragma unroll
-        for (int k = 0; k < K; ++k) Vend[k] = Vl[k][0];
-        // reverse sweep of the block, top level down
-        V4 Xr[B], dAb[B];
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            Xr[r] = vscale(sum_vjp_coef<I>(s0 + r, S, dt), zK);
-            dAb[r] = vzero();
-        }
-#pragma unroll
-        for (int k = K - 1; k >= 0; --k) {
-            V4 G[B];
+    int grp = 0;
+    for (int j = 0; j < nbw; ++j) {
+        if (j < nbn) {
+            const long long s0 = e - (long long)(j + 1) * B;
+            double ab[B][3];
 #pragma unroll
             for (int r = 0; r < B; ++r) {
-                dAb[r] = vadd(dAb[r], qmul(Xr[r], qconj(Vl[k][r])));
-                G[r] = vneg(amul(ab[r], Xr[r]));
+                ab[r][0] = nx[r][0];
+                ab[r][1] = nx[r][1];
+                ab[r][2] = nx[r][2];
             }
-            V4 xb[B];
-            R::rev(G + 1, U[k], dt, xb);
-            V4 us = U[k];
+            if (j + 1 < nb) {  // next block's samples, in flight during this block
 #pragma unroll
-            for (int r = B - 1; r >= 0; --r) us = vadd(us, G[r]);
-            U[k] = us;
-            if (s0 > s || riem0) xb[0] = vaxpy(xb[0], er, us);
-#pragma unroll
-            for (int r = 0; r < B; ++r) Xr[r] = xb[r];
-        }
-#pragma unroll
-        for (int r = 0; r < B; ++r) {
-            const V4 v = vadd(dAb[r], Xr[r]);  // Xr holds Xbar_1
-            if (s0 + r == s) {
-                pend = v;
-            } else {
-                dAo[(s0 + r) * 3 + 0] = v.x[1];
-                dAo[(s0 + r) * 3 + 1] = v.x[2];
-                dAo[(s0 + r) * 3 + 2] = v.x[3];
+                for (int r = 0; r < B; ++r) lda(A3, s0 - B + r, nx[r]);
             }
+            rev_block<I, K, false>(s0, e, ab, aend, Vend, U, zK, dt, row, pend);
+            grp = (j * B) / kRevGroup;
         }
-        aend[0] = ab[0][0];
-        aend[1] = ab[0][1];
-        aend[2] = ab[0][2];
+        if (((j + 1) * B) % kRevGroup == 0 || j + 1 == nbw) {
+            rev_publish(st, wv, lane, pe, len, grp);
+            rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
+        }
     }
+    if (has_first) {
+        rev_block<I, K, true>(s, e, nx, aend, Vend, U, zK, dt, row, pend);
+        grp = (nbn * B) / kRevGroup;
+    }
+    rev_publish(st, wv, lane, pe, len, grp);
+    rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
     // the start role of A_s saw the folded ZV_bar_0; in chunks after the first
     // that sample's sum weight belongs to the previous chunk
     if (c > 0 && I != RIEMANN) pend = vaxpy(pend, -R::start_role(dt), ld4(sp + K * 4));
-    double* ed = a.edge + t * 6;
-    ed[3] = pend.x[1];
-    ed[4] = pend.x[2];
-    ed[5] = pend.x[3];
+    if (valid) {
+        double* ed = a.edge + t * 6;
+        ed[3] = pend.x[1];
+        ed[4] = pend.x[2];
+        ed[5] = pend.x[3];
+    }
 }
 
 // dA at the chunk boundaries: value at e from chunk c plus the start value of chunk c+1

@@ -306,6 +306,36 @@ def test_spacecurve_single_pass_batched_matches_oracle(integrator, op, n):
         assert rel_fro(dA[b], ref_g) <= TOL
 
 
+RAGGED_SAMPLES = {"Simpson": [3, 5, 17, 1003], "Boole": [5, 9, 17, 1005], "Riemann": [2, 3, 17, 1000]}
+
+
+@pytest.mark.parametrize("integrator", ["Boole", "Simpson", "Riemann"])
+@pytest.mark.parametrize("n", [2, 4, 6, 8])
+def test_spacecurve_single_pass_ragged_chunks(integrator, n):
+    """Single-pass kernels on sample counts whose last chunk is shorter than the
+    others (63 chunks of 16 samples per instance, so wavefronts span instances
+    and their lanes finish at different steps) and on one-chunk, one-block
+    instances."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    B = 5
+    for S in RAGGED_SAMPLES[integrator]:
+        a = np.stack([fourier_spacecurve(np.random.default_rng([11, b, n, S]), S) for b in range(B)])
+        cot = np.stack([np.random.default_rng([11, b, n, S, 1]).standard_normal((n, 3)) for b in range(B)])
+        at = torch.from_numpy(a).cuda()
+        out = torch_ops.sc_forward(n, at, 0.0, 1.0, op="many", integrator=integrator).cpu().numpy()
+        dA = torch_ops.sc_vjp(n, at, torch.from_numpy(cot).cuda(), None, 0.0, 1.0, op="many",
+                              integrator=integrator).cpu().numpy()
+        assert capi.last_kernel() == "wb"
+        for b in range(B):
+            ref = O.sc_forward(n, a[b], 0.0, 1.0, op="many", integrator=integrator)
+            ref_g = O.sc_vjp(n, a[b], cot[b], 0.0, 1.0, op="many", integrator=integrator)
+            assert rel_fro(out[b], ref) <= TOL, (S, b)
+            assert rel_fro(dA[b], ref_g) <= TOL, (S, b)
+
+
 def test_batched_matrix_api_matches_per_instance():
     import torch
 

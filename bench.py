@@ -76,8 +76,8 @@ def _counts(Q, K, S, m, m_in, P, n_out, op_one=False):
     return F_fwd, F_vjp, B_fwd, B_carry, B_vjp
 
 
-KERNELS = ("ep_forward", "ep_vjp", "ep_forward_recompute", "wb_fwd_pass1", "wb_combine", "wb_fwd_pass1_vjp",
-           "wb_combine_vjp", "wb_rev", "wb_fixup")
+KERNELS = ("lm_fwd_prod", "lm_fwd_scan", "lm_vjp_init", "lm_vjp_prod", "lm_vjp_scan", "lm_vjp_final",
+           "wb_fwd_pass1", "wb_combine", "wb_fwd_pass1_vjp", "wb_combine_vjp", "wb_rev", "wb_fixup")
 
 # su(2) products on the single-pass path: pure sample x 4-vector = 4 mul + 8 FMA
 # (20 flops), 4-vector x 4-vector = 4 mul + 12 FMA (28 flops).
@@ -101,6 +101,19 @@ def _wb_counts(K, S):
     flops = {"wb_fwd_pass1": f1, "wb_fwd_pass1_vjp": f1, "wb_rev": fr}
     by = {"wb_fwd_pass1": 8 * S * m_in, "wb_fwd_pass1_vjp": 8 * S * m_in, "wb_rev": 8 * S * 2 * m_in}
     return flops, by
+
+
+def _lm_counts(B, Q, K, S, m, m_in, P):
+    """Algorithmic flops / HBM bytes per launch of the level-parallel kernels
+    (one launch per level; jobs = B*Q): a product launch reads the samples and
+    the previous prefix level and writes the products; a scan launch reads the
+    products and writes the prefix level (the carry)."""
+    jobs = B * Q
+    flops = {"lm_fwd_prod": jobs * S * P, "lm_vjp_prod": jobs * S * 2 * P,
+             "lm_fwd_scan": jobs * S * 8 * m, "lm_vjp_scan": jobs * S * 8 * m}
+    by = {"lm_fwd_prod": 8 * (B * S * m_in + 2 * jobs * S * m), "lm_fwd_scan": 8 * 2 * jobs * S * m,
+          "lm_vjp_prod": 8 * (B * S * m_in + 5 * jobs * S * m), "lm_vjp_scan": 8 * 2 * jobs * S * m}
+    return by, flops
 
 
 def _cpu_baseline_sc(a_host, cot_host, n, S, budget_s):
@@ -246,8 +259,7 @@ def main():
           flush=True)
 
     F_fwd, F_vjp, B_fwd, B_carry, B_vjp = _counts(Q, K, S, m, m_in, P, n_out)
-    kernel_bytes = {"ep_forward": B * (B_fwd + B_carry), "ep_vjp": B * (B_vjp + B_carry)}
-    kernel_flops = {"ep_forward": B * F_fwd, "ep_vjp": B * F_vjp}
+    kernel_bytes, kernel_flops = _lm_counts(B, Q, K, S, m, m_in, P)
     wbf, wbb = _wb_counts(K, S)
     for k in wbf:
         kernel_flops[k] = B * wbf[k]

@@ -199,32 +199,6 @@ struct DeviceScratch {
     }
 };
 
-struct EpShape {
-    int tpj, jpw, block, ept;
-};
-
-EpShape ep_shape(uint64_t m) {
-    EpShape s{};
-    if (m <= 256) {
-        s.block = 256;
-        s.tpj = (int)m;
-        s.jpw = 256 / (int)m;
-        s.ept = 1;
-    } else if (m <= 1024) {
-        s.block = 256;
-        s.tpj = 256;
-        s.jpw = 1;
-        s.ept = (m <= 512) ? 2 : 4;
-    } else {
-        s.block = 1024;
-        s.tpj = 1024;
-        s.jpw = 1;
-        const uint64_t e = (m + 1023) / 1024;
-        s.ept = e <= 2 ? 2 : (e <= 4 ? 4 : (e <= 8 ? 8 : 16));
-    }
-    return s;
-}
-
 void fill_gl(const Plan& pl, double* w, double* x) {
     for (int i = 0; i < magnus::kArgMaxQ; ++i) {
         w[i] = 0.0;
@@ -375,6 +349,25 @@ int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, 
 }
 
 // forward: record into `carry` when non-null; outputs=false runs VJP::none
+void fill_lm(const Plan& pl, long long jobs, magnus::LmArgs& a) {
+    a.S = (long long)pl.S;
+    a.jobs = jobs;
+    a.n = (int)pl.n;
+    a.d = (int)pl.d;
+    a.m = (int)pl.m;
+    a.m_in = (int)pl.m_in;
+    a.m_cot = (int)pl.m_cot;
+    a.Q = (int)pl.Q;
+    a.Qj = (int)pl.Qj;
+    a.n_out = (int)pl.n_out;
+    a.op = pl.op;
+    a.sc = pl.sc ? (pl.pad ? 1 : 2) : 0;
+    a.bt_fused = (pl.mat == MAGNUS_MAT_FIXED2 || pl.mat == MAGNUS_MAT_SPACECURVE) ? 1 : 0;
+    a.dt = pl.dt;
+    fill_gl(pl, a.w, a.x);
+}
+
+// forward: record into `carry` when non-null; outputs=false runs VJP::none
 int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, double* carry, bool outputs,
                 hipStream_t stream) {
     DeviceScratch scratch(stream);
@@ -384,38 +377,26 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
     double* partial = outputs ? scratch.alloc<double>((size_t)jobs * pl.n_out * m) : nullptr;
     double* omega1 = outputs ? scratch.alloc<double>((size_t)B * m) : nullptr;
     double* pingpong = nullptr;
+    double* Y = nullptr;
     if (!carry && pl.n >= 2) pingpong = scratch.alloc<double>((size_t)jobs * 2 * S * m);
-    if ((outputs && (!partial || !omega1)) || (!carry && pl.n >= 2 && !pingpong))
+    if (pl.n >= 2) Y = scratch.alloc<double>((size_t)jobs * S * m);
+    if ((outputs && (!partial || !omega1)) || (!carry && pl.n >= 2 && !pingpong) || (pl.n >= 2 && !Y))
         return fail(MAGNUS_ENOMEM, "device allocation failed");
 
-    magnus::EpFwdArgs a{};
+    magnus::LmArgs a{};
+    fill_lm(pl, jobs, a);
     a.in = in;
     a.carry = carry;
     a.pingpong = pingpong;
+    a.Y = Y;
     a.partial = partial;
     a.omega1 = omega1;
-    a.S = (long long)S;
-    a.jobs = jobs;
-    a.n = (int)pl.n;
-    a.d = (int)pl.d;
-    a.m = (int)m;
-    a.m_in = (int)pl.m_in;
-    a.Q = (int)pl.Q;
-    a.Qj = (int)pl.Qj;
-    a.n_out = (int)pl.n_out;
-    a.op = pl.op;
-    a.sc = pl.sc ? (pl.pad ? 1 : 2) : 0;
     a.outputs = outputs ? 1 : 0;
-    a.dt = pl.dt;
-    fill_gl(pl, a.w, a.x);
-    const EpShape sh = ep_shape(m);
-    a.tpj = sh.tpj;
-    a.jpw = sh.jpw;
-    const long long grid = (jobs + sh.jpw - 1) / sh.jpw;
-    KernelTimer tf(outputs ? "ep_forward" : "ep_forward_recompute", stream);
-    hipError_t e = magnus::launch_ep_forward(pl.integ, a, sh.block, grid, sh.ept, stream);
-    tf.done();
-    if (e != hipSuccess) return device_error(e, "forward kernel launch");
+    for (int j = 1; j <= (int)pl.n; ++j) {
+        a.level = j;
+        hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
+        if (e != hipSuccess) return device_error(e, "forward kernel launch");
+    }
     if (outputs) {
         magnus::FwdReduceArgs r{};
         r.partial = partial;
@@ -429,10 +410,10 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
         r.Qj = (int)pl.Qj;
         r.op = pl.op;
         r.sc = pl.pad ? 1 : 0;
-        e = magnus::launch_fwd_reduce(r, stream);
+        hipError_t e = magnus::launch_fwd_reduce(r, stream);
         if (e != hipSuccess) return device_error(e, "reduce kernel launch");
     }
-    g_kernel = "ep";
+    g_kernel = "lm";
     return MAGNUS_OK;
 }
 
@@ -441,10 +422,9 @@ int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot,
     DeviceScratch scratch(stream);
     const long long B = (long long)count;
     const uint64_t S = pl.S, m = pl.m;
-    long long jobs = 0;
     double* dAjob = nullptr;
     if (pl.n >= 2) {
-        jobs = B * (long long)pl.Q;
+        const long long jobs = B * (long long)pl.Q;
         if (!carry) {  // VJP::none recompute (grad.hpp:95-110)
             double* tmp = scratch.alloc<double>((size_t)B * pl.Q * (pl.n - 1) * S * m);
             if (!tmp) return fail(MAGNUS_ENOMEM, "device allocation failed");
@@ -455,33 +435,21 @@ int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot,
         double* gbuf = scratch.alloc<double>((size_t)jobs * 2 * S * m);
         dAjob = scratch.alloc<double>((size_t)jobs * S * m);
         if (!gbuf || !dAjob) return fail(MAGNUS_ENOMEM, "device allocation failed");
-        magnus::EpVjpArgs a{};
+        magnus::LmArgs a{};
+        fill_lm(pl, jobs, a);
         a.in = in;
         a.cot = cot;
-        a.carry = carry;
+        a.carry = const_cast<double*>(carry);  // read only in the VJP kernels
         a.gbuf = gbuf;
         a.dAjob = dAjob;
-        a.S = (long long)S;
-        a.jobs = jobs;
-        a.n = (int)pl.n;
-        a.d = (int)pl.d;
-        a.m = (int)m;
-        a.m_in = (int)pl.m_in;
-        a.m_cot = (int)pl.m_cot;
-        a.Q = (int)pl.Q;
-        a.n_out = (int)pl.n_out;
-        a.op = pl.op;
-        a.sc = pl.sc ? (pl.pad ? 1 : 2) : 0;
-        a.bt_fused = (pl.mat == MAGNUS_MAT_FIXED2 || pl.mat == MAGNUS_MAT_SPACECURVE) ? 1 : 0;
-        a.dt = pl.dt;
-        fill_gl(pl, a.w, a.x);
-        const EpShape sh = ep_shape(m);
-        a.tpj = sh.tpj;
-        a.jpw = sh.jpw;
-        const long long grid = (jobs + sh.jpw - 1) / sh.jpw;
-        KernelTimer tv("ep_vjp", stream);
-        hipError_t e = magnus::launch_ep_vjp(pl.integ, a, sh.block, grid, sh.ept, stream);
-        tv.done();
+        a.ga = 0;
+        hipError_t e = magnus::launch_lm_vjp_init(pl.integ, a, stream);
+        for (int k = (int)pl.n; k >= 2 && e == hipSuccess; --k) {
+            a.level = k;
+            e = magnus::launch_lm_vjp_level(pl.integ, a, stream);
+            a.ga ^= 1;
+        }
+        if (e == hipSuccess) e = magnus::launch_lm_vjp_final(a, stream);
         if (e != hipSuccess) return device_error(e, "vjp kernel launch");
     }
     magnus::VjpReduceArgs r{};
@@ -501,7 +469,7 @@ int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot,
     r.dt = pl.dt;
     hipError_t e = magnus::launch_vjp_reduce(pl.integ, r, stream);
     if (e != hipSuccess) return device_error(e, "vjp reduce launch");
-    g_kernel = "ep";
+    if (pl.n >= 2) g_kernel = "lm";
     return MAGNUS_OK;
 }
 

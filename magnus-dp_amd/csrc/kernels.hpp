@@ -7,47 +7,13 @@ namespace magnus {
 
 constexpr int kArgMaxQ = 64;
 
-// Entry-parallel forward (kernels_ep.hip).  One job = (instance, GL node).
-struct EpFwdArgs {
-    const double* in;    // [B][S][m_in]
-    double* carry;       // [B][Q][n-1][S][m] or nullptr
-    double* pingpong;    // [jobs][2][S][m] level buffers when carry == nullptr
-    double* partial;     // [jobs][n_out][m]
-    double* omega1;      // [B][m]
-    long long S;
-    long long jobs;
-    int n, d, m, m_in, Q, Qj, n_out, op;
-    int sc;       // su(2) 4-vector product
-    int outputs;  // 0: record the carry only (VJP::none)
-    int tpj, jpw; // threads per job, jobs per workgroup
-    double dt;
-    double w[kArgMaxQ];
-    double x[kArgMaxQ];
-};
-
+// GL-lane reductions (kernels_lm.hip)
 struct FwdReduceArgs {
     const double* partial;  // [B][Qj][n_out][m]
     const double* omega1;   // [B][m]
     double* out;            // [B][n_out][m_out]
     long long B;
     int n, m, m_out, n_out, Qj, op, sc;
-};
-
-struct EpVjpArgs {
-    const double* in;     // [B][S][m_in]
-    const double* cot;    // [B][n_out][m_cot]
-    const double* carry;  // [B][Q][n-1][S][m]
-    double* gbuf;         // [jobs][2][S][m]
-    double* dAjob;        // [jobs][S][m]
-    long long S;
-    long long jobs;
-    int n, d, m, m_in, m_cot, Q, n_out, op;
-    int sc;
-    int bt_fused;  // Fixed2/SpaceCurve bar_total = temp * (1 + x); generic = temp + temp * x
-    int tpj, jpw;
-    double dt;
-    double w[kArgMaxQ];
-    double x[kArgMaxQ];
 };
 
 struct VjpReduceArgs {
@@ -60,15 +26,46 @@ struct VjpReduceArgs {
     double dt;
 };
 
+// Level-parallel kernels (kernels_lm.hip).  One job = (instance, GL node); a
+// level's per-sample products run for all (job, sample, entry) at once, its
+// prefix scans and sums per (job, entry).
+struct LmArgs {
+    const double* in;    // [B][S][m_in]
+    const double* cot;   // [B][n_out][m_cot]        (VJP)
+    double* carry;       // [B][Q][n-1][S][m] or nullptr
+    double* pingpong;    // [jobs][2][S][m] level buffers when carry == nullptr
+    double* Y;           // [jobs][S][m] products of the current level (forward)
+    double* partial;     // [jobs][n_out][m]
+    double* omega1;      // [B][m]
+    double* gbuf;        // [jobs][2][S][m] adjoint level buffers (VJP)
+    double* dAjob;       // [jobs][S][m]               (VJP)
+    long long S;
+    long long jobs;
+    int n, d, m, m_in, m_cot, Q, Qj, n_out, op;
+    int sc;        // 0 generic, 1 padded su(2) (S,3) input, 2 su(2) on (S,2,2) input
+    int outputs;   // forward: 0 records the carry only
+    int bt_fused;  // VJP: Fixed2/SpaceCurve bar_total = temp * (1 + x); generic temp + temp * x
+    int level;     // forward level j (1..n) / VJP level k (n..2)
+    int ga;        // VJP: gbuf half holding the current adjoint
+    double dt;
+    double w[kArgMaxQ];
+    double x[kArgMaxQ];
+};
+
 // Per-kernel timing hooks (implemented by the C-ABI layer).  When timing is
 // enabled, timer_begin records a start event on `s` and returns a handle that
 // timer_end closes; both are no-ops (nullptr) otherwise.
 void* timer_begin(const char* name, hipStream_t s);
 void timer_end(void* handle);
 
-hipError_t launch_ep_forward(int integ, const EpFwdArgs& a, int block, long long grid, int ept, hipStream_t s);
-hipError_t launch_ep_vjp(int integ, const EpVjpArgs& a, int block, long long grid, int ept, hipStream_t s);
 hipError_t launch_fwd_reduce(const FwdReduceArgs& a, hipStream_t s);
 hipError_t launch_vjp_reduce(int integ, const VjpReduceArgs& a, hipStream_t s);
+
+// forward level a.level: products (level >= 2) then scan/sums
+hipError_t launch_lm_forward_level(int integ, const LmArgs& a, hipStream_t s);
+// VJP: init (adjoint of the last level's sum), one level a.level, final dA += adjoint
+hipError_t launch_lm_vjp_init(int integ, const LmArgs& a, hipStream_t s);
+hipError_t launch_lm_vjp_level(int integ, const LmArgs& a, hipStream_t s);
+hipError_t launch_lm_vjp_final(const LmArgs& a, hipStream_t s);
 
 }  // namespace magnus

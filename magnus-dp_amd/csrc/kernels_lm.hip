@@ -1,0 +1,484 @@
+// kernels_lm.hip -- level-parallel Magnus kernels: the reference's per-node
+// level recursion (src/magnus.hpp:12-261) and its VJP (src/grad.hpp:14-349)
+// for any matrix policy and integrator.
+//
+// Work decomposition.  One job = (instance b, Gauss-Legendre node q).  A level
+// of the recursion is one product per sample followed by one prefix scan per
+// matrix entry.  The products of a level depend only on the previous level's
+// prefix array, so they run for all (job, sample, entry) at once
+// (lm_fwd_prod, lm_vjp_prod); the scans and sums, which are sequential in the
+// sample index, run one thread per (job, entry) with the reference's
+// per-entry recurrences and order of operations (rules.hpp): lm_fwd_scan,
+// lm_vjp_scan.  Products keep the reference's summation order too, so every
+// scalar is computed as the reference computes it; only the q-lane reduction
+// is done in fixed q order by the reduce kernels.
+#include "kernels.hpp"
+#include "products.hpp"
+#include "rules.hpp"
+
+namespace magnus {
+
+namespace {
+
+// P^{(j)} of job (b, q): carry[b][q][j-1] or a ping-pong buffer
+__device__ __forceinline__ double* lm_slot(const LmArgs& a, long long job, long long b, int q, int j) {
+    if (a.carry) return a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (j - 1)) * a.S * a.m;
+    return a.pingpong + (job * 2 + (j & 1)) * a.S * a.m;
+}
+
+// ---------------------------------------------------------------------------
+// forward
+// ---------------------------------------------------------------------------
+
+// Y_i = A_i (P^{(j-1)}_i + s_q T), T = P^{(j-1)}_{S-1}  (magnus.hpp:230-254)
+__global__ void __launch_bounds__(256) lm_fwd_prod(LmArgs a) {
+    const long long per = a.S * a.m;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.jobs * per) return;
+    const long long job = idx / per;
+    const long long rem = idx - job * per;
+    const long long i = rem / a.m;
+    const int e = (int)(rem - i * a.m);
+    const long long b = job / a.Qj;
+    const int q = (int)(job - b * a.Qj);
+    const double shift = a.x[q] - 1;  // magnus.hpp:65
+    const double* P = lm_slot(a, job, b, q, a.level - 1);
+    const double* T = P + (a.S - 1) * a.m;
+    const double* Ai = a.in + (b * a.S + i) * a.m_in;
+    double f;
+    if (a.sc) f = prod_sc(Ai + (a.sc == 2 ? 1 : 0), P + i * a.m, T, e, shift);
+    else f = prod_generic(Ai, P + i * a.m, T, a.d, e / a.d, e % a.d, shift);
+    a.Y[idx] = f;
+}
+
+// Tiled products for d = D in {16, 32, 64}: a workgroup stages PPB samples'
+// operands in LDS (64 KiB) and each thread computes a 4 x 4 block of one
+// product, every output accumulated over l in order exactly as prod_generic
+// does (so the results are identical).  Products are numbered p = job*S + i.
+constexpr int kTileLds = 8192;  // doubles of LDS per workgroup
+
+template <int D>
+__global__ void __launch_bounds__(256) lm_fwd_prod_tiled(LmArgs a) {
+    constexpr int TPP = (D / 4) * (D / 4);  // threads per product
+    constexpr int PPB = 256 / TPP;          // products per workgroup
+    constexpr int MM = D * D;
+    __shared__ double At[PPB][MM];  // A_i transposed: At[l][r] = a[r][l]
+    __shared__ double Bs[PPB][MM];  // P_i + s T
+    const long long total = a.jobs * a.S;
+    const long long p0 = (long long)blockIdx.x * PPB;
+    for (int x = threadIdx.x; x < PPB * MM; x += 256) {
+        const int g = x / MM, e = x - g * MM;
+        const long long p = p0 + g;
+        if (p >= total) continue;
+        const long long job = p / a.S, i = p - job * a.S;
+        const long long b = job / a.Qj;
+        const int q = (int)(job - b * a.Qj);
+        const double shift = a.x[q] - 1;
+        const double* P = lm_slot(a, job, b, q, a.level - 1);
+        double v = P[i * MM + e];
+        v += P[(a.S - 1) * MM + e] * shift;
+        Bs[g][e] = v;
+        const int r = e / D, l = e - r * D;
+        At[g][l * D + r] = a.in[(b * a.S + i) * (long long)MM + e];
+    }
+    __syncthreads();
+    const int g = threadIdx.x / TPP, t = threadIdx.x - g * TPP;
+    const long long p = p0 + g;
+    if (p >= total) return;
+    const int r0 = 4 * (t / (D / 4)), c0 = 4 * (t % (D / 4));
+    double acc[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
+    for (int l = 0; l < D; ++l) {
+        double av[4], bv[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) av[u] = At[g][l * D + r0 + u];
+#pragma unroll
+        for (int v = 0; v < 4; ++v) bv[v] = Bs[g][l * D + c0 + v];
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+    }
+    double* y = a.Y + p * MM;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) y[(r0 + u) * D + c0 + v] = acc[u][v];
+}
+
+// P^{(j)} = Prefix(Y) (j < n) and the level's GL-weighted sum, one entry of one job
+template <int I>
+__global__ void __launch_bounds__(256) lm_fwd_scan(LmArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.jobs * a.m) return;
+    const long long job = idx / a.m;
+    const int e = (int)(idx - job * a.m);
+    const long long b = job / a.Qj;
+    const int q = (int)(job - b * a.Qj);
+    const long long S = a.S;
+    const int m = a.m, n = a.n, j = a.level;
+    const double dt = a.dt;
+    const double fscale = dt * a.w[q];  // magnus.hpp:66
+    const bool om1 = a.outputs && j == 1 && q == 0 && (a.op != OP_ONE || n == 1);
+    const bool lvl = a.outputs && j >= 2 && (a.op != OP_ONE || j == n);
+    const double c = (j == 1) ? dt : fscale;
+    double* Pout = (j < n) ? lm_slot(a, job, b, q, j) : nullptr;
+    const double* A = a.in + b * S * a.m_in;
+    const double* Yj = a.Y + job * S * m;
+    PrefixState<I> st;
+    double acc = 0.0;
+    constexpr int U = 8;  // samples whose inputs are loaded ahead of the recurrence
+    for (long long i0 = 0; i0 < S; i0 += U) {
+        double f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u;
+            f[u] = 0.0;
+            if (i < S) f[u] = (j == 1) ? raw_entry(A + i * a.m_in, e, a.sc) : Yj[i * m + e];
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const long long i = i0 + u;
+            if (i >= S) break;
+            if (om1 || lvl) sum_acc<I>(acc, i, S, f[u], c);
+            if (Pout) Pout[i * m + e] = st.step(i, f[u], dt);
+        }
+    }
+    if (om1) a.omega1[b * m + e] = acc;
+    if (lvl) {
+        if (a.op == OP_MANY) {
+            a.partial[(job * a.n_out + (j - 1)) * m + e] = acc;
+        } else {
+            double* p = a.partial + job * a.n_out * m + e;
+            const bool first = (a.op == OP_SUM) ? (j == 2) : true;  // sum: levels 2..n accumulate
+            if (first) *p = acc;
+            else *p += acc;
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// VJP
+// ---------------------------------------------------------------------------
+
+__device__ __forceinline__ double lm_cot(const LmArgs& a, long long b, int k, int e) {
+    const double* row = a.cot + (b * a.n_out + k) * (long long)a.m_cot;
+    if (a.sc == 1) return e == 0 ? 0.0 : row[e - 1];
+    return row[e];
+}
+
+// barY = sum_vjp(dOut_last, dscale) (grad.hpp:127, 216, 317); dA_job = 0
+template <int I>
+__global__ void __launch_bounds__(256) lm_vjp_init(LmArgs a) {
+    const long long per = a.S * a.m;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.jobs * per) return;
+    const long long job = idx / per;
+    const long long rem = idx - job * per;
+    const long long i = rem / a.m;
+    const int e = (int)(rem - i * a.m);
+    const long long b = job / a.Q;
+    const int q = (int)(job - b * a.Q);
+    const double dscale = a.dt * a.w[q];  // grad.hpp:315
+    const double o = lm_cot(a, b, (a.op == OP_MANY) ? a.n - 1 : 0, e);
+    a.gbuf[(job * 2 + a.ga) * per + rem] = o * sum_vjp_coef<I>(i, a.S, dscale);
+    a.dAjob[idx] = 0.0;
+}
+
+// dA_job_i += barY_i b_i^T (one entry) and temp_i = A_i^T barY_i for level k
+__global__ void __launch_bounds__(256) lm_vjp_prod(LmArgs a) {
+    const long long per = a.S * a.m;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.jobs * per) return;
+    const long long job = idx / per;
+    const long long rem = idx - job * per;
+    const long long i = rem / a.m;
+    const int e = (int)(rem - i * a.m);
+    const long long b = job / a.Q;
+    const int q = (int)(job - b * a.Q);
+    const double shift = a.x[q] - 1.0;  // grad.hpp:314
+    const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
+    const double* T = P + (a.S - 1) * a.m;
+    const double* g = a.gbuf + (job * 2 + a.ga) * per + i * a.m;
+    const double* Ai = a.in + (b * a.S + i) * a.m_in + (a.sc == 2 ? 1 : 0);
+    double dav, temp;
+    prod_vjp(Ai, P + i * a.m, T, g, e, a.d, a.sc != 0, shift, dav, temp);
+    if (!(a.sc && e == 0)) a.dAjob[idx] += dav;
+    a.gbuf[(job * 2 + (1 - a.ga)) * per + rem] = temp;
+}
+
+// Tiled VJP products for d = D in {16, 32}: dav = g b^T (b = P_i + s T) and
+// temp = A_i^T g, each output accumulated in the order of prod_vjp.  128
+// threads; every LDS operand is read along rows (g is staged twice).
+template <int D>
+__global__ void __launch_bounds__(128) lm_vjp_prod_tiled(LmArgs a) {
+    constexpr int TPP = (D / 4) * (D / 4);
+    constexpr int PPB = 128 / TPP;
+    constexpr int MM = D * D;
+    __shared__ double As[PPB][MM];  // A_i:                 As[l][r] = a[l][r]
+    __shared__ double Gs[PPB][MM];  // barY_i:              Gs[l][c] = g[l][c]
+    __shared__ double Gt[PPB][MM];  // barY_i transposed:   Gt[l][r] = g[r][l]
+    __shared__ double Bt[PPB][MM];  // P_i + s T transposed: Bt[l][c] = b[c][l]
+    const long long per = a.S * MM;
+    const long long total = a.jobs * a.S;
+    const long long p0 = (long long)blockIdx.x * PPB;
+    for (int x = threadIdx.x; x < PPB * MM; x += 128) {
+        const int g = x / MM, e = x - g * MM;
+        const long long p = p0 + g;
+        if (p >= total) continue;
+        const long long job = p / a.S, i = p - job * a.S;
+        const long long b = job / a.Q;
+        const int q = (int)(job - b * a.Q);
+        const double shift = a.x[q] - 1.0;
+        const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
+        double v = P[i * MM + e];
+        v += P[(a.S - 1) * MM + e] * shift;
+        const int r = e / D, c = e - r * D;
+        Bt[g][c * D + r] = v;
+        As[g][e] = a.in[(b * a.S + i) * (long long)MM + e];
+        const double gv = a.gbuf[(job * 2 + a.ga) * per + i * MM + e];
+        Gs[g][e] = gv;
+        Gt[g][c * D + r] = gv;
+    }
+    __syncthreads();
+    const int g = threadIdx.x / TPP, t = threadIdx.x - g * TPP;
+    const long long p = p0 + g;
+    if (p >= total) return;
+    const long long job = p / a.S, i = p - job * a.S;
+    const int r0 = 4 * (t / (D / 4)), c0 = 4 * (t % (D / 4));
+    double dav[4][4], tmp[4][4];
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dav[u][v] = 0.0;
+            tmp[u][v] = 0.0;
+        }
+    for (int l = 0; l < D; ++l) {
+        double gr[4], bc[4], ar[4], gc[4];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            gr[u] = Gt[g][l * D + r0 + u];  // g[r][l]
+            ar[u] = As[g][l * D + r0 + u];  // a[l][r]
+        }
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            bc[v] = Bt[g][l * D + c0 + v];  // b[c][l]
+            gc[v] = Gs[g][l * D + c0 + v];  // g[l][c]
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int v = 0; v < 4; ++v) {
+                dav[u][v] += gr[u] * bc[v];
+                tmp[u][v] += ar[u] * gc[v];
+            }
+    }
+    double* dA = a.dAjob + job * per + i * MM;
+    double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * MM;
+#pragma unroll
+    for (int u = 0; u < 4; ++u)
+#pragma unroll
+        for (int v = 0; v < 4; ++v) {
+            dA[(r0 + u) * D + c0 + v] += dav[u][v];
+            Gb[(r0 + u) * D + c0 + v] = tmp[u][v];
+        }
+}
+
+// bar_total, prefix_vjp and sum_vjp_add of level k for one entry of one job
+// (grad.hpp:225-235, 305-335)
+template <int I>
+__global__ void __launch_bounds__(256) lm_vjp_scan(LmArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.jobs * a.m) return;
+    const long long job = idx / a.m;
+    const int e = (int)(idx - job * a.m);
+    const long long b = job / a.Q;
+    const int q = (int)(job - b * a.Q);
+    const long long S = a.S;
+    const int m = a.m, k = a.level;
+    const double shift = a.x[q] - 1.0;
+    const double dscale = a.dt * a.w[q];
+    double* G = a.gbuf + (job * 2 + (1 - a.ga)) * S * m + e;
+    // bar_total accumulated from the last sample down, as the reference's reverse loop does
+    const double tl = G[(S - 1) * m];
+    double bt;
+    if (a.bt_fused) {
+        bt = tl * (1.0 + shift);
+    } else {
+        bt = tl;
+        bt += tl * shift;
+    }
+    constexpr int U = 8;
+    for (long long i0 = S - 2; i0 >= 0; i0 -= U) {
+        double t[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) t[u] = (i0 - u >= 0) ? G[(i0 - u) * m] : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            if (i0 - u < 0) break;
+            bt += t[u] * shift;
+        }
+    }
+    G[(S - 1) * m] = bt;
+    prefix_vjp<I>(G, m, S, a.dt);
+    if (k > 2 && a.op != OP_ONE) {  // sum_vjp_add (grad.hpp:232, 332)
+        const double o = lm_cot(a, b, a.op == OP_MANY ? k - 2 : 0, e);
+        for (long long i = 0; i < S; ++i) G[i * m] = sum_vjp_add<I>(G[i * m], i, S, o, dscale);
+    }
+}
+
+// dA_job += barY  (grad.hpp:144, 247, 347)
+__global__ void __launch_bounds__(256) lm_vjp_final(LmArgs a) {
+    const long long per = a.S * a.m;
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= a.jobs * per) return;
+    const long long job = idx / per;
+    a.dAjob[idx] += a.gbuf[(job * 2 + a.ga) * per + (idx - job * per)];
+}
+
+// ---------------------------------------------------------------------------
+// GL-lane reductions in fixed q order
+// ---------------------------------------------------------------------------
+// out[b] = Omega_1 + sum_q partial[b][q]   (lane reduction, magnus.hpp:86-90)
+__global__ void fwd_reduce_kernel(FwdReduceArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = (long long)a.n_out * a.m_out;
+    if (idx >= a.B * per) return;
+    const long long b = idx / per;
+    const int rem = (int)(idx - b * per);
+    const int k = rem / a.m_out;
+    const int eo = rem - k * a.m_out;
+    const int e = a.sc ? eo + 1 : eo;  // SpaceCurve output drops component 0
+    double v = 0.0;
+    if (a.n == 1) {
+        v = (k == 0) ? a.omega1[b * a.m + e] : 0.0;
+    } else {
+        if (a.op != OP_ONE && k == 0) v = a.omega1[b * a.m + e];
+        if (!(a.op == OP_MANY && k == 0))  // Omega_1 has no GL-node contributions
+            for (int q = 0; q < a.Qj; ++q) v += a.partial[((b * a.Qj + q) * a.n_out + k) * a.m + e];
+    }
+    a.out[idx] = v;
+}
+
+// dA[b] = sum_vjp(dOut_first, dt) + sum_q dA_q[b]
+template <int I>
+__global__ void vjp_reduce_kernel(VjpReduceArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long per = a.S * a.m_out;
+    if (idx >= a.B * per) return;
+    const long long b = idx / per;
+    const long long rem = idx - b * per;
+    const long long i = rem / a.m_out;
+    const int eo = (int)(rem - i * a.m_out);
+    const int e = a.sc ? eo + 1 : eo;
+    double v = 0.0;
+    if (!(a.op == OP_ONE && a.n > 1)) {
+        const double* row = a.cot + b * (long long)a.n_out * a.m_cot;  // dOut[0] / dOut
+        const double o = a.sc ? row[e - 1] : row[e];
+        v = o * sum_vjp_coef<I>(i, a.S, a.dt);
+    }
+    for (int q = 0; q < a.Qj; ++q) v += a.dAjob[((b * a.Qj + q) * a.S + i) * a.m + e];
+    a.dA[idx] = v;
+}
+
+inline long long blocks_for(long long n) { return (n + 255) / 256; }
+
+template <int I>
+hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
+    if (a.level >= 2) {
+        void* h = timer_begin("lm_fwd_prod", st);
+        const long long prods = a.jobs * a.S;
+        if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_fwd_prod_tiled<16>, dim3((prods + 15) / 16), dim3(256), 0, st, a);
+        else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_fwd_prod_tiled<32>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
+        else if (!a.sc && a.d == 64) hipLaunchKernelGGL(lm_fwd_prod_tiled<64>, dim3(prods), dim3(256), 0, st, a);
+        else hipLaunchKernelGGL(lm_fwd_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
+        timer_end(h);
+    }
+    void* h = timer_begin("lm_fwd_scan", st);
+    hipLaunchKernelGGL((lm_fwd_scan<I>), dim3(blocks_for(a.jobs * a.m)), dim3(256), 0, st, a);
+    timer_end(h);
+    return hipGetLastError();
+}
+
+template <int I>
+hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
+    void* h = timer_begin("lm_vjp_prod", st);
+    const long long prods = a.jobs * a.S;
+    if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_vjp_prod_tiled<16>, dim3((prods + 7) / 8), dim3(128), 0, st, a);
+    else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_vjp_prod_tiled<32>, dim3((prods + 1) / 2), dim3(128), 0, st, a);
+    else hipLaunchKernelGGL(lm_vjp_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
+    timer_end(h);
+    h = timer_begin("lm_vjp_scan", st);
+    hipLaunchKernelGGL((lm_vjp_scan<I>), dim3(blocks_for(a.jobs * a.m)), dim3(256), 0, st, a);
+    timer_end(h);
+    return hipGetLastError();
+}
+
+}  // namespace
+
+hipError_t launch_fwd_reduce(const FwdReduceArgs& a, hipStream_t s) {
+    const long long total = a.B * (long long)a.n_out * a.m_out;
+    if (total == 0) return hipSuccess;
+    const int block = 256;
+    const long long grid = (total + block - 1) / block;
+    hipLaunchKernelGGL(fwd_reduce_kernel, dim3(grid), dim3(block), 0, s, a);
+    return hipGetLastError();
+}
+
+hipError_t launch_vjp_reduce(int integ, const VjpReduceArgs& a, hipStream_t s) {
+    const long long total = a.B * a.S * a.m_out;
+    if (total == 0) return hipSuccess;
+    const int block = 256;
+    const long long grid = (total + block - 1) / block;
+    switch (integ) {
+        case BOOLE: hipLaunchKernelGGL((vjp_reduce_kernel<BOOLE>), dim3(grid), dim3(block), 0, s, a); break;
+        case SIMPSON: hipLaunchKernelGGL((vjp_reduce_kernel<SIMPSON>), dim3(grid), dim3(block), 0, s, a); break;
+        case TRAPEZOID: hipLaunchKernelGGL((vjp_reduce_kernel<TRAPEZOID>), dim3(grid), dim3(block), 0, s, a); break;
+        default: hipLaunchKernelGGL((vjp_reduce_kernel<RIEMANN>), dim3(grid), dim3(block), 0, s, a); break;
+    }
+    return hipGetLastError();
+}
+
+hipError_t launch_lm_forward_level(int integ, const LmArgs& a, hipStream_t s) {
+    switch (integ) {
+        case BOOLE: return lm_forward_level_i<BOOLE>(a, s);
+        case SIMPSON: return lm_forward_level_i<SIMPSON>(a, s);
+        case TRAPEZOID: return lm_forward_level_i<TRAPEZOID>(a, s);
+        default: return lm_forward_level_i<RIEMANN>(a, s);
+    }
+}
+
+hipError_t launch_lm_vjp_init(int integ, const LmArgs& a, hipStream_t s) {
+    const dim3 grid(blocks_for(a.jobs * a.S * a.m));
+    void* h = timer_begin("lm_vjp_init", s);
+    switch (integ) {
+        case BOOLE: hipLaunchKernelGGL((lm_vjp_init<BOOLE>), grid, dim3(256), 0, s, a); break;
+        case SIMPSON: hipLaunchKernelGGL((lm_vjp_init<SIMPSON>), grid, dim3(256), 0, s, a); break;
+        case TRAPEZOID: hipLaunchKernelGGL((lm_vjp_init<TRAPEZOID>), grid, dim3(256), 0, s, a); break;
+        default: hipLaunchKernelGGL((lm_vjp_init<RIEMANN>), grid, dim3(256), 0, s, a); break;
+    }
+    timer_end(h);
+    return hipGetLastError();
+}
+
+hipError_t launch_lm_vjp_level(int integ, const LmArgs& a, hipStream_t s) {
+    switch (integ) {
+        case BOOLE: return lm_vjp_level_i<BOOLE>(a, s);
+        case SIMPSON: return lm_vjp_level_i<SIMPSON>(a, s);
+        case TRAPEZOID: return lm_vjp_level_i<TRAPEZOID>(a, s);
+        default: return lm_vjp_level_i<RIEMANN>(a, s);
+    }
+}
+
+hipError_t launch_lm_vjp_final(const LmArgs& a, hipStream_t s) {
+    void* h = timer_begin("lm_vjp_final", s);
+    hipLaunchKernelGGL(lm_vjp_final, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, s, a);
+    timer_end(h);
+    return hipGetLastError();
+}
+
+}  // namespace magnus

@@ -279,12 +279,39 @@ def test_config_dense_d32_instance_full():
     assert rel_fro(dA[0].cpu().numpy(), ref_g) <= TOL
 
 
+@pytest.mark.parametrize("d", [16, 32, 64])
+@pytest.mark.parametrize("integrator", ["Boole", "Simpson"])
+def test_tiled_products_match_oracle(d, integrator):
+    """Dimensions with LDS-tiled level products (forward d in {16, 32, 64},
+    VJP d in {16, 32}): batched forward with carry, VJP from the carry and by
+    recompute, several GL nodes and levels."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, n = 3, 33, 5
+    A = np.stack([fourier_matrices(np.random.default_rng([13, b, d]), S, d, modes=2) for b in range(B)])
+    cot = np.stack([np.random.default_rng([13, b, d, 1]).standard_normal((n, d, d)) for b in range(B)])
+    At, ct = torch.from_numpy(A).cuda(), torch.from_numpy(cot).cuda()
+    out, carry = torch_ops.forward(n, At, 0.0, 1.0, op="many", integrator=integrator, record_vjp=True)
+    dA = torch_ops.vjp(n, At, ct, carry, 0.0, 1.0, op="many", integrator=integrator)
+    dA2 = torch_ops.vjp(n, At, ct, None, 0.0, 1.0, op="many", integrator=integrator)
+    out, carry, dA, dA2 = (x.cpu().numpy() for x in (out, carry, dA, dA2))
+    for b in range(B):
+        ref, ref_carry = O.forward(n, A[b], 0.0, 1.0, op="many", integrator=integrator, record_vjp=True)
+        ref_g = O.vjp(n, A[b], cot[b], 0.0, 1.0, op="many", integrator=integrator, vjp_data=ref_carry)
+        assert rel_fro(out[b], ref) <= TOL
+        assert rel_fro(carry[b], ref_carry) <= TOL
+        assert rel_fro(dA[b], ref_g) <= TOL
+        assert rel_fro(dA2[b], ref_g) <= TOL
+
+
 @pytest.mark.parametrize("integrator", ["Boole", "Simpson", "Riemann"])
 @pytest.mark.parametrize("op", OPS)
 @pytest.mark.parametrize("n", [2, 3, 5, 8, 9])
 def test_spacecurve_single_pass_batched_matches_oracle(integrator, op, n):
     """Batched SpaceCurve forward + VJP without a carry (single-pass kernels for
-    n <= 8, entry-parallel kernels for n = 9), multi-chunk sample ranges."""
+    n <= 8, level-parallel kernels for n = 9), multi-chunk sample ranges."""
     import torch
 
     from magnus import capi, torch_ops
@@ -297,7 +324,7 @@ def test_spacecurve_single_pass_batched_matches_oracle(integrator, op, n):
     out = torch_ops.sc_forward(n, at, 0.0, 1.0, op=op, integrator=integrator)
     kernel = capi.last_kernel()
     dA = torch_ops.sc_vjp(n, at, torch.from_numpy(cot).cuda(), None, 0.0, 1.0, op=op, integrator=integrator)
-    assert kernel == ("wb" if n <= 8 else "ep") and capi.last_kernel() == kernel
+    assert kernel == ("wb" if n <= 8 else "lm") and capi.last_kernel() == kernel
     out, dA = out.cpu().numpy(), dA.cpu().numpy()
     for b in range(B):
         ref = O.sc_forward(n, a[b], 0.0, 1.0, op=op, integrator=integrator)

@@ -51,23 +51,49 @@ __global__ void __launch_bounds__(256) lm_fwd_prod(LmArgs a) {
     a.Y[idx] = f;
 }
 
-// Tiled products for d = D in {16, 32, 64}: a workgroup stages PPB samples'
-// operands in LDS (64 KiB) and each thread computes a 4 x 4 block of one
-// product, every output accumulated over l in order exactly as prod_generic
-// does (so the results are identical).  Products are numbered p = job*S + i.
-constexpr int kTileLds = 8192;  // doubles of LDS per workgroup
+// ---------------------------------------------------------------------------
+// MFMA products for d = D in {16, 32, 64}.  A 256-thread workgroup stages the
+// operands of PPB samples in LDS (k-major, so a wavefront's operand reads are
+// contiguous) and computes every 16 x 16 output tile with
+// v_mfma_f64_16x16x4f64 (lane l, register r of a tile holds row 4r + l/16,
+// column l%16).  The MFMA sums k in its own order, so these products differ
+// from the reference's l-ordered sums by rounding only.
+// ---------------------------------------------------------------------------
+typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 template <int D>
-__global__ void __launch_bounds__(256) lm_fwd_prod_tiled(LmArgs a) {
-    constexpr int TPP = (D / 4) * (D / 4);  // threads per product
-    constexpr int PPB = 256 / TPP;          // products per workgroup
-    constexpr int MM = D * D;
-    __shared__ double At[PPB][MM];  // A_i transposed: At[l][r] = a[r][l]
-    __shared__ double Bs[PPB][MM];  // P_i + s T
+struct MfmaShape {
+    static constexpr int TILES = (D / 16) * (D / 16);      // 16x16 tiles per product
+    static constexpr int WPP = TILES >= 4 ? TILES / 4 : 1;  // wavefronts per product
+    static constexpr int TPW = TILES / WPP;                // tiles per wavefront
+    static constexpr int PPB = 4 / WPP;                    // products per workgroup
+    static constexpr int MM = D * D;
+};
+
+// C_tile += Aop * Bop over k = 0..D-1 with Aop[i][k] = LA[k*D + i0 + i], Bop[k][j] = LB[k*D + j0 + j]
+template <int D>
+__device__ __forceinline__ dbl4 mfma_tile(const double* LA, const double* LB, int i0, int j0, int lane) {
+    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+    const int li = lane & 15, lk = lane >> 4;
+#pragma unroll
+    for (int kk = 0; kk < D / 4; ++kk) {
+        const double av = LA[(4 * kk + lk) * D + i0 + li];
+        const double bv = LB[(4 * kk + lk) * D + j0 + li];
+        acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
+// Y_i = A_i (P_i + s T) for products p = job*S + i
+template <int D>
+__global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
+    using M = MfmaShape<D>;
+    __shared__ double LA[M::PPB][M::MM];  // A_i transposed: LA[k][i] = a[i][k]
+    __shared__ double LB[M::PPB][M::MM];  // P_i + s T
     const long long total = a.jobs * a.S;
-    const long long p0 = (long long)blockIdx.x * PPB;
-    for (int x = threadIdx.x; x < PPB * MM; x += 256) {
-        const int g = x / MM, e = x - g * MM;
+    const long long p0 = (long long)blockIdx.x * M::PPB;
+    for (int x = threadIdx.x; x < M::PPB * M::MM; x += 256) {
+        const int g = x / M::MM, e = x - g * M::MM;
         const long long p = p0 + g;
         if (p >= total) continue;
         const long long job = p / a.S, i = p - job * a.S;
@@ -75,38 +101,94 @@ __global__ void __launch_bounds__(256) lm_fwd_prod_tiled(LmArgs a) {
         const int q = (int)(job - b * a.Qj);
         const double shift = a.x[q] - 1;
         const double* P = lm_slot(a, job, b, q, a.level - 1);
-        double v = P[i * MM + e];
-        v += P[(a.S - 1) * MM + e] * shift;
-        Bs[g][e] = v;
-        const int r = e / D, l = e - r * D;
-        At[g][l * D + r] = a.in[(b * a.S + i) * (long long)MM + e];
+        double v = P[i * M::MM + e];
+        v += P[(a.S - 1) * M::MM + e] * shift;
+        LB[g][e] = v;
+        const int r = e / D, c = e - r * D;
+        LA[g][c * D + r] = a.in[(b * a.S + i) * (long long)M::MM + e];
     }
     __syncthreads();
-    const int g = threadIdx.x / TPP, t = threadIdx.x - g * TPP;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int g = wave / M::WPP, wp = wave - g * M::WPP;
     const long long p = p0 + g;
     if (p >= total) return;
-    const int r0 = 4 * (t / (D / 4)), c0 = 4 * (t % (D / 4));
-    double acc[4][4];
+    double* y = a.Y + p * M::MM;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+    for (int tt = 0; tt < M::TPW; ++tt) {
+        const int tile = wp * M::TPW + tt;
+        const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
+        const dbl4 acc = mfma_tile<D>(LA[g], LB[g], i0, j0, lane);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) acc[u][v] = 0.0;
-    for (int l = 0; l < D; ++l) {
-        double av[4], bv[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) av[u] = At[g][l * D + r0 + u];
-#pragma unroll
-        for (int v = 0; v < 4; ++v) bv[v] = Bs[g][l * D + c0 + v];
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) acc[u][v] += av[u] * bv[v];
+        for (int r = 0; r < 4; ++r) y[(i0 + 4 * r + (lane >> 4)) * D + j0 + (lane & 15)] = acc[r];
     }
-    double* y = a.Y + p * MM;
+}
+
+// VJP products in two phases over the same LDS: dav = g b^T (b = P_i + s T),
+// added to dA_job, then temp = A_i^T g
+template <int D>
+__global__ void __launch_bounds__(256) lm_vjp_prod_mfma(LmArgs a) {
+    using M = MfmaShape<D>;
+    __shared__ double LA[M::PPB][M::MM];
+    __shared__ double LB[M::PPB][M::MM];
+    const long long per = a.S * M::MM;
+    const long long total = a.jobs * a.S;
+    const long long p0 = (long long)blockIdx.x * M::PPB;
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gw = wave / M::WPP, wp = wave - gw * M::WPP;
+    const long long pw = p0 + gw;
+    // phase 1: LA[k][i] = g[i][k], LB[k][j] = b[j][k]
+    for (int x = threadIdx.x; x < M::PPB * M::MM; x += 256) {
+        const int g = x / M::MM, e = x - g * M::MM;
+        const long long p = p0 + g;
+        if (p >= total) continue;
+        const long long job = p / a.S, i = p - job * a.S;
+        const long long b = job / a.Q;
+        const int q = (int)(job - b * a.Q);
+        const double shift = a.x[q] - 1.0;
+        const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
+        double v = P[i * M::MM + e];
+        v += P[(a.S - 1) * M::MM + e] * shift;
+        const int r = e / D, c = e - r * D;
+        LB[g][c * D + r] = v;
+        LA[g][c * D + r] = a.gbuf[(job * 2 + a.ga) * per + i * M::MM + e];
+    }
+    __syncthreads();
+    if (pw < total) {
+        const long long job = pw / a.S, i = pw - job * a.S;
+        double* dA = a.dAjob + job * per + i * M::MM;
 #pragma unroll
-    for (int u = 0; u < 4; ++u)
+        for (int tt = 0; tt < M::TPW; ++tt) {
+            const int tile = wp * M::TPW + tt;
+            const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
+            const dbl4 acc = mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane);
 #pragma unroll
-        for (int v = 0; v < 4; ++v) y[(r0 + u) * D + c0 + v] = acc[u][v];
+            for (int r = 0; r < 4; ++r) dA[(i0 + 4 * r + (lane >> 4)) * D + j0 + (lane & 15)] += acc[r];
+        }
+    }
+    __syncthreads();
+    // phase 2: LA[k][i] = a[k][i], LB[k][j] = g[k][j]
+    for (int x = threadIdx.x; x < M::PPB * M::MM; x += 256) {
+        const int g = x / M::MM, e = x - g * M::MM;
+        const long long p = p0 + g;
+        if (p >= total) continue;
+        const long long job = p / a.S, i = p - job * a.S;
+        const long long b = job / a.Q;
+        LA[g][e] = a.in[(b * a.S + i) * (long long)M::MM + e];
+        LB[g][e] = a.gbuf[(job * 2 + a.ga) * per + i * M::MM + e];
+    }
+    __syncthreads();
+    if (pw < total) {
+        const long long job = pw / a.S, i = pw - job * a.S;
+        double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * M::MM;
+#pragma unroll
+        for (int tt = 0; tt < M::TPW; ++tt) {
+            const int tile = wp * M::TPW + tt;
+            const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
+            const dbl4 acc = mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane);
+#pragma unroll
+            for (int r = 0; r < 4; ++r) Gb[(i0 + 4 * r + (lane >> 4)) * D + j0 + (lane & 15)] = acc[r];
+        }
+    }
 }
 
 // P^{(j)} = Prefix(Y) (j < n) and the level's GL-weighted sum, one entry of one job
@@ -208,84 +290,6 @@ __global__ void __launch_bounds__(256) lm_vjp_prod(LmArgs a) {
     prod_vjp(Ai, P + i * a.m, T, g, e, a.d, a.sc != 0, shift, dav, temp);
     if (!(a.sc && e == 0)) a.dAjob[idx] += dav;
     a.gbuf[(job * 2 + (1 - a.ga)) * per + rem] = temp;
-}
-
-// Tiled VJP products for d = D in {16, 32}: dav = g b^T (b = P_i + s T) and
-// temp = A_i^T g, each output accumulated in the order of prod_vjp.  128
-// threads; every LDS operand is read along rows (g is staged twice).
-template <int D>
-__global__ void __launch_bounds__(128) lm_vjp_prod_tiled(LmArgs a) {
-    constexpr int TPP = (D / 4) * (D / 4);
-    constexpr int PPB = 128 / TPP;
-    constexpr int MM = D * D;
-    __shared__ double As[PPB][MM];  // A_i:                 As[l][r] = a[l][r]
-    __shared__ double Gs[PPB][MM];  // barY_i:              Gs[l][c] = g[l][c]
-    __shared__ double Gt[PPB][MM];  // barY_i transposed:   Gt[l][r] = g[r][l]
-    __shared__ double Bt[PPB][MM];  // P_i + s T transposed: Bt[l][c] = b[c][l]
-    const long long per = a.S * MM;
-    const long long total = a.jobs * a.S;
-    const long long p0 = (long long)blockIdx.x * PPB;
-    for (int x = threadIdx.x; x < PPB * MM; x += 128) {
-        const int g = x / MM, e = x - g * MM;
-        const long long p = p0 + g;
-        if (p >= total) continue;
-        const long long job = p / a.S, i = p - job * a.S;
-        const long long b = job / a.Q;
-        const int q = (int)(job - b * a.Q);
-        const double shift = a.x[q] - 1.0;
-        const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
-        double v = P[i * MM + e];
-        v += P[(a.S - 1) * MM + e] * shift;
-        const int r = e / D, c = e - r * D;
-        Bt[g][c * D + r] = v;
-        As[g][e] = a.in[(b * a.S + i) * (long long)MM + e];
-        const double gv = a.gbuf[(job * 2 + a.ga) * per + i * MM + e];
-        Gs[g][e] = gv;
-        Gt[g][c * D + r] = gv;
-    }
-    __syncthreads();
-    const int g = threadIdx.x / TPP, t = threadIdx.x - g * TPP;
-    const long long p = p0 + g;
-    if (p >= total) return;
-    const long long job = p / a.S, i = p - job * a.S;
-    const int r0 = 4 * (t / (D / 4)), c0 = 4 * (t % (D / 4));
-    double dav[4][4], tmp[4][4];
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            dav[u][v] = 0.0;
-            tmp[u][v] = 0.0;
-        }
-    for (int l = 0; l < D; ++l) {
-        double gr[4], bc[4], ar[4], gc[4];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            gr[u] = Gt[g][l * D + r0 + u];  // g[r][l]
-            ar[u] = As[g][l * D + r0 + u];  // a[l][r]
-        }
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            bc[v] = Bt[g][l * D + c0 + v];  // b[c][l]
-            gc[v] = Gs[g][l * D + c0 + v];  // g[l][c]
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u)
-#pragma unroll
-            for (int v = 0; v < 4; ++v) {
-                dav[u][v] += gr[u] * bc[v];
-                tmp[u][v] += ar[u] * gc[v];
-            }
-    }
-    double* dA = a.dAjob + job * per + i * MM;
-    double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * MM;
-#pragma unroll
-    for (int u = 0; u < 4; ++u)
-#pragma unroll
-        for (int v = 0; v < 4; ++v) {
-            dA[(r0 + u) * D + c0 + v] += dav[u][v];
-            Gb[(r0 + u) * D + c0 + v] = tmp[u][v];
-        }
 }
 
 // bar_total, prefix_vjp and sum_vjp_add of level k for one entry of one job
@@ -392,9 +396,9 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
     if (a.level >= 2) {
         void* h = timer_begin("lm_fwd_prod", st);
         const long long prods = a.jobs * a.S;
-        if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_fwd_prod_tiled<16>, dim3((prods + 15) / 16), dim3(256), 0, st, a);
-        else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_fwd_prod_tiled<32>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
-        else if (!a.sc && a.d == 64) hipLaunchKernelGGL(lm_fwd_prod_tiled<64>, dim3(prods), dim3(256), 0, st, a);
+        if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_fwd_prod_mfma<16>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
+        else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_fwd_prod_mfma<32>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
+        else if (!a.sc && a.d == 64) hipLaunchKernelGGL(lm_fwd_prod_mfma<64>, dim3(prods), dim3(256), 0, st, a);
         else hipLaunchKernelGGL(lm_fwd_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
         timer_end(h);
     }
@@ -408,8 +412,9 @@ template <int I>
 hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
     void* h = timer_begin("lm_vjp_prod", st);
     const long long prods = a.jobs * a.S;
-    if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_vjp_prod_tiled<16>, dim3((prods + 7) / 8), dim3(128), 0, st, a);
-    else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_vjp_prod_tiled<32>, dim3((prods + 1) / 2), dim3(128), 0, st, a);
+    if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_vjp_prod_mfma<16>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
+    else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_vjp_prod_mfma<32>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
+    else if (!a.sc && a.d == 64) hipLaunchKernelGGL(lm_vjp_prod_mfma<64>, dim3(prods), dim3(256), 0, st, a);
     else hipLaunchKernelGGL(lm_vjp_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
     timer_end(h);
     h = timer_begin("lm_vjp_scan", st);

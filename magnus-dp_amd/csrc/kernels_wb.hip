@@ -272,6 +272,11 @@ __global__ void __launch_bounds__(256) wb_fwd_pass1(WbArgs a) {
         for (int l = 0; l < L; ++l) {
             V4 Pi[B > 1 ? B - 1 : 1];
             R::step(f0[l], fn, base[l], dt, Pi);
+            if (l == 0) {  // V_1 = Prefix(A) is a pure vector: drop its zero component
+                base[0].x[0] = 0.0;
+#pragma unroll
+                for (int r = 0; r + 1 < B; ++r) Pi[r].x[0] = 0.0;
+            }
             f0[l] = fn[B - 1];
             if (l + 1 < L) {
 #pragma unroll
@@ -645,6 +650,11 @@ __device__ __forceinline__ void rev_block(long long s0, long long e, const doubl
         if (zero_start) Vend[k] = vzero();
         else R::back(fs, fn, Vend[k], dt);
         R::interior(fs, fn, Vend[k], dt, Vi[k]);
+        if (k == 0) {  // V_1 = Prefix(A) is a pure vector: drop its zero component
+            Vend[0].x[0] = 0.0;
+#pragma unroll
+            for (int r = 0; r + 1 < B; ++r) Vi[0][r].x[0] = 0.0;
+        }
     }
     // reverse sweep of the block, top level down.  (The sum weight at s in a
     // FIRST block never reaches the output: there V = 0 and the end role is off.)

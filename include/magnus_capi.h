@@ -136,10 +136,27 @@ int magnus_sc_forward_f64(const magnus_problem* p, uint64_t count, const double*
 int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3,
                       const double* cot3, const double* carry, double* dA3, void* stream);
 
+/* ---- sampling A(t) on the device, on the reference's grid
+ *      t_i = np.linspace(t0, tf, samples) (replaces evaluating a Python callable
+ *      there, src/api/magnus/__init__.py:58-85, for these two families).
+ *      Arithmetic follows numpy's order; samples differ from a numpy
+ *      evaluation only through sin/cos (a few ulp).  modes <= 32. ---- */
+
+/* out[b][i][e] = C[b][0][e] + sum_{k=1..modes} ( sin(2 pi k t_i) C[b][2k-1][e]
+ *                                               + cos(2 pi k t_i) C[b][2k][e] ),
+ * coeffs [count][2*modes+1][entries], out [count][samples][entries]; entries = d*d. */
+int magnus_sample_fourier_f64(uint64_t count, uint64_t samples, double t0, double tf, uint64_t entries,
+                              uint64_t modes, const double* coeffs, double* out, void* stream);
+
+/* out[b][i][c] = sum_{k=1..modes} amp[b][k-1][c] sin(2 pi k t_i + phase[b][k-1][c]),
+ * amp/phase [count][modes][3], out [count][samples][3] (SpaceCurve input). */
+int magnus_sample_sc_fourier_f64(uint64_t count, uint64_t samples, double t0, double tf, uint64_t modes,
+                                 const double* amp, const double* phase, double* out, void* stream);
+
 /* ---- kernel timing (diagnostics).  While enabled, every main kernel launch
  *      is bracketed by a HIP event pair recorded on its launch stream.
  *      magnus_timing_read waits for the recorded events and returns, for the
- *      kernel family `name` (e.g. "ep_forward", "ep_vjp"), the number of
+ *      kernel family `name` (e.g. "wb_rev", "lm_fwd_prod"), the number of
  *      launches and their total duration in milliseconds since the last reset. ---- */
 int magnus_timing_enable(int on);
 int magnus_timing_read(const char* name, uint64_t* launches, double* total_ms);
@@ -149,7 +166,7 @@ void magnus_timing_reset(void);
 const char* magnus_last_error(void);
 
 /* name of the kernel family the last successful call on this thread used
- * (diagnostics: "ep", "tile", "mfma", ...) */
+ * (diagnostics: "wb" single-pass SpaceCurve, "lm" level-parallel) */
 const char* magnus_last_kernel(void);
 
 #ifdef __cplusplus

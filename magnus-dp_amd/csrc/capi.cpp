@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "kernels_sample.hpp"
 #include "kernels_wb.hpp"
 
 namespace {
@@ -590,6 +591,50 @@ int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3
     const double* c = pl.n >= 2 ? carry : nullptr;
     if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, cot3, nullptr, dA3, (hipStream_t)stream);
     return run_vjp(pl, count, in3, cot3, c, dA3, (hipStream_t)stream);
+}
+
+int magnus_sample_fourier_f64(uint64_t count, uint64_t samples, double t0, double tf, uint64_t entries,
+                              uint64_t modes, const double* coeffs, double* out, void* stream) {
+    if (!coeffs || !out) return fail(MAGNUS_EINVAL, "null argument");
+    if (samples < 2) return fail(MAGNUS_EINVAL, "samples must be at least 2");
+    if (entries == 0) return fail(MAGNUS_EINVAL, "entries must be positive");
+    if (modes > (uint64_t)magnus::sample_max_modes()) return fail(MAGNUS_ERANGE, "too many Fourier modes (max 32)");
+    if (count == 0) return MAGNUS_OK;
+    magnus::SampleArgs a{};
+    a.coeffs = coeffs;
+    a.out = out;
+    a.B = (long long)count;
+    a.S = (long long)samples;
+    a.m = (int)entries;
+    a.modes = (int)modes;
+    a.t0 = t0;
+    a.tf = tf;
+    a.step = (tf - t0) / (double)(samples - 1);
+    hipError_t e = magnus::launch_sample_fourier(a, (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "sampling kernel launch");
+    return MAGNUS_OK;
+}
+
+int magnus_sample_sc_fourier_f64(uint64_t count, uint64_t samples, double t0, double tf, uint64_t modes,
+                                 const double* amp, const double* phase, double* out, void* stream) {
+    if (!amp || !phase || !out) return fail(MAGNUS_EINVAL, "null argument");
+    if (samples < 2) return fail(MAGNUS_EINVAL, "samples must be at least 2");
+    if (modes > (uint64_t)magnus::sample_max_modes()) return fail(MAGNUS_ERANGE, "too many Fourier modes (max 32)");
+    if (count == 0) return MAGNUS_OK;
+    magnus::SampleArgs a{};
+    a.amp = amp;
+    a.phase = phase;
+    a.out = out;
+    a.B = (long long)count;
+    a.S = (long long)samples;
+    a.m = 3;
+    a.modes = (int)modes;
+    a.t0 = t0;
+    a.tf = tf;
+    a.step = (tf - t0) / (double)(samples - 1);
+    hipError_t e = magnus::launch_sample_sc_fourier(a, (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "sampling kernel launch");
+    return MAGNUS_OK;
 }
 
 int magnus_timing_enable(int on) {

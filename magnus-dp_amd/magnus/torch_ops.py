@@ -147,6 +147,36 @@ class SpaceCurveFunction(torch.autograd.Function):
         return da, None, None, None, None, None, None
 
 
+def sample_fourier(coeffs: torch.Tensor, samples: int, t0: float = 0.0, tf: float = 1.0) -> torch.Tensor:
+    """A(t_i) = C_0 + sum_k sin(2 pi k t_i) C_{2k-1} + cos(2 pi k t_i) C_{2k} on
+    np.linspace(t0, tf, samples), for coeffs (B, 2M+1, d, d) float64 on the GPU."""
+    _check_input(coeffs, 4, "coeffs")
+    coeffs = coeffs.contiguous()
+    B, terms, d, d2 = coeffs.shape
+    if d != d2 or terms % 2 != 1:
+        raise ValueError("coeffs must have shape (B, 2*modes+1, d, d)")
+    out = torch.empty((B, samples, d, d), dtype=torch.float64, device=coeffs.device)
+    capi.check(capi.lib().magnus_sample_fourier_f64(B, samples, t0, tf, d * d, (terms - 1) // 2, _ptr(coeffs),
+                                                    _ptr(out), _stream()))
+    return out
+
+
+def sample_sc_fourier(amp: torch.Tensor, phase: torch.Tensor, samples: int, t0: float = 0.0,
+                      tf: float = 1.0) -> torch.Tensor:
+    """a_c(t_i) = sum_k amp[k, c] sin(2 pi k t_i + phase[k, c]) on np.linspace(t0, tf, samples),
+    for amp, phase (B, M, 3) float64 on the GPU; returns (B, samples, 3)."""
+    _check_input(amp, 3, "amp")
+    _check_input(phase, 3, "phase")
+    amp, phase = amp.contiguous(), phase.contiguous()
+    if amp.shape != phase.shape or amp.shape[2] != 3:
+        raise ValueError("amp and phase must have the same shape (B, modes, 3)")
+    B, modes, _ = amp.shape
+    out = torch.empty((B, samples, 3), dtype=torch.float64, device=amp.device)
+    capi.check(capi.lib().magnus_sample_sc_fourier_f64(B, samples, t0, tf, modes, _ptr(amp), _ptr(phase), _ptr(out),
+                                                       _stream()))
+    return out
+
+
 def magnus(A, n, t0=0.0, tf=1.0, *, op="sum", matrix_backend="Auto", integrator="Auto", record_vjp=True):
     """Differentiable batched Magnus on GPU tensors (A: (B, S, d, d))."""
     return MagnusFunction.apply(A, n, t0, tf, op, matrix_backend, integrator, record_vjp)

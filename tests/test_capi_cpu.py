@@ -87,6 +87,20 @@ def test_sizes():
     assert capi.lib().magnus_carry_elems(ctypes.byref(p1)) == 0
 
 
+def test_sampling_argument_checks_before_device_work():
+    L = capi.lib()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation
+    assert L.magnus_sample_fourier_f64(1, 1, 0.0, 1.0, 4, 2, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_sample_fourier_f64(1, 9, 0.0, 1.0, 0, 2, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_sample_fourier_f64(1, 9, 0.0, 1.0, 4, 33, dummy, dummy, None) == capi.ERANGE
+    assert L.magnus_sample_fourier_f64(1, 9, 0.0, 1.0, 4, 2, None, dummy, None) == capi.EINVAL
+    assert L.magnus_sample_sc_fourier_f64(1, 9, 0.0, 1.0, 40, dummy, dummy, dummy, None) == capi.ERANGE
+    assert L.magnus_sample_sc_fourier_f64(1, 9, 0.0, 1.0, 4, dummy, None, dummy, None) == capi.EINVAL
+    L.magnus_sample_sc_fourier_f64(1, 9, 0.0, 1.0, 40, dummy, dummy, dummy, None)
+    assert "modes" in L.magnus_last_error().decode()
+    assert L.magnus_sample_fourier_f64(0, 9, 0.0, 1.0, 4, 2, dummy, dummy, None) == capi.OK  # empty batch
+
+
 def test_default_gl_table_is_numpy_leggauss_bitwise():
     L = capi.lib()
     for order in (1, 2, 3, 5, 17, 64):

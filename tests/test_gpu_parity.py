@@ -363,6 +363,50 @@ def test_spacecurve_single_pass_ragged_chunks(integrator, n):
             assert rel_fro(dA[b], ref_g) <= TOL, (S, b)
 
 
+@pytest.mark.parametrize("d,modes,S,t0,tf", [(2, 3, 33, 0.0, 1.0), (4, 3, 1025, 0.0, 1.0), (32, 5, 257, -0.4, 1.7),
+                                             (5, 1, 9, 0.2, 0.3)])
+def test_device_sampling_matches_numpy_fourier_form(d, modes, S, t0, tf):
+    """A(t) sampled on the device equals the numpy evaluation (same seeded
+    coefficients) up to sin/cos rounding; the Magnus output on it matches."""
+    import torch
+
+    from magnus import torch_ops
+
+    B = 3
+    rngs = [np.random.default_rng([17, b, d]) for b in range(B)]
+    s = 1.0 / np.sqrt(d)
+    coeffs = np.stack([r.standard_normal((2 * modes + 1, d, d)) * s for r in rngs])
+    ref = np.stack([fourier_matrices(np.random.default_rng([17, b, d]), S, d, modes, t0, tf) for b in range(B)])
+    got = torch_ops.sample_fourier(torch.from_numpy(coeffs).cuda(), S, t0, tf).cpu().numpy()
+    assert got.shape == ref.shape
+    assert rel_fro(got, ref) <= 1e-14
+    assert np.max(np.abs(got - ref)) <= 1e-14 * max(1.0, np.max(np.abs(ref)))
+
+
+@pytest.mark.parametrize("modes,S,t0,tf", [(4, 16385, 0.0, 1.0), (1, 2, 0.0, 1.0), (7, 101, -1.0, 2.5)])
+def test_device_sampling_matches_numpy_spacecurve_form(modes, S, t0, tf):
+    import torch
+
+    from magnus import torch_ops
+
+    B = 4
+    m = np.arange(1, modes + 1, dtype=np.float64)
+    amp = np.empty((B, modes, 3))
+    phase = np.empty((B, modes, 3))
+    ref = np.empty((B, S, 3))
+    for b in range(B):
+        rng = np.random.default_rng([19, b])
+        amp[b] = rng.standard_normal((modes, 3)) / m[:, None]
+        phase[b] = rng.uniform(0.0, 2 * np.pi, size=(modes, 3))
+        ref[b] = fourier_spacecurve(np.random.default_rng([19, b]), S, modes, t0, tf)
+    got = torch_ops.sample_sc_fourier(torch.from_numpy(amp).cuda(), torch.from_numpy(phase).cuda(), S, t0,
+                                      tf).cpu().numpy()
+    assert rel_fro(got, ref) <= 1e-14
+    out = torch_ops.sc_forward(5, torch.from_numpy(got).cuda(), t0, tf, op="many", integrator="Riemann").cpu().numpy()
+    ref_out = np.stack([O.sc_forward(5, ref[b], t0, tf, op="many", integrator="Riemann") for b in range(B)])
+    assert rel_fro(out, ref_out) <= TOL
+
+
 def test_batched_matrix_api_matches_per_instance():
     import torch
 

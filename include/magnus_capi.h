@@ -153,6 +153,17 @@ int magnus_sample_fourier_f64(uint64_t count, uint64_t samples, double t0, doubl
 int magnus_sample_sc_fourier_f64(uint64_t count, uint64_t samples, double t0, double tf, uint64_t modes,
                                  const double* amp, const double* phase, double* out, void* stream);
 
+/* ---- propagators exp(Omega) of Magnus outputs (the step after the path: the
+ *      reference's tests/bench.py:93 applies scipy.linalg.expm to the Magnus sum). ---- */
+
+/* Dense: out[b] = expm(in[b]) for real d x d matrices, in/out [count][d][d].
+ * Scaling and squaring with Pade degree 3..13 (Higham 2005). */
+int magnus_expm_f64(uint64_t count, uint64_t dim, const double* in, double* out, void* stream);
+
+/* SpaceCurve: omega (3-vector) stands for -i omega.sigma; out[b] = exp(-i omega.sigma)
+ * as a complex 2x2 matrix, [count][2][2][re, im]. */
+int magnus_sc_expm_f64(uint64_t count, const double* in3, double* out, void* stream);
+
 /* ---- kernel timing (diagnostics).  While enabled, every main kernel launch
  *      is bracketed by a HIP event pair recorded on its launch stream.
  *      magnus_timing_read waits for the recorded events and returns, for the

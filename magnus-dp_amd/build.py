@@ -35,7 +35,7 @@ COMMON = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-DEVICE_SOURCES = ["kernels_lm.hip", "kernels_wb.hip", "kernels_sample.hip"]
+DEVICE_SOURCES = ["kernels_lm.hip", "kernels_wb.hip", "kernels_sample.hip", "kernels_expm.hip"]
 HOST_SOURCES = ["capi.cpp"]
 
 

@@ -20,6 +20,7 @@
 #include <vector>
 
 #include "kernels.hpp"
+#include "kernels_expm.hpp"
 #include "kernels_sample.hpp"
 #include "kernels_wb.hpp"
 
@@ -634,6 +635,35 @@ int magnus_sample_sc_fourier_f64(uint64_t count, uint64_t samples, double t0, do
     a.step = (tf - t0) / (double)(samples - 1);
     hipError_t e = magnus::launch_sample_sc_fourier(a, (hipStream_t)stream);
     if (e != hipSuccess) return device_error(e, "sampling kernel launch");
+    return MAGNUS_OK;
+}
+
+int magnus_expm_f64(uint64_t count, uint64_t dim, const double* in, double* out, void* stream) {
+    if (!in || !out) return fail(MAGNUS_EINVAL, "null argument");
+    if (dim == 0) return fail(MAGNUS_EINVAL, "dim must be positive");
+    if (count == 0) return MAGNUS_OK;
+    DeviceScratch scratch((hipStream_t)stream);
+    magnus::ExpmArgs a{};
+    a.in = in;
+    a.out = out;
+    a.B = (long long)count;
+    a.d = (int)dim;
+    a.work = scratch.alloc<double>((size_t)count * 8 * dim * dim);
+    if (!a.work) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    hipError_t e = magnus::launch_expm(a, (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "expm kernel launch");
+    return MAGNUS_OK;
+}
+
+int magnus_sc_expm_f64(uint64_t count, const double* in3, double* out, void* stream) {
+    if (!in3 || !out) return fail(MAGNUS_EINVAL, "null argument");
+    if (count == 0) return MAGNUS_OK;
+    magnus::ExpmArgs a{};
+    a.in = in3;
+    a.out = out;
+    a.B = (long long)count;
+    hipError_t e = magnus::launch_sc_expm(a, (hipStream_t)stream);
+    if (e != hipSuccess) return device_error(e, "expm kernel launch");
     return MAGNUS_OK;
 }
 

@@ -70,6 +70,8 @@ def lib() -> ctypes.CDLL:
         d = ctypes.c_double
         L.magnus_sample_fourier_f64.argtypes = [u64, u64, d, d, u64, u64, vp, vp, vp]
         L.magnus_sample_sc_fourier_f64.argtypes = [u64, u64, d, d, u64, vp, vp, vp, vp]
+        L.magnus_expm_f64.argtypes = [u64, u64, vp, vp, vp]
+        L.magnus_sc_expm_f64.argtypes = [u64, vp, vp, vp]
         L.magnus_timing_enable.argtypes = [ctypes.c_int]
         L.magnus_timing_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)]
         L.magnus_timing_reset.restype = None

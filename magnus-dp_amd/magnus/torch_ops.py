@@ -177,6 +177,31 @@ def sample_sc_fourier(amp: torch.Tensor, phase: torch.Tensor, samples: int, t0: 
     return out
 
 
+def expm(omega: torch.Tensor) -> torch.Tensor:
+    """Batched matrix exponential of real (B, d, d) float64 GPU tensors (the
+    propagator exp(Omega) of a Magnus sum)."""
+    _check_input(omega, 3, "omega")
+    omega = omega.contiguous()
+    B, d, d2 = omega.shape
+    if d != d2:
+        raise ValueError("omega must have shape (B, d, d)")
+    out = torch.empty_like(omega)
+    capi.check(capi.lib().magnus_expm_f64(B, d, _ptr(omega), _ptr(out), _stream()))
+    return out
+
+
+def sc_expm(omega: torch.Tensor) -> torch.Tensor:
+    """exp(-i omega.sigma) for SpaceCurve Magnus sums omega (B, 3) float64 on the
+    GPU; returns the complex128 (B, 2, 2) propagators."""
+    _check_input(omega, 2, "omega")
+    omega = omega.contiguous()
+    if omega.shape[1] != 3:
+        raise ValueError("omega must have shape (B, 3)")
+    out = torch.empty((omega.shape[0], 2, 2, 2), dtype=torch.float64, device=omega.device)
+    capi.check(capi.lib().magnus_sc_expm_f64(omega.shape[0], _ptr(omega), _ptr(out), _stream()))
+    return torch.view_as_complex(out)
+
+
 def magnus(A, n, t0=0.0, tf=1.0, *, op="sum", matrix_backend="Auto", integrator="Auto", record_vjp=True):
     """Differentiable batched Magnus on GPU tensors (A: (B, S, d, d))."""
     return MagnusFunction.apply(A, n, t0, tf, op, matrix_backend, integrator, record_vjp)

@@ -101,6 +101,15 @@ def test_sampling_argument_checks_before_device_work():
     assert L.magnus_sample_fourier_f64(0, 9, 0.0, 1.0, 4, 2, dummy, dummy, None) == capi.OK  # empty batch
 
 
+def test_expm_argument_checks_before_device_work():
+    L = capi.lib()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation or is empty
+    assert L.magnus_expm_f64(2, 3, None, dummy, None) == capi.EINVAL
+    assert L.magnus_expm_f64(2, 0, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_sc_expm_f64(2, dummy, None, None) == capi.EINVAL
+    assert L.magnus_expm_f64(0, 3, dummy, dummy, None) == capi.OK
+
+
 def test_default_gl_table_is_numpy_leggauss_bitwise():
     L = capi.lib()
     for order in (1, 2, 3, 5, 17, 64):

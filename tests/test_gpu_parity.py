@@ -407,6 +407,75 @@ def test_device_sampling_matches_numpy_spacecurve_form(modes, S, t0, tf):
     assert rel_fro(out, ref_out) <= TOL
 
 
+@pytest.mark.parametrize("d", [1, 2, 3, 4, 8, 17, 32, 64])
+def test_expm_matches_scipy(d):
+    """Batched expm on every Pade degree (norms 1e-3 .. 40: m = 3, 5, 7, 9, 13 with
+    and without scaling) against scipy.linalg.expm; exp(0) = I exactly."""
+    import torch
+    from scipy.linalg import expm as sp_expm
+
+    from magnus import torch_ops
+
+    rng = np.random.default_rng([23, d])
+    norms = [1e-3, 0.1, 0.5, 1.5, 4.0, 40.0]
+    mats = []
+    for nrm in norms:
+        X = rng.standard_normal((d, d))
+        mats.append(X * (nrm / np.abs(X).sum(axis=0).max()))
+    mats.append(np.zeros((d, d)))
+    M = np.stack(mats)
+    got = torch_ops.expm(torch.from_numpy(M).cuda()).cpu().numpy()
+    for k in range(len(mats)):
+        ref = sp_expm(M[k])
+        tol = 1e-13 if k < 5 else 1e-11  # norm 40: 2^-3 scaling, error grows with the norm
+        assert rel_fro(got[k], ref) <= tol, (k, rel_fro(got[k], ref))
+    np.testing.assert_array_equal(got[-1], np.eye(d))
+
+
+def test_sc_expm_matches_scipy():
+    import torch
+    from scipy.linalg import expm as sp_expm
+
+    from magnus import torch_ops
+
+    sig = [np.array([[0, 1], [1, 0]], complex), np.array([[0, -1j], [1j, 0]]), np.array([[1, 0], [0, -1]], complex)]
+    rng = np.random.default_rng(29)
+    W = np.concatenate([rng.standard_normal((6, 3)) * s for s in (1e-9, 1e-5, 0.3, 1.0, 3.0, 10.0)] +
+                       [np.zeros((1, 3))])
+    got = torch_ops.sc_expm(torch.from_numpy(W).cuda()).cpu().numpy()
+    for b in range(W.shape[0]):
+        ref = sp_expm(-1j * sum(W[b, c] * sig[c] for c in range(3)))
+        assert np.max(np.abs(got[b] - ref)) <= 2e-15 * max(1.0, np.abs(W[b]).sum()), b
+
+
+def test_propagator_of_reference_bench_problem_converges_to_ode_solution():
+    """tests/bench.py of the reference: Y(t) = t X + (1 - t) Z on [0, 1], S = 4097;
+    the device propagator expm(sum Omega_k) equals scipy's expm of the oracle sum
+    and approaches the solve_ivp propagator as n grows."""
+    import torch
+    from scipy.integrate import solve_ivp
+    from scipy.linalg import expm as sp_expm
+
+    from magnus import torch_ops
+
+    X = np.array([[0.0, 1.0], [1.0, 0.0]])
+    Z = np.array([[1.0, 0.0], [0.0, -1.0]])
+    S = 4097
+    t = np.linspace(0.0, 1.0, S)
+    Ys = t[:, None, None] * X + (1 - t)[:, None, None] * Z
+    sol = solve_ivp(lambda tt, y: ((tt * X + (1 - tt) * Z) @ y.reshape(2, 2)).ravel(), (0.0, 1.0),
+                    np.eye(2).ravel(), t_eval=[1.0], rtol=1e-12, atol=1e-14)
+    ivp = sol.y[:, -1].reshape(2, 2)
+    errs = []
+    for n in (2, 4, 6):
+        om = torch_ops.forward(n, torch.from_numpy(Ys)[None].cuda(), 0.0, 1.0, op="sum")
+        U = torch_ops.expm(om).cpu().numpy()[0]
+        ref = sp_expm(O.forward(n, Ys, 0.0, 1.0, op="sum"))
+        assert rel_fro(U, ref) <= 1e-12
+        errs.append(np.max(np.abs(U - ivp)))
+    assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-4
+
+
 def test_batched_matrix_api_matches_per_instance():
     import torch
 

@@ -3,9 +3,12 @@
 Default workload (one GPU): the SpaceCurve batch -- 4096 independent 3-vector
 trajectories f(t) = sum_{m=1..4} a_m sin(2 pi m t + phi_m), a_m ~ N(0,1)/m,
 phi_m ~ U[0, 2 pi), S = 16385 samples on [0, 1], Simpson, compute_sc with
-n = 6, op = 'many', then compute_sc_vjp (no stored carry: the VJP re-derives
-its forward quantities) with a N(0,1) (6, 3) cotangent per trajectory.  One step = forward + VJP of the whole batch
-in batched launches; inputs are resident in HBM before timing starts.
+n = 6, op = 'many', then compute_sc_vjp with a N(0,1) (6, 3) cotangent per
+trajectory.  No reference-style carry is stored: the forward keeps its compact
+per-chunk checkpoint (11 4-vectors per 128-sample chunk, 184 MB for the batch)
+and the VJP starts from it (--sc-residual recompute: the VJP re-derives its
+forward quantities, the reference's VJP::none).  One step = forward + VJP of the
+whole batch in batched launches; inputs are resident in HBM before timing starts.
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 
@@ -183,6 +186,8 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="spacecurve-batch", choices=["spacecurve-batch", "dense-d32", "plain-d4"])
     ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
+    ap.add_argument("--sc-residual", default="checkpoint", choices=["checkpoint", "recompute"],
+                    help="spacecurve-batch: what the VJP starts from")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--cpu-budget", type=float, default=20.0)
     ap.add_argument("--copy-mib", type=int, default=0,
@@ -213,17 +218,26 @@ def main():
         B, S, n, modes = args.batch or 4096, 16385, 6, 4
         a, cot = _sc_inputs(torch, B, S, modes, dev)
 
-        def step():
-            # compute_sc (op='many') then compute_sc_vjp without a stored carry: the
-            # VJP re-derives its forward quantities (the reference's VJP::none path)
-            out = torch_ops.sc_forward(n, a, 0.0, 1.0, op="many", integrator="Simpson")
-            dA = torch_ops.sc_vjp(n, a, cot, None, 0.0, 1.0, op="many", integrator="Simpson")
-            return out, dA
+        if args.sc_residual == "checkpoint":
+            def step():
+                # compute_sc (op='many') keeping the per-chunk checkpoint, then
+                # compute_sc_vjp from it
+                out, ckpt = torch_ops.sc_forward_ckpt(n, a, 0.0, 1.0, op="many", integrator="Simpson")
+                dA = torch_ops.sc_vjp_ckpt(n, a, cot, ckpt, 0.0, 1.0, op="many", integrator="Simpson")
+                return out, dA
+        else:
+            def step():
+                # compute_sc (op='many') then compute_sc_vjp without a stored carry: the
+                # VJP re-derives its forward quantities (the reference's VJP::none path)
+                out = torch_ops.sc_forward(n, a, 0.0, 1.0, op="many", integrator="Simpson")
+                dA = torch_ops.sc_vjp(n, a, cot, None, 0.0, 1.0, op="many", integrator="Simpson")
+                return out, dA
 
         Q, K, m, m_in, P, n_out = (n + 3) // 2, n - 1, 4, 3, 20, n
         config = {"workload": "spacecurve-batch", "instances": B, "samples": S, "n": n, "integrator": "Simpson",
                   "op": "many", "vector": "su(2) 3-vector", "fourier_modes": modes, "cotangent": "N(0,1) (6,3)",
-                  "carry": "none (VJP recomputes its forward quantities)"}
+                  "carry": ("none; compact per-chunk checkpoint kept by the forward" if args.sc_residual == "checkpoint"
+                            else "none (VJP recomputes its forward quantities)")}
         do_vjp = True
     elif wl == "dense-d32":
         B, S, d, n, modes = args.batch or 256, 4097, 32, 4, 5

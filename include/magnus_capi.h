@@ -136,6 +136,21 @@ int magnus_sc_forward_f64(const magnus_problem* p, uint64_t count, const double*
 int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3,
                       const double* cot3, const double* carry, double* dA3, void* stream);
 
+/* ---- compact checkpoints of the single-pass SpaceCurve path (no counterpart in
+ *      the reference, whose saved state is the full carry above).  The forward
+ *      keeps its per-chunk summaries (2(n-1)+1 4-vectors per chunk of samples,
+ *      magnus_sc_checkpoint_elems doubles for `count` instances); a VJP given them
+ *      skips its forward sweep and returns the same dA as magnus_sc_vjp_f64 with
+ *      carry == NULL.  For the autograd front end (torch_ops.magnus_sc).  The
+ *      checkpoint is valid only with the same problem, count and input. ---- */
+
+/* 0 when the problem does not take the single-pass path (Trapezoid, n = 1, n > 8) */
+uint64_t magnus_sc_checkpoint_elems(const magnus_problem* p, uint64_t count);
+int magnus_sc_forward_ckpt_f64(const magnus_problem* p, uint64_t count, const double* in3, double* out3,
+                               double* ckpt, void* stream);
+int magnus_sc_vjp_ckpt_f64(const magnus_problem* p, uint64_t count, const double* in3, const double* cot3,
+                           const double* ckpt, double* dA3, void* stream);
+
 /* ---- sampling A(t) on the device, on the reference's grid
  *      t_i = np.linspace(t0, tf, samples) (replaces evaluating a Python callable
  *      there, src/api/magnus/__init__.py:58-85, for these two families).

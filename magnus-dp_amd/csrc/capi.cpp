@@ -316,8 +316,16 @@ int wb_chunk_len(const Plan& pl, uint64_t count) {
     return LC;
 }
 
+uint64_t wb_summary_elems(const Plan& pl, uint64_t count) {
+    const uint64_t LC = (uint64_t)wb_chunk_len(pl, count);
+    const uint64_t C = (pl.S - 1 + LC - 1) / LC;
+    return count * C * (2 * (pl.n - 1) + 1) * 4;
+}
+
+// Single-pass path.  `ckpt` (optional) holds the chunk summaries: the forward
+// writes them there instead of scratch; a VJP given them skips the chunk pass.
 int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, double* out, double* dA,
-           hipStream_t stream) {
+           double* ckpt, hipStream_t stream) {
     DeviceScratch scratch(stream);
     const bool vjp = dA != nullptr;
     magnus::WbArgs a{};
@@ -336,7 +344,7 @@ int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, 
     a.cot = cot;
     a.out = out;
     a.dA = dA;
-    a.fsum = scratch.alloc<double>(BC * (2 * K + 1) * 4);
+    a.fsum = ckpt ? ckpt : scratch.alloc<double>(BC * (2 * K + 1) * 4);
     if (!a.fsum) return fail(MAGNUS_ENOMEM, "device allocation failed");
     if (vjp) {
         a.src = scratch.alloc<double>(BC * (2 * K + 1) * 4);
@@ -344,13 +352,12 @@ int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, 
         a.edge = scratch.alloc<double>(BC * 6);
         if (!a.src || !a.bases || !a.edge) return fail(MAGNUS_ENOMEM, "device allocation failed");
     }
-    hipError_t e = magnus::launch_wb(pl.integ, a, vjp, stream);
+    hipError_t e = magnus::launch_wb(pl.integ, a, vjp, /*summaries_given=*/vjp && ckpt != nullptr, stream);
     if (e != hipSuccess) return device_error(e, "wb kernel launch");
     g_kernel = "wb";
     return MAGNUS_OK;
 }
 
-// forward: record into `carry` when non-null; outputs=false runs VJP::none
 void fill_lm(const Plan& pl, long long jobs, magnus::LmArgs& a) {
     a.S = (long long)pl.S;
     a.jobs = jobs;
@@ -577,7 +584,7 @@ int magnus_sc_forward_f64(const magnus_problem* p, uint64_t count, const double*
     if (!in3) return fail(MAGNUS_EINVAL, "input buffer must not be null");
     if (!out3) return fail(MAGNUS_EINVAL, "output buffer must not be null");
     double* c = pl.n >= 2 ? carry : nullptr;
-    if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, nullptr, out3, nullptr, (hipStream_t)stream);
+    if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, nullptr, out3, nullptr, nullptr, (hipStream_t)stream);
     return run_forward(pl, count, in3, out3, c, true, (hipStream_t)stream);
 }
 
@@ -590,8 +597,40 @@ int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3
     if (!dA3) return fail(MAGNUS_EINVAL, "output buffer must not be null");
     if (!cot3) return fail(MAGNUS_EINVAL, "cotangent buffer must not be null");
     const double* c = pl.n >= 2 ? carry : nullptr;
-    if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, cot3, nullptr, dA3, (hipStream_t)stream);
+    if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, cot3, nullptr, dA3, nullptr, (hipStream_t)stream);
     return run_vjp(pl, count, in3, cot3, c, dA3, (hipStream_t)stream);
+}
+
+uint64_t magnus_sc_checkpoint_elems(const magnus_problem* p, uint64_t count) {
+    Plan pl;
+    if (make_plan(p, true, true, &pl)) return 0;
+    if (!use_wb(pl, false)) return 0;
+    return wb_summary_elems(pl, count);
+}
+
+int magnus_sc_forward_ckpt_f64(const magnus_problem* p, uint64_t count, const double* in3, double* out3,
+                               double* ckpt, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, true, true, &pl)) return rc;
+    if (!use_wb(pl, false))
+        return fail(MAGNUS_EINVAL, "checkpointed forward needs the single-pass path (su(2) input, Boole/Simpson/Riemann, "
+                                   "2 <= n <= 8)");
+    if (count == 0) return MAGNUS_OK;
+    if (!in3 || !out3 || !ckpt) return fail(MAGNUS_EINVAL, "null argument");
+    return run_wb(pl, count, in3, nullptr, out3, nullptr, ckpt, (hipStream_t)stream);
+}
+
+int magnus_sc_vjp_ckpt_f64(const magnus_problem* p, uint64_t count, const double* in3, const double* cot3,
+                           const double* ckpt, double* dA3, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, true, false, &pl)) return rc;
+    if (!use_wb(pl, false))
+        return fail(MAGNUS_EINVAL, "checkpointed VJP needs the single-pass path (su(2) input, Boole/Simpson/Riemann, "
+                                   "2 <= n <= 8)");
+    if (count == 0) return MAGNUS_OK;
+    if (!in3 || !cot3 || !ckpt || !dA3) return fail(MAGNUS_EINVAL, "null argument");
+    // the summaries are only read by the kernels of this call
+    return run_wb(pl, count, in3, cot3, nullptr, dA3, const_cast<double*>(ckpt), (hipStream_t)stream);
 }
 
 int magnus_sample_fourier_f64(uint64_t count, uint64_t samples, double t0, double tf, uint64_t entries,

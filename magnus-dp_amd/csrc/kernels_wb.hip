@@ -871,14 +871,17 @@ __global__ void __launch_bounds__(256) wb_fixup(WbArgs a) {
 }
 
 template <int I, int K>
-hipError_t launch_wb_k(const WbArgs& a, bool vjp, hipStream_t st) {
+hipError_t launch_wb_k(const WbArgs& a, bool vjp, bool summaries_given, hipStream_t st) {
     const long long threads = a.B * a.C;
     const int blk = 256;
     const long long grid = (threads + blk - 1) / blk;
     const long long cgrid = (a.B + 3) / 4;  // one wavefront per instance
-    void* h = timer_begin(vjp ? "wb_fwd_pass1_vjp" : "wb_fwd_pass1", st);
-    hipLaunchKernelGGL((wb_fwd_pass1<I, K>), dim3(grid), dim3(blk), 0, st, a);
-    timer_end(h);
+    void* h;
+    if (!summaries_given) {
+        h = timer_begin(vjp ? "wb_fwd_pass1_vjp" : "wb_fwd_pass1", st);
+        hipLaunchKernelGGL((wb_fwd_pass1<I, K>), dim3(grid), dim3(blk), 0, st, a);
+        timer_end(h);
+    }
     if (!vjp) {
         h = timer_begin("wb_combine", st);
         hipLaunchKernelGGL((wb_combine<K, false>), dim3(cgrid), dim3(256), 0, st, a);
@@ -898,26 +901,26 @@ hipError_t launch_wb_k(const WbArgs& a, bool vjp, hipStream_t st) {
 }
 
 template <int I>
-hipError_t launch_wb_i(const WbArgs& a, bool vjp, hipStream_t st) {
+hipError_t launch_wb_i(const WbArgs& a, bool vjp, bool sg, hipStream_t st) {
     switch (a.K) {
-        case 1: return launch_wb_k<I, 1>(a, vjp, st);
-        case 2: return launch_wb_k<I, 2>(a, vjp, st);
-        case 3: return launch_wb_k<I, 3>(a, vjp, st);
-        case 4: return launch_wb_k<I, 4>(a, vjp, st);
-        case 5: return launch_wb_k<I, 5>(a, vjp, st);
-        case 6: return launch_wb_k<I, 6>(a, vjp, st);
-        case 7: return launch_wb_k<I, 7>(a, vjp, st);
+        case 1: return launch_wb_k<I, 1>(a, vjp, sg, st);
+        case 2: return launch_wb_k<I, 2>(a, vjp, sg, st);
+        case 3: return launch_wb_k<I, 3>(a, vjp, sg, st);
+        case 4: return launch_wb_k<I, 4>(a, vjp, sg, st);
+        case 5: return launch_wb_k<I, 5>(a, vjp, sg, st);
+        case 6: return launch_wb_k<I, 6>(a, vjp, sg, st);
+        case 7: return launch_wb_k<I, 7>(a, vjp, sg, st);
         default: return hipErrorInvalidValue;
     }
 }
 
 }  // namespace
 
-hipError_t launch_wb(int integ, const WbArgs& a, bool vjp, hipStream_t st) {
+hipError_t launch_wb(int integ, const WbArgs& a, bool vjp, bool summaries_given, hipStream_t st) {
     switch (integ) {
-        case BOOLE: return launch_wb_i<BOOLE>(a, vjp, st);
-        case SIMPSON: return launch_wb_i<SIMPSON>(a, vjp, st);
-        case RIEMANN: return launch_wb_i<RIEMANN>(a, vjp, st);
+        case BOOLE: return launch_wb_i<BOOLE>(a, vjp, summaries_given, st);
+        case SIMPSON: return launch_wb_i<SIMPSON>(a, vjp, summaries_given, st);
+        case RIEMANN: return launch_wb_i<RIEMANN>(a, vjp, summaries_given, st);
         default: return hipErrorInvalidValue;
     }
 }

@@ -30,7 +30,8 @@ struct WbArgs {
 };
 
 // forward only (vjp == false): chunk pass + combine -> out.
-// vjp == true: chunk pass + combine (cotangents per chunk) + reverse chunk pass + fixup -> dA.
-hipError_t launch_wb(int integ, const WbArgs& a, bool vjp, hipStream_t st);
+// vjp == true: chunk pass + combine (cotangents per chunk) + reverse chunk pass + fixup -> dA;
+// summaries_given: fsum already holds the chunk summaries of this input (no chunk pass).
+hipError_t launch_wb(int integ, const WbArgs& a, bool vjp, bool summaries_given, hipStream_t st);
 
 }  // namespace magnus

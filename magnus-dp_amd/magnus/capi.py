@@ -53,6 +53,10 @@ def lib() -> ctypes.CDLL:
             getattr(L, name).argtypes = [P, u64, vp, vp, vp, vp]
         for name in ("magnus_vjp_f64", "magnus_sc_vjp_f64"):
             getattr(L, name).argtypes = [P, u64, vp, vp, vp, vp, vp]
+        L.magnus_sc_forward_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp]
+        L.magnus_sc_vjp_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp, vp]
+        L.magnus_sc_checkpoint_elems.argtypes = [P, u64]
+        L.magnus_sc_checkpoint_elems.restype = u64
         for name in ("magnus_carry_elems", "magnus_output_elems", "magnus_sc_carry_elems", "magnus_sc_output_elems"):
             getattr(L, name).argtypes = [P]
             getattr(L, name).restype = u64

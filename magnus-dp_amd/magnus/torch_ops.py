@@ -5,7 +5,8 @@ and work is enqueued on torch's current HIP stream.  `MagnusFunction` /
 `SpaceCurveFunction` are torch.autograd.Functions whose backward is the HIP
 reverse sweep, the counterpart of the reference's JAX custom_vjp
 (src/api/magnus/jax.py:209-342): the forward residual is the input plus,
-when recorded, the carry.
+when recorded, the carry; without it, SpaceCurve problems on the single-pass
+path keep a compact per-chunk checkpoint instead of recomputing.
 
 Shapes (B = batch of independent instances):
   matrix:     A (B, S, d, d) -> out (B, d, d) or (B, n, d, d)
@@ -105,6 +106,49 @@ def sc_vjp(n: int, a: torch.Tensor, cot: torch.Tensor, carry: torch.Tensor | Non
     return da
 
 
+def sc_checkpoint_elems(n: int, a: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
+                        integrator: str = "Auto") -> int:
+    """Doubles in the compact checkpoint of sc_forward_ckpt for input a (B, S, 3);
+    0 when the problem does not take the single-pass path."""
+    B, S, _ = a.shape
+    p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
+    return int(capi.lib().magnus_sc_checkpoint_elems(ctypes.byref(p), B))
+
+
+def sc_forward_ckpt(n: int, a: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
+                    integrator: str = "Auto"):
+    """sc_forward that also returns the per-chunk summaries of the single-pass
+    path (a flat float64 tensor), from which sc_vjp_ckpt starts its reverse sweep."""
+    _check_input(a, 3, "a")
+    a = a.contiguous()
+    B, S, w = a.shape
+    if w != 3:
+        raise ValueError("data must have shape (samples, 3)")
+    p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
+    elems = int(capi.lib().magnus_sc_checkpoint_elems(ctypes.byref(p), B))
+    out = torch.empty((B, n, 3) if op == "many" else (B, 3), dtype=a.dtype, device=a.device)
+    ckpt = torch.empty((max(elems, 1),), dtype=a.dtype, device=a.device)
+    capi.check(capi.lib().magnus_sc_forward_ckpt_f64(ctypes.byref(p), B, _ptr(a), _ptr(out), _ptr(ckpt), _stream()))
+    return out, ckpt
+
+
+def sc_vjp_ckpt(n: int, a: torch.Tensor, cot: torch.Tensor, ckpt: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *,
+                op: str = "sum", integrator: str = "Auto"):
+    """sc_vjp from the summaries sc_forward_ckpt returned for the same a."""
+    _check_input(a, 3, "a")
+    a = a.contiguous()
+    cot = cot.contiguous().to(torch.float64)
+    B, S, _ = a.shape
+    p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
+    elems = int(capi.lib().magnus_sc_checkpoint_elems(ctypes.byref(p), B))
+    if not ckpt.is_cuda or ckpt.dtype != torch.float64 or ckpt.numel() < elems or not ckpt.is_contiguous():
+        raise ValueError(f"ckpt must be a contiguous float64 GPU tensor of at least {elems} elements")
+    da = torch.empty_like(a)
+    capi.check(capi.lib().magnus_sc_vjp_ckpt_f64(ctypes.byref(p), B, _ptr(a), _ptr(cot), _ptr(ckpt), _ptr(da),
+                                                 _stream()))
+    return da
+
+
 class MagnusFunction(torch.autograd.Function):
     """out = Magnus(A); backward = HIP reverse sweep.  record_vjp keeps the carry
     as the residual; otherwise the backward recomputes it (VJP::none)."""
@@ -129,21 +173,34 @@ class MagnusFunction(torch.autograd.Function):
 
 
 class SpaceCurveFunction(torch.autograd.Function):
+    """out = Magnus(a) for su(2) inputs.  record_vjp keeps the reference's carry;
+    otherwise, where the single-pass path applies, the residual is its compact
+    per-chunk checkpoint (2n-1 4-vectors per chunk of samples) and the backward
+    skips the forward sweep; elsewhere the backward recomputes (VJP::none)."""
+
     @staticmethod
     def forward(ctx, a, n, t0, tf, op, integrator, record_vjp):
+        mode = "none"
         if record_vjp:
-            out, carry = sc_forward(n, a, t0, tf, op=op, integrator=integrator, record_vjp=True)
+            out, res = sc_forward(n, a, t0, tf, op=op, integrator=integrator, record_vjp=True)
+            mode = "carry" if n > 1 else "none"
+        elif a.is_cuda and sc_checkpoint_elems(n, a, t0, tf, op=op, integrator=integrator) > 0:
+            out, res = sc_forward_ckpt(n, a, t0, tf, op=op, integrator=integrator)
+            mode = "ckpt"
         else:
-            out, carry = sc_forward(n, a, t0, tf, op=op, integrator=integrator), None
-        ctx.save_for_backward(a, carry if carry is not None else torch.empty(0, device=a.device))
-        ctx.cfg = (n, t0, tf, op, integrator, carry is not None)
+            out, res = sc_forward(n, a, t0, tf, op=op, integrator=integrator), None
+        ctx.save_for_backward(a, res if mode != "none" else torch.empty(0, device=a.device))
+        ctx.cfg = (n, t0, tf, op, integrator, mode)
         return out
 
     @staticmethod
     def backward(ctx, g):
-        a, carry = ctx.saved_tensors
-        n, t0, tf, op, integ, has_carry = ctx.cfg
-        da = sc_vjp(n, a, g, carry if has_carry else None, t0, tf, op=op, integrator=integ)
+        a, res = ctx.saved_tensors
+        n, t0, tf, op, integ, mode = ctx.cfg
+        if mode == "ckpt":
+            da = sc_vjp_ckpt(n, a, g, res, t0, tf, op=op, integrator=integ)
+        else:
+            da = sc_vjp(n, a, g, res if mode == "carry" else None, t0, tf, op=op, integrator=integ)
         return da, None, None, None, None, None, None
 
 

@@ -101,6 +101,27 @@ def test_sampling_argument_checks_before_device_work():
     assert L.magnus_sample_fourier_f64(0, 9, 0.0, 1.0, 4, 2, dummy, dummy, None) == capi.OK  # empty batch
 
 
+def test_spacecurve_checkpoint_size_and_argument_checks():
+    """Compact checkpoint: 2(n-1)+1 4-vectors per chunk of samples on the
+    single-pass path; none (size 0, EINVAL) elsewhere.  Host-only calls."""
+    L = capi.lib()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation or is empty
+    p = capi.problem(3, 2, 1025, 0.0, 1.0, op="sum", matrix_backend="SpaceCurve", integrator="Simpson")
+    # one instance: 16-sample chunks -> 64 chunks of 5 4-vectors
+    assert L.magnus_sc_checkpoint_elems(ctypes.byref(p), 1) == 64 * 5 * 4
+    # headline batch shape (4096 x 16385 samples, n = 6): 128-sample chunks, 184 MB
+    ph = capi.problem(6, 2, 16385, 0.0, 1.0, op="sum", matrix_backend="SpaceCurve", integrator="Boole")
+    assert L.magnus_sc_checkpoint_elems(ctypes.byref(ph), 4096) * 8 == 4096 * 128 * 11 * 4 * 8
+    for n, integ in [(3, "Trapezoid"), (1, "Simpson"), (9, "Boole")]:
+        q = capi.problem(n, 2, 1025, 0.0, 1.0, matrix_backend="SpaceCurve", integrator=integ)
+        assert L.magnus_sc_checkpoint_elems(ctypes.byref(q), 2) == 0
+        assert L.magnus_sc_forward_ckpt_f64(ctypes.byref(q), 2, dummy, dummy, dummy, None) == capi.EINVAL
+        assert L.magnus_sc_vjp_ckpt_f64(ctypes.byref(q), 2, dummy, dummy, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_sc_forward_ckpt_f64(ctypes.byref(p), 2, dummy, dummy, None, None) == capi.EINVAL
+    assert L.magnus_sc_vjp_ckpt_f64(ctypes.byref(p), 2, dummy, None, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_sc_forward_ckpt_f64(ctypes.byref(p), 0, None, None, None, None) == capi.OK  # empty batch
+
+
 def test_expm_argument_checks_before_device_work():
     L = capi.lib()
     dummy = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation or is empty

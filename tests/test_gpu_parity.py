@@ -502,3 +502,72 @@ def test_autograd_function_matches_vjp():
     for b in range(B):
         ref = O.vjp(n, A[b].detach().cpu().numpy(), w[b].cpu().numpy(), op="many")
         assert rel_fro(A.grad[b].cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("integrator", ["Boole", "Simpson", "Riemann"])
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [2, 5, 8])
+def test_spacecurve_checkpoint_matches_recompute_bitwise(integrator, op, n):
+    """The VJP that starts from the forward's compact per-chunk checkpoint runs
+    the same kernels on the same summaries as the recompute VJP, so its outputs
+    are bit-identical; two calls from one checkpoint agree too (the VJP does not
+    write the checkpoint)."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    B = 4
+    S = _samples_for(integrator, 2049)
+    a = np.stack([fourier_spacecurve(np.random.default_rng([13, b, n]), S) for b in range(B)])
+    out_shape = (n, 3) if op == "many" else (3,)
+    cot = torch.from_numpy(np.stack([np.random.default_rng([13, b, n, 1]).standard_normal(out_shape)
+                                     for b in range(B)])).cuda()
+    at = torch.from_numpy(a).cuda()
+    out_ref = torch_ops.sc_forward(n, at, 0.0, 1.0, op=op, integrator=integrator)
+    dA_ref = torch_ops.sc_vjp(n, at, cot, None, 0.0, 1.0, op=op, integrator=integrator)
+    elems = torch_ops.sc_checkpoint_elems(n, at, 0.0, 1.0, op=op, integrator=integrator)
+    out, ckpt = torch_ops.sc_forward_ckpt(n, at, 0.0, 1.0, op=op, integrator=integrator)
+    assert capi.last_kernel() == "wb"
+    assert 0 < elems == ckpt.numel()
+    dA1 = torch_ops.sc_vjp_ckpt(n, at, cot, ckpt, 0.0, 1.0, op=op, integrator=integrator)
+    dA2 = torch_ops.sc_vjp_ckpt(n, at, cot, ckpt, 0.0, 1.0, op=op, integrator=integrator)
+    assert torch.equal(out, out_ref)
+    assert torch.equal(dA1, dA_ref) and torch.equal(dA2, dA_ref)
+    ref_g = O.sc_vjp(n, a[0], cot[0].cpu().numpy(), 0.0, 1.0, op=op, integrator=integrator)
+    assert rel_fro(dA1[0].cpu().numpy(), ref_g) <= TOL
+
+
+def test_spacecurve_checkpoint_rejects_other_paths():
+    """Problems off the single-pass path (Trapezoid, n = 1, n > 8) have no compact
+    checkpoint: the size query returns 0 and the entry points return EINVAL."""
+    import torch
+
+    from magnus import torch_ops
+
+    a = torch.from_numpy(np.stack([fourier_spacecurve(np.random.default_rng([17]), 65)])).cuda()
+    for n, integ in [(3, "Trapezoid"), (1, "Simpson"), (9, "Boole")]:
+        assert torch_ops.sc_checkpoint_elems(n, a, integrator=integ) == 0
+        with pytest.raises(Exception):
+            torch_ops.sc_forward_ckpt(n, a, integrator=integ)
+
+
+@pytest.mark.parametrize("integrator", ["Boole", "Trapezoid"])
+def test_spacecurve_autograd_without_carry_matches_oracle(integrator):
+    """magnus_sc(record_vjp=False): the compact checkpoint (Boole) or the
+    recompute path (Trapezoid) as the residual, gradients against the oracle."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, n = 3, 1025, 6
+    a = torch.from_numpy(np.stack([fourier_spacecurve(np.random.default_rng([19, b]), S) for b in range(B)])).cuda()
+    a.requires_grad_(True)
+    out = torch_ops.magnus_sc(a, n, 0.0, 1.0, op="many", integrator=integrator, record_vjp=False)
+    w = torch.from_numpy(gradient_values(tuple(out.shape), (0.3, 0.9))).cuda()
+    (out * w).sum().backward()
+    for b in range(B):
+        ab = a[b].detach().cpu().numpy()
+        assert rel_fro(out[b].detach().cpu().numpy(), O.sc_forward(n, ab, 0.0, 1.0, op="many",
+                                                                  integrator=integrator)) <= TOL
+        ref = O.sc_vjp(n, ab, w[b].cpu().numpy(), 0.0, 1.0, op="many", integrator=integrator)
+        assert rel_fro(a.grad[b].cpu().numpy(), ref) <= TOL

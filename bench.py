@@ -179,12 +179,220 @@ def _cpu_baseline_sc(a_host, cot_host, n, S, budget_s):
     }
 
 
+UNEVEN_DIMS = (2, 4, 8, 16, 32, 64)
+UNEVEN_SAMPLES = (257, 1025, 4097, 8193)
+
+
+def _uneven_problems(count, seed=4):
+    """Uneven batch: d ~ U{2..64}, samples ~ U{257..8193}, n ~ U{2..6}, Boole, op='sum'."""
+    import numpy as np
+
+    rng = np.random.default_rng([seed, 0])
+    return [dict(n=int(rng.integers(2, 7)), dim=int(rng.choice(UNEVEN_DIMS)), samples=int(rng.choice(UNEVEN_SAMPLES)),
+                 t0=0.0, tf=1.0, op="sum", integrator="Boole") for _ in range(count)]
+
+
+def _uneven_inputs(torch, probs, device, modes=3, seed=4):
+    """Fourier-form A(t) (3 modes, entries N(0,1)/sqrt(d)) rescaled so that the
+    largest ||A(t)||_2 on a 65-point grid times T is U[0.1, 3]; sampled on the
+    device per (dim, samples) group; cotangents N(0,1) (d, d)."""
+    import numpy as np
+
+    from magnus import torch_ops
+
+    tg = np.linspace(0.0, 1.0, 65)
+    k = np.arange(1, modes + 1)
+    basis = np.concatenate([np.ones((65, 1)), np.stack([np.sin(2 * np.pi * k[None, :] * tg[:, None]),
+                                                       np.cos(2 * np.pi * k[None, :] * tg[:, None])],
+                                                      axis=2).reshape(65, 2 * modes)], axis=1)
+    coeffs, cots = [], []
+    for i, p in enumerate(probs):
+        rng = np.random.default_rng([seed, i])
+        d = p["dim"]
+        c = rng.standard_normal((2 * modes + 1, d, d)) / np.sqrt(d)
+        grid = np.einsum("tk,kij->tij", basis, c)
+        target = rng.uniform(0.1, 3.0)
+        c *= target / (np.linalg.norm(grid, 2, axis=(1, 2)).max() * (p["tf"] - p["t0"]))
+        coeffs.append(c)
+        cots.append(np.random.default_rng([seed, i, 1]).standard_normal((d, d)))
+    A = [None] * len(probs)
+    groups = {}
+    for i, p in enumerate(probs):
+        groups.setdefault((p["dim"], p["samples"]), []).append(i)
+    for (d, S), ids in groups.items():
+        out = torch_ops.sample_fourier(torch.from_numpy(np.stack([coeffs[i] for i in ids])).to(device), S)
+        for j, i in enumerate(ids):
+            A[i] = out[j]
+    return A, [torch.from_numpy(c).to(device) for c in cots]
+
+
+def _uneven_flops(p):
+    """Algorithmic flops of forward (outputs) + VJP with recompute for one instance."""
+    n, S, d = p["n"], p["samples"], p["dim"]
+    Q, K, m, P = (n + 3) // 2, n - 1, d * d, 2 * d ** 3
+    F_fwd, F_vjp, _, _, _ = _counts(Q, K, S, m, m, P, 1)
+    F_rec = Q * K * S * (P + 6 * m)
+    return F_fwd + F_rec + F_vjp
+
+
+def _cpu_baseline_uneven(probs, A, cot, budget_s):
+    """Oracle on host cores over a time-bounded sample of the uneven batch,
+    extrapolated to the whole batch by the flop model; the faster of
+    reference-faithful (instances in sequence, Auto GL lanes) and all-cores
+    (one instance per core)."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    cores = min(len(os.sched_getaffinity(0)), 16)
+    cost = [_uneven_flops(p) for p in probs]
+    total_cost = sum(cost)
+    order = sorted(range(len(probs)), key=lambda i: cost[i])  # cheapest first, so the sample is bounded
+
+    def solve(i, lanes):
+        p = probs[i]
+        a = A[i].cpu().numpy()
+        c = cot[i].cpu().numpy()
+        O.forward(p["n"], a, 0.0, 1.0, op="sum", integrator="Boole", lanes=lanes)
+        O.vjp(p["n"], a, c, 0.0, 1.0, op="sum", integrator="Boole")
+
+    results = {}
+    for mode in ("reference-faithful", "all-cores"):
+        budget = budget_s / 2
+        stride = max(1, len(order) // 64)
+        sample = order[::stride]
+        t0 = time.perf_counter()
+        done_cost, done = 0.0, 0
+        if mode == "reference-faithful":
+            for i in sample:
+                if time.perf_counter() - t0 > budget:
+                    break
+                solve(i, O.auto_lanes(probs[i]["n"], probs[i]["samples"], cores))
+                done_cost += cost[i]
+                done += 1
+        else:
+            with cf.ThreadPoolExecutor(cores) as ex:
+                for w in range(0, len(sample), cores):
+                    if time.perf_counter() - t0 > budget:
+                        break
+                    part = sample[w:w + cores]
+                    list(ex.map(lambda i: solve(i, 1), part))
+                    done_cost += sum(cost[i] for i in part)
+                    done += len(part)
+        el = time.perf_counter() - t0
+        est_total = el * total_cost / done_cost if done_cost else float("inf")
+        results[mode] = (len(probs) / est_total, done, el)
+    best = max(results, key=lambda k: results[k][0])
+    v, done, el = results[best]
+    return {
+        "value": v,
+        "unit": "solves/s",
+        "cores": cores if best == "all-cores" else min(cores, 4),
+        "kind": "port",
+        "sample": f"{done} forward+VJP solves of the batch (every {max(1, len(order) // 64)}th instance in "
+                  f"order of cost, cheapest first) in {el:.1f} s, extrapolated to the {len(probs)} instances "
+                  f"by algorithmic flops; mode={best}; other mode "
+                  f"{[(k, round(r[0], 3)) for k, r in results.items() if k != best]}",
+    }
+
+
+def _bench_uneven(args, torch, dev):
+    from magnus import capi, torch_ops
+    from magnus.uneven import UnevenBatch
+
+    count = args.batch or 2048
+    probs = _uneven_problems(count)
+    print(f"[bench] uneven: generating {count} instances", file=sys.stderr, flush=True)
+    A, cot = _uneven_inputs(torch, probs, dev)
+    torch.cuda.synchronize()
+    ws = int(args.uneven_workspace_gb * (1 << 30))
+    plan = UnevenBatch(probs, workspace_bytes=ws, branches=args.uneven_branches)
+    t = time.perf_counter()
+    plan.autotune(A, cot)
+    torch.cuda.synchronize()
+    tune_s = time.perf_counter() - t
+    print(f"[bench] uneven: autotune {tune_s:.1f} s, plan {plan.info()}", file=sys.stderr, flush=True)
+
+    def step(graph):
+        outs = plan.forward(A, use_graph=graph)
+        dAs = plan.vjp(A, cot, use_graph=graph)
+        return outs, dAs
+
+    timing = {}
+    for graph in (False, True):
+        for _ in range(args.warmup):
+            step(graph)
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(args.steps):
+            outs, dAs = step(graph)
+        torch.cuda.synchronize()
+        timing[graph] = (time.perf_counter() - t) / args.steps * 1e3
+        print(f"[bench] uneven: {'graph' if graph else 'eager'} {timing[graph]:.2f} ms/step", file=sys.stderr,
+              flush=True)
+
+    # per-instance sequential launches of the one-problem entry points
+    torch.cuda.synchronize()
+    t = time.perf_counter()
+    seq_out = []
+    for i, p in enumerate(probs):
+        seq_out.append(torch_ops.forward(p["n"], A[i][None], 0.0, 1.0, op="sum", integrator="Boole")[0])
+        dA_i = torch_ops.vjp(p["n"], A[i][None], cot[i][None], None, 0.0, 1.0, op="sum", integrator="Boole")[0]
+        if i < 16:
+            assert torch.equal(dA_i, dAs[i]), f"uneven dA differs from the per-instance call at {i}"
+    torch.cuda.synchronize()
+    seq_ms = (time.perf_counter() - t) * 1e3
+    for i in range(count):
+        assert torch.equal(seq_out[i], outs[i]), f"uneven output differs from the per-instance call at {i}"
+    print(f"[bench] uneven: per-instance sequential {seq_ms:.1f} ms", file=sys.stderr, flush=True)
+
+    flops = sum(_uneven_flops(p) for p in probs)
+    ms = timing[True]
+    cpu = None if args.no_cpu else _cpu_baseline_uneven(probs, A, cot, args.cpu_budget)
+    ach = flops / (ms / 1e3) / 1e12
+    dims = {d: sum(1 for p in probs if p["dim"] == d) for d in UNEVEN_DIMS}
+    line = {
+        "metric": "Magnus fwd+VJP solves/s (uneven batch)",
+        "value": round(count / (ms / 1e3), 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(ms, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Fourier-form A(t), norm-rescaled, sampled on the device by the library)",
+        "config": {"workload": "uneven-batch", "instances": count, "dims": dims, "samples": list(UNEVEN_SAMPLES),
+                   "n": "2..6", "integrator": "Boole", "op": "sum", "vjp": "no carry (recompute)",
+                   "branches": args.uneven_branches, "plan": plan.info(), "graph": True},
+        "roofline": {"bound": "fp64", "achieved": round(ach, 3), "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                     "frac": round(ach / MI355X_FP64_TFLOPS, 4), "traffic": None, "kernel": "whole step",
+                     "algorithmic_flops_per_step": flops},
+        "cpu_baseline": cpu,
+        "per_instance_sequential": {"ms": round(seq_ms, 2), "solves_per_s": round(count / (seq_ms / 1e3), 3)},
+        "eager_bucketed": {"ms": round(timing[False], 3)},
+        "graph_vs_sequential": round(seq_ms / ms, 3),
+        "autotune_s": round(tune_s, 2),
+        "tuning": plan.tuning.strip().splitlines(),
+    }
+    plan.close()
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
-    ap.add_argument("--workload", default="spacecurve-batch", choices=["spacecurve-batch", "dense-d32", "plain-d4"])
+    ap.add_argument("--workload", default="spacecurve-batch",
+                    choices=["spacecurve-batch", "dense-d32", "plain-d4", "uneven"])
+    ap.add_argument("--uneven-branches", type=int, default=4)
+    ap.add_argument("--uneven-workspace-gb", type=float, default=48.0)
     ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
     ap.add_argument("--sc-residual", default="checkpoint", choices=["checkpoint", "recompute"],
                     help="spacecurve-batch: what the VJP starts from")
@@ -214,6 +422,9 @@ def main():
         print(json.dumps({"copy_bytes_read": x.numel() * 8, "copy_bytes_written": y.numel() * 8}))
         return
     wl = args.workload
+    if wl == "uneven":
+        _bench_uneven(args, torch, dev)
+        return
     if wl == "spacecurve-batch":
         B, S, n, modes = args.batch or 4096, 16385, 6, 4
         a, cot = _sc_inputs(torch, B, S, modes, dev)

@@ -179,6 +179,51 @@ int magnus_expm_f64(uint64_t count, uint64_t dim, const double* in, double* out,
  * as a complex 2x2 matrix, [count][2][2][re, im]. */
 int magnus_sc_expm_f64(uint64_t count, const double* in3, double* out, void* stream);
 
+/* ---- uneven batches (no counterpart in the reference, which solves one problem
+ *      per call).  A plan holds `count` matrix problems of any dim, samples,
+ *      order, op and integrator.  It groups them into buckets of equal shape,
+ *      splits each bucket into sub-batches that fit its workspace, spreads the
+ *      sub-batches over `branches` HIP streams (longest first), and enqueues,
+ *      per sub-batch, a gather of the instances' buffers, the level-parallel
+ *      kernels and a scatter of the results.  With use_graph != 0 the whole
+ *      sequence is captured once into a hipGraph and each call is one graph
+ *      launch.  Buffers are per-instance DEVICE pointers passed as HOST arrays
+ *      of `count` pointers (layouts as above with count = 1).  Results are the
+ *      same as per-instance magnus_forward_f64 / magnus_vjp_f64 calls (carry ==
+ *      NULL) with the plan's product path per bucket.  Calls on one plan must
+ *      not overlap (one stream). ---- */
+typedef struct magnus_uneven_plan magnus_uneven_plan;
+
+/* workspace_bytes: device scratch for all branches (0: half of the free device
+ * memory); branches: 1..8 concurrent streams.  EINVAL names the first invalid
+ * problem ("instance i: ..."); ENOMEM if one instance does not fit. */
+int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t workspace_bytes, int branches,
+                         magnus_uneven_plan** out);
+void magnus_uneven_destroy(magnus_uneven_plan* plan);
+int magnus_uneven_forward_f64(magnus_uneven_plan* plan, const double* const* in, double* const* out,
+                              int use_graph, void* stream);
+int magnus_uneven_vjp_f64(magnus_uneven_plan* plan, const double* const* in, const double* const* cot,
+                          double* const* dA, int use_graph, void* stream);
+
+/* Autotuner: for every bucket, times the VJP (forward recompute + reverse
+ * sweep) of its first sub-batch under each product path that applies to its
+ * dim (VALU; FP64 MFMA with 1, 2 or 4 wavefronts per product) on the given
+ * buffers, keeps the fastest, and rebalances the branches with the measured
+ * times.  Synchronizes the stream. */
+int magnus_uneven_autotune(magnus_uneven_plan* plan, const double* const* in, const double* const* cot,
+                           double* const* dA, void* stream);
+
+/* Tuning table as text, one bucket per line: "dim samples n tile ms_per_instance".
+ * magnus_uneven_tuning writes at most `len` bytes (NUL-terminated) and returns
+ * the full length; magnus_uneven_set_tuning applies the lines whose
+ * (dim, samples, n) match a bucket (tile: 0 default, 1 VALU, 2 / 3 MFMA with
+ * 2 / 4 wavefronts per product at dim 32). */
+uint64_t magnus_uneven_tuning(const magnus_uneven_plan* plan, char* buf, uint64_t len);
+int magnus_uneven_set_tuning(magnus_uneven_plan* plan, const char* text);
+
+/* plan facts: 0 buckets, 1 sub-batches, 2 branches in use, 3 workspace bytes */
+uint64_t magnus_uneven_info(const magnus_uneven_plan* plan, int what);
+
 /* ---- kernel timing (diagnostics).  While enabled, every main kernel launch
  *      is bracketed by a HIP event pair recorded on its launch stream.
  *      magnus_timing_read waits for the recorded events and returns, for the

@@ -35,8 +35,10 @@ COMMON = [
     f"-I{INCLUDE}", f"-I{CSRC}",
 ]
 
-DEVICE_SOURCES = ["kernels_lm.hip", "kernels_wb.hip", "kernels_sample.hip", "kernels_expm.hip"]
-HOST_SOURCES = ["capi.cpp"]
+DEVICE_SOURCES = [
+    "kernels_lm.hip", "kernels_wb.hip", "kernels_sample.hip", "kernels_expm.hip", "kernels_uneven.hip",
+]
+HOST_SOURCES = ["capi.cpp", "uneven.cpp"]
 
 
 def _run(cmd):

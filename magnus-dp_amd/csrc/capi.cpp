@@ -19,12 +19,13 @@
 #include <string>
 #include <vector>
 
+#include "host.hpp"
 #include "kernels.hpp"
 #include "kernels_expm.hpp"
 #include "kernels_sample.hpp"
 #include "kernels_wb.hpp"
 
-namespace {
+namespace magnus_host {
 
 #include "gl_table_data.inc"
 
@@ -35,6 +36,8 @@ int fail(int code, const std::string& msg) {
     g_err = msg;
     return code;
 }
+
+void set_last_kernel(const char* name) { g_kernel = name; }
 
 // ---------------------------------------------------------------------------
 // registries
@@ -72,11 +75,6 @@ int resolve(const char* name, const char* const* names, int count, bool auto_ena
 // ---------------------------------------------------------------------------
 // Gauss-Legendre table
 // ---------------------------------------------------------------------------
-struct GLTable {
-    uint64_t max_order;
-    std::vector<double> w, x;  // flattened by order, order o at o(o-1)/2
-};
-
 std::mutex g_gl_mu;
 std::shared_ptr<const GLTable> g_gl;
 
@@ -98,14 +96,6 @@ size_t gl_offset(uint64_t order) { return (size_t)(order - 1) * order / 2; }
 // ---------------------------------------------------------------------------
 // validation
 // ---------------------------------------------------------------------------
-struct Plan {
-    uint64_t n, S, d, m, m_in, m_cot, Q, Qj, n_out;
-    int op, mat, integ;
-    bool sc, pad;
-    double dt;
-    std::shared_ptr<const GLTable> gl;
-};
-
 int divisibility(int integ) {  // boole.hpp:18, simpson.hpp:18, trapezoid.hpp:18, default.hpp:15
     switch (integ) {
         case MAGNUS_INT_BOOLE: return 4;
@@ -180,27 +170,6 @@ int make_plan(const magnus_problem* p, bool spacecurve, bool forward, Plan* out)
     return MAGNUS_OK;
 }
 
-// stream-ordered scratch allocation, freed on scope exit on the same stream
-struct DeviceScratch {
-    hipStream_t stream;
-    std::vector<void*> ptrs;
-    explicit DeviceScratch(hipStream_t s) : stream(s) {}
-    ~DeviceScratch() {
-        for (void* p : ptrs) (void)hipFreeAsync(p, stream);
-    }
-    template <class T>
-    T* alloc(size_t count) {
-        if (count == 0) return nullptr;
-        void* p = nullptr;
-        if (hipMallocAsync(&p, count * sizeof(T), stream) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
-    }
-};
-
 void fill_gl(const Plan& pl, double* w, double* x) {
     for (int i = 0; i < magnus::kArgMaxQ; ++i) {
         w[i] = 0.0;
@@ -227,6 +196,7 @@ struct TimingTotals {
 };
 std::mutex g_timing_mu;
 bool g_timing_on = false;
+thread_local bool g_timing_suspended = false;
 std::vector<TimedLaunch> g_pending;
 std::vector<std::pair<std::string, TimingTotals>> g_totals;
 
@@ -235,6 +205,7 @@ struct KernelTimer {
     hipStream_t stream;
     const char* name;
     KernelTimer(const char* n, hipStream_t s) : stream(s), name(n) {
+        if (g_timing_suspended) return;
         std::lock_guard<std::mutex> lock(g_timing_mu);
         if (!g_timing_on) return;
         if (hipEventCreate(&start) != hipSuccess || hipEventCreate(&stop) != hipSuccess) {
@@ -252,7 +223,15 @@ struct KernelTimer {
     }
 };
 
-}  // namespace
+bool timing_suspend(bool on) {
+    const bool prev = g_timing_suspended;
+    g_timing_suspended = on;
+    return prev;
+}
+
+}  // namespace magnus_host
+
+using namespace magnus_host;
 
 namespace magnus {
 void* timer_begin(const char* name, hipStream_t s) {
@@ -271,7 +250,7 @@ void timer_end(void* h) {
 }
 }  // namespace magnus
 
-namespace {
+namespace magnus_host {
 
 void timing_drain_locked() {
     for (auto& t : g_pending) {
@@ -373,13 +352,30 @@ void fill_lm(const Plan& pl, long long jobs, magnus::LmArgs& a) {
     a.sc = pl.sc ? (pl.pad ? 1 : 2) : 0;
     a.bt_fused = (pl.mat == MAGNUS_MAT_FIXED2 || pl.mat == MAGNUS_MAT_SPACECURVE) ? 1 : 0;
     a.dt = pl.dt;
+    a.tile = pl.tile;
     fill_gl(pl, a.w, a.x);
+}
+
+size_t forward_scratch_bytes(const Plan& pl, uint64_t count) {
+    const size_t jobs = count * pl.Qj, S = pl.S, m = pl.m;
+    size_t b = scratch_round(jobs * pl.n_out * m * 8) + scratch_round(count * m * 8);
+    if (pl.n >= 2) b += scratch_round(jobs * 2 * S * m * 8) + scratch_round(jobs * S * m * 8);
+    return b;
+}
+
+size_t vjp_scratch_bytes(const Plan& pl, uint64_t count) {
+    if (pl.n < 2) return 0;
+    const size_t jobs = count * pl.Q, S = pl.S, m = pl.m;
+    const size_t carry = scratch_round(count * pl.Q * (pl.n - 1) * S * m * 8);
+    const size_t fwd = scratch_round(jobs * S * m * 8);  // VJP::none forward: Y only
+    const size_t rev = scratch_round(jobs * 2 * S * m * 8) + scratch_round(jobs * S * m * 8);
+    return carry + (fwd > rev ? fwd : rev);
 }
 
 // forward: record into `carry` when non-null; outputs=false runs VJP::none
 int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, double* carry, bool outputs,
-                hipStream_t stream) {
-    DeviceScratch scratch(stream);
+                hipStream_t stream, Arena* arena) {
+    DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const long long jobs = B * (long long)pl.Qj;
     const uint64_t S = pl.S, m = pl.m;
@@ -427,8 +423,8 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
 }
 
 int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, const double* carry,
-            double* dA, hipStream_t stream) {
-    DeviceScratch scratch(stream);
+            double* dA, hipStream_t stream, Arena* arena) {
+    DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const uint64_t S = pl.S, m = pl.m;
     double* dAjob = nullptr;
@@ -437,7 +433,7 @@ int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot,
         if (!carry) {  // VJP::none recompute (grad.hpp:95-110)
             double* tmp = scratch.alloc<double>((size_t)B * pl.Q * (pl.n - 1) * S * m);
             if (!tmp) return fail(MAGNUS_ENOMEM, "device allocation failed");
-            int rc = run_forward(pl, count, in, nullptr, tmp, false, stream);
+            int rc = run_forward(pl, count, in, nullptr, tmp, false, stream, arena);
             if (rc) return rc;
             carry = tmp;
         }
@@ -487,7 +483,7 @@ uint64_t carry_elems(const magnus_problem* p, uint64_t dim) {
     return ((p->n + 3) / 2) * (p->n - 1) * p->samples * dim * dim;
 }
 
-}  // namespace
+}  // namespace magnus_host
 
 extern "C" {
 

@@ -47,6 +47,7 @@ struct LmArgs {
     int bt_fused;  // VJP: Fixed2/SpaceCurve bar_total = temp * (1 + x); generic temp + temp * x
     int level;     // forward level j (1..n) / VJP level k (n..2)
     int ga;        // VJP: gbuf half holding the current adjoint
+    int tile;      // product path: 0 default, 1 VALU, 2 / 3 d = 32 MFMA split over 2 / 4 wavefronts
     double dt;
     double w[kArgMaxQ];
     double x[kArgMaxQ];

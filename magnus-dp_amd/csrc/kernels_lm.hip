@@ -61,14 +61,20 @@ __global__ void __launch_bounds__(256) lm_fwd_prod(LmArgs a) {
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-template <int D>
+// WPP wavefronts share one product; a 4-wavefront workgroup holds 4 / WPP products.
+template <int D, int W>
 struct MfmaShape {
-    static constexpr int TILES = (D / 16) * (D / 16);      // 16x16 tiles per product
-    static constexpr int WPP = TILES >= 4 ? TILES / 4 : 1;  // wavefronts per product
-    static constexpr int TPW = TILES / WPP;                // tiles per wavefront
-    static constexpr int PPB = 4 / WPP;                    // products per workgroup
+    static constexpr int TILES = (D / 16) * (D / 16);  // 16x16 tiles per product
+    static constexpr int WPP = W;                      // wavefronts per product
+    static constexpr int TPW = TILES / WPP;            // tiles per wavefront
+    static constexpr int PPB = 4 / WPP;                // products per workgroup
     static constexpr int MM = D * D;
+    static_assert(TILES % WPP == 0 && 4 % WPP == 0, "tile split");
 };
+
+// default split: one product per wavefront up to d = 32, four wavefronts at d = 64
+template <int D>
+constexpr int default_wpp() { return D == 64 ? 4 : 1; }
 
 // C_tile += Aop * Bop over k = 0..D-1 with Aop[i][k] = LA[k*D + i0 + i], Bop[k][j] = LB[k*D + j0 + j]
 template <int D>
@@ -85,9 +91,9 @@ __device__ __forceinline__ dbl4 mfma_tile(const double* LA, const double* LB, in
 }
 
 // Y_i = A_i (P_i + s T) for products p = job*S + i
-template <int D>
+template <int D, int W>
 __global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
-    using M = MfmaShape<D>;
+    using M = MfmaShape<D, W>;
     __shared__ double LA[M::PPB][M::MM];  // A_i transposed: LA[k][i] = a[i][k]
     __shared__ double LB[M::PPB][M::MM];  // P_i + s T
     const long long total = a.jobs * a.S;
@@ -125,9 +131,9 @@ __global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
 
 // VJP products in two phases over the same LDS: dav = g b^T (b = P_i + s T),
 // added to dA_job, then temp = A_i^T g
-template <int D>
+template <int D, int W>
 __global__ void __launch_bounds__(256) lm_vjp_prod_mfma(LmArgs a) {
-    using M = MfmaShape<D>;
+    using M = MfmaShape<D, W>;
     __shared__ double LA[M::PPB][M::MM];
     __shared__ double LB[M::PPB][M::MM];
     const long long per = a.S * M::MM;
@@ -391,15 +397,37 @@ __global__ void vjp_reduce_kernel(VjpReduceArgs a) {
 
 inline long long blocks_for(long long n) { return (n + 255) / 256; }
 
+// Product path of a level: D (MFMA, default split), D + W (MFMA, W wavefronts
+// per product), or 0 (VALU, one thread per output entry).  a.tile selects:
+// 0 default (MFMA for d in {16, 32, 64}), 1 VALU, 2 / 3 d = 32 split over 2 / 4
+// wavefronts; a choice that does not apply to d falls back to the default.
+inline int prod_path(const LmArgs& a) {
+    const bool mf = !a.sc && (a.d == 16 || a.d == 32 || a.d == 64);
+    if (!mf || a.tile == 1) return 0;
+    if (a.d == 32 && a.tile == 2) return 32 + 2;
+    if (a.d == 32 && a.tile == 3) return 32 + 4;
+    return a.d;
+}
+
+template <int D, int W>
+inline void launch_mfma(void (*k)(LmArgs), const LmArgs& a, hipStream_t st) {
+    constexpr int ppb = MfmaShape<D, W>::PPB;
+    const long long prods = a.jobs * a.S;
+    hipLaunchKernelGGL(k, dim3((prods + ppb - 1) / ppb), dim3(256), 0, st, a);
+}
+
 template <int I>
 hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
     if (a.level >= 2) {
         void* h = timer_begin("lm_fwd_prod", st);
-        const long long prods = a.jobs * a.S;
-        if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_fwd_prod_mfma<16>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
-        else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_fwd_prod_mfma<32>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
-        else if (!a.sc && a.d == 64) hipLaunchKernelGGL(lm_fwd_prod_mfma<64>, dim3(prods), dim3(256), 0, st, a);
-        else hipLaunchKernelGGL(lm_fwd_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
+        switch (prod_path(a)) {
+            case 16: launch_mfma<16, 1>(lm_fwd_prod_mfma<16, 1>, a, st); break;
+            case 32: launch_mfma<32, 1>(lm_fwd_prod_mfma<32, 1>, a, st); break;
+            case 32 + 2: launch_mfma<32, 2>(lm_fwd_prod_mfma<32, 2>, a, st); break;
+            case 32 + 4: launch_mfma<32, 4>(lm_fwd_prod_mfma<32, 4>, a, st); break;
+            case 64: launch_mfma<64, 4>(lm_fwd_prod_mfma<64, 4>, a, st); break;
+            default: hipLaunchKernelGGL(lm_fwd_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
+        }
         timer_end(h);
     }
     void* h = timer_begin("lm_fwd_scan", st);
@@ -411,11 +439,14 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
 template <int I>
 hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
     void* h = timer_begin("lm_vjp_prod", st);
-    const long long prods = a.jobs * a.S;
-    if (!a.sc && a.d == 16) hipLaunchKernelGGL(lm_vjp_prod_mfma<16>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
-    else if (!a.sc && a.d == 32) hipLaunchKernelGGL(lm_vjp_prod_mfma<32>, dim3((prods + 3) / 4), dim3(256), 0, st, a);
-    else if (!a.sc && a.d == 64) hipLaunchKernelGGL(lm_vjp_prod_mfma<64>, dim3(prods), dim3(256), 0, st, a);
-    else hipLaunchKernelGGL(lm_vjp_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
+    switch (prod_path(a)) {
+        case 16: launch_mfma<16, 1>(lm_vjp_prod_mfma<16, 1>, a, st); break;
+        case 32: launch_mfma<32, 1>(lm_vjp_prod_mfma<32, 1>, a, st); break;
+        case 32 + 2: launch_mfma<32, 2>(lm_vjp_prod_mfma<32, 2>, a, st); break;
+        case 32 + 4: launch_mfma<32, 4>(lm_vjp_prod_mfma<32, 4>, a, st); break;
+        case 64: launch_mfma<64, 4>(lm_vjp_prod_mfma<64, 4>, a, st); break;
+        default: hipLaunchKernelGGL(lm_vjp_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
+    }
     timer_end(h);
     h = timer_begin("lm_vjp_scan", st);
     hipLaunchKernelGGL((lm_vjp_scan<I>), dim3(blocks_for(a.jobs * a.m)), dim3(256), 0, st, a);

@@ -76,6 +76,18 @@ def lib() -> ctypes.CDLL:
         L.magnus_sample_sc_fourier_f64.argtypes = [u64, u64, d, d, u64, vp, vp, vp, vp]
         L.magnus_expm_f64.argtypes = [u64, u64, vp, vp, vp]
         L.magnus_sc_expm_f64.argtypes = [u64, vp, vp, vp]
+        pp = ctypes.POINTER(ctypes.c_void_p)
+        L.magnus_uneven_create.argtypes = [P, u64, u64, ctypes.c_int, ctypes.POINTER(ctypes.c_void_p)]
+        L.magnus_uneven_destroy.argtypes = [vp]
+        L.magnus_uneven_destroy.restype = None
+        L.magnus_uneven_forward_f64.argtypes = [vp, pp, pp, ctypes.c_int, vp]
+        L.magnus_uneven_vjp_f64.argtypes = [vp, pp, pp, pp, ctypes.c_int, vp]
+        L.magnus_uneven_autotune.argtypes = [vp, pp, pp, pp, vp]
+        L.magnus_uneven_tuning.argtypes = [vp, ctypes.c_char_p, u64]
+        L.magnus_uneven_tuning.restype = u64
+        L.magnus_uneven_set_tuning.argtypes = [vp, ctypes.c_char_p]
+        L.magnus_uneven_info.argtypes = [vp, ctypes.c_int]
+        L.magnus_uneven_info.restype = u64
         L.magnus_timing_enable.argtypes = [ctypes.c_int]
         L.magnus_timing_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)]
         L.magnus_timing_reset.restype = None

@@ -1,0 +1,175 @@
+"""GPU: uneven batches (magnus_uneven_*) -- many matrix problems of different
+dim, samples, order, op and integrator in one enqueue or one hipGraph launch.
+
+Every instance's forward output and dA must equal a per-instance call of the
+one-problem entry points bit for bit (same kernels, same per-job arithmetic),
+in eager mode and on every replay of the captured graph; product-path
+variants chosen by the tuner are checked against the oracle (1e-12 relative
+Frobenius).
+"""
+from __future__ import annotations
+
+import numpy as np
+import pytest
+
+import oracle as O
+from helpers import fourier_matrices, rel_fro
+
+pytestmark = pytest.mark.gpu
+
+TOL = 1e-12
+
+
+@pytest.fixture(scope="module", autouse=True)
+def numpy_gl_table():
+    total = 64 * 65 // 2
+    w = np.empty(total)
+    x = np.empty(total)
+    off = 0
+    for order in range(1, 65):
+        xr, wr = np.polynomial.legendre.leggauss(order)
+        x[off:off + order] = (xr + 1) / 2
+        w[off:off + order] = wr / 2
+        off += order
+    O.gl_replace(64, w, x)
+    yield
+    O.gl_reset()
+
+
+def _mixed_problems(seed, count):
+    rng = np.random.default_rng(seed)
+    dims = [2, 3, 4, 8, 16, 32]
+    samples = [33, 65, 129]
+    integs = ["Boole", "Simpson", "Trapezoid", "Riemann"]
+    ops = ["sum", "many", "one"]
+    probs = []
+    for i in range(count):
+        probs.append(dict(n=int(rng.integers(1, 6)), dim=int(rng.choice(dims)), samples=int(rng.choice(samples)),
+                          t0=0.0, tf=1.0, op=str(rng.choice(ops)), integrator=str(rng.choice(integs))))
+    # repeated shapes so that buckets hold several instances, and one bucket of
+    # instances too large to share a small workspace
+    big = dict(n=5, dim=32, samples=129, t0=0.0, tf=1.0, op="sum", integrator="Boole")
+    return probs + [dict(p) for p in probs[: count // 2]] + [dict(big) for _ in range(3)]
+
+
+def _inputs(probs, seed):
+    import torch
+
+    A = [torch.from_numpy(fourier_matrices(np.random.default_rng([seed, i]), p["samples"], p["dim"], modes=3)).cuda()
+         for i, p in enumerate(probs)]
+    cot = []
+    for i, p in enumerate(probs):
+        shape = (p["n"], p["dim"], p["dim"]) if p["op"] == "many" else (p["dim"], p["dim"])
+        cot.append(torch.from_numpy(np.random.default_rng([seed, i, 1]).standard_normal(shape)).cuda())
+    return A, cot
+
+
+def _per_instance(probs, A, cot):
+    from magnus import torch_ops
+
+    outs, dAs = [], []
+    for p, a, c in zip(probs, A, cot):
+        kw = dict(op=p["op"], integrator=p["integrator"])
+        outs.append(torch_ops.forward(p["n"], a[None], p["t0"], p["tf"], **kw)[0])
+        dAs.append(torch_ops.vjp(p["n"], a[None], c[None], None, p["t0"], p["tf"], **kw)[0])
+    return outs, dAs
+
+
+@pytest.mark.parametrize("branches,workspace", [(1, 0), (3, 0), (4, 200 << 20)])
+def test_uneven_matches_per_instance_calls_bitwise(branches, workspace):
+    """Eager and graph modes against per-instance calls; the small workspace
+    (50 MB per branch, about 32 MB per dim-32 instance) forces sub-batches and
+    arena reuse."""
+    import torch
+
+    from magnus.uneven import UnevenBatch
+
+    probs = _mixed_problems(3, 24)
+    A, cot = _inputs(probs, 3)
+    ref_out, ref_dA = _per_instance(probs, A, cot)
+    plan = UnevenBatch(probs, workspace_bytes=workspace, branches=branches)
+    info = plan.info()
+    assert info["buckets"] <= len(probs) and info["sub_batches"] >= info["buckets"]
+    if workspace:
+        assert info["sub_batches"] > info["buckets"]
+    for use_graph in (False, True):
+        for rep in range(3 if use_graph else 1):
+            outs = plan.forward(A, use_graph=use_graph)
+            dAs = plan.vjp(A, cot, use_graph=use_graph)
+            torch.cuda.synchronize()
+            for i in range(len(probs)):
+                assert torch.equal(outs[i], ref_out[i]), (use_graph, rep, i, probs[i])
+                assert torch.equal(dAs[i], ref_dA[i]), (use_graph, rep, i, probs[i])
+            # eager work of the program between graph replays
+            _ = sum(float(x.sum()) for x in outs)
+    plan.close()
+
+
+@pytest.mark.parametrize("dim,tile", [(16, 1), (32, 1), (32, 2), (32, 3), (64, 1), (64, 0)])
+def test_uneven_product_paths_match_oracle(dim, tile):
+    """Every product path the tuner can pick, forced through the tuning table."""
+    import torch
+
+    from magnus.uneven import UnevenBatch
+
+    probs = [dict(n=4, dim=dim, samples=65, t0=0.0, tf=1.0, op="many", integrator="Boole") for _ in range(3)]
+    probs.append(dict(n=3, dim=dim, samples=33, t0=0.0, tf=1.0, op="sum", integrator="Simpson"))
+    A, cot = _inputs(probs, 5)
+    plan = UnevenBatch(probs, branches=2)
+    plan.set_tuning(f"{dim} 65 4 {tile}\n{dim} 33 3 {tile}\n")
+    assert all(line.split()[3] == str(tile) for line in plan.tuning.splitlines())
+    outs = plan.forward(A)
+    dAs = plan.vjp(A, cot)
+    torch.cuda.synchronize()
+    for i, p in enumerate(probs):
+        a = A[i].cpu().numpy()
+        ref = O.forward(p["n"], a, 0.0, 1.0, op=p["op"], integrator=p["integrator"])
+        ref_g = O.vjp(p["n"], a, cot[i].cpu().numpy(), 0.0, 1.0, op=p["op"], integrator=p["integrator"])
+        assert rel_fro(outs[i].cpu().numpy(), ref) <= TOL
+        assert rel_fro(dAs[i].cpu().numpy(), ref_g) <= TOL
+    plan.close()
+
+
+def test_uneven_autotune_picks_measured_paths_and_keeps_parity():
+    import torch
+
+    from magnus.uneven import UnevenBatch
+
+    probs = [dict(n=3, dim=d, samples=129, t0=0.0, tf=1.0, op="sum", integrator="Boole")
+             for d in (4, 16, 32, 32, 64)]
+    A, cot = _inputs(probs, 7)
+    plan = UnevenBatch(probs, branches=2)
+    plan.autotune(A, cot)
+    lines = [ln.split() for ln in plan.tuning.splitlines()]
+    assert len(lines) == 4 and all(float(ln[4]) > 0 for ln in lines)
+    outs = plan.forward(A)
+    dAs = plan.vjp(A, cot)
+    torch.cuda.synchronize()
+    for i, p in enumerate(probs):
+        a = A[i].cpu().numpy()
+        assert rel_fro(outs[i].cpu().numpy(), O.forward(3, a, 0.0, 1.0, integrator="Boole")) <= TOL
+        ref_g = O.vjp(3, a, cot[i].cpu().numpy(), 0.0, 1.0, integrator="Boole")
+        assert rel_fro(dAs[i].cpu().numpy(), ref_g) <= TOL
+    # a second plan takes the table as text
+    plan2 = UnevenBatch(probs, branches=1)
+    plan2.set_tuning(plan.tuning)
+    assert [ln.split()[3] for ln in plan2.tuning.splitlines()] == [ln[3] for ln in lines]
+    plan.close()
+    plan2.close()
+
+
+def test_uneven_argument_errors():
+    import torch
+
+    from magnus import capi
+    from magnus.uneven import UnevenBatch
+
+    with pytest.raises(Exception, match="instance 1"):
+        UnevenBatch([dict(n=2, dim=4, samples=33), dict(n=2, dim=4, samples=32, integrator="Boole")])
+    with pytest.raises(Exception):
+        UnevenBatch([dict(n=6, dim=64, samples=8193)], workspace_bytes=1 << 20)
+    plan = UnevenBatch([dict(n=2, dim=4, samples=33)])
+    with pytest.raises(ValueError):
+        plan.forward([torch.zeros((33, 4, 4), dtype=torch.float64, device="cuda")] * 2)
+    plan.close()
+    assert capi.lib().magnus_uneven_info(None, 0) == 0

@@ -226,10 +226,13 @@ def _uneven_inputs(torch, probs, device, modes=3, seed=4):
     return A, [torch.from_numpy(c).to(device) for c in cots]
 
 
-def _uneven_flops(p):
-    """Algorithmic flops of forward (outputs) + VJP with recompute for one instance."""
+def _uneven_flops(p, nodes=None):
+    """Flops of forward (outputs) + VJP with recompute for one instance: with
+    nodes=None the reference algorithm's count (Q GL nodes per level); with
+    nodes=1 the dense W-basis path's (one basis sweep serves every node)."""
     n, S, d = p["n"], p["samples"], p["dim"]
     Q, K, m, P = (n + 3) // 2, n - 1, d * d, 2 * d ** 3
+    Q = Q if nodes is None else nodes
     F_fwd, F_vjp, _, _, _ = _counts(Q, K, S, m, m, P, 1)
     F_rec = Q * K * S * (P + 6 * m)
     return F_fwd + F_rec + F_vjp
@@ -349,7 +352,22 @@ def _bench_uneven(args, torch, dev):
         assert torch.equal(seq_out[i], outs[i]), f"uneven output differs from the per-instance call at {i}"
     print(f"[bench] uneven: per-instance sequential {seq_ms:.1f} ms", file=sys.stderr, flush=True)
 
-    flops = sum(_uneven_flops(p) for p in probs)
+    # per-kernel totals of one eager step (the branches overlap, so these add
+    # up to more than the step)
+    capi.timing_reset()
+    capi.timing_enable(True)
+    step(False)
+    torch.cuda.synchronize()
+    capi.timing_enable(False)
+    per_kernel = {}
+    for name in KERNELS + ("wbd_fold", "wbd_fold_vjp", "uneven_gather", "uneven_scatter"):
+        cnt, tot = capi.timing_read(name)
+        if cnt:
+            per_kernel[name] = {"launches": cnt, "total_ms": round(tot, 3)}
+    print(f"[bench] uneven: kernels {per_kernel}", file=sys.stderr, flush=True)
+
+    ref_flops = sum(_uneven_flops(p) for p in probs)
+    flops = sum(_uneven_flops(p, nodes=1) for p in probs)  # executed: W-basis sweep
     ms = timing[True]
     cpu = None if args.no_cpu else _cpu_baseline_uneven(probs, A, cot, args.cpu_budget)
     ach = flops / (ms / 1e3) / 1e12
@@ -372,7 +390,10 @@ def _bench_uneven(args, torch, dev):
                    "branches": args.uneven_branches, "plan": plan.info(), "graph": True},
         "roofline": {"bound": "fp64", "achieved": round(ach, 3), "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
                      "frac": round(ach / MI355X_FP64_TFLOPS, 4), "traffic": None, "kernel": "whole step",
-                     "algorithmic_flops_per_step": flops},
+                     "algorithmic_flops_per_step": flops,
+                     "reference_algorithm_flops_per_step": ref_flops,
+                     "reference_algorithm_rate_tflops": round(ref_flops / (ms / 1e3) / 1e12, 3)},
+        "kernels_eager_step": per_kernel,
         "cpu_baseline": cpu,
         "per_instance_sequential": {"ms": round(seq_ms, 2), "solves_per_s": round(count / (seq_ms / 1e3), 3)},
         "eager_bucketed": {"ms": round(timing[False], 3)},

@@ -37,6 +37,7 @@ COMMON = [
 
 DEVICE_SOURCES = [
     "kernels_lm.hip", "kernels_wb.hip", "kernels_sample.hip", "kernels_expm.hip", "kernels_uneven.hip",
+    "kernels_wbd.hip",
 ]
 HOST_SOURCES = ["capi.cpp", "uneven.cpp"]
 

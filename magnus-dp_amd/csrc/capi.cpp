@@ -356,8 +356,24 @@ void fill_lm(const Plan& pl, long long jobs, magnus::LmArgs& a) {
     fill_gl(pl, a.w, a.x);
 }
 
+// Dense W-basis path: matrix problems of order >= 2 with no reference-layout
+// carry recorded or supplied (kernels_wbd.hip).  Q times fewer products than
+// the per-node recursion and a carry of n-1 levels instead of Q (n-1).  op
+// "one" stays on the per-node recursion: Omega_n alone can be many orders of
+// magnitude below the terms the fold combines, and the reassociated sums then
+// differ from the reference's by up to ~1e-12 of Omega_n (n = 7 and 8).
+bool use_wbd(const Plan& pl) { return !pl.sc && pl.n >= 2 && pl.op != MAGNUS_OP_ONE; }
+
+size_t wbd_fold_bytes(const Plan& pl, uint64_t B) {  // z, totals, work (+ omega1)
+    const size_t m = pl.m, n = pl.n;
+    return 2 * scratch_round(B * n * m * 8) + scratch_round(B * m * 8) + scratch_round(B * (4 * (n - 1) + 1) * m * 8);
+}
+
 size_t forward_scratch_bytes(const Plan& pl, uint64_t count) {
-    const size_t jobs = count * pl.Qj, S = pl.S, m = pl.m;
+    const size_t S = pl.S, m = pl.m;
+    if (use_wbd(pl))
+        return wbd_fold_bytes(pl, count) + scratch_round(count * 2 * S * m * 8) + scratch_round(count * S * m * 8);
+    const size_t jobs = count * pl.Qj;
     size_t b = scratch_round(jobs * pl.n_out * m * 8) + scratch_round(count * m * 8);
     if (pl.n >= 2) b += scratch_round(jobs * 2 * S * m * 8) + scratch_round(jobs * S * m * 8);
     return b;
@@ -365,16 +381,174 @@ size_t forward_scratch_bytes(const Plan& pl, uint64_t count) {
 
 size_t vjp_scratch_bytes(const Plan& pl, uint64_t count) {
     if (pl.n < 2) return 0;
-    const size_t jobs = count * pl.Q, S = pl.S, m = pl.m;
+    const size_t S = pl.S, m = pl.m;
+    if (use_wbd(pl)) {
+        const size_t W = scratch_round(count * (pl.n - 1) * S * m * 8);
+        const size_t fwd = scratch_round(count * S * m * 8);  // Y
+        const size_t rev = 2 * scratch_round(count * pl.n * m * 8) + scratch_round(count * 2 * S * m * 8) +
+                           scratch_round(count * S * m * 8);  // zbar, wbar, gbuf, dAjob
+        return W + wbd_fold_bytes(pl, count) + (fwd > rev ? fwd : rev);
+    }
+    const size_t jobs = count * pl.Q;
     const size_t carry = scratch_round(count * pl.Q * (pl.n - 1) * S * m * 8);
     const size_t fwd = scratch_round(jobs * S * m * 8);  // VJP::none forward: Y only
     const size_t rev = scratch_round(jobs * 2 * S * m * 8) + scratch_round(jobs * S * m * 8);
     return carry + (fwd > rev ? fwd : rev);
 }
 
+// LmArgs of the W-basis sweep: one job per instance, shift 0, weight 1, all
+// level sums kept (op many layout)
+void fill_wbd_lm(const Plan& pl, long long B, magnus::LmArgs& a) {
+    fill_lm(pl, B, a);
+    a.jobs = B;
+    a.Q = 1;
+    a.Qj = 1;
+    a.op = MAGNUS_OP_MANY;
+    a.n_out = (int)pl.n;
+    a.m_cot = (int)pl.m;
+    for (int i = 0; i < magnus::kArgMaxQ; ++i) a.w[i] = a.x[i] = 0.0;
+    a.w[0] = 1.0;
+    a.x[0] = 1.0;
+}
+
+void fill_wbd_fold(const Plan& pl, long long B, magnus::WbdArgs& f) {
+    f.B = B;
+    f.n = (int)pl.n;
+    f.d = (int)pl.d;
+    f.m = (int)pl.m;
+    f.Q = (int)pl.Q;
+    f.op = pl.op;
+    f.n_out = (int)pl.n_out;
+    fill_gl(pl, f.w, f.x);
+}
+
+int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* out, hipStream_t stream,
+                    Arena* arena) {
+    DeviceScratch scratch(stream, arena);
+    const long long B = (long long)count;
+    const size_t S = pl.S, m = pl.m, n = pl.n;
+    double* z = scratch.alloc<double>(B * n * m);
+    double* z0 = scratch.alloc<double>(B * m);
+    double* totals = scratch.alloc<double>(B * n * m);
+    double* work = scratch.alloc<double>(B * (4 * (n - 1) + 1) * m);
+    double* pingpong = scratch.alloc<double>(B * 2 * S * m);
+    double* Y = scratch.alloc<double>(B * S * m);
+    if (!z || !z0 || !totals || !work || !pingpong || !Y) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    magnus::LmArgs a{};
+    fill_wbd_lm(pl, B, a);
+    a.in = in;
+    a.pingpong = pingpong;
+    a.Y = Y;
+    a.partial = z;
+    a.omega1 = z0;
+    a.totals = totals;
+    a.outputs = 1;
+    for (int j = 1; j <= (int)n; ++j) {
+        a.level = j;
+        hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
+        if (e != hipSuccess) return device_error(e, "forward kernel launch");
+    }
+    magnus::WbdArgs f{};
+    fill_wbd_fold(pl, B, f);
+    f.z = z;
+    f.z0 = z0;
+    f.totals = totals;
+    f.out = out;
+    f.work = work;
+    hipError_t e = magnus::launch_wbd_fold(f, stream);
+    if (e != hipSuccess) return device_error(e, "fold kernel launch");
+    g_kernel = "wbd";
+    return MAGNUS_OK;
+}
+
+int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA, hipStream_t stream,
+                Arena* arena) {
+    DeviceScratch scratch(stream, arena);
+    const long long B = (long long)count;
+    const size_t S = pl.S, m = pl.m, n = pl.n;
+    double* W = scratch.alloc<double>(B * (n - 1) * S * m);
+    double* z = scratch.alloc<double>(B * n * m);
+    double* z0 = scratch.alloc<double>(B * m);
+    double* totals = scratch.alloc<double>(B * n * m);
+    double* work = scratch.alloc<double>(B * (4 * (n - 1) + 1) * m);
+    if (!W || !z || !z0 || !totals || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    magnus::LmArgs a{};
+    fill_wbd_lm(pl, B, a);
+    a.in = in;
+    {
+        DeviceScratch fs(stream, arena);
+        double* Y = fs.alloc<double>(B * S * m);
+        if (!Y) return fail(MAGNUS_ENOMEM, "device allocation failed");
+        a.carry = W;
+        a.Y = Y;
+        a.partial = z;
+        a.omega1 = z0;
+        a.totals = totals;
+        a.outputs = 1;
+        for (int j = 1; j <= (int)n; ++j) {
+            a.level = j;
+            hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
+            if (e != hipSuccess) return device_error(e, "forward kernel launch");
+        }
+    }
+    double* zbar = scratch.alloc<double>(B * n * m);
+    double* wbar = scratch.alloc<double>(B * n * m);
+    double* gbuf = scratch.alloc<double>(B * 2 * S * m);
+    double* dAjob = scratch.alloc<double>(B * S * m);
+    if (!zbar || !wbar || !gbuf || !dAjob) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    magnus::WbdArgs f{};
+    fill_wbd_fold(pl, B, f);
+    f.z = z;
+    f.z0 = z0;
+    f.totals = totals;
+    f.cot = cot;
+    f.zbar = zbar;
+    f.wbar = wbar;
+    f.work = work;
+    hipError_t e = magnus::launch_wbd_fold_vjp(f, stream);
+    if (e != hipSuccess) return device_error(e, "fold kernel launch");
+    a.Y = nullptr;
+    a.partial = nullptr;
+    a.omega1 = nullptr;
+    a.totals = nullptr;
+    a.cot = zbar;
+    a.gbuf = gbuf;
+    a.dAjob = dAjob;
+    a.wbar = wbar;
+    a.ga = 0;
+    e = magnus::launch_lm_vjp_init(pl.integ, a, stream);
+    for (int k = (int)n; k >= 2 && e == hipSuccess; --k) {
+        a.level = k;
+        e = magnus::launch_lm_vjp_level(pl.integ, a, stream);
+        a.ga ^= 1;
+    }
+    if (e == hipSuccess) e = magnus::launch_lm_vjp_final(a, stream);
+    if (e != hipSuccess) return device_error(e, "vjp kernel launch");
+    magnus::VjpReduceArgs r{};
+    r.dAjob = dAjob;
+    r.cot = zbar;
+    r.dA = dA;
+    r.B = B;
+    r.S = (long long)S;
+    r.n = (int)n;
+    r.m = (int)m;
+    r.m_out = (int)m;
+    r.m_cot = (int)m;
+    r.n_out = (int)n;
+    r.Qj = 1;
+    r.op = MAGNUS_OP_MANY;
+    r.sc = 0;
+    r.dt = pl.dt;
+    e = magnus::launch_vjp_reduce(pl.integ, r, stream);
+    if (e != hipSuccess) return device_error(e, "vjp reduce launch");
+    g_kernel = "wbd";
+    return MAGNUS_OK;
+}
+
 // forward: record into `carry` when non-null; outputs=false runs VJP::none
 int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, double* carry, bool outputs,
                 hipStream_t stream, Arena* arena) {
+    if (!carry && outputs && use_wbd(pl)) return run_wbd_forward(pl, count, in, out, stream, arena);
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const long long jobs = B * (long long)pl.Qj;
@@ -424,6 +598,7 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
 
 int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, const double* carry,
             double* dA, hipStream_t stream, Arena* arena) {
+    if (!carry && use_wbd(pl)) return run_wbd_vjp(pl, count, in, cot, dA, stream, arena);
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const uint64_t S = pl.S, m = pl.m;

@@ -39,6 +39,8 @@ struct LmArgs {
     double* omega1;      // [B][m]
     double* gbuf;        // [jobs][2][S][m] adjoint level buffers (VJP)
     double* dAjob;       // [jobs][S][m]               (VJP)
+    double* totals;      // [jobs][n][m] last prefix value of each level, or nullptr (forward)
+    const double* wbar;  // [jobs][n][m] cotangents of the level totals, or nullptr (VJP)
     long long S;
     long long jobs;
     int n, d, m, m_in, m_cot, Q, Qj, n_out, op;
@@ -68,5 +70,27 @@ hipError_t launch_lm_forward_level(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_init(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_level(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_final(const LmArgs& a, hipStream_t s);
+
+// Dense W-basis GL fold (kernels_wbd.hip).  The level kernels run one job per
+// instance with shift 0 and weight 1, giving the basis streams W_j = Prefix(A W_{j-1})
+// (W_1 = Prefix(A)), their totals What_j and the sums Z_l = Sum(A W_l) (Z_0 = Sum(A)).
+// Per GL node q: T_1 = What_1, T_j = What_j + s_q sum_{l<j} What_l T_{j-l};
+// Omega_1 = Z_0, Omega_j = sum_q w_q (Z_{j-1} + s_q sum_{l<=j-2} Z_l T_{j-1-l}).
+struct WbdArgs {
+    const double* z;       // [B][n][m]  rows 1..K: Z_1..Z_K (row 0 unused)
+    const double* z0;      // [B][m]     Z_0
+    const double* totals;  // [B][n][m]  rows 0..K-1: What_1..What_K
+    const double* cot;     // [B][n_out][m]              (VJP)
+    double* out;           // [B][n_out][m]              (forward)
+    double* zbar;          // [B][n][m]  rows 0..K: Zbar_0..Zbar_K      (VJP)
+    double* wbar;          // [B][n][m]  rows 0..K-1: What_bar_1..K     (VJP)
+    double* work;          // [B][4K+1][m] scratch
+    long long B;
+    int n, d, m, Q, op, n_out;
+    double w[kArgMaxQ];
+    double x[kArgMaxQ];
+};
+hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s);      // -> out
+hipError_t launch_wbd_fold_vjp(const WbdArgs& a, hipStream_t s);  // -> zbar, wbar
 
 }  // namespace magnus

@@ -235,6 +235,7 @@ __global__ void __launch_bounds__(256) lm_fwd_scan(LmArgs a) {
             if (Pout) Pout[i * m + e] = st.step(i, f[u], dt);
         }
     }
+    if (a.totals && Pout) a.totals[(job * n + (j - 1)) * m + e] = Pout[(S - 1) * m + e];
     if (om1) a.omega1[b * m + e] = acc;
     if (lvl) {
         if (a.op == OP_MANY) {
@@ -333,6 +334,7 @@ __global__ void __launch_bounds__(256) lm_vjp_scan(LmArgs a) {
             bt += t[u] * shift;
         }
     }
+    if (a.wbar) bt += a.wbar[(job * a.n + (k - 2)) * (long long)m + e];  // cotangent of the level total
     G[(S - 1) * m] = bt;
     prefix_vjp<I>(G, m, S, a.dt);
     if (k > 2 && a.op != OP_ONE) {  // sum_vjp_add (grad.hpp:232, 332)

@@ -571,3 +571,53 @@ def test_spacecurve_autograd_without_carry_matches_oracle(integrator):
                                                                   integrator=integrator)) <= TOL
         ref = O.sc_vjp(n, ab, w[b].cpu().numpy(), 0.0, 1.0, op="many", integrator=integrator)
         assert rel_fro(a.grad[b].cpu().numpy(), ref) <= TOL
+
+
+@pytest.mark.parametrize("integrator", INTEGRATORS)
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [2, 3, 5, 8])
+@pytest.mark.parametrize("dim", [2, 3, 5, 16, 32, 64])
+def test_dense_wbasis_matches_oracle(dim, n, op, integrator):
+    """Forward and VJP without a carry take the dense W-basis path (one basis
+    sweep for all GL nodes, per-instance GL fold and its adjoint); op 'one'
+    keeps the per-node recursion."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    B = 2
+    S = _samples_for(integrator, 65 if dim >= 32 else 129)
+    A = np.stack([fourier_matrices(np.random.default_rng([23, b, dim, n]), S, dim, modes=3) for b in range(B)])
+    out_shape = (n, dim, dim) if op == "many" else (dim, dim)
+    cot = np.stack([np.random.default_rng([23, b, dim, n, 1]).standard_normal(out_shape) for b in range(B)])
+    At = torch.from_numpy(A).cuda()
+    family = "lm" if op == "one" else "wbd"
+    out = torch_ops.forward(n, At, 0.0, 1.0, op=op, integrator=integrator)
+    assert capi.last_kernel() == family
+    dA = torch_ops.vjp(n, At, torch.from_numpy(cot).cuda(), None, 0.0, 1.0, op=op, integrator=integrator)
+    assert capi.last_kernel() == family
+    out, dA = out.cpu().numpy(), dA.cpu().numpy()
+    for b in range(B):
+        assert rel_fro(out[b], O.forward(n, A[b], 0.0, 1.0, op=op, integrator=integrator)) <= TOL
+        ref_g = O.vjp(n, A[b], cot[b], 0.0, 1.0, op=op, integrator=integrator)
+        assert rel_fro(dA[b], ref_g) <= TOL
+
+
+@pytest.mark.parametrize("scale", [0.5, 2.0, 4.0])
+def test_dense_wbasis_large_norm(scale):
+    """||A||_2 T up to 4 (the controller's training range), n = 8, op = many:
+    the reassociated sums stay within the bar."""
+    import torch
+
+    from magnus import torch_ops
+
+    S, d, n = 1025, 8, 8
+    A = fourier_matrices(np.random.default_rng([29, int(scale * 10)]), S, d, modes=6)
+    A *= scale / np.linalg.norm(A, 2, axis=(1, 2)).max()
+    cot = np.random.default_rng([29, 1]).standard_normal((n, d, d))
+    At = torch.from_numpy(A[None]).cuda()
+    out = torch_ops.forward(n, At, 0.0, 1.0, op="many", integrator="Boole")[0].cpu().numpy()
+    dA = torch_ops.vjp(n, At, torch.from_numpy(cot[None]).cuda(), None, 0.0, 1.0, op="many",
+                       integrator="Boole")[0].cpu().numpy()
+    assert rel_fro(out, O.forward(n, A, 0.0, 1.0, op="many", integrator="Boole")) <= TOL
+    assert rel_fro(dA, O.vjp(n, A, cot, 0.0, 1.0, op="many", integrator="Boole")) <= TOL

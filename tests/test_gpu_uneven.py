@@ -75,10 +75,10 @@ def _per_instance(probs, A, cot):
     return outs, dAs
 
 
-@pytest.mark.parametrize("branches,workspace", [(1, 0), (3, 0), (4, 200 << 20)])
+@pytest.mark.parametrize("branches,workspace", [(1, 0), (3, 0), (4, 64 << 20)])
 def test_uneven_matches_per_instance_calls_bitwise(branches, workspace):
     """Eager and graph modes against per-instance calls; the small workspace
-    (50 MB per branch, about 32 MB per dim-32 instance) forces sub-batches and
+    (16 MB per branch, about 11 MB per dim-32 instance) forces sub-batches and
     arena reuse."""
     import torch
 

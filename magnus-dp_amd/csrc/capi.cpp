@@ -364,9 +364,10 @@ void fill_lm(const Plan& pl, long long jobs, magnus::LmArgs& a) {
 // differ from the reference's by up to ~1e-12 of Omega_n (n = 7 and 8).
 bool use_wbd(const Plan& pl) { return !pl.sc && pl.n >= 2 && pl.op != MAGNUS_OP_ONE; }
 
-size_t wbd_fold_bytes(const Plan& pl, uint64_t B) {  // z, totals, work (+ omega1)
-    const size_t m = pl.m, n = pl.n;
-    return 2 * scratch_round(B * n * m * 8) + scratch_round(B * m * 8) + scratch_round(B * (4 * (n - 1) + 1) * m * 8);
+size_t wbd_fold_bytes(const Plan& pl, uint64_t B) {  // z, totals, omega1, part, part2, work
+    const size_t m = pl.m, n = pl.n, Q = pl.Q;
+    return 2 * scratch_round(B * n * m * 8) + scratch_round(B * m * 8) + 2 * scratch_round(B * Q * n * m * 8) +
+           scratch_round(B * Q * (2 * (n - 1) + 1) * m * 8);
 }
 
 size_t forward_scratch_bytes(const Plan& pl, uint64_t count) {
@@ -427,13 +428,17 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const size_t S = pl.S, m = pl.m, n = pl.n;
+    const size_t Q = pl.Q;
     double* z = scratch.alloc<double>(B * n * m);
     double* z0 = scratch.alloc<double>(B * m);
     double* totals = scratch.alloc<double>(B * n * m);
-    double* work = scratch.alloc<double>(B * (4 * (n - 1) + 1) * m);
+    double* part = scratch.alloc<double>(B * Q * n * m);
+    double* part2 = scratch.alloc<double>(B * Q * n * m);
+    double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
     double* pingpong = scratch.alloc<double>(B * 2 * S * m);
     double* Y = scratch.alloc<double>(B * S * m);
-    if (!z || !z0 || !totals || !work || !pingpong || !Y) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    if (!z || !z0 || !totals || !part || !part2 || !work || !pingpong || !Y)
+        return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
     a.in = in;
@@ -454,6 +459,7 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
     f.z0 = z0;
     f.totals = totals;
     f.out = out;
+    f.part = part;
     f.work = work;
     hipError_t e = magnus::launch_wbd_fold(f, stream);
     if (e != hipSuccess) return device_error(e, "fold kernel launch");
@@ -466,12 +472,15 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const size_t S = pl.S, m = pl.m, n = pl.n;
+    const size_t Q = pl.Q;
     double* W = scratch.alloc<double>(B * (n - 1) * S * m);
     double* z = scratch.alloc<double>(B * n * m);
     double* z0 = scratch.alloc<double>(B * m);
     double* totals = scratch.alloc<double>(B * n * m);
-    double* work = scratch.alloc<double>(B * (4 * (n - 1) + 1) * m);
-    if (!W || !z || !z0 || !totals || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    double* part = scratch.alloc<double>(B * Q * n * m);
+    double* part2 = scratch.alloc<double>(B * Q * n * m);
+    double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
+    if (!W || !z || !z0 || !totals || !part || !part2 || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
     a.in = in;
@@ -504,6 +513,8 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     f.cot = cot;
     f.zbar = zbar;
     f.wbar = wbar;
+    f.part = part;
+    f.part2 = part2;
     f.work = work;
     hipError_t e = magnus::launch_wbd_fold_vjp(f, stream);
     if (e != hipSuccess) return device_error(e, "fold kernel launch");

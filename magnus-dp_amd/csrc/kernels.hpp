@@ -84,7 +84,9 @@ struct WbdArgs {
     double* out;           // [B][n_out][m]              (forward)
     double* zbar;          // [B][n][m]  rows 0..K: Zbar_0..Zbar_K      (VJP)
     double* wbar;          // [B][n][m]  rows 0..K-1: What_bar_1..K     (VJP)
-    double* work;          // [B][4K+1][m] scratch
+    double* part;          // [B][Q][n][m] node partials: Omega_j (forward), Zbar_l (VJP)
+    double* part2;         // [B][Q][n][m] node partials of What_bar_j  (VJP)
+    double* work;          // [B][Q][2K+1][m] per-node scratch (T, T_bar)
     long long B;
     int n, d, m, Q, op, n_out;
     double w[kArgMaxQ];

@@ -218,7 +218,7 @@ __global__ void __launch_bounds__(256) lm_fwd_scan(LmArgs a) {
     const double* Yj = a.Y + job * S * m;
     PrefixState<I> st;
     double acc = 0.0;
-    constexpr int U = 8;  // samples whose inputs are loaded ahead of the recurrence
+    constexpr int U = 16;  // samples whose inputs are loaded ahead of the recurrence
     for (long long i0 = 0; i0 < S; i0 += U) {
         double f[U];
 #pragma unroll
@@ -323,15 +323,16 @@ __global__ void __launch_bounds__(256) lm_vjp_scan(LmArgs a) {
         bt = tl;
         bt += tl * shift;
     }
-    constexpr int U = 8;
-    for (long long i0 = S - 2; i0 >= 0; i0 -= U) {
-        double t[U];
+    constexpr int U = 16;
+    // shift 0 (the W-basis sweep): every term is a signed zero, bt is unchanged
+    if (shift != 0.0) {
+        for (long long i0 = S - 2; i0 >= 0; i0 -= U) {
+            double t[U];
 #pragma unroll
-        for (int u = 0; u < U; ++u) t[u] = (i0 - u >= 0) ? G[(i0 - u) * m] : 0.0;
+            for (int u = 0; u < U; ++u) t[u] = (i0 - u >= 0) ? G[(i0 - u) * m] : 0.0;
 #pragma unroll
-        for (int u = 0; u < U; ++u) {
-            if (i0 - u < 0) break;
-            bt += t[u] * shift;
+            for (int u = 0; u < U; ++u)
+                if (i0 - u >= 0) bt += t[u] * shift;
         }
     }
     if (a.wbar) bt += a.wbar[(job * a.n + (k - 2)) * (long long)m + e];  // cotangent of the level total
@@ -339,7 +340,15 @@ __global__ void __launch_bounds__(256) lm_vjp_scan(LmArgs a) {
     prefix_vjp<I>(G, m, S, a.dt);
     if (k > 2 && a.op != OP_ONE) {  // sum_vjp_add (grad.hpp:232, 332)
         const double o = lm_cot(a, b, a.op == OP_MANY ? k - 2 : 0, e);
-        for (long long i = 0; i < S; ++i) G[i * m] = sum_vjp_add<I>(G[i * m], i, S, o, dscale);
+        constexpr int V = 8;  // loads of a batch issued before its stores
+        for (long long i0 = 0; i0 < S; i0 += V) {
+            double g[V];
+#pragma unroll
+            for (int u = 0; u < V; ++u) g[u] = (i0 + u < S) ? G[(i0 + u) * m] : 0.0;
+#pragma unroll
+            for (int u = 0; u < V; ++u)
+                if (i0 + u < S) G[(i0 + u) * m] = sum_vjp_add<I>(g[u], i0 + u, S, o, dscale);
+        }
     }
 }
 
@@ -433,7 +442,9 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
         timer_end(h);
     }
     void* h = timer_begin("lm_fwd_scan", st);
-    hipLaunchKernelGGL((lm_fwd_scan<I>), dim3(blocks_for(a.jobs * a.m)), dim3(256), 0, st, a);
+    // one wavefront per workgroup: the scans are latency-bound chains, so they
+    // are spread over as many CUs (memory queues) as there are wavefronts
+    hipLaunchKernelGGL((lm_fwd_scan<I>), dim3((a.jobs * a.m + 63) / 64), dim3(64), 0, st, a);
     timer_end(h);
     return hipGetLastError();
 }
@@ -451,7 +462,7 @@ hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
     }
     timer_end(h);
     h = timer_begin("lm_vjp_scan", st);
-    hipLaunchKernelGGL((lm_vjp_scan<I>), dim3(blocks_for(a.jobs * a.m)), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((lm_vjp_scan<I>), dim3((a.jobs * a.m + 63) / 64), dim3(64), 0, st, a);
     timer_end(h);
     return hipGetLastError();
 }

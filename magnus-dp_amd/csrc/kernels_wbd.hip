@@ -9,9 +9,14 @@
 // sweep, and what depends on q is d x d algebra per instance:
 //   T_1 = What_1,  T_j = What_j + s_q sum_{l=1}^{j-1} What_l T_{j-l},
 //   Omega_1 = Z_0, Omega_j = sum_q w_q (Z_{j-1} + s_q sum_{l=0}^{j-2} Z_l T_{j-1-l}).
-// One 256-thread workgroup per instance; every thread owns the entries
-// e = tid + 256 t of every matrix, so accumulations need no atomics and a
-// barrier separates a matrix's last write from its first read as an operand.
+// One 256-thread workgroup per (instance, node q) writes node q's partial
+// results; a reduce kernel adds the nodes in fixed q order.  Products stage
+// both operands in LDS (transposed on the way in) and, for d in {16, 32, 48,
+// 64}, each thread computes a (d/16) x (d/16) register tile; other d take
+// one entry per thread, and d > 64 reads the operands from global memory.
+// Every thread owns the same entries of every matrix it accumulates, so the
+// accumulations need no atomics; a barrier separates a matrix's last write
+// from its first read as an operand.
 #include "kernels.hpp"
 #include "rules.hpp"
 
@@ -20,39 +25,115 @@ namespace magnus {
 namespace {
 
 constexpr int kFoldThreads = 256;
+constexpr int kFoldLds = 64;  // largest d staged in LDS
 
-// C[e] += alpha * (X' Y')[e] for the entries this thread owns;
-// X' = X or X^T, Y' = Y or Y^T (d x d, row-major)
-__device__ void mat_acc(double* __restrict__ C, double alpha, const double* __restrict__ X, bool tx,
+struct FoldLds {
+    double x[kFoldLds * kFoldLds];
+    double y[kFoldLds * kFoldLds];
+};
+
+// C[e] += alpha * (X' Y')[e] for the entries this thread owns (X' = X or X^T,
+// Y' = Y or Y^T; d x d row-major).  Ends with a barrier: C and the staging
+// buffers are free for the next call.
+__device__ void mat_acc(FoldLds& L, double* __restrict__ C, double alpha, const double* __restrict__ X, bool tx,
                         const double* __restrict__ Y, bool ty, int d) {
-    const int m = d * d;
-    for (int e = threadIdx.x; e < m; e += kFoldThreads) {
-        const int r = e / d, c = e - r * d;
-        double acc = 0.0;
-        for (int k = 0; k < d; ++k) {
-            const double xv = tx ? X[k * d + r] : X[r * d + k];
-            const double yv = ty ? Y[c * d + k] : Y[k * d + c];
-            acc = fma(xv, yv, acc);
+    const int m = d * d, tid = threadIdx.x;
+    if (d > kFoldLds) {
+        for (int e = tid; e < m; e += kFoldThreads) {
+            const int r = e / d, c = e - r * d;
+            double acc = 0.0;
+            for (int k = 0; k < d; ++k)
+                acc = fma(tx ? X[k * d + r] : X[r * d + k], ty ? Y[c * d + k] : Y[k * d + c], acc);
+            C[e] = fma(alpha, acc, C[e]);
         }
-        C[e] = fma(alpha, acc, C[e]);
+        __syncthreads();
+        return;
     }
+    // stage X' and Y' row-major
+    for (int e = tid; e < m; e += kFoldThreads) {
+        const int r = e / d, c = e - r * d;
+        L.x[e] = tx ? X[c * d + r] : X[e];
+        L.y[e] = ty ? Y[c * d + r] : Y[e];
+    }
+    __syncthreads();
+    if ((d & 15) == 0) {
+        const int t = d >> 4;  // register tile t x t, threads on a 16 x 16 grid
+        const int r0 = (tid >> 4) * t, c0 = (tid & 15) * t;
+        double acc[4][4];
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j) acc[i][j] = 0.0;
+        for (int k = 0; k < d; ++k) {
+            double xv[4], yv[4];
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                xv[i] = i < t ? L.x[(r0 + i) * d + k] : 0.0;
+                yv[i] = i < t ? L.y[k * d + c0 + i] : 0.0;
+            }
+#pragma unroll
+            for (int i = 0; i < 4; ++i)
+#pragma unroll
+                for (int j = 0; j < 4; ++j) acc[i][j] = fma(xv[i], yv[j], acc[i][j]);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i)
+#pragma unroll
+            for (int j = 0; j < 4; ++j)
+                if (i < t && j < t) {
+                    double* c = C + (r0 + i) * d + c0 + j;
+                    *c = fma(alpha, acc[i][j], *c);
+                }
+    } else {
+        for (int e = tid; e < m; e += kFoldThreads) {
+            const int r = e / d, c = e - r * d;
+            double acc = 0.0;
+            for (int k = 0; k < d; ++k) acc = fma(L.x[r * d + k], L.y[k * d + c], acc);
+            C[e] = fma(alpha, acc, C[e]);
+        }
+    }
+    __syncthreads();
 }
 
-__device__ void mat_set(double* __restrict__ C, double alpha, const double* __restrict__ X, int m) {
+// Entry ownership of mat_set / mat_axpy must match mat_acc's: the tiled
+// layout gives thread tid the entries (r0 + i, c0 + j).
+__device__ __forceinline__ int owned_count(int d) { return ((d & 15) == 0 && d <= kFoldLds) ? (d >> 4) * (d >> 4) : 0; }
+__device__ __forceinline__ int owned_entry(int d, int u) {
+    const int t = d >> 4, tid = threadIdx.x;
+    return ((tid >> 4) * t + u / t) * d + (tid & 15) * t + u % t;
+}
+
+__device__ void mat_set(double* __restrict__ C, double alpha, const double* __restrict__ X, int d) {
+    const int m = d * d, oc = owned_count(d);
+    if (oc) {
+        for (int u = 0; u < oc; ++u) {
+            const int e = owned_entry(d, u);
+            C[e] = X ? alpha * X[e] : 0.0;
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < m; e += kFoldThreads) C[e] = X ? alpha * X[e] : 0.0;
 }
 
-__device__ void mat_axpy(double* __restrict__ C, double alpha, const double* __restrict__ X, int m) {
+__device__ void mat_axpy(double* __restrict__ C, double alpha, const double* __restrict__ X, int d) {
+    const int m = d * d, oc = owned_count(d);
+    if (oc) {
+        for (int u = 0; u < oc; ++u) {
+            const int e = owned_entry(d, u);
+            C[e] = fma(alpha, X[e], C[e]);
+        }
+        return;
+    }
     for (int e = threadIdx.x; e < m; e += kFoldThreads) C[e] = fma(alpha, X[e], C[e]);
 }
 
 // T_1..T_K of node q; T_j at T + j m (T_0 unused)
-__device__ void fold_totals(const WbdArgs& a, const double* What, double* T, double s) {
-    const int K = a.n - 1, m = a.m, d = a.d;
-    const long long M = m;
+__device__ void fold_totals(FoldLds& L, const WbdArgs& a, const double* What, double* T, double s) {
+    const int K = a.n - 1, d = a.d;
+    const long long M = a.m;
     for (int j = 1; j <= K; ++j) {
-        mat_set(T + j * M, 1.0, What + (j - 1) * M, m);
-        for (int l = 1; l < j; ++l) mat_acc(T + j * M, s, What + (l - 1) * M, false, T + (j - l) * M, false, d);
+        mat_set(T + j * M, 1.0, What + (j - 1) * M, d);
+        for (int l = 1; l < j; ++l) mat_acc(L, T + j * M, s, What + (l - 1) * M, false, T + (j - l) * M, false, d);
         __syncthreads();
     }
 }
@@ -70,81 +151,112 @@ __device__ __forceinline__ const double* obar(const WbdArgs& a, long long b, int
     return j == a.n ? c : nullptr;
 }
 
+// node q's contribution to Omega_2..Omega_n: work rows [b][q][n-1][m]
 __global__ void __launch_bounds__(kFoldThreads) wbd_fold(WbdArgs a) {
-    const long long b = blockIdx.x;
-    const int n = a.n, K = n - 1, m = a.m, d = a.d;
-    const long long M = m;
-    double* work = a.work + b * (long long)(4 * K + 1) * M;
-    double* T = work;                 // T_j at T + j M, j = 1..K
-    double* Om = work + (K + 1) * M;  // Omega_j at Om + (j-1) M, j = 1..n
-    const double* What = a.totals + b * (long long)a.n * M;
-    mat_set(Om, 1.0, zrow(a, b, 0), m);
-    for (int j = 2; j <= n; ++j) mat_set(Om + (j - 1) * M, 0.0, nullptr, m);
-    __syncthreads();
-    for (int q = 0; q < a.Q; ++q) {
-        const double s = a.x[q] - 1.0, wq = a.w[q];
-        fold_totals(a, What, T, s);
-        for (int j = 2; j <= n; ++j) {
-            double* O = Om + (j - 1) * M;
-            mat_axpy(O, wq, zrow(a, b, j - 1), m);
-            for (int l = 0; l <= j - 2; ++l) mat_acc(O, wq * s, zrow(a, b, l), false, T + (j - 1 - l) * M, false, d);
-        }
-        __syncthreads();
-    }
-    double* out = a.out + b * (long long)a.n_out * m;
-    for (int e = threadIdx.x; e < m; e += kFoldThreads) {
-        if (a.op == OP_MANY) {
-            for (int j = 0; j < n; ++j) out[j * (long long)m + e] = Om[j * (long long)m + e];
-        } else if (a.op == OP_SUM) {
-            double v = Om[e];
-            for (int j = 1; j < n; ++j) v += Om[j * (long long)m + e];
-            out[e] = v;
-        } else {
-            out[e] = Om[(n - 1) * (long long)m + e];
-        }
+    __shared__ FoldLds L;
+    const long long b = blockIdx.x / a.Q;
+    const int q = (int)(blockIdx.x - b * a.Q);
+    const int n = a.n, K = n - 1, d = a.d;
+    const long long M = a.m;
+    double* T = a.work + (long long)blockIdx.x * (2 * K + 1) * M;  // T_j at T + j M
+    double* Om = a.part + ((long long)blockIdx.x * n) * M;          // Omega_j at Om + (j-1) M, j = 2..n
+    const double* What = a.totals + b * (long long)n * M;
+    const double s = a.x[q] - 1.0, wq = a.w[q];
+    fold_totals(L, a, What, T, s);
+    for (int j = 2; j <= n; ++j) {
+        double* O = Om + (j - 1) * M;
+        mat_set(O, wq, zrow(a, b, j - 1), d);
+        for (int l = 0; l <= j - 2; ++l) mat_acc(L, O, wq * s, zrow(a, b, l), false, T + (j - 1 - l) * M, false, d);
     }
 }
 
-// Adjoint of the fold: Zbar_0..Zbar_K and What_bar_1..What_bar_K from the
-// output cotangent (reverse of wbd_fold, node by node).
+// out from Omega_1 = Z_0 and the node partials, nodes added in q order
+__global__ void __launch_bounds__(256) wbd_fold_reduce(WbdArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long M = a.m;
+    if (idx >= a.B * M) return;
+    const long long b = idx / M;
+    const long long e = idx - b * M;
+    const int n = a.n;
+    auto omega = [&](int j) {  // Omega_j, j >= 2
+        double v = 0.0;
+        for (int q = 0; q < a.Q; ++q) v += a.part[((b * a.Q + q) * n + (j - 1)) * M + e];
+        return v;
+    };
+    double* out = a.out + b * (long long)a.n_out * M + e;
+    const double o1 = a.z0[b * M + e];
+    if (a.op == OP_MANY) {
+        out[0] = o1;
+        for (int j = 2; j <= n; ++j) out[(j - 1) * M] = omega(j);
+    } else if (a.op == OP_SUM) {
+        double v = o1;
+        for (int j = 2; j <= n; ++j) v += omega(j);
+        out[0] = v;
+    } else {
+        out[0] = omega(n);
+    }
+}
+
+// Adjoint of node q's fold: its Zbar_l and What_bar_j contributions, rows
+// [b][q][n][m] of part (Zbar) and part2 (What_bar).
 __global__ void __launch_bounds__(kFoldThreads) wbd_fold_vjp(WbdArgs a) {
-    const long long b = blockIdx.x;
-    const int n = a.n, K = n - 1, m = a.m, d = a.d;
-    const long long M = m;
-    double* work = a.work + b * (long long)(4 * K + 1) * M;
-    double* T = work;                  // T_j at T + j M
-    double* Tb = work + (K + 1) * M;   // T_bar_j at Tb + j M
-    const double* What = a.totals + b * (long long)a.n * M;
-    double* Zb = a.zbar + b * (long long)a.n * M;  // row l = Zbar_l
-    double* Wb = a.wbar + b * (long long)a.n * M;  // row j-1 = What_bar_j
-    mat_set(Zb, 1.0, obar(a, b, 1), m);  // Omega_1 = Z_0
-    for (int l = 1; l <= K; ++l) mat_set(Zb + l * M, 0.0, nullptr, m);
-    for (int j = 1; j <= n; ++j) mat_set(Wb + (j - 1) * M, 0.0, nullptr, m);
-    __syncthreads();
-    for (int q = 0; q < a.Q; ++q) {
-        const double s = a.x[q] - 1.0, wq = a.w[q];
-        fold_totals(a, What, T, s);
-        for (int j = 1; j <= K; ++j) mat_set(Tb + j * M, 0.0, nullptr, m);
-        __syncthreads();
-        for (int j = 2; j <= n; ++j) {
-            const double* Ob = obar(a, b, j);
-            if (!Ob) continue;
-            mat_axpy(Zb + (j - 1) * M, wq, Ob, m);
-            for (int l = 0; l <= j - 2; ++l) {
-                mat_acc(Zb + l * M, wq * s, Ob, false, T + (j - 1 - l) * M, true, d);
-                mat_acc(Tb + (j - 1 - l) * M, wq * s, zrow(a, b, l), true, Ob, false, d);
-            }
-        }
-        __syncthreads();
-        for (int j = K; j >= 1; --j) {
-            mat_axpy(Wb + (j - 1) * M, 1.0, Tb + j * M, m);
-            for (int l = 1; l < j; ++l) {
-                mat_acc(Wb + (l - 1) * M, s, Tb + j * M, false, T + (j - l) * M, true, d);
-                mat_acc(Tb + (j - l) * M, s, What + (l - 1) * M, true, Tb + j * M, false, d);
-            }
-            __syncthreads();
+    __shared__ FoldLds L;
+    const long long b = blockIdx.x / a.Q;
+    const int q = (int)(blockIdx.x - b * a.Q);
+    const int n = a.n, K = n - 1, d = a.d;
+    const long long M = a.m;
+    double* T = a.work + (long long)blockIdx.x * (2 * K + 1) * M;  // T_j at T + j M
+    double* Tb = T + K * M;                                        // T_bar_j at Tb + j M
+    const double* What = a.totals + b * (long long)n * M;
+    double* Zb = a.part + (long long)blockIdx.x * n * M;   // row l = Zbar_l (node q)
+    double* Wb = a.part2 + (long long)blockIdx.x * n * M;  // row j-1 = What_bar_j (node q)
+    const double s = a.x[q] - 1.0, wq = a.w[q];
+    for (int l = 0; l < n; ++l) {
+        mat_set(Zb + l * M, 0.0, nullptr, d);
+        mat_set(Wb + l * M, 0.0, nullptr, d);
+    }
+    for (int j = 1; j <= K; ++j) mat_set(Tb + j * M, 0.0, nullptr, d);
+    fold_totals(L, a, What, T, s);
+    for (int j = 2; j <= n; ++j) {
+        const double* Ob = obar(a, b, j);
+        if (!Ob) continue;
+        mat_axpy(Zb + (j - 1) * M, wq, Ob, d);
+        for (int l = 0; l <= j - 2; ++l) {
+            mat_acc(L, Zb + l * M, wq * s, Ob, false, T + (j - 1 - l) * M, true, d);
+            mat_acc(L, Tb + (j - 1 - l) * M, wq * s, zrow(a, b, l), true, Ob, false, d);
         }
     }
+    __syncthreads();
+    for (int j = K; j >= 1; --j) {
+        mat_axpy(Wb + (j - 1) * M, 1.0, Tb + j * M, d);
+        for (int l = 1; l < j; ++l) {
+            mat_acc(L, Wb + (l - 1) * M, s, Tb + j * M, false, T + (j - l) * M, true, d);
+            mat_acc(L, Tb + (j - l) * M, s, What + (l - 1) * M, true, Tb + j * M, false, d);
+        }
+        __syncthreads();
+    }
+}
+
+// zbar / wbar = sum of the node partials in q order (+ Omega_1's cotangent on Zbar_0)
+__global__ void __launch_bounds__(256) wbd_fold_vjp_reduce(WbdArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long M = a.m;
+    const int n = a.n;
+    if (idx >= a.B * n * M) return;
+    const long long b = idx / (n * M);
+    const long long r = idx - b * n * M;  // row * M + e
+    double z = 0.0, w = 0.0;
+    if (r < M) {
+        const double* o1 = obar(a, b, 1);
+        z = o1 ? o1[r] : 0.0;
+    }
+    for (int q = 0; q < a.Q; ++q) {
+        const long long off = ((b * a.Q + q) * n) * M + r;
+        z += a.part[off];
+        w += a.part2[off];
+    }
+    a.zbar[idx] = z;
+    a.wbar[idx] = w;
 }
 
 }  // namespace
@@ -152,7 +264,8 @@ __global__ void __launch_bounds__(kFoldThreads) wbd_fold_vjp(WbdArgs a) {
 hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     void* h = timer_begin("wbd_fold", s);
-    hipLaunchKernelGGL(wbd_fold, dim3((unsigned)a.B), dim3(kFoldThreads), 0, s, a);
+    hipLaunchKernelGGL(wbd_fold, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
+    hipLaunchKernelGGL(wbd_fold_reduce, dim3((unsigned)((a.B * a.m + 255) / 256)), dim3(256), 0, s, a);
     timer_end(h);
     return hipGetLastError();
 }
@@ -160,7 +273,8 @@ hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s) {
 hipError_t launch_wbd_fold_vjp(const WbdArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     void* h = timer_begin("wbd_fold_vjp", s);
-    hipLaunchKernelGGL(wbd_fold_vjp, dim3((unsigned)a.B), dim3(kFoldThreads), 0, s, a);
+    hipLaunchKernelGGL(wbd_fold_vjp, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
+    hipLaunchKernelGGL(wbd_fold_vjp_reduce, dim3((unsigned)((a.B * a.n * a.m + 255) / 256)), dim3(256), 0, s, a);
     timer_end(h);
     return hipGetLastError();
 }

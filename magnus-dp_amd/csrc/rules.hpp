@@ -220,43 +220,60 @@ struct PrefixState<RIEMANN> {
 // Reverse sweeps (prefix_vjp) on one scalar stream G[i * stride], in place.
 // ---------------------------------------------------------------------------
 
-// boole.hpp:108-151
+// boole.hpp:108-151.  Blocks are taken in groups of kVjpGroup, highest
+// first; a group's raw values are loaded before any of its stores (a block
+// writes G[s+1..s+4], which the lower blocks of the group do not read: their
+// G[s+4] is the chained g4), so the loads of a group are in flight together.
+constexpr int kVjpGroup = 8;
 __device__ __forceinline__ void prefix_vjp_boole(double* G, long long stride, long long S, double dt) {
     double pending = 0.0;
-    long long s = S - 5;
-    double g4 = G[(s + 4) * stride];
-    for (;;) {
-        const double g1 = G[(s + 1) * stride];
-        const double g2 = G[(s + 2) * stride];
-        const double g3 = G[(s + 3) * stride];
-        double a0 = G[s * stride];
-        a0 += g1;
-        a0 += g2;
-        a0 += g3;
-        a0 += g4;
-        double a1 = g1 * (dt / 2);
-        a1 += g2 * (4 * dt / 3);
-        a1 += g3 * (9 * dt / 8);
-        a1 += g4 * (64 * dt / 45);
-        double a2 = g2 * (dt / 3);
-        a2 += g3 * (9 * dt / 8);
-        a2 += g4 * (24 * dt / 45);
-        double a3 = g3 * (3 * dt / 8);
-        a3 += g4 * (64 * dt / 45);
-        double a4 = g4 * (14 * dt / 45);
-        a4 += pending;
-        G[(s + 1) * stride] = a1;
-        G[(s + 2) * stride] = a2;
-        G[(s + 3) * stride] = a3;
-        G[(s + 4) * stride] = a4;
-        double p = g1 * (dt / 2);
-        p += g2 * (dt / 3);
-        p += g3 * (3 * dt / 8);
-        p += g4 * (14 * dt / 45);
-        pending = p;
-        g4 = a0;  // G[s] after its update is g4 of the previous block
-        if (s == 0) break;
-        s -= 4;
+    const long long nb = (S - 1) / 4;  // blocks [4b, 4b+4], b = nb-1 .. 0
+    double g4 = G[(S - 1) * stride];
+    for (long long top = nb - 1; top >= 0; top -= kVjpGroup) {
+        double r0[kVjpGroup], r1[kVjpGroup], r2[kVjpGroup], r3[kVjpGroup];
+#pragma unroll
+        for (int u = 0; u < kVjpGroup; ++u) {
+            const long long s = 4 * (top - u);
+            r0[u] = r1[u] = r2[u] = r3[u] = 0.0;
+            if (top - u >= 0) {
+                r0[u] = G[s * stride];
+                r1[u] = G[(s + 1) * stride];
+                r2[u] = G[(s + 2) * stride];
+                r3[u] = G[(s + 3) * stride];
+            }
+        }
+#pragma unroll
+        for (int u = 0; u < kVjpGroup; ++u) {
+            if (top - u < 0) break;
+            const long long s = 4 * (top - u);
+            const double g1 = r1[u], g2 = r2[u], g3 = r3[u];
+            double a0 = r0[u];
+            a0 += g1;
+            a0 += g2;
+            a0 += g3;
+            a0 += g4;
+            double a1 = g1 * (dt / 2);
+            a1 += g2 * (4 * dt / 3);
+            a1 += g3 * (9 * dt / 8);
+            a1 += g4 * (64 * dt / 45);
+            double a2 = g2 * (dt / 3);
+            a2 += g3 * (9 * dt / 8);
+            a2 += g4 * (24 * dt / 45);
+            double a3 = g3 * (3 * dt / 8);
+            a3 += g4 * (64 * dt / 45);
+            double a4 = g4 * (14 * dt / 45);
+            a4 += pending;
+            G[(s + 1) * stride] = a1;
+            G[(s + 2) * stride] = a2;
+            G[(s + 3) * stride] = a3;
+            G[(s + 4) * stride] = a4;
+            double p = g1 * (dt / 2);
+            p += g2 * (dt / 3);
+            p += g3 * (3 * dt / 8);
+            p += g4 * (14 * dt / 45);
+            pending = p;
+            g4 = a0;  // G[s] after its update is g4 of the block below
+        }
     }
     G[0] = pending;
 }

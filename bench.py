@@ -405,13 +405,170 @@ def _bench_uneven(args, torch, dev):
     print(json.dumps(line))
 
 
+def _cpu_baseline_controller(coeffs, cot, budget_s):
+    """The reference CPU path at the plain resolution (S = 16385, n = 8,
+    forward op='sum' + VJP) on host cores, one instance per core, over a
+    time-bounded sample of the evaluation set."""
+    import concurrent.futures as cf
+
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+    from magnus import controller as C, torch_ops
+
+    S, n = C.PLAIN
+    cores = min(len(os.sched_getaffinity(0)), 16)
+
+    def one(i):
+        a = torch_ops.sample_fourier(coeffs[i:i + 1], S)[0].cpu().numpy()
+        c = cot[i].cpu().numpy()
+        O.forward(n, a, 0.0, 1.0, op="sum", integrator="Boole")
+        O.vjp(n, a, c, 0.0, 1.0, op="sum", integrator="Boole")
+
+    done, t0 = 0, time.perf_counter()
+    with cf.ThreadPoolExecutor(cores) as ex:
+        while time.perf_counter() - t0 < budget_s and done + cores <= coeffs.shape[0]:
+            list(ex.map(one, range(done, done + cores)))
+            done += cores
+    el = time.perf_counter() - t0
+    return {"value": done / el, "unit": "solves/s", "cores": cores, "kind": "port",
+            "sample": f"{done} plain-resolution (S={S}, n={n}) forward+VJP solves of the evaluation set, one "
+                      f"instance per core, in {el:.1f} s"}
+
+
+def _bench_controller(args, torch, dev):
+    import numpy as np
+
+    from magnus import controller as C
+    from magnus import torch_ops
+
+    tol = 1e-8
+    n_tr, n_ev = args.ctrl_train, args.ctrl_eval
+    log = lambda msg: print(f"[bench] controller: {msg}", file=sys.stderr, flush=True)  # noqa: E731
+    t = time.perf_counter()
+    c_tr = C.make_coefficients(n_tr, seed=11)
+    c_ev = C.make_coefficients(n_ev, seed=12)
+    f_tr = C.features(torch_ops.sample_fourier(c_tr, C.FEATURE_GRID), 0.0, 1.0)
+    lab_tr, min_tr, reach_tr, _, _ = C.label(c_tr, tol)
+    lab_ev, min_ev, reach_ev, errs_ev, refs_ev = C.label(c_ev, tol)
+    torch.cuda.synchronize()
+    label_s = time.perf_counter() - t
+    hist = np.bincount(lab_tr, minlength=len(C.CLASSES))
+    top = [(C.CLASSES[k], int(hist[k])) for k in np.argsort(-hist)[:8]]
+    log(f"{n_tr}+{n_ev} instances labelled in {label_s:.1f} s; reachable {reach_tr.mean():.3f}; top labels {top}")
+    ctl = C.Controller(f_tr.shape[1], seed=1)
+    (loss_hist), train_s = C.timed(lambda: ctl.fit(f_tr, torch.from_numpy(lab_tr).to(dev), epochs=args.ctrl_epochs,
+                                                   batch=1024, lr=2e-3))
+    log(f"trained {args.ctrl_epochs} epochs in {train_s:.1f} s; loss {loss_hist[0]:.3f} -> {loss_hist[-1]:.3f}")
+    f_ev = C.features(torch_ops.sample_fourier(c_ev, C.FEATURE_GRID), 0.0, 1.0)
+    pred = ctl.predict(f_ev).cpu().numpy()
+    costs = np.array([C.cost(*c) for c in C.CLASSES])
+    flat = errs_ev.reshape(n_ev, -1)
+    sufficient = flat[np.arange(n_ev), pred] <= tol
+    acc = {"exact": float((pred == lab_ev).mean()), "sufficient": float(sufficient.mean()),
+           "cost_ratio_vs_label": float(costs[pred].sum() / costs[lab_ev].sum()),
+           "label_cost_vs_minimal": float(costs[lab_ev].sum() / costs[min_ev].sum())}
+    log(f"eval prediction {acc}")
+    cot = torch.from_numpy(np.stack([np.random.default_rng([12, i, 1]).standard_normal((8, 8))
+                                     for i in range(n_ev)])).to(dev)
+
+    def plain():
+        return C.solve(c_ev, [C.PLAIN] * n_ev, cot)
+
+    stages = {}
+
+    def learned():
+        start, t1 = C.timed(lambda: [C.CLASSES[k] for k in ctl.predict(
+            C.features(torch_ops.sample_fourier(c_ev, C.FEATURE_GRID), 0.0, 1.0)).cpu().numpy()])
+        (res, fwd, st), t2 = C.timed(lambda: C.accept(c_ev, start, tol))
+        (out, _, w), t3 = C.timed(lambda: C.solve(c_ev, res, cot, forward_out=fwd))
+        stages["learned"] = {"predict_s": round(t1, 3), "judge_s": round(t2, 3), "solve_s": round(t3, 3),
+                             "judge_calls": st["calls"]}
+        return res, out, st, w
+
+    def nonet():
+        (res, fwd, st), t2 = C.timed(lambda: C.accept(c_ev, [(C.S_GRID[0], C.N_GRID[0])] * n_ev, tol))
+        (out, _, w), t3 = C.timed(lambda: C.solve(c_ev, res, cot, forward_out=fwd))
+        stages["no_network"] = {"judge_s": round(t2, 3), "solve_s": round(t3, 3), "judge_calls": st["calls"]}
+        return res, out, st, w
+
+    from magnus import capi
+
+    kernel_ms = {}
+
+    def best(fn, reps=2):  # after one warm-up run, the faster of `reps` timed runs
+        fn()
+        runs = [C.timed(fn) for _ in range(reps)]
+        # one more run with the per-kernel timers on (diagnostics only)
+        capi.timing_reset()
+        capi.timing_enable(True)
+        fn()
+        torch.cuda.synchronize()
+        capi.timing_enable(False)
+        tot = {}
+        for name in KERNELS + ("wbd_fold", "wbd_fold_vjp"):
+            cnt, ms = capi.timing_read(name)
+            if cnt:
+                tot[name] = (cnt, round(ms, 1))
+        kernel_ms[fn.__name__] = tot
+        log(f"{fn.__name__} kernels {tot}")
+        return min(runs, key=lambda r: r[1])
+
+    (out_p, _, w_p), t_p = best(plain)
+    log(f"plain {t_p:.2f} s")
+    (res_l, out_l, st_l, w_l), t_l = best(learned)
+    log(f"learned {t_l:.2f} s, rounds {np.bincount(st_l['rounds'])}")
+    (res_0, out_0, st_0, w_0), t_0 = best(nonet)
+    log(f"without network {t_0:.2f} s, rounds mean {st_0['rounds'].mean():.2f}")
+    hit = lambda o: float((C.rel_err(o, refs_ev) <= tol).float().mean())  # noqa: E731
+    cpu = None if args.no_cpu else _cpu_baseline_controller(c_ev, cot, args.cpu_budget)
+    value = n_ev / t_l
+    line = {
+        "metric": "Magnus fwd+VJP solves/s at rel err <=1e-8, learned vs plain, 1 GPU",
+        "value": round(value, 3),
+        "unit": "solves/s",
+        "n_gpus": 1,
+        "steps": 2,
+        "warmup": 1,
+        "ms_per_step": round(t_l * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (seeded Fourier-form A(t), d=8, 3-8 modes, ||A||_2 T ~ U[0.1, 4]; device-sampled)",
+        "config": {"workload": "controller", "train": n_tr, "eval": n_ev, "dim": 8, "tol": tol,
+                   "samples": list(C.S_GRID), "orders": list(C.N_GRID), "reference": [C.REF_S, C.REF_N],
+                   "plain": list(C.PLAIN), "mlp": "24-128-128-63, bf16 weights, fp32 accumulation, Adam"},
+        "learned_vs_plain": round(t_p / t_l, 3),
+        "learned_vs_no_network": round(t_0 / t_l, 3),
+        "plain": {"s": round(t_p, 3), "solves_per_s": round(n_ev / t_p, 3), "hit_rate": hit(out_p)},
+        "learned": {"s": round(t_l, 3), "hit_rate": hit(out_l), "judge_work": int(st_l["work"]),
+                    "solve_work": int(w_l), "mean_rounds": float(st_l["rounds"].mean())},
+        "no_network": {"s": round(t_0, 3), "solves_per_s": round(n_ev / t_0, 3), "hit_rate": hit(out_0),
+                       "judge_work": int(st_0["work"]), "solve_work": int(w_0),
+                       "mean_rounds": float(st_0["rounds"].mean())},
+        "prediction": acc,
+        "kernel_ms": kernel_ms,
+        "stages": stages,
+        "label_s": round(label_s, 2),
+        "train_s": round(train_s, 2),
+        "roofline": None,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(line))
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="spacecurve-batch",
-                    choices=["spacecurve-batch", "dense-d32", "plain-d4", "uneven"])
+                    choices=["spacecurve-batch", "dense-d32", "plain-d4", "uneven", "controller"])
+    ap.add_argument("--ctrl-train", type=int, default=200000)
+    ap.add_argument("--ctrl-eval", type=int, default=10000)
+    ap.add_argument("--ctrl-epochs", type=int, default=20)
     ap.add_argument("--uneven-branches", type=int, default=4)
     ap.add_argument("--uneven-workspace-gb", type=float, default=48.0)
     ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
@@ -445,6 +602,9 @@ def main():
     wl = args.workload
     if wl == "uneven":
         _bench_uneven(args, torch, dev)
+        return
+    if wl == "controller":
+        _bench_controller(args, torch, dev)
         return
     if wl == "spacecurve-batch":
         B, S, n, modes = args.batch or 4096, 16385, 6, 4

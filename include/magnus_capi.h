@@ -136,6 +136,14 @@ int magnus_sc_forward_f64(const magnus_problem* p, uint64_t count, const double*
 int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3,
                       const double* cot3, const double* carry, double* dA3, void* stream);
 
+/* Magnus sums of every order from one basis sweep (no counterpart in the
+ * reference, which solves one order per call): out [count][n][dim][dim], row
+ * k-1 = the op="sum" output of order k = 1..p->n (each with its own GL nodes),
+ * the same values magnus_forward_f64 returns for order k and op "sum" up to
+ * the reassociation of the dense W-basis path.  p->op is ignored; matrix
+ * problems only.  For resolution control (choosing n per instance). */
+int magnus_forward_orders_f64(const magnus_problem* p, uint64_t count, const double* in, double* out, void* stream);
+
 /* ---- compact checkpoints of the single-pass SpaceCurve path (no counterpart in
  *      the reference, whose saved state is the full carry above).  The forward
  *      keeps its per-chunk summaries (2(n-1)+1 4-vectors per chunk of samples,
@@ -223,6 +231,37 @@ int magnus_uneven_set_tuning(magnus_uneven_plan* plan, const char* text);
 
 /* plan facts: 0 buckets, 1 sub-batches, 2 branches in use, 3 workspace bytes */
 uint64_t magnus_uneven_info(const magnus_uneven_plan* plan, int what);
+
+/* ---- resolution controller (no counterpart in the reference).  Features of a
+ *      coarse sampling and a 3-layer MLP (in -> hidden -> hidden -> out, ReLU)
+ *      trained on the device: FP32 master parameters, bfloat16 weight copies
+ *      read by the layer kernels, FP32 activations and accumulation, Adam
+ *      (beta 0.9 / 0.999, eps 1e-8).  x, logits, labels and loss_rows are
+ *      device pointers; parameter and gradient copies are host arrays laid out
+ *      W1 [hidden][in], b1, W2 [hidden][hidden], b2, W3 [out][hidden], b3.
+ *      Calls on one network must not overlap. ---- */
+
+/* features [count][magnus_ctrl_feature_count()] (float) of A sampled on a
+ * grid of 2^k + 1 points (9..129) of [t0, tf], in [count][grid][dim][dim], dim <= 16 */
+uint64_t magnus_ctrl_feature_count(void);
+int magnus_ctrl_features_f64(uint64_t count, uint64_t grid, uint64_t dim, double t0, double tf, const double* in,
+                             float* out, void* stream);
+
+typedef struct magnus_mlp magnus_mlp;
+int magnus_mlp_create(uint32_t in_dim, uint32_t hidden, uint32_t out_dim, uint64_t max_batch, uint64_t seed,
+                      magnus_mlp** out);
+void magnus_mlp_destroy(magnus_mlp* net);
+uint64_t magnus_mlp_param_count(const magnus_mlp* net);
+int magnus_mlp_get_params(const magnus_mlp* net, float* host, uint64_t len);
+/* also refreshes the bfloat16 copies and resets the optimizer state */
+int magnus_mlp_set_params(magnus_mlp* net, const float* host, uint64_t len);
+/* gradients of the last training step (before its parameter update) */
+int magnus_mlp_get_grads(const magnus_mlp* net, float* host, uint64_t len);
+int magnus_mlp_forward(magnus_mlp* net, uint64_t batch, const float* x, float* logits, void* stream);
+/* softmax cross-entropy (mean over the batch) on int32 class labels, backward,
+ * one Adam step at learning rate lr; loss_rows [batch] gets each row's loss */
+int magnus_mlp_train_step(magnus_mlp* net, uint64_t batch, const float* x, const int32_t* labels, float lr,
+                          float* loss_rows, void* stream);
 
 /* ---- kernel timing (diagnostics).  While enabled, every main kernel launch
  *      is bracketed by a HIP event pair recorded on its launch stream.

@@ -37,9 +37,9 @@ COMMON = [
 
 DEVICE_SOURCES = [
     "kernels_lm.hip", "kernels_wb.hip", "kernels_sample.hip", "kernels_expm.hip", "kernels_uneven.hip",
-    "kernels_wbd.hip",
+    "kernels_wbd.hip", "kernels_ctrl.hip", "kernels_mlp.hip",
 ]
-HOST_SOURCES = ["capi.cpp", "uneven.cpp"]
+HOST_SOURCES = ["capi.cpp", "uneven.cpp", "mlp.cpp"]
 
 
 def _run(cmd):

@@ -39,6 +39,25 @@ int fail(int code, const std::string& msg) {
 
 void set_last_kernel(const char* name) { g_kernel = name; }
 
+void keep_pool_memory() {
+    static std::mutex mu;
+    static uint64_t done = 0;  // bit per device
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+        (void)hipGetLastError();
+        return;
+    }
+    std::lock_guard<std::mutex> lock(mu);
+    if (done & (1ull << dev)) return;
+    done |= 1ull << dev;
+    hipMemPool_t pool;
+    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
+        uint64_t threshold = UINT64_MAX;
+        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+    }
+    (void)hipGetLastError();
+}
+
 // ---------------------------------------------------------------------------
 // registries
 // ---------------------------------------------------------------------------
@@ -556,6 +575,62 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     return MAGNUS_OK;
 }
 
+// Magnus sums of every order 1..n from one basis sweep: row k-1 of out is the
+// op="sum" output of order k (its own Q(k) GL nodes).
+int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hipStream_t stream) {
+    DeviceScratch scratch(stream);
+    const long long B = (long long)count;
+    const size_t S = pl.S, m = pl.m, n = pl.n, Q = pl.Q;
+    double* z = scratch.alloc<double>(B * n * m);
+    double* z0 = scratch.alloc<double>(B * m);
+    double* totals = scratch.alloc<double>(B * n * m);
+    double* part = scratch.alloc<double>(B * Q * n * m);
+    double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
+    double* pingpong = n >= 2 ? scratch.alloc<double>(B * 2 * S * m) : nullptr;
+    double* Y = n >= 2 ? scratch.alloc<double>(B * S * m) : nullptr;
+    if (!z || !z0 || !totals || !part || !work || (n >= 2 && (!pingpong || !Y)))
+        return fail(MAGNUS_ENOMEM, "device allocation failed");
+    magnus::LmArgs a{};
+    fill_wbd_lm(pl, B, a);
+    a.in = in;
+    a.pingpong = pingpong;
+    a.Y = Y;
+    a.partial = z;
+    a.omega1 = z0;
+    a.totals = totals;
+    a.outputs = 1;
+    for (int j = 1; j <= (int)n; ++j) {
+        a.level = j;
+        hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
+        if (e != hipSuccess) return device_error(e, "forward kernel launch");
+    }
+    hipError_t e = hipMemcpy2DAsync(out, n * m * sizeof(double), z0, m * sizeof(double), m * sizeof(double), B,
+                                    hipMemcpyDeviceToDevice, stream);
+    if (e != hipSuccess) return device_error(e, "order-1 copy");
+    for (size_t k = 2; k <= n; ++k) {
+        Plan pk = pl;
+        pk.n = k;
+        pk.Q = (k + 3) / 2;
+        pk.Qj = pk.Q;
+        pk.op = MAGNUS_OP_SUM;
+        pk.n_out = 1;
+        magnus::WbdArgs f{};
+        fill_wbd_fold(pk, B, f);
+        f.rows = (int)n;
+        f.z = z;
+        f.z0 = z0;
+        f.totals = totals;
+        f.out = out + (k - 1) * m;
+        f.ostride = (long long)(n * m);
+        f.part = part;
+        f.work = work;
+        e = magnus::launch_wbd_fold(f, stream);
+        if (e != hipSuccess) return device_error(e, "fold kernel launch");
+    }
+    g_kernel = "wbd";
+    return MAGNUS_OK;
+}
+
 // forward: record into `carry` when non-null; outputs=false runs VJP::none
 int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, double* carry, bool outputs,
                 hipStream_t stream, Arena* arena) {
@@ -781,6 +856,15 @@ int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3
     const double* c = pl.n >= 2 ? carry : nullptr;
     if (use_wb(pl, c != nullptr)) return run_wb(pl, count, in3, cot3, nullptr, dA3, nullptr, (hipStream_t)stream);
     return run_vjp(pl, count, in3, cot3, c, dA3, (hipStream_t)stream);
+}
+
+int magnus_forward_orders_f64(const magnus_problem* p, uint64_t count, const double* in, double* out, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, true, &pl)) return rc;
+    if (pl.sc) return fail(MAGNUS_EINVAL, "all-orders forward takes matrix problems (not the SpaceCurve backend)");
+    if (count == 0) return MAGNUS_OK;
+    if (!in || !out) return fail(MAGNUS_EINVAL, "null argument");
+    return run_orders(pl, count, in, out, (hipStream_t)stream);
 }
 
 uint64_t magnus_sc_checkpoint_elems(const magnus_problem* p, uint64_t count) {

@@ -49,6 +49,12 @@ struct Arena {
 constexpr size_t kScratchAlign = 256;
 inline size_t scratch_round(size_t bytes) { return (bytes + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
 
+// Keep memory freed by stream-ordered frees in the current device's default
+// pool (release threshold "unlimited"), once per device: with the default
+// threshold 0 every synchronization returns it to the driver and the next
+// call maps gigabytes of scratch again.
+void keep_pool_memory();
+
 // Temporary device buffers: stream-ordered allocations freed on the same
 // stream at scope exit, or, with an arena, slices of it (no allocation calls,
 // so the work can be captured into a graph).
@@ -75,6 +81,7 @@ struct DeviceScratch {
             arena->used += bytes;
             return static_cast<T*>(p);
         }
+        keep_pool_memory();
         void* p = nullptr;
         if (hipMallocAsync(&p, count * sizeof(T), stream) != hipSuccess) {
             (void)hipGetLastError();

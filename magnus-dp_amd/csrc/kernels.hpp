@@ -88,7 +88,9 @@ struct WbdArgs {
     double* part2;         // [B][Q][n][m] node partials of What_bar_j  (VJP)
     double* work;          // [B][Q][2K+1][m] per-node scratch (T, T_bar)
     long long B;
+    long long ostride;     // doubles between instances in out (0: n_out * m)
     int n, d, m, Q, op, n_out;
+    int rows;              // row stride of z / totals / zbar / wbar (0: n); rows >= n
     double w[kArgMaxQ];
     double x[kArgMaxQ];
 };

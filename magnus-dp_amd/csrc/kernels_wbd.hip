@@ -139,8 +139,10 @@ __device__ void fold_totals(FoldLds& L, const WbdArgs& a, const double* What, do
 }
 
 // Z_l of the instance: row l of z for l >= 1, z0 for l = 0
+__device__ __forceinline__ int row_stride(const WbdArgs& a) { return a.rows ? a.rows : a.n; }
+
 __device__ __forceinline__ const double* zrow(const WbdArgs& a, long long b, int l) {
-    return l == 0 ? a.z0 + b * a.m : a.z + (b * a.n + l) * (long long)a.m;
+    return l == 0 ? a.z0 + b * a.m : a.z + (b * row_stride(a) + l) * (long long)a.m;
 }
 
 // cotangent of Omega_j (j = 1..n) from the output cotangent, or nullptr
@@ -160,7 +162,7 @@ __global__ void __launch_bounds__(kFoldThreads) wbd_fold(WbdArgs a) {
     const long long M = a.m;
     double* T = a.work + (long long)blockIdx.x * (2 * K + 1) * M;  // T_j at T + j M
     double* Om = a.part + ((long long)blockIdx.x * n) * M;          // Omega_j at Om + (j-1) M, j = 2..n
-    const double* What = a.totals + b * (long long)n * M;
+    const double* What = a.totals + b * (long long)row_stride(a) * M;
     const double s = a.x[q] - 1.0, wq = a.w[q];
     fold_totals(L, a, What, T, s);
     for (int j = 2; j <= n; ++j) {
@@ -183,7 +185,7 @@ __global__ void __launch_bounds__(256) wbd_fold_reduce(WbdArgs a) {
         for (int q = 0; q < a.Q; ++q) v += a.part[((b * a.Q + q) * n + (j - 1)) * M + e];
         return v;
     };
-    double* out = a.out + b * (long long)a.n_out * M + e;
+    double* out = a.out + b * (a.ostride ? a.ostride : (long long)a.n_out * M) + e;
     const double o1 = a.z0[b * M + e];
     if (a.op == OP_MANY) {
         out[0] = o1;
@@ -207,7 +209,7 @@ __global__ void __launch_bounds__(kFoldThreads) wbd_fold_vjp(WbdArgs a) {
     const long long M = a.m;
     double* T = a.work + (long long)blockIdx.x * (2 * K + 1) * M;  // T_j at T + j M
     double* Tb = T + K * M;                                        // T_bar_j at Tb + j M
-    const double* What = a.totals + b * (long long)n * M;
+    const double* What = a.totals + b * (long long)row_stride(a) * M;
     double* Zb = a.part + (long long)blockIdx.x * n * M;   // row l = Zbar_l (node q)
     double* Wb = a.part2 + (long long)blockIdx.x * n * M;  // row j-1 = What_bar_j (node q)
     const double s = a.x[q] - 1.0, wq = a.w[q];

@@ -53,6 +53,7 @@ def lib() -> ctypes.CDLL:
             getattr(L, name).argtypes = [P, u64, vp, vp, vp, vp]
         for name in ("magnus_vjp_f64", "magnus_sc_vjp_f64"):
             getattr(L, name).argtypes = [P, u64, vp, vp, vp, vp, vp]
+        L.magnus_forward_orders_f64.argtypes = [P, u64, vp, vp, vp]
         L.magnus_sc_forward_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp]
         L.magnus_sc_vjp_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp, vp]
         L.magnus_sc_checkpoint_elems.argtypes = [P, u64]
@@ -88,6 +89,19 @@ def lib() -> ctypes.CDLL:
         L.magnus_uneven_set_tuning.argtypes = [vp, ctypes.c_char_p]
         L.magnus_uneven_info.argtypes = [vp, ctypes.c_int]
         L.magnus_uneven_info.restype = u64
+        f32p = ctypes.POINTER(ctypes.c_float)
+        u32 = ctypes.c_uint32
+        L.magnus_ctrl_feature_count.restype = u64
+        L.magnus_ctrl_features_f64.argtypes = [u64, u64, u64, d, d, vp, vp, vp]
+        L.magnus_mlp_create.argtypes = [u32, u32, u32, u64, u64, ctypes.POINTER(ctypes.c_void_p)]
+        L.magnus_mlp_destroy.argtypes = [vp]
+        L.magnus_mlp_destroy.restype = None
+        L.magnus_mlp_param_count.argtypes = [vp]
+        L.magnus_mlp_param_count.restype = u64
+        for name in ("magnus_mlp_get_params", "magnus_mlp_set_params", "magnus_mlp_get_grads"):
+            getattr(L, name).argtypes = [vp, f32p, u64]
+        L.magnus_mlp_forward.argtypes = [vp, u64, vp, vp, vp]
+        L.magnus_mlp_train_step.argtypes = [vp, u64, vp, vp, ctypes.c_float, vp, vp]
         L.magnus_timing_enable.argtypes = [ctypes.c_int]
         L.magnus_timing_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)]
         L.magnus_timing_reset.restype = None

@@ -61,6 +61,21 @@ def forward(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: st
     return out
 
 
+def forward_orders(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, matrix_backend: str = "Auto",
+                   integrator: str = "Auto"):
+    """Magnus sums of orders 1..n from one basis sweep: (B, n, d, d), row k-1 =
+    forward(k, A, op="sum")."""
+    _check_input(A, 4, "A")
+    A = A.contiguous()
+    B, S, d, d2 = A.shape
+    if d != d2:
+        raise ValueError("data must contain square matrices")
+    p = capi.problem(n, d, S, t0, tf, "sum", matrix_backend, integrator)
+    out = torch.empty((B, n, d, d), dtype=A.dtype, device=A.device)
+    capi.check(capi.lib().magnus_forward_orders_f64(ctypes.byref(p), B, _ptr(A), _ptr(out), _stream()))
+    return out
+
+
 def vjp(n: int, A: torch.Tensor, cot: torch.Tensor, carry: torch.Tensor | None = None, t0: float = 0.0,
         tf: float = 1.0, *, op: str = "sum", matrix_backend: str = "Auto", integrator: str = "Auto"):
     _check_input(A, 4, "A")

@@ -497,7 +497,7 @@ def _bench_controller(args, torch, dev):
 
     kernel_ms = {}
 
-    def best(fn, reps=2):  # after one warm-up run, the faster of `reps` timed runs
+    def best(fn, reps=3):  # after one warm-up run, the fastest of `reps` timed runs
         fn()
         runs = [C.timed(fn) for _ in range(reps)]
         # one more run with the per-kernel timers on (diagnostics only)
@@ -529,7 +529,7 @@ def _bench_controller(args, torch, dev):
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": 2,
+        "steps": 3,
         "warmup": 1,
         "ms_per_step": round(t_l * 1e3, 3),
         "higher_is_better": True,

@@ -383,10 +383,39 @@ void fill_lm(const Plan& pl, long long jobs, magnus::LmArgs& a) {
 // differ from the reference's by up to ~1e-12 of Omega_n (n = 7 and 8).
 bool use_wbd(const Plan& pl) { return !pl.sc && pl.n >= 2 && pl.op != MAGNUS_OP_ONE; }
 
-size_t wbd_fold_bytes(const Plan& pl, uint64_t B) {  // z, totals, omega1, part, part2, work
+// Chunk length of the sample-parallel scan (0: sequential scan).  It depends
+// on the sample count only, so an instance's results do not depend on what
+// it is batched with: at most 256 chunks, at least 32 samples per chunk.
+int scan_chunk_len(const Plan& pl, uint64_t) {
+    if (pl.integ == MAGNUS_INT_TRAPEZOID) return 0;
+    uint64_t clen = (pl.S - 1 + 255) / 256;
+    clen = (clen + 3) / 4 * 4;
+    if (clen < 32) clen = 32;
+    if (pl.S - 1 < 2 * clen) return 0;
+    return (int)clen;
+}
+
+size_t scan_chunk_bytes(const Plan& pl, uint64_t B) {
+    const int clen = scan_chunk_len(pl, B);
+    if (!clen) return 0;
+    const uint64_t C = (pl.S - 1 + clen - 1) / clen;
+    return scratch_round(2 * C * B * pl.m * 8);
+}
+
+double* alloc_scan_chunks(DeviceScratch& scratch, const Plan& pl, uint64_t B, magnus::LmArgs& a) {
+    a.clen = scan_chunk_len(pl, B);
+    a.cscan = nullptr;
+    if (!a.clen) return nullptr;
+    const uint64_t C = (pl.S - 1 + a.clen - 1) / a.clen;
+    a.cscan = scratch.alloc<double>(2 * C * B * pl.m);
+    if (!a.cscan) a.clen = 0;
+    return a.cscan;
+}
+
+size_t wbd_fold_bytes(const Plan& pl, uint64_t B) {  // z, totals, omega1, part, part2, work, scan chunks
     const size_t m = pl.m, n = pl.n, Q = pl.Q;
     return 2 * scratch_round(B * n * m * 8) + scratch_round(B * m * 8) + 2 * scratch_round(B * Q * n * m * 8) +
-           scratch_round(B * Q * (2 * (n - 1) + 1) * m * 8);
+           scratch_round(B * Q * (2 * (n - 1) + 1) * m * 8) + scan_chunk_bytes(pl, B);
 }
 
 size_t forward_scratch_bytes(const Plan& pl, uint64_t count) {
@@ -460,6 +489,7 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
         return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
+    alloc_scan_chunks(scratch, pl, B, a);
     a.in = in;
     a.pingpong = pingpong;
     a.Y = Y;
@@ -502,6 +532,7 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     if (!W || !z || !z0 || !totals || !part || !part2 || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
+    alloc_scan_chunks(scratch, pl, B, a);
     a.in = in;
     {
         DeviceScratch fs(stream, arena);
@@ -592,6 +623,7 @@ int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hi
         return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
+    alloc_scan_chunks(scratch, pl, B, a);
     a.in = in;
     a.pingpong = pingpong;
     a.Y = Y;

@@ -41,6 +41,8 @@ struct LmArgs {
     double* dAjob;       // [jobs][S][m]               (VJP)
     double* totals;      // [jobs][n][m] last prefix value of each level, or nullptr (forward)
     const double* wbar;  // [jobs][n][m] cotangents of the level totals, or nullptr (VJP)
+    double* cscan;       // [2][nchunks][jobs*m] chunk scratch of the sample-parallel scan, or nullptr
+    int clen;            // samples per chunk of the sample-parallel scan (multiple of 4)
     long long S;
     long long jobs;
     int n, d, m, m_in, m_cot, Q, Qj, n_out, op;

@@ -250,6 +250,125 @@ __global__ void __launch_bounds__(256) lm_fwd_scan(LmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Sample-parallel forward scan (one job per instance, op "many": the W-basis
+// sweep).  Chunks c of clen samples, [s_c, e_c] with e_c = min(s_c + clen, S-1),
+// share their end points.  Pass 1 runs each chunk's recurrence from a zero
+// base (chunk 0 from the true start) and keeps its end value and its share of
+// the level sum; pass 2 turns the end values into chunk bases (exclusive scan
+// in chunk order) and adds the shares in chunk order; pass 3 reruns each
+// chunk's recurrence from its base and writes the prefix (and, in the last
+// chunk, the level total).  Inside a chunk the operations are the sequential
+// kernel's; only the chunk bases are summed in another order.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double lm_scan_input(const LmArgs& a, int j, long long b, long long job, long long i,
+                                                int e) {
+    if (j == 1) return raw_entry(a.in + (b * a.S + i) * a.m_in, e, a.sc);
+    return a.Y[(job * a.S + i) * a.m + e];
+}
+
+__host__ __device__ __forceinline__ int lm_nchunks(const LmArgs& a) {
+    return (int)((a.S - 1 + a.clen - 1) / a.clen);
+}
+
+// walk chunk c from base `at` (chunk 0: the true start), calling out(i, P, f_i)
+template <int I, typename F>
+__device__ __forceinline__ double lm_chunk_walk(const LmArgs& a, int j, long long b, long long job, int e, int c,
+                                                double at, F&& out) {
+    const long long S = a.S, s0 = (long long)c * a.clen;
+    const long long e0 = (s0 + a.clen < S - 1) ? s0 + a.clen : S - 1;
+    PrefixState<I> st;
+    double P;
+    const double fs = lm_scan_input(a, j, b, job, s0, e);
+    if (c == 0) {
+        P = st.step(0, fs, a.dt);
+    } else {
+        st.start(fs, at);
+        P = at;
+    }
+    out(s0, P, fs);
+    constexpr int U = 8;
+    for (long long i0 = s0 + 1; i0 <= e0; i0 += U) {
+        double f[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) f[u] = (i0 + u <= e0) ? lm_scan_input(a, j, b, job, i0 + u, e) : 0.0;
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (i0 + u <= e0) {
+                P = st.step(i0 + u, f[u], a.dt);
+                out(i0 + u, P, f[u]);
+            }
+    }
+    return P;
+}
+
+template <int I>
+__global__ void __launch_bounds__(256) lm_fwd_scan_c1(LmArgs a) {
+    const long long JM = a.jobs * a.m;
+    const int C = lm_nchunks(a);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= JM * C) return;
+    const int c = (int)(idx / JM);
+    const long long je = idx - (long long)c * JM;
+    const long long job = je / a.m;
+    const int e = (int)(je - job * a.m);
+    const long long b = job;  // one job per instance
+    const long long S = a.S;
+    const bool last = (c == C - 1);
+    double acc = 0.0;
+    const double cw = a.dt * (a.level == 1 ? 1.0 : a.w[0]);
+    const double end = lm_chunk_walk<I>(a, a.level, b, job, e, c, 0.0, [&](long long i, double, double f) {
+        if (i < (long long)(c + 1) * a.clen || last) sum_acc<I>(acc, i, S, f, cw);
+    });
+    a.cscan[(long long)c * JM + je] = end;
+    a.cscan[((long long)C + c) * JM + je] = acc;
+}
+
+template <int I>
+__global__ void __launch_bounds__(64) lm_fwd_scan_c2(LmArgs a) {
+    const long long JM = a.jobs * a.m;
+    const int C = lm_nchunks(a);
+    const long long je = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (je >= JM) return;
+    const long long job = je / a.m;
+    const int e = (int)(je - job * a.m);
+    const int n = a.n, j = a.level;
+    double off = 0.0, acc = 0.0;
+    for (int c = 0; c < C; ++c) {
+        const double inc = a.cscan[(long long)c * JM + je];
+        a.cscan[(long long)c * JM + je] = off;
+        off += inc;
+        acc += a.cscan[((long long)C + c) * JM + je];
+    }
+    if (!a.outputs) return;
+    if (j == 1) {
+        a.omega1[job * a.m + e] = acc;
+    } else {
+        a.partial[(job * a.n_out + (j - 1)) * (long long)a.m + e] = acc;
+    }
+    (void)n;
+}
+
+template <int I>
+__global__ void __launch_bounds__(256) lm_fwd_scan_c3(LmArgs a) {
+    const long long JM = a.jobs * a.m;
+    const int C = lm_nchunks(a);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= JM * C) return;
+    const int c = (int)(idx / JM);
+    const long long je = idx - (long long)c * JM;
+    const long long job = je / a.m;
+    const int e = (int)(je - job * a.m);
+    const int j = a.level, m = a.m;
+    const bool last = (c == C - 1);
+    double* Pout = lm_slot(a, job, job, 0, j);
+    const double at = a.cscan[(long long)c * JM + je];
+    const double end = lm_chunk_walk<I>(a, j, job, job, e, c, at, [&](long long i, double P, double) {
+        if (i < (long long)(c + 1) * a.clen || last) Pout[i * m + e] = P;
+    });
+    if (last && a.totals) a.totals[(job * a.n + (j - 1)) * (long long)m + e] = end;
+}
+
+// ---------------------------------------------------------------------------
 // VJP
 // ---------------------------------------------------------------------------
 
@@ -420,6 +539,15 @@ inline int prod_path(const LmArgs& a) {
     return a.d;
 }
 
+// the sample-parallel scan applies to the W-basis sweep (one job per
+// instance, all level sums kept) with a chunk scratch, block rules that resume
+// from a base value, and at least two chunks
+template <int I>
+inline bool chunked_scan(const LmArgs& a) {
+    return I != TRAPEZOID && a.cscan && a.clen > 0 && a.clen % 4 == 0 && a.Qj == 1 && a.op == OP_MANY &&
+           a.S - 1 >= 2 * (long long)a.clen;
+}
+
 template <int D, int W>
 inline void launch_mfma(void (*k)(LmArgs), const LmArgs& a, hipStream_t st) {
     constexpr int ppb = MfmaShape<D, W>::PPB;
@@ -442,9 +570,22 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
         timer_end(h);
     }
     void* h = timer_begin("lm_fwd_scan", st);
-    // one wavefront per workgroup: the scans are latency-bound chains, so they
-    // are spread over as many CUs (memory queues) as there are wavefronts
-    hipLaunchKernelGGL((lm_fwd_scan<I>), dim3((a.jobs * a.m + 63) / 64), dim3(64), 0, st, a);
+    bool chunked = false;
+    if constexpr (I != TRAPEZOID) {
+        chunked = chunked_scan<I>(a);
+        if (chunked) {
+            const long long JM = a.jobs * a.m, work = JM * lm_nchunks(a);
+            hipLaunchKernelGGL((lm_fwd_scan_c1<I>), dim3((work + 255) / 256), dim3(256), 0, st, a);
+            hipLaunchKernelGGL((lm_fwd_scan_c2<I>), dim3((JM + 63) / 64), dim3(64), 0, st, a);
+            if (a.level < a.n)
+                hipLaunchKernelGGL((lm_fwd_scan_c3<I>), dim3((work + 255) / 256), dim3(256), 0, st, a);
+        }
+    }
+    if (!chunked) {
+        // one wavefront per workgroup: the scans are latency-bound chains, so
+        // they are spread over as many CUs (memory queues) as there are wavefronts
+        hipLaunchKernelGGL((lm_fwd_scan<I>), dim3((a.jobs * a.m + 63) / 64), dim3(64), 0, st, a);
+    }
     timer_end(h);
     return hipGetLastError();
 }

@@ -141,6 +141,11 @@ struct PrefixState<BOOLE> {
         }
         return P;
     }
+    // resume at a block start s (s % 4 == 0) whose prefix value is `at`
+    __device__ __forceinline__ void start(double fs, double at) {
+        f0 = fs;
+        base = at;
+    }
 };
 
 // simpson.hpp:24-58: odd i = trapezoid from i-1, even i = Simpson panel from i-2.
@@ -180,6 +185,11 @@ struct PrefixState<SIMPSON> {
         Pm1 = P;
         return P;
     }
+    // resume at an even sample s >= 2 whose prefix value is `at`
+    __device__ __forceinline__ void start(double fs, double at) {
+        prev1 = fs;
+        Pm1 = at;
+    }
 };
 
 // trapezoid.hpp:23-33, including the reconstruction of f_{i} from prefix
@@ -214,6 +224,7 @@ struct PrefixState<RIEMANN> {
         Pm1 = P;
         return P;
     }
+    __device__ __forceinline__ void start(double, double at) { Pm1 = at; }
 };
 
 // ---------------------------------------------------------------------------

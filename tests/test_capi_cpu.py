@@ -122,6 +122,41 @@ def test_spacecurve_checkpoint_size_and_argument_checks():
     assert L.magnus_sc_forward_ckpt_f64(ctypes.byref(p), 0, None, None, None, None) == capi.OK  # empty batch
 
 
+def test_controller_and_uneven_argument_checks_before_device_work():
+    """New entry points reject bad arguments before any device call."""
+    L = capi.lib()
+    dummy = ctypes.c_void_p(16)  # never dereferenced
+    # features: dim 1..16, grid 2^k + 1 in 9..129
+    assert L.magnus_ctrl_features_f64(1, 65, 0, 0.0, 1.0, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_ctrl_features_f64(1, 65, 17, 0.0, 1.0, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_ctrl_features_f64(1, 66, 8, 0.0, 1.0, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_ctrl_features_f64(1, 257, 8, 0.0, 1.0, dummy, dummy, None) == capi.EINVAL
+    assert L.magnus_ctrl_features_f64(1, 65, 8, 0.0, 1.0, None, dummy, None) == capi.EINVAL
+    assert L.magnus_ctrl_features_f64(0, 65, 8, 0.0, 1.0, dummy, dummy, None) == capi.OK
+    assert L.magnus_ctrl_feature_count() == 24
+    # MLP creation: positive widths and batch
+    h = ctypes.c_void_p()
+    assert L.magnus_mlp_create(0, 128, 63, 1024, 0, ctypes.byref(h)) == capi.EINVAL
+    assert L.magnus_mlp_create(24, 128, 63, 0, 0, ctypes.byref(h)) == capi.EINVAL
+    assert L.magnus_mlp_create(24, 5000, 63, 1024, 0, ctypes.byref(h)) == capi.EINVAL
+    assert L.magnus_mlp_param_count(None) == 0
+    # all-orders forward: matrix problems only
+    sc = capi.problem(3, 2, 33, 0.0, 1.0, matrix_backend="SpaceCurve")
+    assert L.magnus_forward_orders_f64(ctypes.byref(sc), 1, dummy, dummy, None) == capi.EINVAL
+    p = capi.problem(3, 4, 33, 0.0, 1.0)
+    assert L.magnus_forward_orders_f64(ctypes.byref(p), 0, None, None, None) == capi.OK
+    assert L.magnus_forward_orders_f64(ctypes.byref(p), 1, None, dummy, None) == capi.EINVAL
+    # uneven plans: branch count, then per-instance validation naming the instance
+    probs = (capi.Problem * 2)(capi.problem(2, 4, 33, 0.0, 1.0),
+                               capi.problem(2, 4, 32, 0.0, 1.0, integrator="Boole"))
+    plan = ctypes.c_void_p()
+    assert L.magnus_uneven_create(probs, 2, 0, 0, ctypes.byref(plan)) == capi.EINVAL
+    assert L.magnus_uneven_create(probs, 2, 0, 9, ctypes.byref(plan)) == capi.EINVAL
+    assert L.magnus_uneven_create(probs, 2, 0, 2, ctypes.byref(plan)) == capi.EINVAL
+    assert L.magnus_last_error().decode().startswith("instance 1: ")
+    assert L.magnus_uneven_info(None, 0) == 0
+
+
 def test_expm_argument_checks_before_device_work():
     L = capi.lib()
     dummy = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation or is empty

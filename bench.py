@@ -450,8 +450,12 @@ def _bench_controller(args, torch, dev):
     c_tr = C.make_coefficients(n_tr, seed=11)
     c_ev = C.make_coefficients(n_ev, seed=12)
     f_tr = C.features(torch_ops.sample_fourier(c_tr, C.FEATURE_GRID), 0.0, 1.0)
-    lab_tr, min_tr, reach_tr, _, _ = C.label(c_tr, tol)
-    lab_ev, min_ev, reach_ev, errs_ev, refs_ev = C.label(c_ev, tol)
+    def progress(done, total):
+        if done % 10240 < 512 or done == total:
+            log(f"labelled {done}/{total} ({time.perf_counter() - t:.0f} s)")
+
+    lab_tr, min_tr, reach_tr, _, _ = C.label(c_tr, tol, progress=progress)
+    lab_ev, min_ev, reach_ev, errs_ev, refs_ev = C.label(c_ev, tol, progress=progress)
     torch.cuda.synchronize()
     label_s = time.perf_counter() - t
     hist = np.bincount(lab_tr, minlength=len(C.CLASSES))

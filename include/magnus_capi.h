@@ -272,6 +272,11 @@ int magnus_timing_enable(int on);
 int magnus_timing_read(const char* name, uint64_t* launches, double* total_ms);
 void magnus_timing_reset(void);
 
+/* Library scratch comes from a per-(device, stream) cache of device blocks that
+ * grows to the largest call seen and is kept for reuse (no allocation on the
+ * steady-state path).  This waits for the devices and frees every cache. */
+int magnus_release_scratch(void);
+
 /* thread-local message of the last failing call on this thread */
 const char* magnus_last_error(void);
 

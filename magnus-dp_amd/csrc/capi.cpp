@@ -39,22 +39,64 @@ int fail(int code, const std::string& msg) {
 
 void set_last_kernel(const char* name) { g_kernel = name; }
 
-void keep_pool_memory() {
-    static std::mutex mu;
-    static uint64_t done = 0;  // bit per device
-    int dev = 0;
-    if (hipGetDevice(&dev) != hipSuccess || dev < 0 || dev >= 64) {
+void* ScratchCache::bump(size_t bytes) {
+    for (; bi < blocks.size(); ++bi, off = 0) {
+        if (off + bytes <= blocks[bi].size) {
+            void* p = blocks[bi].p + off;
+            off += bytes;
+            return p;
+        }
+    }
+    size_t size = bytes < (size_t(256) << 20) ? (size_t(256) << 20) : bytes;
+    if (!blocks.empty() && size < 2 * blocks.back().size) size = 2 * blocks.back().size;
+    void* p = nullptr;
+    if (hipMalloc(&p, size) != hipSuccess) {
         (void)hipGetLastError();
-        return;
+        p = nullptr;
+        if (size > bytes && hipMalloc(&p, bytes) == hipSuccess) {
+            size = bytes;
+        } else {
+            (void)hipGetLastError();
+            return nullptr;
+        }
     }
-    std::lock_guard<std::mutex> lock(mu);
-    if (done & (1ull << dev)) return;
-    done |= 1ull << dev;
-    hipMemPool_t pool;
-    if (hipDeviceGetDefaultMemPool(&pool, dev) == hipSuccess) {
-        uint64_t threshold = UINT64_MAX;
-        (void)hipMemPoolSetAttribute(pool, hipMemPoolAttrReleaseThreshold, &threshold);
+    blocks.push_back({static_cast<char*>(p), size});
+    bi = blocks.size() - 1;
+    off = bytes;
+    return p;
+}
+
+namespace {
+std::mutex g_cache_mu;
+std::vector<std::pair<std::pair<int, hipStream_t>, std::unique_ptr<ScratchCache>>> g_caches;
+}  // namespace
+
+ScratchCache* scratch_cache(hipStream_t s) {
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) {
+        (void)hipGetLastError();
+        return nullptr;
     }
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    for (auto& c : g_caches)
+        if (c.first.first == dev && c.first.second == s) return c.second.get();
+    g_caches.push_back({{dev, s}, std::make_unique<ScratchCache>()});
+    return g_caches.back().second.get();
+}
+
+void release_scratch_caches() {
+    std::lock_guard<std::mutex> lock(g_cache_mu);
+    int cur = 0;
+    (void)hipGetDevice(&cur);
+    for (auto& c : g_caches) {
+        std::lock_guard<std::recursive_mutex> l2(c.second->mu);
+        (void)hipSetDevice(c.first.first);
+        (void)hipDeviceSynchronize();
+        for (auto& b : c.second->blocks) (void)hipFree(b.p);
+        c.second->blocks.clear();
+        c.second->bi = c.second->off = 0;
+    }
+    (void)hipSetDevice(cur);
     (void)hipGetLastError();
 }
 
@@ -1029,6 +1071,11 @@ void magnus_timing_reset(void) {
     std::lock_guard<std::mutex> lock(g_timing_mu);
     timing_drain_locked();
     g_totals.clear();
+}
+
+int magnus_release_scratch(void) {
+    release_scratch_caches();
+    return MAGNUS_OK;
 }
 
 const char* magnus_last_error(void) { return g_err.c_str(); }

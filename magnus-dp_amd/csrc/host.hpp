@@ -7,6 +7,7 @@
 
 #include <cstdint>
 #include <memory>
+#include <mutex>
 #include <string>
 #include <vector>
 
@@ -49,46 +50,67 @@ struct Arena {
 constexpr size_t kScratchAlign = 256;
 inline size_t scratch_round(size_t bytes) { return (bytes + kScratchAlign - 1) / kScratchAlign * kScratchAlign; }
 
-// Keep memory freed by stream-ordered frees in the current device's default
-// pool (release threshold "unlimited"), once per device: with the default
-// threshold 0 every synchronization returns it to the driver and the next
-// call maps gigabytes of scratch again.
-void keep_pool_memory();
+// Per-(device, stream) cache of device blocks for library scratch.  A call's
+// buffers are bump-allocated from it and released in stack order when the
+// call returns; the next call on the same stream reuses the memory, which
+// stream order makes safe.  Blocks grow geometrically and are kept until
+// magnus_release_scratch(): no allocation call (and no stall inside one) on
+// the steady-state path.
+struct ScratchCache {
+    struct Block {
+        char* p;
+        size_t size;
+    };
+    std::recursive_mutex mu;
+    std::vector<Block> blocks;
+    size_t bi = 0, off = 0;  // cursor: block index, offset
+    void* bump(size_t bytes);
+};
+ScratchCache* scratch_cache(hipStream_t s);
+void release_scratch_caches();
 
-// Temporary device buffers: stream-ordered allocations freed on the same
-// stream at scope exit, or, with an arena, slices of it (no allocation calls,
-// so the work can be captured into a graph).
+// Temporary device buffers of one call: slices of the stream's scratch cache,
+// or of a caller-owned arena (no allocation calls either way, so the work can
+// be captured into a graph once the cache has grown to size).
 struct DeviceScratch {
     hipStream_t stream;
     Arena* arena;
-    size_t mark;
-    std::vector<void*> ptrs;
-    explicit DeviceScratch(hipStream_t s, Arena* a = nullptr) : stream(s), arena(a), mark(a ? a->used : 0) {}
+    ScratchCache* cache = nullptr;
+    size_t mark = 0, mark_bi = 0;
+    explicit DeviceScratch(hipStream_t s, Arena* a = nullptr) : stream(s), arena(a) {
+        if (arena) {
+            mark = arena->used;
+        } else {
+            cache = scratch_cache(s);
+            if (cache) {
+                cache->mu.lock();
+                mark_bi = cache->bi;
+                mark = cache->off;
+            }
+        }
+    }
     ~DeviceScratch() {
         if (arena) {
             arena->used = mark;
-            return;
+        } else if (cache) {
+            cache->bi = mark_bi;
+            cache->off = mark;
+            cache->mu.unlock();
         }
-        for (void* p : ptrs) (void)hipFreeAsync(p, stream);
     }
+    DeviceScratch(const DeviceScratch&) = delete;
+    DeviceScratch& operator=(const DeviceScratch&) = delete;
     template <class T>
     T* alloc(size_t count) {
         if (count == 0) return nullptr;
+        const size_t bytes = scratch_round(count * sizeof(T));
         if (arena) {
-            const size_t bytes = scratch_round(count * sizeof(T));
             if (arena->used + bytes > arena->size) return nullptr;
             void* p = arena->base + arena->used;
             arena->used += bytes;
             return static_cast<T*>(p);
         }
-        keep_pool_memory();
-        void* p = nullptr;
-        if (hipMallocAsync(&p, count * sizeof(T), stream) != hipSuccess) {
-            (void)hipGetLastError();
-            return nullptr;
-        }
-        ptrs.push_back(p);
-        return static_cast<T*>(p);
+        return cache ? static_cast<T*>(cache->bump(bytes)) : nullptr;
     }
 };
 

@@ -102,6 +102,7 @@ def lib() -> ctypes.CDLL:
             getattr(L, name).argtypes = [vp, f32p, u64]
         L.magnus_mlp_forward.argtypes = [vp, u64, vp, vp, vp]
         L.magnus_mlp_train_step.argtypes = [vp, u64, vp, vp, ctypes.c_float, vp, vp]
+        L.magnus_release_scratch.argtypes = []
         L.magnus_timing_enable.argtypes = [ctypes.c_int]
         L.magnus_timing_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)]
         L.magnus_timing_reset.restype = None

@@ -92,9 +92,9 @@ def rel_err(x: torch.Tensor, ref: torch.Tensor) -> torch.Tensor:
     return torch.linalg.vector_norm((x - ref).flatten(1), dim=1) / torch.linalg.vector_norm(ref.flatten(1), dim=1)
 
 
-def label(coeffs: torch.Tensor, tol: float, batch: int = 512):
+def label(coeffs: torch.Tensor, tol: float, batch: int = 512, progress=None):
     """Per instance: (labels, minimal, reachable, errors [B][9][7], reference
-    sums [B][d][d]).  `minimal` is the cheapest class within tol of the
+    sums [B][d][d]); progress(done, total) is called after every batch.  `minimal` is the cheapest class within tol of the
     reference; `labels` -- what the network learns -- is the cheapest class
     the judge accepts in one round with a result within tol (its doubled-samples
     and next-order checks pass), so a correct prediction costs one round."""
@@ -134,6 +134,8 @@ def label(coeffs: torch.Tensor, tol: float, batch: int = 512):
             reach[b0 + r] = suff.size > 0
             minimal[b0 + r] = order[suff[0]] if suff.size else class_index(*PLAIN)
             labels[b0 + r] = order[acc[0]] if acc.size else class_index(*PLAIN)
+        if progress is not None:
+            progress(b0 + nb, B)
     return labels, minimal, reach, errs, torch.cat(refs)
 
 

@@ -621,3 +621,4 @@ def test_dense_wbasis_large_norm(scale):
                        integrator="Boole")[0].cpu().numpy()
     assert rel_fro(out, O.forward(n, A, 0.0, 1.0, op="many", integrator="Boole")) <= TOL
     assert rel_fro(dA, O.vjp(n, A, cot, 0.0, 1.0, op="many", integrator="Boole")) <= TOL
+

@@ -5,7 +5,7 @@ trajectories f(t) = sum_{m=1..4} a_m sin(2 pi m t + phi_m), a_m ~ N(0,1)/m,
 phi_m ~ U[0, 2 pi), S = 16385 samples on [0, 1], Simpson, compute_sc with
 n = 6, op = 'many', then compute_sc_vjp with a N(0,1) (6, 3) cotangent per
 trajectory.  No reference-style carry is stored: the forward keeps its compact
-per-chunk checkpoint (11 4-vectors per 128-sample chunk, 184 MB for the batch)
+per-chunk checkpoint (11 4-vectors per 512-sample chunk, 46 MB for the batch)
 and the VJP starts from it (--sc-residual recompute: the VJP re-derives its
 forward quantities, the reference's VJP::none).  One step = forward + VJP of the
 whole batch in batched launches; inputs are resident in HBM before timing starts.

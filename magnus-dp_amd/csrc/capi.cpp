@@ -346,12 +346,13 @@ bool use_wb(const Plan& pl, bool has_carry) {
 }
 
 int wb_chunk_len(const Plan& pl, uint64_t count) {
-    // aim for ~512k chunk-threads, chunks of 16..256 samples, multiple of 4
-    const double target = 524288.0;
+    // aim for ~128k chunk-threads (two wavefronts per SIMD), chunks of 16..512
+    // samples, multiple of 4: longer chunks shrink the per-instance combine
+    const double target = 131072.0;
     double lc = (double)(pl.S - 1) * (double)count / target;
     int LC = (int)lc;
     if (LC < 16) LC = 16;
-    if (LC > 256) LC = 256;
+    if (LC > 512) LC = 512;
     LC = (LC + 3) / 4 * 4;
     return LC;
 }

@@ -52,6 +52,121 @@ __global__ void __launch_bounds__(256) lm_fwd_prod(LmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Row products for small d (<= 16, generic matrices): one thread per (sample,
+// output row), the sample's operands staged once in LDS, so a product costs
+// its operands' reads once instead of once per output entry.  Sums run in the
+// same index order as prod_generic / prod_vjp.
+// ---------------------------------------------------------------------------
+constexpr int kRowMaxD = 16;
+
+__host__ __device__ __forceinline__ int rows_spb(int d) { return 256 / d; }
+// the VJP stages three matrices per sample; stay within 64 KB of dynamic LDS
+__host__ __device__ __forceinline__ int vjp_rows_spb(int d) {
+    const int s = 256 / d, lim = 65536 / (3 * d * d * 8);
+    return s < lim ? s : lim;
+}
+
+// Y_i = A_i (P_i + s T): LDS holds b = P_i + s T per sample
+__global__ void __launch_bounds__(256) lm_fwd_prod_rows(LmArgs a) {
+    extern __shared__ double lds[];
+    const int d = a.d, m = a.m, spb = rows_spb(d);
+    const long long total = a.jobs * a.S;
+    const long long p0 = (long long)blockIdx.x * spb;
+    for (int x = threadIdx.x; x < spb * m; x += 256) {
+        const int g = x / m, e = x - g * m;
+        const long long p = p0 + g;
+        if (p >= total) continue;
+        const long long job = p / a.S, i = p - job * a.S;
+        const long long b = job / a.Qj;
+        const int q = (int)(job - b * a.Qj);
+        const double shift = a.x[q] - 1;
+        const double* P = lm_slot(a, job, b, q, a.level - 1);
+        double v = P[i * m + e];
+        v += P[(a.S - 1) * m + e] * shift;
+        lds[x] = v;
+    }
+    __syncthreads();
+    const int g = threadIdx.x / d, r = threadIdx.x - g * d;
+    const long long p = p0 + g;
+    if (g >= spb || p >= total) return;
+    const long long job = p / a.S, i = p - job * a.S;
+    const long long b = job / a.Qj;
+    const double* Ai = a.in + (b * a.S + i) * a.m_in + r * d;
+    double ar[kRowMaxD], acc[kRowMaxD];
+#pragma unroll
+    for (int l = 0; l < kRowMaxD; ++l) {
+        ar[l] = l < d ? Ai[l] : 0.0;
+        acc[l] = 0.0;
+    }
+    const double* bs = lds + g * m;
+#pragma unroll
+    for (int l = 0; l < kRowMaxD; ++l) {
+        if (l >= d) break;
+#pragma unroll
+        for (int c = 0; c < kRowMaxD; ++c)
+            if (c < d) acc[c] += ar[l] * bs[l * d + c];
+    }
+    double* y = a.Y + p * m + r * d;
+#pragma unroll
+    for (int c = 0; c < kRowMaxD; ++c)
+        if (c < d) y[c] = acc[c];
+}
+
+// dA_job_i += g b^T (row r) and temp_i = A_i^T g (row r); LDS per sample: b, g, A
+__global__ void __launch_bounds__(256) lm_vjp_prod_rows(LmArgs a) {
+    extern __shared__ double lds[];
+    const int d = a.d, m = a.m, spb = vjp_rows_spb(d);
+    const long long per = a.S * m;
+    const long long total = a.jobs * a.S;
+    const long long p0 = (long long)blockIdx.x * spb;
+    for (int x = threadIdx.x; x < spb * m; x += 256) {
+        const int g = x / m, e = x - g * m;
+        const long long p = p0 + g;
+        if (p >= total) continue;
+        const long long job = p / a.S, i = p - job * a.S;
+        const long long b = job / a.Q;
+        const int q = (int)(job - b * a.Q);
+        const double shift = a.x[q] - 1.0;
+        const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
+        double v = P[i * m + e];
+        v += P[(a.S - 1) * m + e] * shift;
+        lds[x] = v;                                                                  // b
+        lds[spb * m + x] = a.gbuf[(job * 2 + a.ga) * per + i * m + e];               // g
+        lds[2 * spb * m + x] = a.in[(b * a.S + i) * (long long)a.m_in + e];          // A
+    }
+    __syncthreads();
+    const int g = threadIdx.x / d, r = threadIdx.x - g * d;
+    const long long p = p0 + g;
+    if (g >= spb || p >= total) return;
+    const long long job = p / a.S, i = p - job * a.S;
+    const double* bs = lds + g * m;
+    const double* gs = lds + spb * m + g * m;
+    const double* as = lds + 2 * spb * m + g * m;
+    double gr[kRowMaxD], ac[kRowMaxD];
+#pragma unroll
+    for (int j = 0; j < kRowMaxD; ++j) {
+        gr[j] = j < d ? gs[r * d + j] : 0.0;  // g[r][:]
+        ac[j] = j < d ? as[j * d + r] : 0.0;  // a[:][r]
+    }
+    double* dA = a.dAjob + job * per + i * m + r * d;
+    double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * m + r * d;
+#pragma unroll
+    for (int cc = 0; cc < kRowMaxD; ++cc) {
+        if (cc >= d) break;
+        double v = 0.0;  // sum_j g[r][j] b[cc][j]
+#pragma unroll
+        for (int j = 0; j < kRowMaxD; ++j)
+            if (j < d) v += gr[j] * bs[cc * d + j];
+        double t2 = 0.0;  // sum_i a[i][r] g[i][cc]
+#pragma unroll
+        for (int ii = 0; ii < kRowMaxD; ++ii)
+            if (ii < d) t2 += ac[ii] * gs[ii * d + cc];
+        dA[cc] += v;
+        Gb[cc] = t2;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // MFMA products for d = D in {16, 32, 64}.  A 256-thread workgroup stages the
 // operands of PPB samples in LDS (k-major, so a wavefront's operand reads are
 // contiguous) and computes every 16 x 16 output tile with
@@ -528,12 +643,14 @@ __global__ void vjp_reduce_kernel(VjpReduceArgs a) {
 inline long long blocks_for(long long n) { return (n + 255) / 256; }
 
 // Product path of a level: D (MFMA, default split), D + W (MFMA, W wavefronts
-// per product), or 0 (VALU, one thread per output entry).  a.tile selects:
+// per product), 1 (VALU row products, d <= 16) or 0 (VALU, one thread per
+// output entry).  a.tile selects:
 // 0 default (MFMA for d in {16, 32, 64}), 1 VALU, 2 / 3 d = 32 split over 2 / 4
 // wavefronts; a choice that does not apply to d falls back to the default.
 inline int prod_path(const LmArgs& a) {
     const bool mf = !a.sc && (a.d == 16 || a.d == 32 || a.d == 64);
-    if (!mf || a.tile == 1) return 0;
+    const bool rows = !a.sc && a.d <= kRowMaxD;  // row products: VALU with LDS-staged operands
+    if (!mf || a.tile == 1) return rows ? 1 : 0;
     if (a.d == 32 && a.tile == 2) return 32 + 2;
     if (a.d == 32 && a.tile == 3) return 32 + 4;
     return a.d;
@@ -565,6 +682,12 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
             case 32 + 2: launch_mfma<32, 2>(lm_fwd_prod_mfma<32, 2>, a, st); break;
             case 32 + 4: launch_mfma<32, 4>(lm_fwd_prod_mfma<32, 4>, a, st); break;
             case 64: launch_mfma<64, 4>(lm_fwd_prod_mfma<64, 4>, a, st); break;
+            case 1: {
+                const int spb = rows_spb(a.d);
+                hipLaunchKernelGGL(lm_fwd_prod_rows, dim3((a.jobs * a.S + spb - 1) / spb), dim3(256),
+                                   (size_t)spb * a.m * sizeof(double), st, a);
+                break;
+            }
             default: hipLaunchKernelGGL(lm_fwd_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
         }
         timer_end(h);
@@ -599,6 +722,12 @@ hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
         case 32 + 2: launch_mfma<32, 2>(lm_vjp_prod_mfma<32, 2>, a, st); break;
         case 32 + 4: launch_mfma<32, 4>(lm_vjp_prod_mfma<32, 4>, a, st); break;
         case 64: launch_mfma<64, 4>(lm_vjp_prod_mfma<64, 4>, a, st); break;
+        case 1: {
+            const int spb = vjp_rows_spb(a.d);
+            hipLaunchKernelGGL(lm_vjp_prod_rows, dim3((a.jobs * a.S + spb - 1) / spb), dim3(256),
+                               (size_t)3 * spb * a.m * sizeof(double), st, a);
+            break;
+        }
         default: hipLaunchKernelGGL(lm_vjp_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
     }
     timer_end(h);

@@ -109,9 +109,9 @@ def test_spacecurve_checkpoint_size_and_argument_checks():
     p = capi.problem(3, 2, 1025, 0.0, 1.0, op="sum", matrix_backend="SpaceCurve", integrator="Simpson")
     # one instance: 16-sample chunks -> 64 chunks of 5 4-vectors
     assert L.magnus_sc_checkpoint_elems(ctypes.byref(p), 1) == 64 * 5 * 4
-    # headline batch shape (4096 x 16385 samples, n = 6): 128-sample chunks, 184 MB
+    # headline batch shape (4096 x 16385 samples, n = 6): 512-sample chunks, 46 MB
     ph = capi.problem(6, 2, 16385, 0.0, 1.0, op="sum", matrix_backend="SpaceCurve", integrator="Boole")
-    assert L.magnus_sc_checkpoint_elems(ctypes.byref(ph), 4096) * 8 == 4096 * 128 * 11 * 4 * 8
+    assert L.magnus_sc_checkpoint_elems(ctypes.byref(ph), 4096) * 8 == 4096 * 32 * 11 * 4 * 8
     for n, integ in [(3, "Trapezoid"), (1, "Simpson"), (9, "Boole")]:
         q = capi.problem(n, 2, 1025, 0.0, 1.0, matrix_backend="SpaceCurve", integrator=integ)
         assert L.magnus_sc_checkpoint_elems(ctypes.byref(q), 2) == 0

@@ -566,8 +566,8 @@ def _bench_controller(args, torch, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=5)
-    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=20)
+    ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="spacecurve-batch",
                     choices=["spacecurve-batch", "dense-d32", "plain-d4", "uneven", "controller"])
     ap.add_argument("--ctrl-train", type=int, default=200000)

@@ -214,10 +214,10 @@ int magnus_uneven_vjp_f64(magnus_uneven_plan* plan, const double* const* in, con
                           double* const* dA, int use_graph, void* stream);
 
 /* Autotuner: for every bucket, times the VJP (forward recompute + reverse
- * sweep) of its first sub-batch under each product path that applies to its
- * dim (VALU; FP64 MFMA with 1, 2 or 4 wavefronts per product) on the given
- * buffers, keeps the fastest, and rebalances the branches with the measured
- * times.  Synchronizes the stream. */
+ * sweep) of its first sub-batch under each product path worth trying for its
+ * dim (dim 16: FP64 MFMA or VALU row products; dim 32: MFMA with 1, 2 or 4
+ * wavefronts per product) on the given buffers, keeps the fastest, and
+ * rebalances the branches with the measured times.  Synchronizes the stream. */
 int magnus_uneven_autotune(magnus_uneven_plan* plan, const double* const* in, const double* const* cot,
                            double* const* dA, void* stream);
 

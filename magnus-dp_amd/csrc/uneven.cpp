@@ -65,9 +65,12 @@ double model_cost(const Plan& pl) {
     return Q * K * S * (2.0 * P + 14.0 * m) + 2.0 * m * S + 1e5;  // + per-launch overhead
 }
 
+// product paths worth timing: at d = 32 the three MFMA splits, at d = 16 MFMA
+// against the VALU row products; the per-entry VALU path is never faster at
+// d >= 32 (tile 1 stays selectable through the tuning table)
 std::vector<int> tile_candidates(const Plan& pl) {
-    if (pl.d == 32) return {0, 1, 2, 3};
-    if (pl.d == 16 || pl.d == 64) return {0, 1};
+    if (pl.d == 32) return {0, 2, 3};
+    if (pl.d == 16) return {0, 1};
     return {0};
 }
 

@@ -438,11 +438,13 @@ int scan_chunk_len(const Plan& pl, uint64_t) {
     return (int)clen;
 }
 
+// forward scan: end values and sum shares; reverse scan (Boole): chain
+// increments, pendings and raw chunk starts
 size_t scan_chunk_bytes(const Plan& pl, uint64_t B) {
     const int clen = scan_chunk_len(pl, B);
     if (!clen) return 0;
     const uint64_t C = (pl.S - 1 + clen - 1) / clen;
-    return scratch_round(2 * C * B * pl.m * 8);
+    return scratch_round(3 * C * B * pl.m * 8);
 }
 
 double* alloc_scan_chunks(DeviceScratch& scratch, const Plan& pl, uint64_t B, magnus::LmArgs& a) {
@@ -450,7 +452,7 @@ double* alloc_scan_chunks(DeviceScratch& scratch, const Plan& pl, uint64_t B, ma
     a.cscan = nullptr;
     if (!a.clen) return nullptr;
     const uint64_t C = (pl.S - 1 + a.clen - 1) / a.clen;
-    a.cscan = scratch.alloc<double>(2 * C * B * pl.m);
+    a.cscan = scratch.alloc<double>(3 * C * B * pl.m);
     if (!a.cscan) a.clen = 0;
     return a.cscan;
 }

@@ -41,7 +41,7 @@ struct LmArgs {
     double* dAjob;       // [jobs][S][m]               (VJP)
     double* totals;      // [jobs][n][m] last prefix value of each level, or nullptr (forward)
     const double* wbar;  // [jobs][n][m] cotangents of the level totals, or nullptr (VJP)
-    double* cscan;       // [2][nchunks][jobs*m] chunk scratch of the sample-parallel scan, or nullptr
+    double* cscan;       // [3][nchunks][jobs*m] chunk scratch of the sample-parallel scans, or nullptr
     int clen;            // samples per chunk of the sample-parallel scan (multiple of 4)
     long long S;
     long long jobs;

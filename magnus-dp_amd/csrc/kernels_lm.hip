@@ -586,6 +586,124 @@ __global__ void __launch_bounds__(256) lm_vjp_scan(LmArgs a) {
     }
 }
 
+// ---------------------------------------------------------------------------
+// Sample-parallel reverse scan, Boole, W-basis sweep (shift 0).  The reverse
+// block loop of prefix_vjp_boole carries g4 (the adjoint reaching a block's
+// end, additive: a block passes r0 + g1 + g2 + g3 + g4 down) and `pending`
+// (linear in the lowest block's g4).  Pass 1 runs each chunk with zero inputs
+// and keeps its chain increment, its outgoing pending and its raw lowest-block
+// start (which the chunk below overwrites); pass 2 walks the chunks top-down
+// per entry to get each chunk's true inputs; pass 3 reruns each chunk from
+// them and applies sum_vjp_add to the samples it writes.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ void boole_rev_block(double r0, double g1, double g2, double g3, double& g4,
+                                                double& pending, double dt, double& a1, double& a2, double& a3,
+                                                double& a4) {
+    double a0 = r0;
+    a0 += g1;
+    a0 += g2;
+    a0 += g3;
+    a0 += g4;
+    a1 = g1 * (dt / 2);
+    a1 += g2 * (4 * dt / 3);
+    a1 += g3 * (9 * dt / 8);
+    a1 += g4 * (64 * dt / 45);
+    a2 = g2 * (dt / 3);
+    a2 += g3 * (9 * dt / 8);
+    a2 += g4 * (24 * dt / 45);
+    a3 = g3 * (3 * dt / 8);
+    a3 += g4 * (64 * dt / 45);
+    a4 = g4 * (14 * dt / 45);
+    a4 += pending;
+    double p = g1 * (dt / 2);
+    p += g2 * (dt / 3);
+    p += g3 * (3 * dt / 8);
+    p += g4 * (14 * dt / 45);
+    pending = p;
+    g4 = a0;
+}
+
+__global__ void __launch_bounds__(256) lm_vjp_scan_b1(LmArgs a) {
+    const long long JM = a.jobs * a.m;
+    const int C = lm_nchunks(a);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= JM * C) return;
+    const int c = (int)(idx / JM);
+    const long long je = idx - (long long)c * JM;
+    const long long job = je / a.m;
+    const int e = (int)(je - job * a.m), m = a.m;
+    const long long S = a.S, s0 = (long long)c * a.clen;
+    const long long e0 = (s0 + a.clen < S - 1) ? s0 + a.clen : S - 1;
+    const double* G = a.gbuf + (job * 2 + (1 - a.ga)) * S * m + e;
+    double g4 = 0.0, pending = 0.0, a1, a2, a3, a4;
+    for (long long s = e0 - 4; s >= s0; s -= 4) {
+        boole_rev_block(G[s * m], G[(s + 1) * m], G[(s + 2) * m], G[(s + 3) * m], g4, pending, a.dt, a1, a2, a3,
+                        a4);
+    }
+    a.cscan[(long long)c * JM + je] = g4;                  // chain increment
+    a.cscan[((long long)C + c) * JM + je] = pending;       // outgoing pending with zero inputs
+    a.cscan[((long long)2 * C + c) * JM + je] = G[s0 * m]; // raw lowest-block start
+}
+
+__global__ void __launch_bounds__(64) lm_vjp_scan_b2(LmArgs a) {
+    const long long JM = a.jobs * a.m;
+    const int C = lm_nchunks(a);
+    const long long je = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (je >= JM) return;
+    const long long job = je / a.m;
+    const int e = (int)(je - job * a.m), m = a.m, k = a.level;
+    const long long S = a.S;
+    const double* G = a.gbuf + (job * 2 + (1 - a.ga)) * S * m + e;
+    double g4 = G[(S - 1) * m];  // bar_total with shift 0: the last sample's value
+    if (a.wbar) g4 += a.wbar[(job * a.n + (k - 2)) * (long long)m + e];
+    double pending = 0.0;
+    const double c14 = 14 * a.dt / 45;
+    for (int c = C - 1; c >= 0; --c) {
+        const double inc = a.cscan[(long long)c * JM + je];
+        const double pl = a.cscan[((long long)C + c) * JM + je];
+        a.cscan[(long long)c * JM + je] = g4;
+        a.cscan[((long long)C + c) * JM + je] = pending;
+        pending = pl + g4 * c14;
+        g4 = inc + g4;
+    }
+}
+
+__global__ void __launch_bounds__(256) lm_vjp_scan_b3(LmArgs a) {
+    const long long JM = a.jobs * a.m;
+    const int C = lm_nchunks(a);
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= JM * C) return;
+    const int c = (int)(idx / JM);
+    const long long je = idx - (long long)c * JM;
+    const long long job = je / a.m;
+    const int e = (int)(je - job * a.m), m = a.m, k = a.level;
+    const long long S = a.S, s0 = (long long)c * a.clen;
+    const long long e0 = (s0 + a.clen < S - 1) ? s0 + a.clen : S - 1;
+    double* G = a.gbuf + (job * 2 + (1 - a.ga)) * S * m + e;
+    const bool add = (k > 2 && a.op != OP_ONE);
+    const double o = add ? lm_cot(a, job, a.op == OP_MANY ? k - 2 : 0, e) : 0.0;
+    const double dscale = a.dt * a.w[0];
+    double g4 = a.cscan[(long long)c * JM + je];
+    double pending = a.cscan[((long long)C + c) * JM + je];
+    const double raw0 = a.cscan[((long long)2 * C + c) * JM + je];
+    double a1, a2, a3, a4;
+    for (long long s = e0 - 4; s >= s0; s -= 4) {
+        const double r0 = (s == s0) ? raw0 : G[s * m];
+        boole_rev_block(r0, G[(s + 1) * m], G[(s + 2) * m], G[(s + 3) * m], g4, pending, a.dt, a1, a2, a3, a4);
+        if (add) {
+            a1 = sum_vjp_add<BOOLE>(a1, s + 1, S, o, dscale);
+            a2 = sum_vjp_add<BOOLE>(a2, s + 2, S, o, dscale);
+            a3 = sum_vjp_add<BOOLE>(a3, s + 3, S, o, dscale);
+            a4 = sum_vjp_add<BOOLE>(a4, s + 4, S, o, dscale);
+        }
+        G[(s + 1) * m] = a1;
+        G[(s + 2) * m] = a2;
+        G[(s + 3) * m] = a3;
+        G[(s + 4) * m] = a4;
+    }
+    if (c == 0) G[0] = add ? sum_vjp_add<BOOLE>(pending, 0, S, o, dscale) : pending;
+}
+
 // dA_job += barY  (grad.hpp:144, 247, 347)
 __global__ void __launch_bounds__(256) lm_vjp_final(LmArgs a) {
     const long long per = a.S * a.m;
@@ -732,7 +850,14 @@ hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
     }
     timer_end(h);
     h = timer_begin("lm_vjp_scan", st);
-    hipLaunchKernelGGL((lm_vjp_scan<I>), dim3((a.jobs * a.m + 63) / 64), dim3(64), 0, st, a);
+    if (I == BOOLE && chunked_scan<I>(a) && a.Q == 1 && a.x[0] == 1.0) {
+        const long long JM = a.jobs * a.m, work = JM * lm_nchunks(a);
+        hipLaunchKernelGGL(lm_vjp_scan_b1, dim3((work + 255) / 256), dim3(256), 0, st, a);
+        hipLaunchKernelGGL(lm_vjp_scan_b2, dim3((JM + 63) / 64), dim3(64), 0, st, a);
+        hipLaunchKernelGGL(lm_vjp_scan_b3, dim3((work + 255) / 256), dim3(256), 0, st, a);
+    } else {
+        hipLaunchKernelGGL((lm_vjp_scan<I>), dim3((a.jobs * a.m + 63) / 64), dim3(64), 0, st, a);
+    }
     timer_end(h);
     return hipGetLastError();
 }

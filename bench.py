@@ -576,6 +576,7 @@ def main():
     ap.add_argument("--uneven-branches", type=int, default=4)
     ap.add_argument("--uneven-workspace-gb", type=float, default=48.0)
     ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
+    ap.add_argument("--dim", type=int, default=32, help="dense workload: matrix dimension (profiling only)")
     ap.add_argument("--sc-residual", default="checkpoint", choices=["checkpoint", "recompute"],
                     help="spacecurve-batch: what the VJP starts from")
     ap.add_argument("--no-cpu", action="store_true")
@@ -636,7 +637,7 @@ def main():
                             else "none (VJP recomputes its forward quantities)")}
         do_vjp = True
     elif wl == "dense-d32":
-        B, S, d, n, modes = args.batch or 256, 4097, 32, 4, 5
+        B, S, d, n, modes = args.batch or 256, 4097, args.dim, 4, 5
         a, cot = _dense_inputs(torch, B, S, d, modes, dev, seed=3)
 
         def step():
@@ -645,7 +646,8 @@ def main():
             return out, dA
 
         Q, K, m, m_in, P, n_out = (n + 3) // 2, n - 1, d * d, d * d, 2 * d ** 3, 1
-        config = {"workload": "dense-d32", "instances": B, "samples": S, "dim": d, "n": n, "integrator": "Boole",
+        config = {"workload": "dense-d32" if d == 32 else f"dense-d{d}", "instances": B, "samples": S, "dim": d,
+                  "n": n, "integrator": "Boole",
                   "op": "sum", "carry": "recorded"}
         do_vjp = True
     else:

@@ -622,3 +622,35 @@ def test_dense_wbasis_large_norm(scale):
     assert rel_fro(out, O.forward(n, A, 0.0, 1.0, op="many", integrator="Boole")) <= TOL
     assert rel_fro(dA, O.vjp(n, A, cot, 0.0, 1.0, op="many", integrator="Boole")) <= TOL
 
+
+
+def test_dense_d32_vjp_central_differences():
+    """Dense MFMA path (d=32, Boole, n=4, op sum): the VJP from the recorded
+    carry and the carry-free (W-basis) VJP against central differences with
+    h = 1e-6 on 64 random sample entries of one instance."""
+    import torch
+
+    from magnus import torch_ops
+
+    S, d, n = 257, 32, 4
+    A = fourier_matrices(np.random.default_rng([47]), S, d, modes=5)
+    cot = np.random.default_rng([47, 1]).standard_normal((d, d))
+    At = torch.from_numpy(A[None]).cuda()
+    out, carry = torch_ops.forward(n, At, 0.0, 1.0, op="sum", integrator="Boole", record_vjp=True)
+    g_carry = torch_ops.vjp(n, At, torch.from_numpy(cot[None]).cuda(), carry, 0.0, 1.0, op="sum",
+                            integrator="Boole")[0].cpu().numpy()
+    g_free = torch_ops.vjp(n, At, torch.from_numpy(cot[None]).cuda(), None, 0.0, 1.0, op="sum",
+                           integrator="Boole")[0].cpu().numpy()
+    rng = np.random.default_rng([47, 2])
+    picks = [(int(rng.integers(S)), int(rng.integers(d)), int(rng.integers(d))) for _ in range(64)]
+    h = 1e-6
+    Ap = np.repeat(A[None], 2 * len(picks), axis=0)
+    for k, (i, r, c) in enumerate(picks):
+        Ap[2 * k, i, r, c] += h
+        Ap[2 * k + 1, i, r, c] -= h
+    outs = torch_ops.forward(n, torch.from_numpy(Ap).cuda(), 0.0, 1.0, op="sum", integrator="Boole").cpu().numpy()
+    fd = np.array([((outs[2 * k] - outs[2 * k + 1]) * cot).sum() / (2 * h) for k in range(len(picks))])
+    for g in (g_carry, g_free):
+        an = np.array([g[i, r, c] for i, r, c in picks])
+        assert np.max(np.abs(an - fd)) <= 1e-6 * max(1.0, np.max(np.abs(fd))), np.max(np.abs(an - fd))
+    assert rel_fro(g_free, g_carry) <= TOL

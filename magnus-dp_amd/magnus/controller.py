@@ -239,11 +239,27 @@ def _chunks(ids, S, d, budget=4e9):
     return [ids[k:k + per] for k in range(0, len(ids), per)]
 
 
+_STREAMS: list = []
+
+
+def _streams(k: int = 4):
+    """A small pool of side streams (default priority) for independent groups."""
+    while len(_STREAMS) < k:
+        _STREAMS.append(torch.cuda.Stream())
+    return _STREAMS[:k]
+
+
+def _index(ids, device):
+    return torch.tensor(ids, dtype=torch.long).pin_memory().to(device, non_blocking=True)
+
+
 def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24):
     """The judge: from per-instance starting resolutions (list of (S, n)),
     verify and escalate until every instance passes.  Returns (accepted
     resolutions, accepted sums [B][d][d], stats) with stats = {work: sum of
-    (S-1)(n-1) over the sweeps run, rounds: per-instance rounds, calls}."""
+    (S-1)(n-1) over the sweeps run, rounds: per-instance rounds, calls}.
+    The groups of a round run on a pool of streams; the host waits once per
+    round."""
     B, d = coeffs.shape[0], coeffs.shape[-1]
     res = list(start)
     done = [False] * B
@@ -251,6 +267,8 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24):
     out = torch.empty_like(coeffs[:, 0])
     work = 0
     calls = 0
+    main = torch.cuda.current_stream()
+    pool = _streams()
     for _ in range(max_rounds):
         groups = {}
         for i in range(B):
@@ -258,42 +276,56 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24):
                 groups.setdefault(res[i], []).append(i)
         if not groups:
             break
+        pending = []
+        k = 0
         for (S, n), all_ids in groups.items():
             for ids in _chunks(all_ids, S, d):
-                idx = torch.tensor(ids, device=coeffs.device)
-                c = coeffs[idx]
-                nm = min(n + 1, REF_N)
-                hi = sums(c, S, nm)
-                cur = hi[:, n - 1]
-                if S < REF_S:
-                    up = sums(c, 2 * S - 1, n)[:, n - 1]
-                    calls += 1
-                    work += len(ids) * cost(2 * S - 1, n)
-                    s_ok = (rel_err(up, cur) <= JUDGE_MARGIN * tol).cpu().numpy()
-                else:
-                    s_ok = np.ones(len(ids), bool)
-                calls += 1
-                work += len(ids) * cost(S, nm)
-                n_ok = ((rel_err(hi[:, nm - 1], cur) <= JUDGE_MARGIN * tol).cpu().numpy() if nm > n
-                        else np.ones(len(ids), bool))
-                out[idx] = cur
-                for k, i in enumerate(ids):
-                    rounds[i] += 1
-                    if (s_ok[k] and n_ok[k]) or (S >= REF_S and n >= REF_N):
-                        done[i] = True
-                        res[i] = (S, n)
+                st = pool[k % len(pool)]
+                k += 1
+                st.wait_stream(main)
+                with torch.cuda.stream(st):
+                    idx = _index(ids, coeffs.device)
+                    c = coeffs[idx]
+                    nm = min(n + 1, REF_N)
+                    hi = sums(c, S, nm)
+                    cur = hi[:, n - 1]
+                    if S < REF_S:
+                        up = sums(c, 2 * S - 1, n)[:, n - 1]
+                        s_ok = rel_err(up, cur) <= JUDGE_MARGIN * tol
+                        calls += 1
+                        work += len(ids) * cost(2 * S - 1, n)
                     else:
-                        S2 = S if s_ok[k] or S >= REF_S else 2 * S - 1
-                        n2 = n if n_ok[k] or n >= REF_N else n + 1
-                        res[i] = (S2, n2)
+                        s_ok = torch.ones(len(ids), dtype=torch.bool, device=coeffs.device)
+                    n_ok = (rel_err(hi[:, nm - 1], cur) <= JUDGE_MARGIN * tol if nm > n
+                            else torch.ones(len(ids), dtype=torch.bool, device=coeffs.device))
+                    out[idx] = cur
+                    calls += 1
+                    work += len(ids) * cost(S, nm)
+                    pending.append(((S, n), ids, s_ok, n_ok))
+        for st in pool:
+            main.wait_stream(st)
+        torch.cuda.synchronize()
+        for (S, n), ids, s_ok, n_ok in pending:
+            s_ok = s_ok.cpu().numpy()
+            n_ok = n_ok.cpu().numpy()
+            for kk, i in enumerate(ids):
+                rounds[i] += 1
+                if (s_ok[kk] and n_ok[kk]) or (S >= REF_S and n >= REF_N):
+                    done[i] = True
+                    res[i] = (S, n)
+                else:
+                    S2 = S if s_ok[kk] or S >= REF_S else 2 * S - 1
+                    n2 = n if n_ok[kk] or n >= REF_N else n + 1
+                    res[i] = (S2, n2)
     return res, out, {"work": work, "rounds": rounds, "calls": calls}
 
 
 def solve(coeffs: torch.Tensor, res, cot: torch.Tensor, keep_dA: bool = False, forward_out=None):
     """Forward (op sum) + VJP of every instance at its resolution, grouped by
-    resolution; returns (outputs [B][d][d], dA list or None, work in sweep
-    units).  forward_out: outputs the judge already computed at these
-    resolutions (the forward is then not repeated)."""
+    resolution (groups spread over a pool of streams); returns (outputs
+    [B][d][d], dA list or None, work in sweep units).  forward_out: outputs
+    the judge already computed at these resolutions (the forward is then not
+    repeated)."""
     d = coeffs.shape[-1]
     groups = {}
     for i, r in enumerate(res):
@@ -301,18 +333,27 @@ def solve(coeffs: torch.Tensor, res, cot: torch.Tensor, keep_dA: bool = False, f
     out = torch.empty_like(coeffs[:, 0]) if forward_out is None else forward_out
     dA = [None] * len(res) if keep_dA else None
     work = 0
+    main = torch.cuda.current_stream()
+    pool = _streams()
+    k = 0
     for (S, n), all_ids in groups.items():
         for ids in _chunks(all_ids, S, d):
-            idx = torch.tensor(ids, device=coeffs.device)
-            A = torch_ops.sample_fourier(coeffs[idx], S)
-            if forward_out is None:
-                out[idx] = torch_ops.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole")
-                work += len(ids) * cost(S, n)
-            g = torch_ops.vjp(n, A, cot[idx], None, 0.0, 1.0, op="sum", integrator="Boole")
-            work += len(ids) * 3 * cost(S, n)  # forward recompute + reverse sweep
-            if keep_dA:
-                for k, i in enumerate(ids):
-                    dA[i] = g[k]
+            st = pool[k % len(pool)]
+            k += 1
+            st.wait_stream(main)
+            with torch.cuda.stream(st):
+                idx = _index(ids, coeffs.device)
+                A = torch_ops.sample_fourier(coeffs[idx], S)
+                if forward_out is None:
+                    out[idx] = torch_ops.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole")
+                    work += len(ids) * cost(S, n)
+                g = torch_ops.vjp(n, A, cot[idx], None, 0.0, 1.0, op="sum", integrator="Boole")
+                work += len(ids) * 3 * cost(S, n)  # forward recompute + reverse sweep
+                if keep_dA:
+                    for kk, i in enumerate(ids):
+                        dA[i] = g[kk]
+    for st in pool:
+        main.wait_stream(st)
     return out, dA, work
 
 

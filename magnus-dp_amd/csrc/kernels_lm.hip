@@ -191,42 +191,64 @@ struct MfmaShape {
 template <int D>
 constexpr int default_wpp() { return D == 64 ? 4 : 1; }
 
-// C_tile += Aop * Bop over k = 0..D-1 with Aop[i][k] = LA[k*D + i0 + i], Bop[k][j] = LB[k*D + j0 + j]
+// Operands sit in LDS k-major with the leading dimension padded to D + 1
+// doubles: transposed stores (consecutive lanes step one row) and the tile
+// reads (consecutive lanes step one column, four rows apart) then fall on
+// different banks.
+// C_tile += Aop * Bop over k = 0..D-1 with Aop[i][k] = LA[k*LD + i0 + i], Bop[k][j] = LB[k*LD + j0 + j]
 template <int D>
 __device__ __forceinline__ dbl4 mfma_tile(const double* LA, const double* LB, int i0, int j0, int lane) {
+    constexpr int LD = D + 1;
     dbl4 acc = {0.0, 0.0, 0.0, 0.0};
     const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
     for (int kk = 0; kk < D / 4; ++kk) {
-        const double av = LA[(4 * kk + lk) * D + i0 + li];
-        const double bv = LB[(4 * kk + lk) * D + j0 + li];
+        const double av = LA[(4 * kk + lk) * LD + i0 + li];
+        const double bv = LB[(4 * kk + lk) * LD + j0 + li];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
     return acc;
 }
 
-// Y_i = A_i (P_i + s T) for products p = job*S + i
+template <int D>
+__device__ __forceinline__ void store_tile(double* out, const dbl4& acc, int i0, int j0, int lane, bool add) {
+#pragma unroll
+    for (int r = 0; r < 4; ++r) {
+        double* o = out + (i0 + 4 * r + (lane >> 4)) * D + j0 + (lane & 15);
+        if (add) *o += acc[r];
+        else *o = acc[r];
+    }
+}
+
+// Y_i = A_i (P_i + s T) for products p = job*S + i.  Staging: per product the
+// index arithmetic once (uniform over the workgroup), then its D*D entries.
 template <int D, int W>
 __global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
     using M = MfmaShape<D, W>;
-    __shared__ double LA[M::PPB][M::MM];  // A_i transposed: LA[k][i] = a[i][k]
-    __shared__ double LB[M::PPB][M::MM];  // P_i + s T
+    constexpr int LD = D + 1, LM = D * LD;
+    __shared__ double LA[M::PPB][LM];  // A_i transposed: LA[k][i] = a[i][k]
+    __shared__ double LB[M::PPB][LM];  // P_i + s T: LB[k][j]
     const long long total = a.jobs * a.S;
     const long long p0 = (long long)blockIdx.x * M::PPB;
-    for (int x = threadIdx.x; x < M::PPB * M::MM; x += 256) {
-        const int g = x / M::MM, e = x - g * M::MM;
+#pragma unroll
+    for (int g = 0; g < M::PPB; ++g) {
         const long long p = p0 + g;
-        if (p >= total) continue;
+        if (p >= total) break;
         const long long job = p / a.S, i = p - job * a.S;
         const long long b = job / a.Qj;
         const int q = (int)(job - b * a.Qj);
         const double shift = a.x[q] - 1;
         const double* P = lm_slot(a, job, b, q, a.level - 1);
-        double v = P[i * M::MM + e];
-        v += P[(a.S - 1) * M::MM + e] * shift;
-        LB[g][e] = v;
-        const int r = e / D, c = e - r * D;
-        LA[g][c * D + r] = a.in[(b * a.S + i) * (long long)M::MM + e];
+        const double* Pi = P + i * M::MM;
+        const double* T = P + (a.S - 1) * M::MM;
+        const double* Ai = a.in + (b * a.S + i) * (long long)M::MM;
+        for (int e = threadIdx.x; e < M::MM; e += 256) {
+            const int r = e / D, c = e - r * D;
+            double v = Pi[e];
+            v += T[e] * shift;
+            LB[g][r * LD + c] = v;
+            LA[g][c * LD + r] = Ai[e];
+        }
     }
     __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
@@ -238,9 +260,7 @@ __global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
     for (int tt = 0; tt < M::TPW; ++tt) {
         const int tile = wp * M::TPW + tt;
         const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
-        const dbl4 acc = mfma_tile<D>(LA[g], LB[g], i0, j0, lane);
-#pragma unroll
-        for (int r = 0; r < 4; ++r) y[(i0 + 4 * r + (lane >> 4)) * D + j0 + (lane & 15)] = acc[r];
+        store_tile<D>(y, mfma_tile<D>(LA[g], LB[g], i0, j0, lane), i0, j0, lane, false);
     }
 }
 
@@ -249,8 +269,9 @@ __global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
 template <int D, int W>
 __global__ void __launch_bounds__(256) lm_vjp_prod_mfma(LmArgs a) {
     using M = MfmaShape<D, W>;
-    __shared__ double LA[M::PPB][M::MM];
-    __shared__ double LB[M::PPB][M::MM];
+    constexpr int LD = D + 1, LM = D * LD;
+    __shared__ double LA[M::PPB][LM];
+    __shared__ double LB[M::PPB][LM];
     const long long per = a.S * M::MM;
     const long long total = a.jobs * a.S;
     const long long p0 = (long long)blockIdx.x * M::PPB;
@@ -258,20 +279,25 @@ __global__ void __launch_bounds__(256) lm_vjp_prod_mfma(LmArgs a) {
     const int gw = wave / M::WPP, wp = wave - gw * M::WPP;
     const long long pw = p0 + gw;
     // phase 1: LA[k][i] = g[i][k], LB[k][j] = b[j][k]
-    for (int x = threadIdx.x; x < M::PPB * M::MM; x += 256) {
-        const int g = x / M::MM, e = x - g * M::MM;
+#pragma unroll
+    for (int g = 0; g < M::PPB; ++g) {
         const long long p = p0 + g;
-        if (p >= total) continue;
+        if (p >= total) break;
         const long long job = p / a.S, i = p - job * a.S;
         const long long b = job / a.Q;
         const int q = (int)(job - b * a.Q);
         const double shift = a.x[q] - 1.0;
         const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
-        double v = P[i * M::MM + e];
-        v += P[(a.S - 1) * M::MM + e] * shift;
-        const int r = e / D, c = e - r * D;
-        LB[g][c * D + r] = v;
-        LA[g][c * D + r] = a.gbuf[(job * 2 + a.ga) * per + i * M::MM + e];
+        const double* Pi = P + i * M::MM;
+        const double* T = P + (a.S - 1) * M::MM;
+        const double* Gi = a.gbuf + (job * 2 + a.ga) * per + i * M::MM;
+        for (int e = threadIdx.x; e < M::MM; e += 256) {
+            const int r = e / D, c = e - r * D;
+            double v = Pi[e];
+            v += T[e] * shift;
+            LB[g][c * LD + r] = v;
+            LA[g][c * LD + r] = Gi[e];
+        }
     }
     __syncthreads();
     if (pw < total) {
@@ -281,21 +307,24 @@ __global__ void __launch_bounds__(256) lm_vjp_prod_mfma(LmArgs a) {
         for (int tt = 0; tt < M::TPW; ++tt) {
             const int tile = wp * M::TPW + tt;
             const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
-            const dbl4 acc = mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) dA[(i0 + 4 * r + (lane >> 4)) * D + j0 + (lane & 15)] += acc[r];
+            store_tile<D>(dA, mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane), i0, j0, lane, true);
         }
     }
     __syncthreads();
     // phase 2: LA[k][i] = a[k][i], LB[k][j] = g[k][j]
-    for (int x = threadIdx.x; x < M::PPB * M::MM; x += 256) {
-        const int g = x / M::MM, e = x - g * M::MM;
+#pragma unroll
+    for (int g = 0; g < M::PPB; ++g) {
         const long long p = p0 + g;
-        if (p >= total) continue;
+        if (p >= total) break;
         const long long job = p / a.S, i = p - job * a.S;
         const long long b = job / a.Q;
-        LA[g][e] = a.in[(b * a.S + i) * (long long)M::MM + e];
-        LB[g][e] = a.gbuf[(job * 2 + a.ga) * per + i * M::MM + e];
+        const double* Ai = a.in + (b * a.S + i) * (long long)M::MM;
+        const double* Gi = a.gbuf + (job * 2 + a.ga) * per + i * M::MM;
+        for (int e = threadIdx.x; e < M::MM; e += 256) {
+            const int r = e / D, c = e - r * D;
+            LA[g][r * LD + c] = Ai[e];
+            LB[g][r * LD + c] = Gi[e];
+        }
     }
     __syncthreads();
     if (pw < total) {
@@ -305,9 +334,7 @@ __global__ void __launch_bounds__(256) lm_vjp_prod_mfma(LmArgs a) {
         for (int tt = 0; tt < M::TPW; ++tt) {
             const int tile = wp * M::TPW + tt;
             const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
-            const dbl4 acc = mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane);
-#pragma unroll
-            for (int r = 0; r < 4; ++r) Gb[(i0 + 4 * r + (lane >> 4)) * D + j0 + (lane & 15)] = acc[r];
+            store_tile<D>(Gb, mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane), i0, j0, lane, false);
         }
     }
 }

@@ -208,6 +208,79 @@ struct BlockRule<RIEMANN> {
     __device__ static double weight(int, double dt) { return dt; }  // default.hpp:28-30
 };
 
+// ---------------------------------------------------------------------------
+// Sample feed of the chunk passes (Simpson and Boole).  The lanes of a
+// wavefront walk 64 different chunks, so a per-lane read of one sample touches
+// 64 cache lines and the texture path, not the FP64 pipe, sets the pace.  The
+// feed loads groups of kFeedG samples of every chunk with direct global->LDS
+// loads (global_load_lds_dwordx4, 16 bytes per lane, the LDS address being the
+// slab base + 16 * lane): six lanes read one chunk's group (96 contiguous
+// bytes) and ten chunks share one instruction, whose 960 bytes fill one slab;
+// each lane then reads its own chunk's samples from its slab.  No registers
+// hold data in flight.  Under these rules chunk lengths are even ((S-1) % B ==
+// 0, chunk length a multiple of 4), so a 16-byte piece never straddles the
+// first or last sample of a chunk and every piece read lies in the chunk.
+// Group g covers the chunk's samples counted up from its first feed sample
+// (UP) or down from its end (!UP).
+// ---------------------------------------------------------------------------
+constexpr int kFeedG = 4;                                  // samples per group
+constexpr int kFeedPieces = kFeedG * 3 / 2;                // 16-byte pieces per chunk and group
+constexpr int kFeedPer = 64 / kFeedPieces;                 // chunks per load instruction (lanes 60..63 idle)
+constexpr int kFeedOps = (64 + kFeedPer - 1) / kFeedPer;   // load instructions per group
+constexpr int kFeedSlab = kFeedPer * kFeedG * 3 + 6;       // doubles per slab: 960 bytes + 48 (bank spread)
+
+// Per wavefront the feed holds its slabs (the load targets) and a table of
+// the wavefront's chunks in two separate LDS arrays, so that reads of the
+// table need not wait for loads in flight into the slabs.
+struct alignas(16) FeedSrc {
+    long long off;  // element offset of group 0 (UP: its first sample; !UP: its lowest sample)
+    int len;        // samples of the chunk the feed may read (0: none)
+    int pad;
+};
+using FeedSlabs = double[kFeedOps][kFeedSlab];
+
+// each lane publishes where its chunk's groups start
+__device__ __forceinline__ void feed_init(FeedSrc* src, int lane, long long off, int len) {
+    src[lane] = FeedSrc{off, len, 0};
+    __builtin_amdgcn_wave_barrier();
+}
+
+// issue the loads of group g for all 64 chunks of the wavefront (every lane calls)
+template <bool UP>
+__device__ __forceinline__ void feed_issue(FeedSlabs& slab, const FeedSrc* src, const double* A, int lane, int g) {
+    const int sub = lane / kFeedPieces, k = lane - sub * kFeedPieces;
+    const int first = (2 * k) / 3, last = (2 * k + 1) / 3;  // samples of the group that piece k touches
+    FeedSrc sr[kFeedOps];  // all table reads first, then the loads
+#pragma unroll
+    for (int m = 0; m < kFeedOps; ++m) {
+        const int o = m * kFeedPer + sub;
+        sr[m] = src[o < 64 ? o : 63];
+    }
+    const long long step = (long long)(3 * kFeedG) * g;
+#pragma unroll
+    for (int m = 0; m < kFeedOps; ++m) {
+        const int o = m * kFeedPer + sub;
+        const bool ok = sub < kFeedPer && o < 64 &&
+                        (UP ? (kFeedG * g + last < sr[m].len) : (first >= kFeedG * (g + 1) - sr[m].len));
+        if (ok) {
+            const double* p = A + (UP ? sr[m].off + step : sr[m].off - step) + 2 * k;
+            __builtin_amdgcn_global_load_lds(p, (__attribute__((address_space(3))) void*)&slab[m][0], 16, 0, 0);
+        }
+    }
+}
+
+// the group's loads have landed (vmcnt(0)); every lane of the wavefront may read the slabs
+__device__ __forceinline__ void feed_wait() {
+    __builtin_amdgcn_s_waitcnt(0x0F70);
+    __builtin_amdgcn_wave_barrier();
+}
+
+// this lane's chunk in the slabs: sample i, component c at [3 i + c]
+__device__ __forceinline__ const double* feed_row(const FeedSlabs& slab, int lane) {
+    const int m = lane / kFeedPer;
+    return &slab[m][(lane - m * kFeedPer) * kFeedG * 3];
+}
+
 __device__ __forceinline__ long long chunk_begin(const WbArgs& a, int c) { return (long long)c * a.LC; }
 __device__ __forceinline__ long long chunk_end(const WbArgs& a, int c) {
     long long e = (long long)(c + 1) * a.LC;
@@ -218,15 +291,48 @@ __device__ __forceinline__ long long chunk_end(const WbArgs& a, int c) {
 // chunk pass: local streams V_1..V_{K+1} over (s, e] from zero bases.
 // summary per (b, c): [Vhat_1..Vhat_K, ZV_0..ZV_K] (4 doubles each)
 // ---------------------------------------------------------------------------
+
+// one block [s0, s0+B] of the chunk pass; ab = samples s0+1..s0+B.  On entry
+// base[l] = V_{l+1}[s0] and f0[l] = its integrand at s0; on exit both at s0+B.
 template <int I, int K>
-__global__ void __launch_bounds__(256) wb_fwd_pass1(WbArgs a) {
+__device__ __forceinline__ void fwd_block(const double (*ab)[3], V4* base, V4* f0, double dt) {
     using R = BlockRule<I>;
     constexpr int B = R::B;
     constexpr int L = K + 1;
+    V4 fn[B];  // integrand of the current level at s0+1..s0+B
+#pragma unroll
+    for (int r = 0; r < B; ++r) fn[r] = pure(ab[r]);
+#pragma unroll
+    for (int l = 0; l < L; ++l) {
+        V4 Pi[B > 1 ? B - 1 : 1];
+        R::step(f0[l], fn, base[l], dt, Pi);
+        if (l == 0) {  // V_1 = Prefix(A) is a pure vector: drop its zero component
+            base[0].x[0] = 0.0;
+#pragma unroll
+            for (int r = 0; r + 1 < B; ++r) Pi[r].x[0] = 0.0;
+        }
+        f0[l] = fn[B - 1];
+        if (l + 1 < L) {
+#pragma unroll
+            for (int r = 0; r + 1 < B; ++r) fn[r] = amul(ab[r], Pi[r]);
+            fn[B - 1] = amul(ab[B - 1], base[l]);
+        }
+    }
+}
+
+template <int I, int K>
+__global__ void __launch_bounds__(256, 2) wb_fwd_pass1(WbArgs a) {
+    using R = BlockRule<I>;
+    constexpr int B = R::B;
+    constexpr int L = K + 1;
+    constexpr bool kFed = B % 2 == 0;  // Simpson, Boole: samples through the feed
+    __shared__ FeedSlabs feed_slab[kFed ? 4 : 1];
+    __shared__ FeedSrc feed_src[kFed ? 4 : 1][64];
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= a.B * a.C) return;
-    const long long b = t / a.C;
-    const int c = (int)(t - b * a.C);
+    const bool valid = t < a.B * a.C;  // with the feed every lane stays: its loads are cooperative
+    if (!kFed && !valid) return;
+    const long long b = valid ? t / a.C : 0;
+    const int c = valid ? (int)(t - b * a.C) : 0;
     const long long S = a.S;
     const double dt = a.dt;
     const double* A3 = a.in + b * S * 3;
@@ -249,40 +355,54 @@ __global__ void __launch_bounds__(256) wb_fwd_pass1(WbArgs a) {
             X = amul(as, base[l]);
         }
     }
-    double nx[B][3];
+    if constexpr (kFed) {
+        // group g = samples s+1+4g .. s+4+4g; the wavefront runs the largest group count
+        const int lane = (int)(threadIdx.x & 63);
+        FeedSlabs& slab = feed_slab[threadIdx.x >> 6];
+        FeedSrc* src = feed_src[threadIdx.x >> 6];
+        const int len = valid ? (int)(e - s) : 0;
+        feed_init(src, lane, (b * S + s + 1) * 3, len);
+        int ng = (len + kFeedG - 1) / kFeedG;
+        for (int off = 32; off > 0; off >>= 1) ng = max(ng, __shfl_xor(ng, off, 64));
+        const double* row = feed_row(slab, lane);
+        if (ng > 0) {
+            feed_issue<true>(slab, src, a.in, lane, 0);
+            feed_wait();
+        }
+        for (int g = 0; g < ng; ++g) {
+            double gs[kFeedG][3];
 #pragma unroll
-    for (int r = 0; r < B; ++r) lda(A3, s + 1 + r, nx[r]);
+            for (int i = 0; i < kFeedG; ++i) {
+                gs[i][0] = row[3 * i];
+                gs[i][1] = row[3 * i + 1];
+                gs[i][2] = row[3 * i + 2];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (g + 1 < ng) feed_issue<true>(slab, src, a.in, lane, g + 1);  // lands during this group
+#pragma unroll
+            for (int jb = 0; jb < kFeedG / B; ++jb)
+                if (kFeedG * g + jb * B < len) fwd_block<I, K>(gs + jb * B, base, f0, dt);
+            feed_wait();
+        }
+        if (!valid) return;
+    } else {
+        double nx[B][3];
+#pragma unroll
+        for (int r = 0; r < B; ++r) lda(A3, s + 1 + r, nx[r]);
 #pragma unroll 2
-    for (long long s0 = s; s0 < e; s0 += B) {
-        double ab[B][3];
+        for (long long s0 = s; s0 < e; s0 += B) {
+            double ab[B][3];
 #pragma unroll
-        for (int r = 0; r < B; ++r) {
-            ab[r][0] = nx[r][0];
-            ab[r][1] = nx[r][1];
-            ab[r][2] = nx[r][2];
-        }
-        if (s0 + B < e) {  // next block's samples, in flight during this block
-#pragma unroll
-            for (int r = 0; r < B; ++r) lda(A3, s0 + B + 1 + r, nx[r]);
-        }
-        V4 fn[B];  // integrand of the current level at s0+1..s0+B
-#pragma unroll
-        for (int r = 0; r < B; ++r) fn[r] = pure(ab[r]);
-#pragma unroll
-        for (int l = 0; l < L; ++l) {
-            V4 Pi[B > 1 ? B - 1 : 1];
-            R::step(f0[l], fn, base[l], dt, Pi);
-            if (l == 0) {  // V_1 = Prefix(A) is a pure vector: drop its zero component
-                base[0].x[0] = 0.0;
-#pragma unroll
-                for (int r = 0; r + 1 < B; ++r) Pi[r].x[0] = 0.0;
+            for (int r = 0; r < B; ++r) {
+                ab[r][0] = nx[r][0];
+                ab[r][1] = nx[r][1];
+                ab[r][2] = nx[r][2];
             }
-            f0[l] = fn[B - 1];
-            if (l + 1 < L) {
+            if (s0 + B < e) {  // next block's samples, in flight during this block
 #pragma unroll
-                for (int r = 0; r + 1 < B; ++r) fn[r] = amul(ab[r], Pi[r]);
-                fn[B - 1] = amul(ab[B - 1], base[l]);
+                for (int r = 0; r < B; ++r) lda(A3, s0 + B + 1 + r, nx[r]);
             }
+            fwd_block<I, K>(ab, base, f0, dt);
         }
     }
     // f0[l] = A_e V_l[e] (V_0 = identity), base[l] = V_{l+1}[e]

@@ -227,21 +227,20 @@ constexpr int kFeedG = 4;                                  // samples per group
 constexpr int kFeedPieces = kFeedG * 3 / 2;                // 16-byte pieces per chunk and group
 constexpr int kFeedPer = 64 / kFeedPieces;                 // chunks per load instruction (lanes 60..63 idle)
 constexpr int kFeedOps = (64 + kFeedPer - 1) / kFeedPer;   // load instructions per group
-constexpr int kFeedSlab = kFeedPer * kFeedG * 3 + 6;       // doubles per slab: 960 bytes + 48 (bank spread)
+constexpr int kFeedSlab = kFeedPer * kFeedG * 3 + 2;       // doubles per slab: 960 bytes + 16 (bank spread)
 
 // Per wavefront the feed holds its slabs (the load targets) and a table of
 // the wavefront's chunks in two separate LDS arrays, so that reads of the
-// table need not wait for loads in flight into the slabs.
-struct alignas(16) FeedSrc {
-    long long off;  // element offset of group 0 (UP: its first sample; !UP: its lowest sample)
-    int len;        // samples of the chunk the feed may read (0: none)
-    int pad;
-};
+// table need not wait for loads in flight into the slabs.  A table entry packs
+// the element offset of the chunk's group 0 (UP: its first sample; !UP: its
+// lowest sample) above 16 bits of length (the samples the feed may read,
+// 0: none; chunks hold at most 512).
+using FeedSrc = long long;
 using FeedSlabs = double[kFeedOps][kFeedSlab];
 
 // each lane publishes where its chunk's groups start
 __device__ __forceinline__ void feed_init(FeedSrc* src, int lane, long long off, int len) {
-    src[lane] = FeedSrc{off, len, 0};
+    src[lane] = (long long)(((unsigned long long)off << 16) | (unsigned)len);
     __builtin_amdgcn_wave_barrier();
 }
 
@@ -260,10 +259,11 @@ __device__ __forceinline__ void feed_issue(FeedSlabs& slab, const FeedSrc* src, 
 #pragma unroll
     for (int m = 0; m < kFeedOps; ++m) {
         const int o = m * kFeedPer + sub;
-        const bool ok = sub < kFeedPer && o < 64 &&
-                        (UP ? (kFeedG * g + last < sr[m].len) : (first >= kFeedG * (g + 1) - sr[m].len));
+        const int len = (int)(sr[m] & 0xffff);
+        const long long off = sr[m] >> 16;
+        const bool ok = sub < kFeedPer && o < 64 && (UP ? (kFeedG * g + last < len) : (first >= kFeedG * (g + 1) - len));
         if (ok) {
-            const double* p = A + (UP ? sr[m].off + step : sr[m].off - step) + 2 * k;
+            const double* p = A + (UP ? off + step : off - step) + 2 * k;
             __builtin_amdgcn_global_load_lds(p, (__attribute__((address_space(3))) void*)&slab[m][0], 16, 0, 0);
         }
     }
@@ -723,11 +723,9 @@ constexpr int kRevGroup = 8;                          // samples per lane and fl
 constexpr int kRevRow = kRevGroup * 3;                // doubles per lane row
 constexpr int kRevWaves = 4;                          // wavefronts per workgroup
 
-struct RevDest {
-    long long off;  // element offset in dA of slot 0 of the lane's current group
-    int lim;        // slots q <= lim lie at or below the lane's chunk start s
-    int pad;
-};
+// per lane: the element offset in dA of slot 0 of its current group, packed
+// above 16 bits of lim (signed): slots q <= lim lie at or below the chunk start s
+using RevDest = long long;
 
 struct RevStage {
     double row[kRevWaves][64 * kRevRow];
@@ -848,7 +846,7 @@ __device__ __forceinline__ void rev_flush(const double* rows, const RevDest* dst
 #pragma unroll
         for (int i = 0; i < 8; ++i) {
             const int m = (r0 + i) % 3;
-            if (qq[m] > d[i].lim) dA[d[i].off + kk[m]] = v[i];
+            if (qq[m] > (int)(short)(d[i] & 0xffff)) dA[(d[i] >> 16) + kk[m]] = v[i];
         }
     }
 }
@@ -856,15 +854,19 @@ __device__ __forceinline__ void rev_flush(const double* rows, const RevDest* dst
 // publish this lane's current group before a flush
 __device__ __forceinline__ void rev_publish(RevStage& st, int wv, int lane, long long pe, int len, int grp) {
     const int top = kRevGroup * (grp + 1);
-    st.dst[wv][lane] = RevDest{pe - 3 * top, top - len, 0};
+    st.dst[wv][lane] = (long long)(((unsigned long long)(pe - 3 * top) << 16) | ((unsigned)(top - len) & 0xffffu));
 }
 
 template <int I, int K>
 __global__ void __launch_bounds__(256, (I == BOOLE || (I == SIMPSON && K >= 6)) ? 1 : 2) wb_rev(WbArgs a) {
     using R = BlockRule<I>;
     constexpr int B = R::B;
+    constexpr bool kFed = B % 2 == 0;  // Simpson, Boole: samples through the feed
     static_assert(kRevGroup % B == 0, "a block must not straddle two staging groups");
+    static_assert(kRevGroup % kFeedG == 0, "a feed group must not straddle two staging groups");
     __shared__ RevStage st;
+    __shared__ FeedSlabs feed_slab[kFed ? kRevWaves : 1];
+    __shared__ FeedSrc feed_src[kFed ? kRevWaves : 1][64];
     const int lane = (int)(threadIdx.x & 63), wv = (int)(threadIdx.x >> 6);
     const long long t = (long long)blockIdx.x * blockDim.x + threadIdx.x;
     const bool valid = t < a.B * a.C;  // every lane stays: the flushes are cooperative
@@ -916,43 +918,91 @@ __global__ void __launch_bounds__(256, (I == BOOLE || (I == SIMPSON && K >= 6)) 
     const bool has_first = valid && (I != RIEMANN || c == 0);
     const int nb = valid ? (int)((e - s) / B) - (has_first ? 0 : 1) : 0;  // blocks in all
     const int nbn = has_first ? nb - 1 : nb;                              // blocks above s
-    int nbw = nbn;
-    for (int off = 32; off > 0; off >>= 1) nbw = max(nbw, __shfl_xor(nbw, off, 64));
     V4 pend = vzero();
-    double nx[B][3];
-    if (nb > 0) {
+    if constexpr (kFed) {
+        // feed group g = samples e-4g-4 .. e-4g-1 (blocks j with jB / 4 == g), two
+        // feed groups per staging group; the block at s comes last, from its own loads
+        constexpr int kPer = kRevGroup / kFeedG;
+        FeedSlabs& slab = feed_slab[wv];
+        FeedSrc* src = feed_src[wv];
+        feed_init(src, lane, (b * S + e - kFeedG) * 3, len);
+        int ng = (nbn * B + kFeedG - 1) / kFeedG;
+        for (int off = 32; off > 0; off >>= 1) ng = max(ng, __shfl_xor(ng, off, 64));
+        const double* frow = feed_row(slab, lane);
+        if (ng > 0) {
+            feed_issue<false>(slab, src, a.in, lane, 0);
+            feed_wait();
+        }
+        for (int g = 0; g < ng; ++g) {
+            double gs[kFeedG][3];  // samples e-4g-4+i
 #pragma unroll
-        for (int r = 0; r < B; ++r) lda(A3, e - B + r, nx[r]);
-    }
-    int grp = 0;
-    for (int j = 0; j < nbw; ++j) {
-        if (j < nbn) {
-            const long long s0 = e - (long long)(j + 1) * B;
+            for (int i = 0; i < kFeedG; ++i) {
+                gs[i][0] = frow[3 * i];
+                gs[i][1] = frow[3 * i + 1];
+                gs[i][2] = frow[3 * i + 2];
+            }
+            __builtin_amdgcn_wave_barrier();
+            if (g + 1 < ng) feed_issue<false>(slab, src, a.in, lane, g + 1);  // lands during this group
+#pragma unroll
+            for (int jb = 0; jb < kFeedG / B; ++jb) {
+                const int j = g * (kFeedG / B) + jb;
+                if (j < nbn)
+                    rev_block<I, K, false>(e - (long long)(j + 1) * B, e, gs + kFeedG - (jb + 1) * B, aend, Vend, U,
+                                           zK, dt, row, pend);
+            }
+            feed_wait();  // before the stores, so that it waits for the feed only
+            if ((g + 1) % kPer == 0 || g + 1 == ng) {
+                const int grp = g / kPer;
+                rev_publish(st, wv, lane, pe, kRevGroup * grp < nbn * B ? len : 0, grp);
+                rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
+            }
+        }
+        if (has_first) {
             double ab[B][3];
 #pragma unroll
-            for (int r = 0; r < B; ++r) {
-                ab[r][0] = nx[r][0];
-                ab[r][1] = nx[r][1];
-                ab[r][2] = nx[r][2];
-            }
-            if (j + 1 < nb) {  // next block's samples, in flight during this block
+            for (int r = 0; r < B; ++r) lda(A3, s + r, ab[r]);
+            rev_block<I, K, true>(s, e, ab, aend, Vend, U, zK, dt, row, pend);
+        }
+        rev_publish(st, wv, lane, pe, len, (nbn * B) / kRevGroup);
+        rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
+    } else {
+        int nbw = nbn;
+        for (int off = 32; off > 0; off >>= 1) nbw = max(nbw, __shfl_xor(nbw, off, 64));
+        double nx[B][3];
+        if (nb > 0) {
 #pragma unroll
-                for (int r = 0; r < B; ++r) lda(A3, s0 - B + r, nx[r]);
+            for (int r = 0; r < B; ++r) lda(A3, e - B + r, nx[r]);
+        }
+        int grp = 0;
+        for (int j = 0; j < nbw; ++j) {
+            if (j < nbn) {
+                const long long s0 = e - (long long)(j + 1) * B;
+                double ab[B][3];
+#pragma unroll
+                for (int r = 0; r < B; ++r) {
+                    ab[r][0] = nx[r][0];
+                    ab[r][1] = nx[r][1];
+                    ab[r][2] = nx[r][2];
+                }
+                if (j + 1 < nb) {  // next block's samples, in flight during this block
+#pragma unroll
+                    for (int r = 0; r < B; ++r) lda(A3, s0 - B + r, nx[r]);
+                }
+                rev_block<I, K, false>(s0, e, ab, aend, Vend, U, zK, dt, row, pend);
+                grp = (j * B) / kRevGroup;
             }
-            rev_block<I, K, false>(s0, e, ab, aend, Vend, U, zK, dt, row, pend);
-            grp = (j * B) / kRevGroup;
+            if (((j + 1) * B) % kRevGroup == 0 || j + 1 == nbw) {
+                rev_publish(st, wv, lane, pe, len, grp);
+                rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
+            }
         }
-        if (((j + 1) * B) % kRevGroup == 0 || j + 1 == nbw) {
-            rev_publish(st, wv, lane, pe, len, grp);
-            rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
+        if (has_first) {
+            rev_block<I, K, true>(s, e, nx, aend, Vend, U, zK, dt, row, pend);
+            grp = (nbn * B) / kRevGroup;
         }
+        rev_publish(st, wv, lane, pe, len, grp);
+        rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
     }
-    if (has_first) {
-        rev_block<I, K, true>(s, e, nx, aend, Vend, U, zK, dt, row, pend);
-        grp = (nbn * B) / kRevGroup;
-    }
-    rev_publish(st, wv, lane, pe, len, grp);
-    rev_flush(st.row[wv], st.dst[wv], a.dA, lane);
     // the start role of A_s saw the folded ZV_bar_0; in chunks after the first
     // that sample's sum weight belongs to the previous chunk
     if (c > 0 && I != RIEMANN) pend = vaxpy(pend, -R::start_role(dt), ld4(sp + K * 4));

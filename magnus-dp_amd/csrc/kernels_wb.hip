@@ -481,25 +481,26 @@ __device__ __forceinline__ Pair<K> load_pair(const double* p) {
     return r;
 }
 
-__device__ __forceinline__ V4 shfl4(const V4& v, int src) {
-    return V4{{__shfl(v.x[0], src, 64), __shfl(v.x[1], src, 64), __shfl(v.x[2], src, 64), __shfl(v.x[3], src, 64)}};
+// lane shuffles within segments of w lanes (w a power of two dividing 64)
+__device__ __forceinline__ V4 shfl4(const V4& v, int src, int w) {
+    return V4{{__shfl(v.x[0], src, w), __shfl(v.x[1], src, w), __shfl(v.x[2], src, w), __shfl(v.x[3], src, w)}};
 }
-__device__ __forceinline__ V4 shfl_down4(const V4& v, int d) {
-    return V4{{__shfl_down(v.x[0], d, 64), __shfl_down(v.x[1], d, 64), __shfl_down(v.x[2], d, 64),
-               __shfl_down(v.x[3], d, 64)}};
+__device__ __forceinline__ V4 shfl_down4(const V4& v, int d, int w) {
+    return V4{{__shfl_down(v.x[0], d, w), __shfl_down(v.x[1], d, w), __shfl_down(v.x[2], d, w),
+               __shfl_down(v.x[3], d, w)}};
 }
-__device__ __forceinline__ V4 shfl_up4(const V4& v, int d) {
-    return V4{{__shfl_up(v.x[0], d, 64), __shfl_up(v.x[1], d, 64), __shfl_up(v.x[2], d, 64),
-               __shfl_up(v.x[3], d, 64)}};
+__device__ __forceinline__ V4 shfl_up4(const V4& v, int d, int w) {
+    return V4{{__shfl_up(v.x[0], d, w), __shfl_up(v.x[1], d, w), __shfl_up(v.x[2], d, w),
+               __shfl_up(v.x[3], d, w)}};
 }
 
 template <int K>
-__device__ __forceinline__ Pair<K> shfl_down_pair(const Pair<K>& p, int d) {
+__device__ __forceinline__ Pair<K> shfl_down_pair(const Pair<K>& p, int d, int w) {
     Pair<K> r;
 #pragma unroll
-    for (int j = 0; j < K; ++j) r.V.v[j] = shfl_down4(p.V.v[j], d);
+    for (int j = 0; j < K; ++j) r.V.v[j] = shfl_down4(p.V.v[j], d, w);
 #pragma unroll
-    for (int j = 0; j <= K; ++j) r.Z.v[j] = shfl_down4(p.Z.v[j], d);
+    for (int j = 0; j <= K; ++j) r.Z.v[j] = shfl_down4(p.Z.v[j], d, w);
     return r;
 }
 
@@ -517,11 +518,13 @@ template <int K, bool VJP>
 __global__ void __launch_bounds__(256) wb_combine(WbArgs a) {
     constexpr int n = K + 1;
     constexpr int stride = (2 * K + 1) * 4;
-    const int lane = (int)(threadIdx.x & 63);
-    const long long b = (long long)blockIdx.x * 4 + (threadIdx.x >> 6);
-    if (b >= a.B) return;  // uniform over the wavefront
+    // segment of SW lanes per instance (SW = a.cw, a power of two); shuffles stay in the segment
+    const int SW = a.cw;
+    const int lane = (int)(threadIdx.x & (SW - 1));
+    const long long b = ((long long)blockIdx.x * 4 + (threadIdx.x >> 6)) * (64 / SW) + (threadIdx.x & 63) / SW;
+    if (b >= a.B) return;  // uniform over the segment
     const int C = a.C;
-    const int m = (C + 63) / 64;
+    const int m = (C + SW - 1) / SW;
     const int nl = (C + m - 1) / m;
     const int c0 = lane * m;
     const int c1 = c0 + m < C ? c0 + m : C;
@@ -535,15 +538,15 @@ __global__ void __launch_bounds__(256) wb_combine(WbArgs a) {
     }
     Pair<K> suf = pr;
     for (int d = 1; d < nl; d <<= 1) {
-        const Pair<K> o = shfl_down_pair<K>(suf, d);
+        const Pair<K> o = shfl_down_pair<K>(suf, d, SW);
         if (lane + d < nl) suf = compose<K>(o, suf);
     }
     // instance totals: Z_0..K and What_1..K
     V4 Z[K + 1], Wh[K];
 #pragma unroll
-    for (int j = 0; j <= K; ++j) Z[j] = shfl4(suf.Z.v[j], 0);
+    for (int j = 0; j <= K; ++j) Z[j] = shfl4(suf.Z.v[j], 0, SW);
 #pragma unroll
-    for (int j = 0; j < K; ++j) Wh[j] = shfl4(suf.V.v[j], 0);
+    for (int j = 0; j < K; ++j) Wh[j] = shfl4(suf.V.v[j], 0, SW);
 
     const int Q = a.Q;
     double sq = 0.0, wq = 0.0;
@@ -576,7 +579,7 @@ __global__ void __launch_bounds__(256) wb_combine(WbArgs a) {
     for (int j = 0; j < K; ++j) Om[j] = vzero();
     for (int q = 0; q < Q; ++q) {
 #pragma unroll
-        for (int j = 0; j < K; ++j) Om[j] = vadd(Om[j], shfl4(part[j], q));
+        for (int j = 0; j < K; ++j) Om[j] = vadd(Om[j], shfl4(part[j], q, SW));
     }
     if (a.out && lane == 0) {  // outputs: components 1..3
         double* o = a.out + b * (long long)a.n_out * 3;
@@ -643,23 +646,23 @@ __global__ void __launch_bounds__(256) wb_combine(WbArgs a) {
         for (int j = 0; j < K; ++j) Wbar[j] = vzero();
         for (int q = 0; q < Q; ++q) {
 #pragma unroll
-            for (int j = 0; j <= K; ++j) Zbar[j] = vadd(Zbar[j], shfl4(Zp[j], q));
+            for (int j = 0; j <= K; ++j) Zbar[j] = vadd(Zbar[j], shfl4(Zp[j], q, SW));
 #pragma unroll
-            for (int j = 0; j < K; ++j) Wbar[j] = vadd(Wbar[j], shfl4(Wp[j], q));
+            for (int j = 0; j < K; ++j) Wbar[j] = vadd(Wbar[j], shfl4(Wp[j], q, SW));
         }
         // exclusive suffix (chunks after this lane) and exclusive prefix of V
-        Pair<K> sf = shfl_down_pair<K>(suf, 1);
+        Pair<K> sf = shfl_down_pair<K>(suf, 1, SW);
         if (lane + 1 >= nl) sf = pair_identity<K>();
         USer<K> pre = pr.V;
         for (int d = 1; d < nl; d <<= 1) {
             USer<K> o;
 #pragma unroll
-            for (int j = 0; j < K; ++j) o.v[j] = shfl_up4(pre.v[j], d);
+            for (int j = 0; j < K; ++j) o.v[j] = shfl_up4(pre.v[j], d, SW);
             if (lane >= d) pre = umul<K>(pre, o);
         }
         USer<K> W;
 #pragma unroll
-        for (int j = 0; j < K; ++j) W.v[j] = shfl_up4(pre.v[j], 1);
+        for (int j = 0; j < K; ++j) W.v[j] = shfl_up4(pre.v[j], 1, SW);
         if (lane == 0) {
 #pragma unroll
             for (int j = 0; j < K; ++j) W.v[j] = vzero();
@@ -1045,7 +1048,13 @@ hipError_t launch_wb_k(const WbArgs& a, bool vjp, bool summaries_given, hipStrea
     const long long threads = a.B * a.C;
     const int blk = 256;
     const long long grid = (threads + blk - 1) / blk;
-    const long long cgrid = (a.B + 3) / 4;  // one wavefront per instance
+    // combine: one segment of cw lanes per instance, cw the smallest power of two
+    // holding the chunks and the GL nodes (at most a wavefront)
+    WbArgs ca = a;
+    ca.cw = 1;
+    while (ca.cw < 64 && (ca.cw < a.C || ca.cw < a.Q)) ca.cw <<= 1;
+    const long long per_block = 4 * (64 / ca.cw);
+    const long long cgrid = (a.B + per_block - 1) / per_block;
     void* h;
     if (!summaries_given) {
         h = timer_begin(vjp ? "wb_fwd_pass1_vjp" : "wb_fwd_pass1", st);
@@ -1054,12 +1063,12 @@ hipError_t launch_wb_k(const WbArgs& a, bool vjp, bool summaries_given, hipStrea
     }
     if (!vjp) {
         h = timer_begin("wb_combine", st);
-        hipLaunchKernelGGL((wb_combine<K, false>), dim3(cgrid), dim3(256), 0, st, a);
+        hipLaunchKernelGGL((wb_combine<K, false>), dim3(cgrid), dim3(256), 0, st, ca);
         timer_end(h);
         return hipGetLastError();
     }
     h = timer_begin("wb_combine_vjp", st);
-    hipLaunchKernelGGL((wb_combine<K, true>), dim3(cgrid), dim3(256), 0, st, a);
+    hipLaunchKernelGGL((wb_combine<K, true>), dim3(cgrid), dim3(256), 0, st, ca);
     timer_end(h);
     h = timer_begin("wb_rev", st);
     hipLaunchKernelGGL((wb_rev<I, K>), dim3(grid), dim3(blk), 0, st, a);

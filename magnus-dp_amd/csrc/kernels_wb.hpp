@@ -24,6 +24,7 @@ struct WbArgs {
     int LC;  // samples per chunk (multiple of the integrator block)
     int K;   // n - 1
     int n_out, op, Q;
+    int cw;  // lanes per instance in the combine (set by launch_wb)
     double dt;
     double w[kArgMaxQ];
     double x[kArgMaxQ];

@@ -475,11 +475,26 @@ __global__ void __launch_bounds__(64) lm_fwd_scan_c2(LmArgs a) {
     const int e = (int)(je - job * a.m);
     const int n = a.n, j = a.level;
     double off = 0.0, acc = 0.0;
-    for (int c = 0; c < C; ++c) {
-        const double inc = a.cscan[(long long)c * JM + je];
-        a.cscan[(long long)c * JM + je] = off;
-        off += inc;
-        acc += a.cscan[((long long)C + c) * JM + je];
+    // the chunks in batches: a batch's loads go out together (the stores into
+    // the same array would otherwise order every load behind the previous store)
+    constexpr int U = 16;
+    for (int c0 = 0; c0 < C; c0 += U) {
+        double inc[U], part[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u;
+            inc[u] = c < C ? a.cscan[(long long)c * JM + je] : 0.0;
+            part[u] = c < C ? a.cscan[((long long)C + c) * JM + je] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c0 + u;
+            if (c < C) {
+                a.cscan[(long long)c * JM + je] = off;
+                off += inc[u];
+                acc += part[u];
+            }
+        }
     }
     if (!a.outputs) return;
     if (j == 1) {
@@ -685,13 +700,25 @@ __global__ void __launch_bounds__(64) lm_vjp_scan_b2(LmArgs a) {
     if (a.wbar) g4 += a.wbar[(job * a.n + (k - 2)) * (long long)m + e];
     double pending = 0.0;
     const double c14 = 14 * a.dt / 45;
-    for (int c = C - 1; c >= 0; --c) {
-        const double inc = a.cscan[(long long)c * JM + je];
-        const double pl = a.cscan[((long long)C + c) * JM + je];
-        a.cscan[(long long)c * JM + je] = g4;
-        a.cscan[((long long)C + c) * JM + je] = pending;
-        pending = pl + g4 * c14;
-        g4 = inc + g4;
+    constexpr int U = 16;  // chunks per batch of loads, as in lm_fwd_scan_c2
+    for (int c1 = C - 1; c1 >= 0; c1 -= U) {
+        double inc[U], pl[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c1 - u;
+            inc[u] = c >= 0 ? a.cscan[(long long)c * JM + je] : 0.0;
+            pl[u] = c >= 0 ? a.cscan[((long long)C + c) * JM + je] : 0.0;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            const int c = c1 - u;
+            if (c >= 0) {
+                a.cscan[(long long)c * JM + je] = g4;
+                a.cscan[((long long)C + c) * JM + je] = pending;
+                pending = pl[u] + g4 * c14;
+                g4 = inc[u] + g4;
+            }
+        }
     }
 }
 

@@ -132,6 +132,65 @@ def test_spacecurve_vjp_matches_oracle(integrator, op, n):
     assert rel_fro(got2, ref) <= TOL
 
 
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [16, 33, 64])
+@pytest.mark.parametrize("backend,dim", [("Manual", 3), ("Blas", 4)])
+def test_high_orders_match_oracle(op, n, backend, dim):
+    """Orders far above the benchmark configs, up to the last order of the
+    64-order GL table used by any n here (Q = 33 nodes at n = 64): forward with
+    the carry, the VJP from the carry and by recompute (for many/sum the
+    carry-free dense W-basis sweep with K = n - 1 levels).
+
+    op='one' returns Omega_n alone, which at these orders is the cancellation of
+    GL-node terms many orders of magnitude larger (|Omega_64| ~ 1e-22 to 1e-28
+    here against an expansion of norm ~2): the oracle's own Omega_n moves by
+    1.7e-8 (n = 33) and 8e-2 (n = 64) relative when only its GL-node reduction
+    order changes (one lane against four, both reference modes).  Its forward and
+    VJP are therefore held to the bar in absolute terms, at the scale of the
+    same problem's op='sum' result, whose relative error meets the bar."""
+    S = 17
+    data = dense_matrix_values(S, dim)
+    core = _core()
+    kw = dict(op=op, matrix_backend=backend, integrator="Boole")
+    out, carry = core.compute(n, data, 0.0, 1.0, record_vjp=True, **kw)
+    ref, ref_carry = O.forward(n, data, 0.0, 1.0, record_vjp=True, **kw)
+    cot = gradient_values(out.shape, (0.2, 1.1))
+    ref_g = O.vjp(n, data, cot, **kw)
+    if op == "one":
+        sum_kw = dict(kw, op="sum")
+        scale = np.linalg.norm(O.forward(n, data, 0.0, 1.0, **sum_kw))
+        scale_g = np.linalg.norm(O.vjp(n, data, cot, **sum_kw))
+        assert np.linalg.norm(out - ref) <= TOL * scale
+        assert np.linalg.norm(core.compute(n, data, 0.0, 1.0, **kw) - ref) <= TOL * scale
+        assert np.linalg.norm(core.compute_vjp(n, data, cot, vjp_data=carry, **kw) - ref_g) <= TOL * scale_g
+        assert np.linalg.norm(core.compute_vjp(n, data, cot, **kw) - ref_g) <= TOL * scale_g
+        assert rel_fro(carry, ref_carry) <= TOL
+        return
+    assert rel_fro(out, ref) <= TOL
+    assert rel_fro(carry, ref_carry) <= TOL
+    assert rel_fro(core.compute(n, data, 0.0, 1.0, **kw), ref) <= TOL
+    assert rel_fro(core.compute_vjp(n, data, cot, vjp_data=carry, **kw), ref_g) <= TOL
+    assert rel_fro(core.compute_vjp(n, data, cot, **kw), ref_g) <= TOL
+
+
+@pytest.mark.parametrize("op", OPS)
+@pytest.mark.parametrize("n", [20, 64])
+def test_spacecurve_high_orders_match_oracle(op, n):
+    """SpaceCurve problems above the single-pass kernels' n <= 8 (level-parallel
+    kernels): forward with the carry and both VJP forms."""
+    S = 33
+    data = spacecurve_values(np.linspace(0, 1, S))
+    core = _core()
+    out, carry = core.compute_sc(n, data, 0.0, 1.0, op=op, integrator="Simpson", record_vjp=True)
+    ref, ref_carry = O.sc_forward(n, data, 0.0, 1.0, op=op, integrator="Simpson", record_vjp=True)
+    assert rel_fro(out, ref) <= TOL
+    assert rel_fro(carry, ref_carry) <= TOL
+    cot = gradient_values(out.shape, (0.2, 1.1))
+    ref_g = O.sc_vjp(n, data, cot, op=op, integrator="Simpson")
+    assert rel_fro(core.compute_sc_vjp(n, data, cot, vjp_data=carry, op=op, integrator="Simpson"), ref_g) <= TOL
+    assert rel_fro(core.compute_sc_vjp(n, data, cot, op=op, integrator="Simpson"), ref_g) <= TOL
+
+
 # ------------------------------------------------------- reference tests ---
 # (tests/test_python_api.py of the reference, through the same public API)
 def test_matrix_ops_and_shapes():

@@ -367,6 +367,9 @@ uint64_t wb_summary_elems(const Plan& pl, uint64_t count) {
 // writes them there instead of scratch; a VJP given them skips the chunk pass.
 int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, double* out, double* dA,
            double* ckpt, hipStream_t stream) {
+    // the chunk passes pack element offsets of the input into 48 bits
+    if (count > 0 && pl.S > ((uint64_t)1 << 47) / 3 / count)
+        return fail(MAGNUS_EINVAL, "SpaceCurve batch exceeds 2^47 input elements");
     DeviceScratch scratch(stream);
     const bool vjp = dA != nullptr;
     magnus::WbArgs a{};

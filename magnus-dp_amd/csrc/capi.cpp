@@ -7,7 +7,9 @@
 // (composer/backends.hpp:104-180, composer/backends.cpp:26-61), and the
 // process-global, replaceable Gauss-Legendre table (util/gausslegendre.cpp).
 // Device work is enqueued on the caller's stream; temporary device buffers
-// are stream-ordered allocations freed on the same stream.
+// are slices of a per-(device, stream) scratch cache: grown with hipMalloc,
+// reused by the next call on the same stream (whose kernels run after this
+// call's), freed by magnus_release_scratch.
 #include "magnus_capi.h"
 
 #include <hip/hip_runtime.h>

@@ -249,6 +249,9 @@ template <bool UP>
 __device__ __forceinline__ void feed_issue(FeedSlabs& slab, const FeedSrc* src, const double* A, int lane, int g) {
     const int sub = lane / kFeedPieces, k = lane - sub * kFeedPieces;
     const int first = (2 * k) / 3, last = (2 * k + 1) / 3;  // samples of the group that piece k touches
+    // lgkmcnt(0): the wavefront's reads of the previous group from the slabs have
+    // completed before any load can land there
+    __builtin_amdgcn_s_waitcnt(0xC07F);
     FeedSrc sr[kFeedOps];  // all table reads first, then the loads
 #pragma unroll
     for (int m = 0; m < kFeedOps; ++m) {

@@ -176,7 +176,11 @@ __global__ void __launch_bounds__(256) lm_vjp_prod_rows(LmArgs a) {
 // ---------------------------------------------------------------------------
 typedef double dbl4 __attribute__((ext_vector_type(4)));
 
-// WPP wavefronts share one product; a 4-wavefront workgroup holds 4 / WPP products.
+// WPP wavefronts share one product; a 4-wavefront workgroup holds 4 / WPP
+// products.  The k range is staged in KS slices of DK = D / KS rows: LDS then
+// holds DK of the D rows of both operands, so at d = 32 and 64 a workgroup
+// needs 33-34 KB instead of 67, four workgroups fit a CU, and one workgroup's
+// loads overlap another's MFMAs.
 template <int D, int W>
 struct MfmaShape {
     static constexpr int TILES = (D / 16) * (D / 16);  // 16x16 tiles per product
@@ -184,7 +188,10 @@ struct MfmaShape {
     static constexpr int TPW = TILES / WPP;            // tiles per wavefront
     static constexpr int PPB = 4 / WPP;                // products per workgroup
     static constexpr int MM = D * D;
-    static_assert(TILES % WPP == 0 && 4 % WPP == 0, "tile split");
+    static constexpr int LD = D + 1;                                            // padded LDS row
+    static constexpr int KS = (PPB * 2 * D * LD * 8 > 40 * 1024) ? 2 : 1;       // k slices
+    static constexpr int DK = D / KS;                                           // k rows per slice
+    static_assert(TILES % WPP == 0 && 4 % WPP == 0 && DK % 4 == 0, "tile split");
 };
 
 // default split: one product per wavefront up to d = 32, four wavefronts at d = 64
@@ -195,19 +202,41 @@ constexpr int default_wpp() { return D == 64 ? 4 : 1; }
 // doubles: transposed stores (consecutive lanes step one row) and the tile
 // reads (consecutive lanes step one column, four rows apart) then fall on
 // different banks.
-// C_tile += Aop * Bop over k = 0..D-1 with Aop[i][k] = LA[k*LD + i0 + i], Bop[k][j] = LB[k*LD + j0 + j]
-template <int D>
-__device__ __forceinline__ dbl4 mfma_tile(const double* LA, const double* LB, int i0, int j0, int lane) {
-    constexpr int LD = D + 1;
-    dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+// acc += Aop * Bop over the DK rows of a slice, Aop[i][k] = LA[k*LD + i0 + i], Bop[k][j] = LB[k*LD + j0 + j]
+template <int DK, int LD>
+__device__ __forceinline__ void mfma_tile(const double* LA, const double* LB, int i0, int j0, int lane, dbl4& acc) {
     const int li = lane & 15, lk = lane >> 4;
 #pragma unroll
-    for (int kk = 0; kk < D / 4; ++kk) {
+    for (int kk = 0; kk < DK / 4; ++kk) {
         const double av = LA[(4 * kk + lk) * LD + i0 + li];
         const double bv = LB[(4 * kk + lk) * LD + j0 + li];
         acc = __builtin_amdgcn_mfma_f64_16x16x4f64(av, bv, acc, 0, 0, 0);
     }
-    return acc;
+}
+
+// rows k0..k0+DK-1 of a row-major D x D source: L[k - k0][j] = X[k][j] (+ T[k][j] * s)
+template <int D, int DK>
+__device__ __forceinline__ void stage_rows(double* L, const double* X, const double* T, double s, int k0) {
+    constexpr int LD = D + 1;
+    const double* Xs = X + k0 * D;
+    for (int e = threadIdx.x; e < DK * D; e += 256) {
+        const int r = e / D, c = e - r * D;
+        double v = Xs[e];
+        if (T) v += T[k0 * D + e] * s;
+        L[r * LD + c] = v;
+    }
+}
+
+// columns k0..k0+DK-1 of a row-major D x D source: L[k - k0][i] = X[i][k] (+ T[i][k] * s)
+template <int D, int DK>
+__device__ __forceinline__ void stage_cols(double* L, const double* X, const double* T, double s, int k0) {
+    constexpr int LD = D + 1;
+    for (int e = threadIdx.x; e < DK * D; e += 256) {
+        const int r = e / DK, c = e - r * DK;
+        double v = X[r * D + k0 + c];
+        if (T) v += T[r * D + k0 + c] * s;
+        L[c * LD + r] = v;
+    }
 }
 
 template <int D>
@@ -220,47 +249,63 @@ __device__ __forceinline__ void store_tile(double* out, const dbl4& acc, int i0,
     }
 }
 
-// Y_i = A_i (P_i + s T) for products p = job*S + i.  Staging: per product the
-// index arithmetic once (uniform over the workgroup), then its D*D entries.
+// Y_i = A_i (P_i + s T) for products p = job*S + i; per product the index
+// arithmetic once (uniform over the workgroup), then its slices.
 template <int D, int W>
 __global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
     using M = MfmaShape<D, W>;
-    constexpr int LD = D + 1, LM = D * LD;
-    __shared__ double LA[M::PPB][LM];  // A_i transposed: LA[k][i] = a[i][k]
-    __shared__ double LB[M::PPB][LM];  // P_i + s T: LB[k][j]
+    constexpr int LD = M::LD, LM = M::DK * LD;
+    __shared__ double LA[M::PPB][LM];  // slice of A_i transposed: LA[k][i] = a[i][k0 + k]
+    __shared__ double LB[M::PPB][LM];  // slice of P_i + s T: LB[k][j]
     const long long total = a.jobs * a.S;
     const long long p0 = (long long)blockIdx.x * M::PPB;
+    const double* srcA[M::PPB];
+    const double* srcP[M::PPB];
+    const double* srcT[M::PPB];
+    double shift[M::PPB];
 #pragma unroll
     for (int g = 0; g < M::PPB; ++g) {
         const long long p = p0 + g;
-        if (p >= total) break;
+        srcA[g] = srcP[g] = srcT[g] = nullptr;
+        shift[g] = 0.0;
+        if (p >= total) continue;
         const long long job = p / a.S, i = p - job * a.S;
         const long long b = job / a.Qj;
         const int q = (int)(job - b * a.Qj);
-        const double shift = a.x[q] - 1;
+        shift[g] = a.x[q] - 1;
         const double* P = lm_slot(a, job, b, q, a.level - 1);
-        const double* Pi = P + i * M::MM;
-        const double* T = P + (a.S - 1) * M::MM;
-        const double* Ai = a.in + (b * a.S + i) * (long long)M::MM;
-        for (int e = threadIdx.x; e < M::MM; e += 256) {
-            const int r = e / D, c = e - r * D;
-            double v = Pi[e];
-            v += T[e] * shift;
-            LB[g][r * LD + c] = v;
-            LA[g][c * LD + r] = Ai[e];
-        }
+        srcP[g] = P + i * M::MM;
+        srcT[g] = P + (a.S - 1) * M::MM;
+        srcA[g] = a.in + (b * a.S + i) * (long long)M::MM;
     }
-    __syncthreads();
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int g = wave / M::WPP, wp = wave - g * M::WPP;
+    dbl4 acc[M::TPW];
+#pragma unroll
+    for (int tt = 0; tt < M::TPW; ++tt) acc[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < M::KS; ++ks) {
+        if (ks) __syncthreads();  // the previous slice's tile reads are done
+#pragma unroll
+        for (int h = 0; h < M::PPB; ++h) {
+            if (!srcA[h]) continue;
+            stage_cols<D, M::DK>(LA[h], srcA[h], nullptr, 0.0, ks * M::DK);
+            stage_rows<D, M::DK>(LB[h], srcP[h], srcT[h], shift[h], ks * M::DK);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < M::TPW; ++tt) {
+            const int tile = wp * M::TPW + tt;
+            mfma_tile<M::DK, LD>(LA[g], LB[g], 16 * (tile / (D / 16)), 16 * (tile % (D / 16)), lane, acc[tt]);
+        }
+    }
     const long long p = p0 + g;
     if (p >= total) return;
     double* y = a.Y + p * M::MM;
 #pragma unroll
     for (int tt = 0; tt < M::TPW; ++tt) {
         const int tile = wp * M::TPW + tt;
-        const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
-        store_tile<D>(y, mfma_tile<D>(LA[g], LB[g], i0, j0, lane), i0, j0, lane, false);
+        store_tile<D>(y, acc[tt], 16 * (tile / (D / 16)), 16 * (tile % (D / 16)), lane, false);
     }
 }
 
@@ -269,72 +314,91 @@ __global__ void __launch_bounds__(256) lm_fwd_prod_mfma(LmArgs a) {
 template <int D, int W>
 __global__ void __launch_bounds__(256) lm_vjp_prod_mfma(LmArgs a) {
     using M = MfmaShape<D, W>;
-    constexpr int LD = D + 1, LM = D * LD;
+    constexpr int LD = M::LD, LM = M::DK * LD;
     __shared__ double LA[M::PPB][LM];
     __shared__ double LB[M::PPB][LM];
     const long long per = a.S * M::MM;
     const long long total = a.jobs * a.S;
     const long long p0 = (long long)blockIdx.x * M::PPB;
-    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    const int gw = wave / M::WPP, wp = wave - gw * M::WPP;
-    const long long pw = p0 + gw;
-    // phase 1: LA[k][i] = g[i][k], LB[k][j] = b[j][k]
+    const double* srcA[M::PPB];
+    const double* srcP[M::PPB];
+    const double* srcT[M::PPB];
+    const double* srcG[M::PPB];
+    double shift[M::PPB];
 #pragma unroll
     for (int g = 0; g < M::PPB; ++g) {
         const long long p = p0 + g;
-        if (p >= total) break;
+        srcA[g] = srcP[g] = srcT[g] = srcG[g] = nullptr;
+        shift[g] = 0.0;
+        if (p >= total) continue;
         const long long job = p / a.S, i = p - job * a.S;
         const long long b = job / a.Q;
         const int q = (int)(job - b * a.Q);
-        const double shift = a.x[q] - 1.0;
+        shift[g] = a.x[q] - 1.0;
         const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
-        const double* Pi = P + i * M::MM;
-        const double* T = P + (a.S - 1) * M::MM;
-        const double* Gi = a.gbuf + (job * 2 + a.ga) * per + i * M::MM;
-        for (int e = threadIdx.x; e < M::MM; e += 256) {
-            const int r = e / D, c = e - r * D;
-            double v = Pi[e];
-            v += T[e] * shift;
-            LB[g][c * LD + r] = v;
-            LA[g][c * LD + r] = Gi[e];
+        srcP[g] = P + i * M::MM;
+        srcT[g] = P + (a.S - 1) * M::MM;
+        srcG[g] = a.gbuf + (job * 2 + a.ga) * per + i * M::MM;
+        srcA[g] = a.in + (b * a.S + i) * (long long)M::MM;
+    }
+    const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    const int gw = wave / M::WPP, wp = wave - gw * M::WPP;
+    const long long pw = p0 + gw;
+    dbl4 acc[M::TPW];
+    // phase 1: LA[k][i] = g[i][k], LB[k][j] = b[j][k]
+#pragma unroll
+    for (int tt = 0; tt < M::TPW; ++tt) acc[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < M::KS; ++ks) {
+        if (ks) __syncthreads();
+#pragma unroll
+        for (int h = 0; h < M::PPB; ++h) {
+            if (!srcA[h]) continue;
+            stage_cols<D, M::DK>(LA[h], srcG[h], nullptr, 0.0, ks * M::DK);
+            stage_cols<D, M::DK>(LB[h], srcP[h], srcT[h], shift[h], ks * M::DK);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < M::TPW; ++tt) {
+            const int tile = wp * M::TPW + tt;
+            mfma_tile<M::DK, LD>(LA[gw], LB[gw], 16 * (tile / (D / 16)), 16 * (tile % (D / 16)), lane, acc[tt]);
         }
     }
-    __syncthreads();
     if (pw < total) {
         const long long job = pw / a.S, i = pw - job * a.S;
         double* dA = a.dAjob + job * per + i * M::MM;
 #pragma unroll
         for (int tt = 0; tt < M::TPW; ++tt) {
             const int tile = wp * M::TPW + tt;
-            const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
-            store_tile<D>(dA, mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane), i0, j0, lane, true);
+            store_tile<D>(dA, acc[tt], 16 * (tile / (D / 16)), 16 * (tile % (D / 16)), lane, true);
         }
     }
-    __syncthreads();
     // phase 2: LA[k][i] = a[k][i], LB[k][j] = g[k][j]
 #pragma unroll
-    for (int g = 0; g < M::PPB; ++g) {
-        const long long p = p0 + g;
-        if (p >= total) break;
-        const long long job = p / a.S, i = p - job * a.S;
-        const long long b = job / a.Q;
-        const double* Ai = a.in + (b * a.S + i) * (long long)M::MM;
-        const double* Gi = a.gbuf + (job * 2 + a.ga) * per + i * M::MM;
-        for (int e = threadIdx.x; e < M::MM; e += 256) {
-            const int r = e / D, c = e - r * D;
-            LA[g][r * LD + c] = Ai[e];
-            LB[g][r * LD + c] = Gi[e];
+    for (int tt = 0; tt < M::TPW; ++tt) acc[tt] = dbl4{0.0, 0.0, 0.0, 0.0};
+#pragma unroll
+    for (int ks = 0; ks < M::KS; ++ks) {
+        __syncthreads();  // the previous phase's or slice's tile reads are done
+#pragma unroll
+        for (int h = 0; h < M::PPB; ++h) {
+            if (!srcA[h]) continue;
+            stage_rows<D, M::DK>(LA[h], srcA[h], nullptr, 0.0, ks * M::DK);
+            stage_rows<D, M::DK>(LB[h], srcG[h], nullptr, 0.0, ks * M::DK);
+        }
+        __syncthreads();
+#pragma unroll
+        for (int tt = 0; tt < M::TPW; ++tt) {
+            const int tile = wp * M::TPW + tt;
+            mfma_tile<M::DK, LD>(LA[gw], LB[gw], 16 * (tile / (D / 16)), 16 * (tile % (D / 16)), lane, acc[tt]);
         }
     }
-    __syncthreads();
     if (pw < total) {
         const long long job = pw / a.S, i = pw - job * a.S;
         double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * M::MM;
 #pragma unroll
         for (int tt = 0; tt < M::TPW; ++tt) {
             const int tile = wp * M::TPW + tt;
-            const int i0 = 16 * (tile / (D / 16)), j0 = 16 * (tile % (D / 16));
-            store_tile<D>(Gb, mfma_tile<D>(LA[gw], LB[gw], i0, j0, lane), i0, j0, lane, false);
+            store_tile<D>(Gb, acc[tt], 16 * (tile / (D / 16)), 16 * (tile % (D / 16)), lane, false);
         }
     }
 }

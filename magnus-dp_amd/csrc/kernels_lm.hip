@@ -178,9 +178,10 @@ typedef double dbl4 __attribute__((ext_vector_type(4)));
 
 // WPP wavefronts share one product; a 4-wavefront workgroup holds 4 / WPP
 // products.  The k range is staged in KS slices of DK = D / KS rows: LDS then
-// holds DK of the D rows of both operands, so at d = 32 and 64 a workgroup
-// needs 33-34 KB instead of 67, four workgroups fit a CU, and one workgroup's
-// loads overlap another's MFMAs.
+// holds DK of the D rows of both operands, so more workgroups fit a CU and one
+// workgroup's loads overlap another's MFMAs.  Measured: at d = 32, 17 KB per
+// workgroup (KS = products per workgroup) beat 34 KB and 67 KB; at d = 64, two
+// slices (33 KB) beat one (67 KB) and four, which slowed the VJP.
 template <int D, int W>
 struct MfmaShape {
     static constexpr int TILES = (D / 16) * (D / 16);  // 16x16 tiles per product
@@ -189,7 +190,7 @@ struct MfmaShape {
     static constexpr int PPB = 4 / WPP;                // products per workgroup
     static constexpr int MM = D * D;
     static constexpr int LD = D + 1;                                            // padded LDS row
-    static constexpr int KS = (PPB * 2 * D * LD * 8 > 40 * 1024) ? 2 : 1;       // k slices
+    static constexpr int KS = D == 64 ? 2 : (D == 32 ? PPB : 1);                // k slices
     static constexpr int DK = D / KS;                                           // k rows per slice
     static_assert(TILES % WPP == 0 && 4 % WPP == 0 && DK % 4 == 0, "tile split");
 };

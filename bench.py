@@ -1,19 +1,26 @@
 """Benchmark of the MI355X Magnus path.  Prints ONE JSON line.
 
-Default workload (one GPU): the SpaceCurve batch -- 4096 independent 3-vector
-trajectories f(t) = sum_{m=1..4} a_m sin(2 pi m t + phi_m), a_m ~ N(0,1)/m,
-phi_m ~ U[0, 2 pi), S = 16385 samples on [0, 1], Simpson, compute_sc with
-n = 6, op = 'many', then compute_sc_vjp with a N(0,1) (6, 3) cotangent per
-trajectory.  No reference-style carry is stored: the forward keeps its compact
-per-chunk checkpoint (11 4-vectors per 512-sample chunk, 46 MB for the batch)
-and the VJP starts from it (--sc-residual recompute: the VJP re-derives its
-forward quantities, the reference's VJP::none).  One step = forward + VJP of the
-whole batch in batched launches; inputs are resident in HBM before timing starts.
+Default workload (one GPU): the learned resolution controller, the metric's
+"Magnus fwd+VJP solves/s at rel err <= 1e-8, learned vs plain".  200k training
+and 10k held-out Fourier-form instances (d = 8, 3-8 modes, ||A||_2 T ~ U[0.1, 4]),
+labelled with the minimal (samples in {65..16385}, n in {2..8}) that reaches
+relative error <= 1e-8 against a samples = 32769, n = 8 run; the MLP is trained
+on the device.  One step = the learned forward + VJP of the 10k held-out
+instances (features, MLP prediction, the doubling judge of every accepted
+result, the VJP at the accepted resolution); the plain fixed resolution
+(16385, 8) and the judge without the network are measured with the same
+protocol.  Labelling and training happen before the timed steps.
+
+Other workloads (--workload):
+  spacecurve-batch  4096 trajectories f(t) = sum_{m=1..4} a_m sin(2 pi m t + phi_m),
+                    S = 16385, Simpson, compute_sc n = 6 'many' + compute_sc_vjp; the
+                    forward keeps its compact per-chunk checkpoint and the VJP starts
+                    from it (--sc-residual recompute: the reference's VJP::none)
+  dense-d32         256 x d = 32, S = 4097, Boole, n = 4, sum, record carry + VJP
+  plain-d4          one d = 4 instance, S = 1025, Boole, n = 4, sum, forward only
+  uneven            2048 instances of d 2..64, S 257..8193, n 2..6 in one graph launch
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
-
-Other workloads: dense-d32 (256 x d=32, S=4097, Boole, n=4, sum, record + VJP),
-plain-d4 (one d=4 instance, S=1025, Boole, n=4, sum, forward only).
 """
 from __future__ import annotations
 
@@ -437,6 +444,22 @@ def _cpu_baseline_controller(coeffs, cot, budget_s):
                       f"instance per core, in {el:.1f} s"}
 
 
+def _roctx():
+    """ROCm's marker library, the one rocprofv3 --marker-trace records (ranges are
+    no-ops without a profiler), or None where it is absent."""
+    import ctypes
+
+    lib_dir = os.path.join(os.environ.get("ROCM_PATH", "/opt/rocm"), "lib")
+    for path in ("librocprofiler-sdk-roctx.so", os.path.join(lib_dir, "librocprofiler-sdk-roctx.so")):
+        try:
+            lib = ctypes.CDLL(path)
+        except OSError:
+            continue
+        lib.roctxRangePushA.argtypes = [ctypes.c_char_p]
+        return lib
+    return None
+
+
 def _bench_controller(args, torch, dev):
     import numpy as np
 
@@ -500,31 +523,68 @@ def _bench_controller(args, torch, dev):
     from magnus import capi
 
     kernel_ms = {}
+    kernel_counts = {}
+    roctx = _roctx()
 
-    def best(fn, reps=3):  # after one warm-up run, the fastest of `reps` timed runs
-        fn()
-        runs = [C.timed(fn) for _ in range(reps)]
-        # one more run with the per-kernel timers on (diagnostics only)
+    def measure(fn):
+        """W untimed warm-up steps, then exactly K timed steps between device
+        synchronisations, with the library's per-launch HIP events on; returns the
+        last step's result and the seconds per step."""
+        for _ in range(args.warmup):
+            fn()
+        torch.cuda.synchronize()
         capi.timing_reset()
         capi.timing_enable(True)
-        fn()
+        if roctx:
+            roctx.roctxRangePushA(f"{fn.__name__} timed steps".encode())
         torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.steps):
+            res = fn()
+        torch.cuda.synchronize()
+        per_step = (time.perf_counter() - t0) / args.steps
+        if roctx:
+            roctx.roctxRangePop()
         capi.timing_enable(False)
-        tot = {}
+        tot, cnts = {}, {}
         for name in KERNELS + ("wbd_fold", "wbd_fold_vjp"):
             cnt, ms = capi.timing_read(name)
             if cnt:
                 tot[name] = (cnt, round(ms, 1))
+                cnts[name] = (cnt, ms) + capi.timing_counts(name)
         kernel_ms[fn.__name__] = tot
+        kernel_counts[fn.__name__] = cnts
         log(f"{fn.__name__} kernels {tot}")
-        return min(runs, key=lambda r: r[1])
+        return res, per_step
 
-    (out_p, _, w_p), t_p = best(plain)
+    (out_p, _, w_p), t_p = measure(plain)
     log(f"plain {t_p:.2f} s")
-    (res_l, out_l, st_l, w_l), t_l = best(learned)
+    (res_l, out_l, st_l, w_l), t_l = measure(learned)
     log(f"learned {t_l:.2f} s, rounds {np.bincount(st_l['rounds'])}")
-    (res_0, out_0, st_0, w_0), t_0 = best(nonet)
+    (res_0, out_0, st_0, w_0), t_0 = measure(nonet)
     log(f"without network {t_0:.2f} s, rounds mean {st_0['rounds'].mean():.2f}")
+    # roofline: the family with the most device time in the learned steps, at the
+    # algorithmic bytes / flops its launch sites state over its summed durations
+    # (HIP events on each launch's stream, over the timed steps).  The learned step
+    # runs its groups on four streams, so launches overlap and the rate is a lower bound.
+    roofline = None
+    counted = {k: v for k, v in kernel_counts["learned"].items() if v[2] > 0 or v[3] > 0}
+    if counted:
+        dom = max(counted, key=lambda k: counted[k][1])
+        n_l, ms_l, by, fl = counted[dom]
+        dur_s = ms_l / 1e3
+        if by / (MI355X_HBM_GBS * 1e9) >= fl / (MI355X_FP64_TFLOPS * 1e12):
+            ach = by / dur_s / 1e9
+            roofline = {"bound": "hbm", "achieved": round(ach, 2), "peak": MI355X_HBM_GBS, "unit": "GB/s",
+                        "frac": round(ach / MI355X_HBM_GBS, 4)}
+        else:
+            ach = fl / dur_s / 1e12
+            roofline = {"bound": "fp64", "achieved": round(ach, 3), "peak": MI355X_FP64_TFLOPS, "unit": "TFLOP/s",
+                        "frac": round(ach / MI355X_FP64_TFLOPS, 4)}
+        roofline.update({"traffic": None, "kernel": dom, "launches": n_l, "kernel_avg_ms": round(ms_l / n_l, 4),
+                         "algorithmic_bytes_per_launch": round(by / n_l), "algorithmic_flops_per_launch": round(fl / n_l),
+                         "note": "launches differ in size (one per resolution group and level): achieved = summed "
+                                 "algorithmic work over summed durations; four streams overlap, so a lower bound"})
     hit = lambda o: float((C.rel_err(o, refs_ev) <= tol).float().mean())  # noqa: E731
     cpu = None if args.no_cpu else _cpu_baseline_controller(c_ev, cot, args.cpu_budget)
     value = n_ev / t_l
@@ -533,8 +593,8 @@ def _bench_controller(args, torch, dev):
         "value": round(value, 3),
         "unit": "solves/s",
         "n_gpus": 1,
-        "steps": 3,
-        "warmup": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
         "ms_per_step": round(t_l * 1e3, 3),
         "higher_is_better": True,
         "scaling": "weak",
@@ -557,7 +617,7 @@ def _bench_controller(args, torch, dev):
         "stages": stages,
         "label_s": round(label_s, 2),
         "train_s": round(train_s, 2),
-        "roofline": None,
+        "roofline": roofline,
         "cpu_baseline": cpu,
     }
     print(json.dumps(line))
@@ -566,9 +626,9 @@ def _bench_controller(args, torch, dev):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="spacecurve-batch",
+    ap.add_argument("--steps", type=int, default=None, help="timed steps (default 3 for controller, else 20)")
+    ap.add_argument("--warmup", type=int, default=None, help="warm-up steps (default 1 for controller, else 5)")
+    ap.add_argument("--workload", default="controller",
                     choices=["spacecurve-batch", "dense-d32", "plain-d4", "uneven", "controller"])
     ap.add_argument("--ctrl-train", type=int, default=200000)
     ap.add_argument("--ctrl-eval", type=int, default=10000)
@@ -585,6 +645,10 @@ def main():
                     help="only run a plain device copy (y = 1.0 * x) of this many MiB, three times: the "
                          "reference reading for scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
+    if args.steps is None:
+        args.steps = 3 if args.workload == "controller" else 20
+    if args.warmup is None:
+        args.warmup = 1 if args.workload == "controller" else 5
     if args.gpus != 1:
         print("bench.py measures one GPU only: this build has no multi-GPU path (run with --gpus 1)",
               file=sys.stderr)

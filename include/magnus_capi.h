@@ -270,6 +270,11 @@ int magnus_mlp_train_step(magnus_mlp* net, uint64_t batch, const float* x, const
  *      launches and their total duration in milliseconds since the last reset. ---- */
 int magnus_timing_enable(int on);
 int magnus_timing_read(const char* name, uint64_t* launches, double* total_ms);
+/* the algorithmic HBM bytes and flops of the same launches (FMA = 2 flops), as
+ * their launch sites state them (0 where a site states none); with the
+ * duration from magnus_timing_read this gives a kernel family's achieved rate
+ * when its launches differ in size.  No counterpart in the reference. */
+int magnus_timing_counts(const char* name, double* bytes, double* flops);
 void magnus_timing_reset(void);
 
 /* Library scratch comes from a per-(device, stream) cache of device blocks that

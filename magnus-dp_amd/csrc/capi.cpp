@@ -252,10 +252,12 @@ void fill_gl(const Plan& pl, double* w, double* x) {
 struct TimedLaunch {
     std::string name;
     hipEvent_t start, stop;
+    double bytes, flops;
 };
 struct TimingTotals {
     uint64_t launches = 0;
     double ms = 0.0;
+    double bytes = 0.0, flops = 0.0;
 };
 std::mutex g_timing_mu;
 bool g_timing_on = false;
@@ -267,7 +269,8 @@ struct KernelTimer {
     hipEvent_t start = nullptr, stop = nullptr;
     hipStream_t stream;
     const char* name;
-    KernelTimer(const char* n, hipStream_t s) : stream(s), name(n) {
+    double bytes, flops;
+    KernelTimer(const char* n, hipStream_t s, double b, double f) : stream(s), name(n), bytes(b), flops(f) {
         if (g_timing_suspended) return;
         std::lock_guard<std::mutex> lock(g_timing_mu);
         if (!g_timing_on) return;
@@ -281,7 +284,7 @@ struct KernelTimer {
         if (!start) return;
         (void)hipEventRecord(stop, stream);
         std::lock_guard<std::mutex> lock(g_timing_mu);
-        g_pending.push_back({name, start, stop});
+        g_pending.push_back({name, start, stop, bytes, flops});
         start = stop = nullptr;
     }
 };
@@ -297,8 +300,8 @@ bool timing_suspend(bool on) {
 using namespace magnus_host;
 
 namespace magnus {
-void* timer_begin(const char* name, hipStream_t s) {
-    auto* t = new KernelTimer(name, s);
+void* timer_begin(const char* name, hipStream_t s, double bytes, double flops) {
+    auto* t = new KernelTimer(name, s, bytes, flops);
     if (!t->start) {
         delete t;
         return nullptr;
@@ -328,6 +331,8 @@ void timing_drain_locked() {
             }
             it->second.launches += 1;
             it->second.ms += ms;
+            it->second.bytes += t.bytes;
+            it->second.flops += t.flops;
         }
         (void)hipEventDestroy(t.start);
         (void)hipEventDestroy(t.stop);
@@ -1072,6 +1077,20 @@ int magnus_timing_read(const char* name, uint64_t* launches, double* total_ms) {
         }
     if (launches) *launches = n;
     if (total_ms) *total_ms = ms;
+    return MAGNUS_OK;
+}
+
+int magnus_timing_counts(const char* name, double* bytes, double* flops) {
+    std::lock_guard<std::mutex> lock(g_timing_mu);
+    timing_drain_locked();
+    double b = 0.0, f = 0.0;
+    for (auto& t : g_totals)
+        if (name && t.first == name) {
+            b = t.second.bytes;
+            f = t.second.flops;
+        }
+    if (bytes) *bytes = b;
+    if (flops) *flops = f;
     return MAGNUS_OK;
 }
 

@@ -59,8 +59,10 @@ struct LmArgs {
 
 // Per-kernel timing hooks (implemented by the C-ABI layer).  When timing is
 // enabled, timer_begin records a start event on `s` and returns a handle that
-// timer_end closes; both are no-ops (nullptr) otherwise.
-void* timer_begin(const char* name, hipStream_t s);
+// timer_end closes; both are no-ops (nullptr) otherwise.  A launch site may
+// state the algorithmic HBM bytes and flops of what it launches (FMA = 2
+// flops); the totals are reported beside the durations.
+void* timer_begin(const char* name, hipStream_t s, double bytes = 0.0, double flops = 0.0);
 void timer_end(void* handle);
 
 hipError_t launch_fwd_reduce(const FwdReduceArgs& a, hipStream_t s);

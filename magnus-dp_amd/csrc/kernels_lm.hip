@@ -909,10 +909,32 @@ inline void launch_mfma(void (*k)(LmArgs), const LmArgs& a, hipStream_t st) {
     hipLaunchKernelGGL(k, dim3((prods + ppb - 1) / ppb), dim3(256), 0, st, a);
 }
 
+// Algorithmic work of one level's launches, as the timers record it (8 bytes
+// per value, FMA = 2 flops): a product launch reads the samples once and the
+// previous level's values and writes the products (the VJP also reads the
+// carry and the adjoint and writes dA_job and the next adjoint: 5 values per
+// entry), P = 2 d^3 flops per product (20 for the su(2) product); a scan
+// launch reads the products and writes the prefix level, 8 flops per entry.
+struct LevelCounts {
+    double prod_bytes, prod_flops, scan_bytes, scan_flops;
+};
+inline LevelCounts level_counts(const LmArgs& a, bool vjp) {
+    const double jobs = (double)a.jobs, S = (double)a.S, m = a.m;
+    const double B = jobs / (vjp ? a.Q : a.Qj);
+    const double P = a.sc ? 20.0 : 2.0 * a.d * a.d * (double)a.d;
+    LevelCounts c;
+    c.prod_flops = (vjp ? 2.0 : 1.0) * jobs * S * P;
+    c.prod_bytes = 8.0 * (B * S * a.m_in + (vjp ? 5.0 : 2.0) * jobs * S * m);
+    c.scan_flops = 8.0 * jobs * S * m;
+    c.scan_bytes = 16.0 * jobs * S * m;
+    return c;
+}
+
 template <int I>
 hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
+    const LevelCounts cnt = level_counts(a, false);
     if (a.level >= 2) {
-        void* h = timer_begin("lm_fwd_prod", st);
+        void* h = timer_begin("lm_fwd_prod", st, cnt.prod_bytes, cnt.prod_flops);
         switch (prod_path(a)) {
             case 16: launch_mfma<16, 1>(lm_fwd_prod_mfma<16, 1>, a, st); break;
             case 32: launch_mfma<32, 1>(lm_fwd_prod_mfma<32, 1>, a, st); break;
@@ -929,7 +951,7 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
         }
         timer_end(h);
     }
-    void* h = timer_begin("lm_fwd_scan", st);
+    void* h = timer_begin("lm_fwd_scan", st, cnt.scan_bytes, cnt.scan_flops);
     bool chunked = false;
     if constexpr (I != TRAPEZOID) {
         chunked = chunked_scan<I>(a);
@@ -952,7 +974,8 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
 
 template <int I>
 hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
-    void* h = timer_begin("lm_vjp_prod", st);
+    const LevelCounts cnt = level_counts(a, true);
+    void* h = timer_begin("lm_vjp_prod", st, cnt.prod_bytes, cnt.prod_flops);
     switch (prod_path(a)) {
         case 16: launch_mfma<16, 1>(lm_vjp_prod_mfma<16, 1>, a, st); break;
         case 32: launch_mfma<32, 1>(lm_vjp_prod_mfma<32, 1>, a, st); break;
@@ -968,7 +991,7 @@ hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
         default: hipLaunchKernelGGL(lm_vjp_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
     }
     timer_end(h);
-    h = timer_begin("lm_vjp_scan", st);
+    h = timer_begin("lm_vjp_scan", st, cnt.scan_bytes, cnt.scan_flops);
     if (I == BOOLE && chunked_scan<I>(a) && a.Q == 1 && a.x[0] == 1.0) {
         const long long JM = a.jobs * a.m, work = JM * lm_nchunks(a);
         hipLaunchKernelGGL(lm_vjp_scan_b1, dim3((work + 255) / 256), dim3(256), 0, st, a);

@@ -105,6 +105,8 @@ def lib() -> ctypes.CDLL:
         L.magnus_release_scratch.argtypes = []
         L.magnus_timing_enable.argtypes = [ctypes.c_int]
         L.magnus_timing_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)]
+        L.magnus_timing_counts.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
+                                           ctypes.POINTER(ctypes.c_double)]
         L.magnus_timing_reset.restype = None
         L.magnus_last_error.restype = ctypes.c_char_p
         L.magnus_last_kernel.restype = ctypes.c_char_p
@@ -160,6 +162,15 @@ def timing_read(name: str) -> tuple[int, float]:
     ms = ctypes.c_double()
     check(lib().magnus_timing_read(name.encode(), ctypes.byref(n), ctypes.byref(ms)))
     return n.value, ms.value
+
+
+def timing_counts(name: str) -> tuple[float, float]:
+    """(algorithmic HBM bytes, flops) of the timed launches of kernel family `name`
+    since the last reset, as their launch sites state them."""
+    b = ctypes.c_double()
+    f = ctypes.c_double()
+    check(lib().magnus_timing_counts(name.encode(), ctypes.byref(b), ctypes.byref(f)))
+    return b.value, f.value
 
 
 def last_kernel() -> str:

@@ -535,6 +535,41 @@ def test_propagator_of_reference_bench_problem_converges_to_ode_solution():
     assert errs[0] > errs[1] > errs[2] and errs[2] < 1e-4
 
 
+def test_timing_counts_state_the_level_kernels_algorithmic_work():
+    """The per-launch algorithmic work the level kernels report to the timers
+    (bench.py's roofline for launches of different sizes): a d = 4 forward that
+    records the carry (n = 3, Q = 3 GL nodes, so 3 jobs) runs two product
+    launches (levels 2, 3) and three scan launches; per launch a product reads
+    the samples once and the previous level and writes the products, a scan
+    reads the products and writes the prefix level."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    S, d, n = 33, 4, 3
+    m, jobs = d * d, 3
+    a = torch.from_numpy(dense_matrix_values(S, d)).cuda()[None]
+    capi.timing_reset()
+    capi.timing_enable(True)
+    try:
+        torch_ops.forward(n, a, 0.0, 1.0, op="sum", integrator="Boole", record_vjp=True)
+        torch.cuda.synchronize()
+    finally:
+        capi.timing_enable(False)
+    assert capi.last_kernel() == "lm"
+    launches, _ = capi.timing_read("lm_fwd_prod")
+    by, fl = capi.timing_counts("lm_fwd_prod")
+    assert launches == 2
+    assert by == 2 * 8 * (S * m + 2 * jobs * S * m)
+    assert fl == 2 * jobs * S * 2 * d ** 3
+    launches, _ = capi.timing_read("lm_fwd_scan")
+    by, fl = capi.timing_counts("lm_fwd_scan")
+    assert launches == 3
+    assert by == 3 * 16 * jobs * S * m and fl == 3 * 8 * jobs * S * m
+    capi.timing_reset()
+    assert capi.timing_counts("lm_fwd_prod") == (0.0, 0.0)
+
+
 def test_batched_matrix_api_matches_per_instance():
     import torch
 

@@ -20,6 +20,7 @@
 #include "magnus_oracle.h"
 
 #include <math.h>
+#include <pthread.h>
 #include <stdio.h>
 #include <stdlib.h>
 #include <string.h>
@@ -42,6 +43,10 @@ const char* oracle_last_error(void) { return g_err; }
 /* Here: Newton iteration on P_N from Chebyshev-like starting points.          */
 /* ------------------------------------------------------------------------ */
 #define GL_CAP 128
+/* The table is process-wide and built lazily on first use; callers may drive */
+/* the oracle from several threads at once (one instance per thread), so     */
+/* every read and every swap of the table holds g_gl_mu.                      */
+static pthread_mutex_t g_gl_mu = PTHREAD_MUTEX_INITIALIZER;
 static double* g_gl_w = NULL;
 static double* g_gl_x = NULL;
 static int g_gl_max = 0;
@@ -96,6 +101,19 @@ static void legendre_nodes(int N, double* x_out, double* w_out) {
     if (N % 2 == 1) x_out[N / 2] = 0.0;
 }
 
+/* Installs a freshly built table (w, x own their memory) and frees the old one. */
+static void gl_swap(int max_order, double* w, double* x) {
+    pthread_mutex_lock(&g_gl_mu);
+    double* old_w = g_gl_w;
+    double* old_x = g_gl_x;
+    g_gl_w = w;
+    g_gl_x = x;
+    g_gl_max = max_order;
+    pthread_mutex_unlock(&g_gl_mu);
+    free(old_w);
+    free(old_x);
+}
+
 int oracle_gl_init(int max_order) {
     if (max_order < 1 || max_order > GL_CAP) return fail("max_order out of range");
     size_t total = gl_offset(max_order + 1);
@@ -111,11 +129,7 @@ int oracle_gl_init(int max_order) {
             wo[i] = wo[i] / 2.0;
         }
     }
-    free(g_gl_w);
-    free(g_gl_x);
-    g_gl_w = w;
-    g_gl_x = x;
-    g_gl_max = max_order;
+    gl_swap(max_order, w, x);
     return 0;
 }
 
@@ -128,24 +142,36 @@ int oracle_gl_replace(int max_order, const double* weights, const double* nodes)
     if (!w || !x) { free(w); free(x); return fail("out of memory"); }
     memcpy(w, weights, total * sizeof(double));
     memcpy(x, nodes, total * sizeof(double));
-    free(g_gl_w);
-    free(g_gl_x);
-    g_gl_w = w;
-    g_gl_x = x;
-    g_gl_max = max_order;
+    gl_swap(max_order, w, x);
     return 0;
 }
 
-int oracle_gl_max_order(void) {
-    if (!g_gl_w) oracle_gl_init(64);
-    return g_gl_max;
+/* Largest order of the table, building the default (orders 1..64) if none is */
+/* installed yet; 0 if that build fails. */
+static int gl_ready(void) {
+    pthread_mutex_lock(&g_gl_mu);
+    int have = g_gl_w != NULL, mx = g_gl_max;
+    pthread_mutex_unlock(&g_gl_mu);
+    if (have) return mx;
+    if (oracle_gl_init(64)) return 0;
+    pthread_mutex_lock(&g_gl_mu);
+    mx = g_gl_max;
+    pthread_mutex_unlock(&g_gl_mu);
+    return mx;
 }
 
+int oracle_gl_max_order(void) { return gl_ready(); }
+
 int oracle_gl_get(int order, double* weights, double* nodes) {
-    if (!g_gl_w && oracle_gl_init(64)) return 1;
-    if (order < 1 || order > g_gl_max) return fail("Gauss-Legendre order is outside the available table");
+    if (!gl_ready()) return 1;
+    pthread_mutex_lock(&g_gl_mu);
+    if (order < 1 || order > g_gl_max) {
+        pthread_mutex_unlock(&g_gl_mu);
+        return fail("Gauss-Legendre order is outside the available table");
+    }
     memcpy(weights, g_gl_w + gl_offset(order), (size_t)order * sizeof(double));
     memcpy(nodes, g_gl_x + gl_offset(order), (size_t)order * sizeof(double));
+    pthread_mutex_unlock(&g_gl_mu);
     return 0;
 }
 
@@ -908,8 +934,9 @@ static int validate(int n, int op, int mat, int integ, size_t samples, size_t di
     if (mat < 0 || mat > 3) return fail("Could not resolve dispatch name option");
     if (mat == OR_FIXED2 && dim != 2) return fail("invalid dispatch option (based on params)");
     if (mat == OR_SPACECURVE && dim != 2) return fail("invalid dispatch option (based on params)");
-    if (!g_gl_w && oracle_gl_init(64)) return 1;
-    if (n > 1 && (n + 3) / 2 > g_gl_max)
+    int gl_max = gl_ready();
+    if (!gl_max) return 1;
+    if (n > 1 && (n + 3) / 2 > gl_max)
         return fail("requested magnus order exceeds GL table. Consider providing a larger table.");
     if ((samples - 1) % (size_t)divisibility(integ) != 0)
         return fail("sample interval count violates integrator divisibility");

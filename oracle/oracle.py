@@ -12,6 +12,7 @@ from __future__ import annotations
 
 import ctypes
 import os
+import threading
 from pathlib import Path
 
 import numpy as np
@@ -24,6 +25,7 @@ MATRIX = {"Fixed2": 0, "Manual": 1, "Blas": 2, "SpaceCurve": 3}
 OPS = {"one": 0, "ONE": 0, "many": 1, "MANY": 1, "sum": 2, "SUM": 2}
 
 _lib = None
+_lib_mu = threading.Lock()
 
 
 def build() -> None:
@@ -34,8 +36,13 @@ def build() -> None:
 
 
 def lib() -> ctypes.CDLL:
+    """The loaded library (built on first use if absent); safe to call from several threads."""
     global _lib
-    if _lib is None:
+    if _lib is not None:
+        return _lib
+    with _lib_mu:
+        if _lib is not None:
+            return _lib
         if not _LIB_PATH.exists():
             build()
         L = ctypes.CDLL(str(_LIB_PATH))

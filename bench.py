@@ -81,7 +81,7 @@ def _counts(Q, K, S, m, m_in, P, n_out, op_one=False):
     return F_fwd, F_vjp, B_fwd, B_carry, B_vjp
 
 
-KERNELS = ("sample_fourier", "sample_sc_fourier", "lm_fwd_prod", "lm_fwd_scan", "lm_vjp_init", "lm_vjp_prod", "lm_vjp_scan", "lm_vjp_final",
+KERNELS = ("sample_fourier", "sample_sc_fourier", "lm_fwd_prod", "lm_fwd_scan", "lm_fwd_fused", "lm_vjp_init", "lm_vjp_prod", "lm_vjp_scan", "lm_vjp_final",
            "wb_fwd_pass1", "wb_combine", "wb_fwd_pass1_vjp", "wb_combine_vjp", "wb_rev", "wb_fixup")
 
 # su(2) products on the single-pass path: pure sample x 4-vector = 4 mul + 8 FMA
@@ -112,11 +112,15 @@ def _lm_counts(B, Q, K, S, m, m_in, P):
     """Algorithmic flops / HBM bytes per launch of the level-parallel kernels
     (one launch per level; jobs = B*Q): a product launch reads the samples and
     the previous prefix level and writes the products; a scan launch reads the
-    products and writes the prefix level (the carry)."""
+    products and writes the prefix level (the carry); a fused level (products
+    formed inside the chunk passes) reads the samples and the previous level
+    and writes the level."""
     jobs = B * Q
     flops = {"lm_fwd_prod": jobs * S * P, "lm_vjp_prod": jobs * S * 2 * P,
-             "lm_fwd_scan": jobs * S * 8 * m, "lm_vjp_scan": jobs * S * 8 * m}
+             "lm_fwd_scan": jobs * S * 8 * m, "lm_vjp_scan": jobs * S * 8 * m,
+             "lm_fwd_fused": jobs * S * (P + 8 * m)}
     by = {"lm_fwd_prod": 8 * (B * S * m_in + 2 * jobs * S * m), "lm_fwd_scan": 8 * 2 * jobs * S * m,
+          "lm_fwd_fused": 8 * (B * S * m_in + 2 * jobs * S * m),
           "lm_vjp_prod": 8 * (B * S * m_in + 5 * jobs * S * m), "lm_vjp_scan": 8 * 2 * jobs * S * m}
     return by, flops
 

@@ -59,46 +59,79 @@ __global__ void __launch_bounds__(256) lm_fwd_prod(LmArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int kRowMaxD = 16;
 
+// Where a workgroup's samples live: one thread per sample splits the flat
+// sample index p = job * S + i (two 64-bit divisions) once, so the staging
+// loops, which touch every entry, do no division by S or Q.  The table sits
+// behind the staged matrices in the dynamic LDS, sized by the workgroup's
+// sample count, so it costs the VJP no workgroup per CU.
+struct RowSample {
+    const double* P;  // previous level (or carry slot) of the sample's job, row 0
+    const double* A;  // the sample's input matrix
+    long long job, i;
+    double shift;
+};
+
 __host__ __device__ __forceinline__ int rows_spb(int d) { return 256 / d; }
+// LDS stride of one sample's d x d staging: d^2 + 1 doubles.  The inner loops
+// read one address per sample (broadcast over its d row threads) for the 64 / d
+// samples of a wavefront at once; at a stride of d^2 doubles (a multiple of
+// 32 dwords for even d) all of them fall in one bank pair, and the odd stride
+// spreads up to 16 samples over distinct pairs.
+__host__ __device__ __forceinline__ int rows_stride(int d) { return d * d + 1; }
 // the VJP stages three matrices per sample; stay within 64 KB of dynamic LDS
 __host__ __device__ __forceinline__ int vjp_rows_spb(int d) {
-    const int s = 256 / d, lim = 65536 / (3 * d * d * 8);
+    const int s = 256 / d, lim = 65536 / (3 * rows_stride(d) * 8);
     return s < lim ? s : lim;
+}
+// dynamic LDS of the forward rows launch: the staged matrices, then the table
+__host__ __device__ __forceinline__ size_t rows_lds_bytes(int d, int spb) {
+    return (size_t)spb * (rows_stride(d) * sizeof(double) + sizeof(RowSample));
+}
+
+__device__ __forceinline__ void row_samples(const LmArgs& a, long long p0, int spb, RowSample* tab) {
+    const int g = threadIdx.x;
+    if (g >= spb) return;
+    const long long p = p0 + g;
+    if (p >= a.jobs * a.S) return;
+    RowSample r;
+    r.job = p / a.S;
+    r.i = p - r.job * a.S;
+    const long long b = r.job / a.Qj;
+    const int q = (int)(r.job - b * a.Qj);
+    r.shift = a.x[q] - 1;
+    r.P = lm_slot(a, r.job, b, q, a.level - 1);
+    r.A = a.in + (b * a.S + r.i) * (long long)a.m_in;
+    tab[g] = r;
 }
 
 // Y_i = A_i (P_i + s T): LDS holds b = P_i + s T per sample
 __global__ void __launch_bounds__(256) lm_fwd_prod_rows(LmArgs a) {
     extern __shared__ double lds[];
-    const int d = a.d, m = a.m, spb = rows_spb(d);
+    const int d = a.d, m = a.m, spb = rows_spb(d), sm = rows_stride(d);
+    RowSample* tab = reinterpret_cast<RowSample*>(lds + spb * sm);
     const long long total = a.jobs * a.S;
     const long long p0 = (long long)blockIdx.x * spb;
-    for (int x = threadIdx.x; x < spb * m; x += 256) {
+    const int live = (int)(total - p0 < spb ? total - p0 : spb);
+    row_samples(a, p0, spb, tab);
+    __syncthreads();
+    for (int x = threadIdx.x; x < live * m; x += 256) {
         const int g = x / m, e = x - g * m;
-        const long long p = p0 + g;
-        if (p >= total) continue;
-        const long long job = p / a.S, i = p - job * a.S;
-        const long long b = job / a.Qj;
-        const int q = (int)(job - b * a.Qj);
-        const double shift = a.x[q] - 1;
-        const double* P = lm_slot(a, job, b, q, a.level - 1);
-        double v = P[i * m + e];
-        v += P[(a.S - 1) * m + e] * shift;
-        lds[x] = v;
+        const RowSample& r = tab[g];
+        double v = r.P[r.i * m + e];
+        v += r.P[(a.S - 1) * m + e] * r.shift;
+        lds[g * sm + e] = v;
     }
     __syncthreads();
-    const int g = threadIdx.x / d, r = threadIdx.x - g * d;
-    const long long p = p0 + g;
-    if (g >= spb || p >= total) return;
-    const long long job = p / a.S, i = p - job * a.S;
-    const long long b = job / a.Qj;
-    const double* Ai = a.in + (b * a.S + i) * a.m_in + r * d;
+    const int g = threadIdx.x / d, rr = threadIdx.x - g * d;
+    if (g >= live) return;
+    const double* Ai = tab[g].A + rr * d;
     double ar[kRowMaxD], acc[kRowMaxD];
 #pragma unroll
     for (int l = 0; l < kRowMaxD; ++l) {
         ar[l] = l < d ? Ai[l] : 0.0;
         acc[l] = 0.0;
     }
-    const double* bs = lds + g * m;
+    const double* bs = lds + g * sm;
 #pragma unroll
     for (int l = 0; l < kRowMaxD; ++l) {
         if (l >= d) break;
@@ -106,16 +139,18 @@ __global__ void __launch_bounds__(256) lm_fwd_prod_rows(LmArgs a) {
         for (int c = 0; c < kRowMaxD; ++c)
             if (c < d) acc[c] += ar[l] * bs[l * d + c];
     }
-    double* y = a.Y + p * m + r * d;
+    double* y = a.Y + (p0 + g) * m + rr * d;
 #pragma unroll
     for (int c = 0; c < kRowMaxD; ++c)
         if (c < d) y[c] = acc[c];
 }
 
-// dA_job_i += g b^T (row r) and temp_i = A_i^T g (row r); LDS per sample: b, g, A
+// dA_job_i += g b^T (row r) and temp_i = A_i^T g (row r); LDS per sample: b, g, A.
+// (A per-sample table as in lm_fwd_prod_rows measured slower here: the staging
+// loads then wait on LDS reads of the table.)
 __global__ void __launch_bounds__(256) lm_vjp_prod_rows(LmArgs a) {
     extern __shared__ double lds[];
-    const int d = a.d, m = a.m, spb = vjp_rows_spb(d);
+    const int d = a.d, m = a.m, spb = vjp_rows_spb(d), sm = rows_stride(d);
     const long long per = a.S * m;
     const long long total = a.jobs * a.S;
     const long long p0 = (long long)blockIdx.x * spb;
@@ -130,26 +165,27 @@ __global__ void __launch_bounds__(256) lm_vjp_prod_rows(LmArgs a) {
         const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
         double v = P[i * m + e];
         v += P[(a.S - 1) * m + e] * shift;
-        lds[x] = v;                                                                  // b
-        lds[spb * m + x] = a.gbuf[(job * 2 + a.ga) * per + i * m + e];               // g
-        lds[2 * spb * m + x] = a.in[(b * a.S + i) * (long long)a.m_in + e];          // A
+        const int y = g * sm + e;
+        lds[y] = v;                                                                  // b
+        lds[spb * sm + y] = a.gbuf[(job * 2 + a.ga) * per + i * m + e];              // g
+        lds[2 * spb * sm + y] = a.in[(b * a.S + i) * (long long)a.m_in + e];         // A
     }
     __syncthreads();
-    const int g = threadIdx.x / d, r = threadIdx.x - g * d;
+    const int g = threadIdx.x / d, rr = threadIdx.x - g * d;
     const long long p = p0 + g;
     if (g >= spb || p >= total) return;
     const long long job = p / a.S, i = p - job * a.S;
-    const double* bs = lds + g * m;
-    const double* gs = lds + spb * m + g * m;
-    const double* as = lds + 2 * spb * m + g * m;
+    const double* bs = lds + g * sm;
+    const double* gs = lds + spb * sm + g * sm;
+    const double* as = lds + 2 * spb * sm + g * sm;
     double gr[kRowMaxD], ac[kRowMaxD];
 #pragma unroll
     for (int j = 0; j < kRowMaxD; ++j) {
-        gr[j] = j < d ? gs[r * d + j] : 0.0;  // g[r][:]
-        ac[j] = j < d ? as[j * d + r] : 0.0;  // a[:][r]
+        gr[j] = j < d ? gs[rr * d + j] : 0.0;  // g[r][:]
+        ac[j] = j < d ? as[j * d + rr] : 0.0;  // a[:][r]
     }
-    double* dA = a.dAjob + job * per + i * m + r * d;
-    double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * m + r * d;
+    double* dA = a.dAjob + job * per + i * m + rr * d;
+    double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * m + rr * d;
 #pragma unroll
     for (int cc = 0; cc < kRowMaxD; ++cc) {
         if (cc >= d) break;
@@ -591,6 +627,135 @@ __global__ void __launch_bounds__(256) lm_fwd_scan_c3(LmArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// Chunk passes of levels >= 2 with the level's products formed on the fly
+// (generic d = 8, one job per instance).  lm_fwd_prod_rows writes Y_i =
+// A_i (P_i + s T) for the whole level and both chunk passes read it back; here
+// each chunk pass forms its chunk's products from the input and the previous
+// level directly, so the level reads A and P^(j-1) twice and writes P^(j) once
+// instead of reading them once, writing Y and reading Y twice.  A wavefront
+// walks one chunk (lane = entry); each batch of kFuseU samples is staged in
+// the wavefront's own LDS slice (A rows padded to d + 1 doubles) and every lane
+// sums its entry's product in the rows kernel's order, so Y and the walk
+// (lm_chunk_walk's) are the same operations as the unfused passes.
+// ---------------------------------------------------------------------------
+constexpr int kFuseU = 8;                            // samples per staged batch
+constexpr int kFuseD = 8;                            // the fused dimension
+constexpr int kFuseM = kFuseD * kFuseD;
+constexpr int kFuseSlot = kFuseD * (kFuseD + 1) + kFuseM;  // doubles per staged sample
+
+__host__ __device__ __forceinline__ bool fused_level_ok(const LmArgs& a) {
+    return a.level >= 2 && !a.sc && a.d == kFuseD && a.m_in == a.m && a.Qj == 1;
+}
+__host__ __device__ __forceinline__ long long fused_items(const LmArgs& a) {
+    return (long long)lm_nchunks(a) * a.jobs;
+}
+
+template <int I, bool WRITE>
+__global__ void __launch_bounds__(256, 4) lm_fwd_chunk_fused(LmArgs a) {
+    constexpr int D = kFuseD, M = kFuseM;
+    __shared__ double lds[4][kFuseU * kFuseSlot];
+    const int wid = threadIdx.x >> 6, e = threadIdx.x & 63, r = e / D, cc = e % D;
+    const long long item = (long long)blockIdx.x * 4 + wid;
+    const long long jobs = a.jobs, S = a.S, JM = jobs * M;
+    const int C = lm_nchunks(a);
+    if (item >= (long long)C * jobs) return;  // whole wavefronts leave together
+    const int c = (int)(item / jobs);
+    const long long job = item - (long long)c * jobs;
+    const long long je = job * M + e;
+    const int j = a.level;
+    const long long s0 = (long long)c * a.clen;
+    const long long e0 = (s0 + a.clen < S - 1) ? s0 + a.clen : S - 1;
+    const int len = (int)(e0 - s0);  // samples s0 + 1 .. e0 follow s0
+    const bool last = (c == C - 1);
+    const double* Ab = a.in + (job * S + s0) * M + e;
+    const double* P = lm_slot(a, job, job, 0, j - 1);
+    const double* Pb = P + s0 * M + e;
+    const double shift = a.x[0] - 1;
+    const double Te = P[(S - 1) * M + e];
+    double* slot = lds[wid];
+    // pass 1: end value and level-sum share of the chunk from a zero base;
+    // pass 3: the prefix from the chunk's base, written out
+    double* Pout = WRITE ? lm_slot(a, job, job, 0, j) + s0 * M + e : nullptr;
+    const double cw = a.dt * a.w[0];
+    const double at = WRITE ? a.cscan[(long long)c * JM + je] : 0.0;
+    PrefixState<I> st;
+    double Pv = 0.0, acc = 0.0;
+    // a batch's loads are issued one batch ahead, so they land while the
+    // previous batch is multiplied and walked
+    double av[kFuseU], bv[kFuseU];
+#pragma unroll
+    for (int u = 0; u < kFuseU; ++u) {
+        av[u] = 0.0;
+        bv[u] = 0.0;
+        if (u <= len) {
+            av[u] = Ab[u * M];
+            double v = Pb[u * M];
+            v += Te * shift;
+            bv[u] = v;
+        }
+    }
+    for (int o0 = 0; o0 <= len; o0 += kFuseU) {
+        __builtin_amdgcn_wave_barrier();  // the previous batch's reads stay before these writes
+#pragma unroll
+        for (int u = 0; u < kFuseU; ++u) {
+            slot[u * kFuseSlot + r * (D + 1) + cc] = av[u];
+            slot[u * kFuseSlot + D * (D + 1) + e] = bv[u];
+        }
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the writes have landed
+        __builtin_amdgcn_wave_barrier();
+        const int o1 = o0 + kFuseU;
+        if (o1 <= len) {
+            const double* an = Ab + (long long)o1 * M;
+            const double* pn = Pb + (long long)o1 * M;
+#pragma unroll
+            for (int u = 0; u < kFuseU; ++u) {
+                av[u] = 0.0;
+                bv[u] = 0.0;
+                if (o1 + u <= len) {
+                    av[u] = an[u * M];
+                    double v = pn[u * M];
+                    v += Te * shift;
+                    bv[u] = v;
+                }
+            }
+        }
+        // one sample at a time: its product, then its step (unrolling this loop
+        // lets the compiler hoist all of the batch's LDS reads and spill)
+#pragma unroll 1
+        for (int u = 0; u < kFuseU; ++u) {
+            const int o = o0 + u;
+            if (o > len) break;
+            const double* Ar = slot + u * kFuseSlot + r * (D + 1);
+            const double* Bc = slot + u * kFuseSlot + D * (D + 1) + cc;
+            double y = 0.0;
+#pragma unroll
+            for (int l = 0; l < D; ++l) y += Ar[l] * Bc[l * D];
+            const long long i = s0 + o;
+            if (o == 0) {
+                if (c == 0) {
+                    Pv = st.step(0, y, a.dt);
+                } else {
+                    st.start(y, at);
+                    Pv = at;
+                }
+            } else {
+                Pv = st.step(i, y, a.dt);
+            }
+            if (i < s0 + a.clen || last) {
+                if (WRITE) Pout[(long long)o * M] = Pv;
+                else sum_acc<I>(acc, i, S, y, cw);
+            }
+        }
+    }
+    if (WRITE) {
+        if (last && a.totals) a.totals[(job * a.n + (j - 1)) * (long long)M + e] = Pv;
+    } else {
+        a.cscan[(long long)c * JM + je] = Pv;
+        a.cscan[((long long)C + c) * JM + je] = acc;
+    }
+}
+
+// ---------------------------------------------------------------------------
 // VJP
 // ---------------------------------------------------------------------------
 
@@ -933,6 +1098,22 @@ inline LevelCounts level_counts(const LmArgs& a, bool vjp) {
 template <int I>
 hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
     const LevelCounts cnt = level_counts(a, false);
+    bool fused = false;
+    if constexpr (I != TRAPEZOID) fused = chunked_scan<I>(a) && fused_level_ok(a);
+    if (fused) {
+        // the products are formed inside both chunk passes: the level reads the
+        // samples and the previous level and writes this one
+        void* h = timer_begin("lm_fwd_fused", st, cnt.prod_bytes, cnt.prod_flops + cnt.scan_flops);
+        const long long JM = a.jobs * a.m, waves = fused_items(a);  // one chunk per wavefront
+        const dim3 grid((unsigned)((waves + 3) / 4));
+        if constexpr (I != TRAPEZOID) {
+            hipLaunchKernelGGL((lm_fwd_chunk_fused<I, false>), grid, dim3(256), 0, st, a);
+            hipLaunchKernelGGL((lm_fwd_scan_c2<I>), dim3((JM + 63) / 64), dim3(64), 0, st, a);
+            if (a.level < a.n) hipLaunchKernelGGL((lm_fwd_chunk_fused<I, true>), grid, dim3(256), 0, st, a);
+        }
+        timer_end(h);
+        return hipGetLastError();
+    }
     if (a.level >= 2) {
         void* h = timer_begin("lm_fwd_prod", st, cnt.prod_bytes, cnt.prod_flops);
         switch (prod_path(a)) {
@@ -944,7 +1125,7 @@ hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
             case 1: {
                 const int spb = rows_spb(a.d);
                 hipLaunchKernelGGL(lm_fwd_prod_rows, dim3((a.jobs * a.S + spb - 1) / spb), dim3(256),
-                                   (size_t)spb * a.m * sizeof(double), st, a);
+                                   rows_lds_bytes(a.d, spb), st, a);
                 break;
             }
             default: hipLaunchKernelGGL(lm_fwd_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);
@@ -985,7 +1166,7 @@ hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
         case 1: {
             const int spb = vjp_rows_spb(a.d);
             hipLaunchKernelGGL(lm_vjp_prod_rows, dim3((a.jobs * a.S + spb - 1) / spb), dim3(256),
-                               (size_t)3 * spb * a.m * sizeof(double), st, a);
+                               (size_t)3 * spb * rows_stride(a.d) * sizeof(double), st, a);
             break;
         }
         default: hipLaunchKernelGGL(lm_vjp_prod, dim3(blocks_for(a.jobs * a.S * a.m)), dim3(256), 0, st, a);

@@ -186,6 +186,12 @@ __global__ void __launch_bounds__(256) lm_vjp_prod_rows(LmArgs a) {
     }
     double* dA = a.dAjob + job * per + i * m + rr * d;
     double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i * m + rr * d;
+    // the row of dA is read before any store: the stores into Gb might alias it
+    // as far as the compiler knows, so reads left in the loop would each wait
+    // for a full memory round trip
+    double dav[kRowMaxD];
+#pragma unroll
+    for (int cc = 0; cc < kRowMaxD; ++cc) dav[cc] = cc < d ? dA[cc] : 0.0;
 #pragma unroll
     for (int cc = 0; cc < kRowMaxD; ++cc) {
         if (cc >= d) break;
@@ -197,7 +203,7 @@ __global__ void __launch_bounds__(256) lm_vjp_prod_rows(LmArgs a) {
 #pragma unroll
         for (int ii = 0; ii < kRowMaxD; ++ii)
             if (ii < d) t2 += ac[ii] * gs[ii * d + cc];
-        dA[cc] += v;
+        dA[cc] = dav[cc] + v;
         Gb[cc] = t2;
     }
 }

@@ -633,7 +633,7 @@ def main():
     ap.add_argument("--steps", type=int, default=None, help="timed steps (default 3 for controller, else 20)")
     ap.add_argument("--warmup", type=int, default=None, help="warm-up steps (default 1 for controller, else 5)")
     ap.add_argument("--workload", default="controller",
-                    choices=["spacecurve-batch", "dense-d32", "plain-d4", "uneven", "controller"])
+                    choices=["spacecurve-batch", "dense-d32", "plain-d4", "uneven", "controller", "wbasis-d8"])
     ap.add_argument("--ctrl-train", type=int, default=200000)
     ap.add_argument("--ctrl-eval", type=int, default=10000)
     ap.add_argument("--ctrl-epochs", type=int, default=20)
@@ -717,6 +717,22 @@ def main():
         config = {"workload": "dense-d32" if d == 32 else f"dense-d{d}", "instances": B, "samples": S, "dim": d,
                   "n": n, "integrator": "Boole",
                   "op": "sum", "carry": "recorded"}
+        do_vjp = True
+    elif wl == "wbasis-d8":
+        # one resolution group of the controller's solve: forward (op 'sum') + VJP
+        # without a carry on the dense W-basis path, 1000 d = 8 instances at S = 2049
+        B, S, d, n, modes = args.batch or 1000, 2049, 8, 8, 5
+        a, cot = _dense_inputs(torch, B, S, d, modes, dev, seed=5)
+
+        def step():
+            out = torch_ops.forward(n, a, 0.0, 1.0, op="sum", integrator="Boole")
+            dA = torch_ops.vjp(n, a, cot, None, 0.0, 1.0, op="sum", integrator="Boole")
+            return out, dA
+
+        # one job per instance on this path (the W-basis holds all GL nodes)
+        Q, K, m, m_in, P, n_out = 1, n - 1, d * d, d * d, 2 * d ** 3, 1
+        config = {"workload": "wbasis-d8", "instances": B, "samples": S, "dim": d, "n": n, "integrator": "Boole",
+                  "op": "sum", "carry": "none (dense W-basis sweep)"}
         do_vjp = True
     else:
         B, S, d, n, modes = 1, 1025, 4, 4, 3

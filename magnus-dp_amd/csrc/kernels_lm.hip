@@ -761,6 +761,93 @@ __global__ void __launch_bounds__(256, 4) lm_fwd_chunk_fused(LmArgs a) {
     }
 }
 
+// VJP products for d = 8 streamed by wavefronts: a wavefront takes kVwSamples
+// consecutive samples of one job (lane = entry (r, c)), stages each sample's
+// b = P + s T, adjoint g and input A (rows padded to d + 1) in its own LDS
+// slice, and writes dA[r][c] += sum_j g[r][j] b[c][j] and Gb[r][c] =
+// sum_i A[i][r] g[i][c] with coalesced 512-byte rows; the next batch's loads
+// are issued before the current batch is multiplied.  The sums run in
+// lm_vjp_prod_rows' order, so the results are the same bits.
+constexpr int kVwSamples = 32;                        // samples per wavefront
+constexpr int kVwU = 4;                               // samples per staged batch
+constexpr int kVwSlot = 3 * kFuseD * (kFuseD + 1);    // doubles per staged sample
+
+__host__ __device__ __forceinline__ bool vjp_wave_ok(const LmArgs& a) {
+    return !a.sc && a.d == kFuseD && a.m_in == a.m;
+}
+
+__global__ void __launch_bounds__(256, 4) lm_vjp_prod_wave(LmArgs a) {
+    constexpr int D = kFuseD, M = kFuseM, LD = D + 1;
+    __shared__ double lds[4][kVwU * kVwSlot];
+    const int wid = threadIdx.x >> 6, e = threadIdx.x & 63, r = e / D, cc = e % D;
+    const long long S = a.S, per = S * M;
+    const long long nblk = (S + kVwSamples - 1) / kVwSamples;
+    const long long item = (long long)blockIdx.x * 4 + wid;
+    if (item >= a.jobs * nblk) return;  // whole wavefronts leave together
+    const long long job = item / nblk;
+    const long long i0 = (item - job * nblk) * kVwSamples;
+    const int len = (int)(S - i0 < kVwSamples ? S - i0 : kVwSamples);
+    const long long b = job / a.Q;
+    const int q = (int)(job - b * a.Q);
+    const double shift = a.x[q] - 1.0;
+    const double* P = a.carry + ((b * a.Q + q) * (long long)(a.n - 1) + (a.level - 2)) * per;
+    const double Te = P[(S - 1) * M + e];
+    const double* Pb = P + i0 * M + e;
+    const double* Gi = a.gbuf + (job * 2 + a.ga) * per + i0 * M + e;
+    const double* Ai = a.in + (b * S + i0) * M + e;
+    double* dA = a.dAjob + job * per + i0 * M + e;
+    double* Gb = a.gbuf + (job * 2 + (1 - a.ga)) * per + i0 * M + e;
+    double* slot = lds[wid];
+    double vb[kVwU], vg[kVwU], va[kVwU];
+    auto load = [&](int o0) {
+#pragma unroll
+        for (int u = 0; u < kVwU; ++u) {
+            vb[u] = 0.0;
+            vg[u] = 0.0;
+            va[u] = 0.0;
+            if (o0 + u < len) {
+                double v = Pb[(long long)(o0 + u) * M];
+                v += Te * shift;
+                vb[u] = v;
+                vg[u] = Gi[(long long)(o0 + u) * M];
+                va[u] = Ai[(long long)(o0 + u) * M];
+            }
+        }
+    };
+    load(0);
+    for (int o0 = 0; o0 < len; o0 += kVwU) {
+        __builtin_amdgcn_wave_barrier();  // the previous batch's reads stay before these writes
+#pragma unroll
+        for (int u = 0; u < kVwU; ++u) {
+            double* sl = slot + u * kVwSlot + r * LD + cc;
+            sl[0] = vb[u];
+            sl[D * LD] = vg[u];
+            sl[2 * D * LD] = va[u];
+        }
+        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the writes have landed
+        __builtin_amdgcn_wave_barrier();
+        double dav[kVwU];
+#pragma unroll
+        for (int u = 0; u < kVwU; ++u) dav[u] = (o0 + u < len) ? dA[(long long)(o0 + u) * M] : 0.0;
+        if (o0 + kVwU < len) load(o0 + kVwU);
+#pragma unroll
+        for (int u = 0; u < kVwU; ++u) {
+            if (o0 + u >= len) break;
+            const double* bs = slot + u * kVwSlot;
+            const double* gs = bs + D * LD;
+            const double* as = gs + D * LD;
+            double v = 0.0;  // sum_j g[r][j] b[c][j]
+#pragma unroll
+            for (int j = 0; j < D; ++j) v += gs[r * LD + j] * bs[cc * LD + j];
+            double t2 = 0.0;  // sum_i a[i][r] g[i][c]
+#pragma unroll
+            for (int ii = 0; ii < D; ++ii) t2 += as[ii * LD + r] * gs[ii * LD + cc];
+            dA[(long long)(o0 + u) * M] = dav[u] + v;
+            Gb[(long long)(o0 + u) * M] = t2;
+        }
+    }
+}
+
 // ---------------------------------------------------------------------------
 // VJP
 // ---------------------------------------------------------------------------
@@ -1170,6 +1257,11 @@ hipError_t lm_vjp_level_i(const LmArgs& a, hipStream_t st) {
         case 32 + 4: launch_mfma<32, 4>(lm_vjp_prod_mfma<32, 4>, a, st); break;
         case 64: launch_mfma<64, 4>(lm_vjp_prod_mfma<64, 4>, a, st); break;
         case 1: {
+            if (vjp_wave_ok(a)) {
+                const long long waves = a.jobs * ((a.S + kVwSamples - 1) / kVwSamples);
+                hipLaunchKernelGGL(lm_vjp_prod_wave, dim3((unsigned)((waves + 3) / 4)), dim3(256), 0, st, a);
+                break;
+            }
             const int spb = vjp_rows_spb(a.d);
             hipLaunchKernelGGL(lm_vjp_prod_rows, dim3((a.jobs * a.S + spb - 1) / spb), dim3(256),
                                (size_t)3 * spb * rows_stride(a.d) * sizeof(double), st, a);

@@ -14,6 +14,8 @@
 // both operands in LDS (transposed on the way in) and, for d in {16, 32, 48,
 // 64}, each thread computes a (d/16) x (d/16) register tile; other d take
 // one entry per thread, and d > 64 reads the operands from global memory.
+// d <= 16 runs an instantiation with 16 x 16 staging (4 KB of LDS), so that
+// eight of these latency-bound workgroups share a CU instead of two.
 // Every thread owns the same entries of every matrix it accumulates, so the
 // accumulations need no atomics; a barrier separates a matrix's last write
 // from its first read as an operand.
@@ -25,20 +27,26 @@ namespace magnus {
 namespace {
 
 constexpr int kFoldThreads = 256;
-constexpr int kFoldLds = 64;  // largest d staged in LDS
+constexpr int kFoldLds = 64;       // largest d staged in LDS
+constexpr int kFoldLdsSmall = 16;  // staging capacity of the d <= 16 instantiation (4 KB instead of 64 KB)
 
+// CAP: largest d staged in LDS.  A workgroup holds two CAP x CAP operands, so
+// the d <= 16 instantiation fits eight workgroups per CU where the d <= 64 one
+// fits two; the arithmetic is the same for every d <= CAP.
+template <int CAP>
 struct FoldLds {
-    double x[kFoldLds * kFoldLds];
-    double y[kFoldLds * kFoldLds];
+    double x[CAP * CAP];
+    double y[CAP * CAP];
 };
 
 // C[e] += alpha * (X' Y')[e] for the entries this thread owns (X' = X or X^T,
 // Y' = Y or Y^T; d x d row-major).  Ends with a barrier: C and the staging
 // buffers are free for the next call.
-__device__ void mat_acc(FoldLds& L, double* __restrict__ C, double alpha, const double* __restrict__ X, bool tx,
+template <int CAP>
+__device__ void mat_acc(FoldLds<CAP>& L, double* __restrict__ C, double alpha, const double* __restrict__ X, bool tx,
                         const double* __restrict__ Y, bool ty, int d) {
     const int m = d * d, tid = threadIdx.x;
-    if (d > kFoldLds) {
+    if (d > CAP) {
         for (int e = tid; e < m; e += kFoldThreads) {
             const int r = e / d, c = e - r * d;
             double acc = 0.0;
@@ -97,14 +105,16 @@ __device__ void mat_acc(FoldLds& L, double* __restrict__ C, double alpha, const 
 
 // Entry ownership of mat_set / mat_axpy must match mat_acc's: the tiled
 // layout gives thread tid the entries (r0 + i, c0 + j).
-__device__ __forceinline__ int owned_count(int d) { return ((d & 15) == 0 && d <= kFoldLds) ? (d >> 4) * (d >> 4) : 0; }
+template <int CAP>
+__device__ __forceinline__ int owned_count(int d) { return ((d & 15) == 0 && d <= CAP) ? (d >> 4) * (d >> 4) : 0; }
 __device__ __forceinline__ int owned_entry(int d, int u) {
     const int t = d >> 4, tid = threadIdx.x;
     return ((tid >> 4) * t + u / t) * d + (tid & 15) * t + u % t;
 }
 
+template <int CAP>
 __device__ void mat_set(double* __restrict__ C, double alpha, const double* __restrict__ X, int d) {
-    const int m = d * d, oc = owned_count(d);
+    const int m = d * d, oc = owned_count<CAP>(d);
     if (oc) {
         for (int u = 0; u < oc; ++u) {
             const int e = owned_entry(d, u);
@@ -115,8 +125,9 @@ __device__ void mat_set(double* __restrict__ C, double alpha, const double* __re
     for (int e = threadIdx.x; e < m; e += kFoldThreads) C[e] = X ? alpha * X[e] : 0.0;
 }
 
+template <int CAP>
 __device__ void mat_axpy(double* __restrict__ C, double alpha, const double* __restrict__ X, int d) {
-    const int m = d * d, oc = owned_count(d);
+    const int m = d * d, oc = owned_count<CAP>(d);
     if (oc) {
         for (int u = 0; u < oc; ++u) {
             const int e = owned_entry(d, u);
@@ -128,11 +139,12 @@ __device__ void mat_axpy(double* __restrict__ C, double alpha, const double* __r
 }
 
 // T_1..T_K of node q; T_j at T + j m (T_0 unused)
-__device__ void fold_totals(FoldLds& L, const WbdArgs& a, const double* What, double* T, double s) {
+template <int CAP>
+__device__ void fold_totals(FoldLds<CAP>& L, const WbdArgs& a, const double* What, double* T, double s) {
     const int K = a.n - 1, d = a.d;
     const long long M = a.m;
     for (int j = 1; j <= K; ++j) {
-        mat_set(T + j * M, 1.0, What + (j - 1) * M, d);
+        mat_set<CAP>(T + j * M, 1.0, What + (j - 1) * M, d);
         for (int l = 1; l < j; ++l) mat_acc(L, T + j * M, s, What + (l - 1) * M, false, T + (j - l) * M, false, d);
         __syncthreads();
     }
@@ -154,8 +166,9 @@ __device__ __forceinline__ const double* obar(const WbdArgs& a, long long b, int
 }
 
 // node q's contribution to Omega_2..Omega_n: work rows [b][q][n-1][m]
+template <int CAP>
 __global__ void __launch_bounds__(kFoldThreads) wbd_fold(WbdArgs a) {
-    __shared__ FoldLds L;
+    __shared__ FoldLds<CAP> L;
     const long long b = blockIdx.x / a.Q;
     const int q = (int)(blockIdx.x - b * a.Q);
     const int n = a.n, K = n - 1, d = a.d;
@@ -167,7 +180,7 @@ __global__ void __launch_bounds__(kFoldThreads) wbd_fold(WbdArgs a) {
     fold_totals(L, a, What, T, s);
     for (int j = 2; j <= n; ++j) {
         double* O = Om + (j - 1) * M;
-        mat_set(O, wq, zrow(a, b, j - 1), d);
+        mat_set<CAP>(O, wq, zrow(a, b, j - 1), d);
         for (int l = 0; l <= j - 2; ++l) mat_acc(L, O, wq * s, zrow(a, b, l), false, T + (j - 1 - l) * M, false, d);
     }
 }
@@ -201,8 +214,9 @@ __global__ void __launch_bounds__(256) wbd_fold_reduce(WbdArgs a) {
 
 // Adjoint of node q's fold: its Zbar_l and What_bar_j contributions, rows
 // [b][q][n][m] of part (Zbar) and part2 (What_bar).
+template <int CAP>
 __global__ void __launch_bounds__(kFoldThreads) wbd_fold_vjp(WbdArgs a) {
-    __shared__ FoldLds L;
+    __shared__ FoldLds<CAP> L;
     const long long b = blockIdx.x / a.Q;
     const int q = (int)(blockIdx.x - b * a.Q);
     const int n = a.n, K = n - 1, d = a.d;
@@ -214,15 +228,15 @@ __global__ void __launch_bounds__(kFoldThreads) wbd_fold_vjp(WbdArgs a) {
     double* Wb = a.part2 + (long long)blockIdx.x * n * M;  // row j-1 = What_bar_j (node q)
     const double s = a.x[q] - 1.0, wq = a.w[q];
     for (int l = 0; l < n; ++l) {
-        mat_set(Zb + l * M, 0.0, nullptr, d);
-        mat_set(Wb + l * M, 0.0, nullptr, d);
+        mat_set<CAP>(Zb + l * M, 0.0, nullptr, d);
+        mat_set<CAP>(Wb + l * M, 0.0, nullptr, d);
     }
-    for (int j = 1; j <= K; ++j) mat_set(Tb + j * M, 0.0, nullptr, d);
+    for (int j = 1; j <= K; ++j) mat_set<CAP>(Tb + j * M, 0.0, nullptr, d);
     fold_totals(L, a, What, T, s);
     for (int j = 2; j <= n; ++j) {
         const double* Ob = obar(a, b, j);
         if (!Ob) continue;
-        mat_axpy(Zb + (j - 1) * M, wq, Ob, d);
+        mat_axpy<CAP>(Zb + (j - 1) * M, wq, Ob, d);
         for (int l = 0; l <= j - 2; ++l) {
             mat_acc(L, Zb + l * M, wq * s, Ob, false, T + (j - 1 - l) * M, true, d);
             mat_acc(L, Tb + (j - 1 - l) * M, wq * s, zrow(a, b, l), true, Ob, false, d);
@@ -230,7 +244,7 @@ __global__ void __launch_bounds__(kFoldThreads) wbd_fold_vjp(WbdArgs a) {
     }
     __syncthreads();
     for (int j = K; j >= 1; --j) {
-        mat_axpy(Wb + (j - 1) * M, 1.0, Tb + j * M, d);
+        mat_axpy<CAP>(Wb + (j - 1) * M, 1.0, Tb + j * M, d);
         for (int l = 1; l < j; ++l) {
             mat_acc(L, Wb + (l - 1) * M, s, Tb + j * M, false, T + (j - l) * M, true, d);
             mat_acc(L, Tb + (j - l) * M, s, What + (l - 1) * M, true, Tb + j * M, false, d);
@@ -266,7 +280,10 @@ __global__ void __launch_bounds__(256) wbd_fold_vjp_reduce(WbdArgs a) {
 hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     void* h = timer_begin("wbd_fold", s);
-    hipLaunchKernelGGL(wbd_fold, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
+    if (a.d <= kFoldLdsSmall)
+        hipLaunchKernelGGL(wbd_fold<kFoldLdsSmall>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(wbd_fold<kFoldLds>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
     hipLaunchKernelGGL(wbd_fold_reduce, dim3((unsigned)((a.B * a.m + 255) / 256)), dim3(256), 0, s, a);
     timer_end(h);
     return hipGetLastError();
@@ -275,7 +292,10 @@ hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s) {
 hipError_t launch_wbd_fold_vjp(const WbdArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     void* h = timer_begin("wbd_fold_vjp", s);
-    hipLaunchKernelGGL(wbd_fold_vjp, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
+    if (a.d <= kFoldLdsSmall)
+        hipLaunchKernelGGL(wbd_fold_vjp<kFoldLdsSmall>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(wbd_fold_vjp<kFoldLds>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
     hipLaunchKernelGGL(wbd_fold_vjp_reduce, dim3((unsigned)((a.B * a.n * a.m + 255) / 256)), dim3(256), 0, s, a);
     timer_end(h);
     return hipGetLastError();

@@ -652,11 +652,15 @@ constexpr int kFuseSlot = kFuseD * (kFuseD + 1) + kFuseM;  // doubles per staged
 __host__ __device__ __forceinline__ bool fused_level_ok(const LmArgs& a) {
     return a.level >= 2 && !a.sc && a.d == kFuseD && a.m_in == a.m && a.Qj == 1;
 }
+__host__ __device__ __forceinline__ bool fused_raw_ok(const LmArgs& a) {
+    return a.level == 1 && !a.sc && a.d == kFuseD && a.m_in == a.m && a.Qj == 1;
+}
 __host__ __device__ __forceinline__ long long fused_items(const LmArgs& a) {
     return (long long)lm_nchunks(a) * a.jobs;
 }
 
-template <int I, bool WRITE>
+// RAW: level 1, whose input is the sample itself (no product, no previous level)
+template <int I, bool WRITE, bool RAW>
 __global__ void __launch_bounds__(256, 4) lm_fwd_chunk_fused(LmArgs a) {
     constexpr int D = kFuseD, M = kFuseM;
     __shared__ double lds[4][kFuseU * kFuseSlot];
@@ -674,15 +678,15 @@ __global__ void __launch_bounds__(256, 4) lm_fwd_chunk_fused(LmArgs a) {
     const int len = (int)(e0 - s0);  // samples s0 + 1 .. e0 follow s0
     const bool last = (c == C - 1);
     const double* Ab = a.in + (job * S + s0) * M + e;
-    const double* P = lm_slot(a, job, job, 0, j - 1);
-    const double* Pb = P + s0 * M + e;
+    const double* P = RAW ? nullptr : lm_slot(a, job, job, 0, j - 1);
+    const double* Pb = RAW ? nullptr : P + s0 * M + e;
     const double shift = a.x[0] - 1;
-    const double Te = P[(S - 1) * M + e];
+    const double Te = RAW ? 0.0 : P[(S - 1) * M + e];
     double* slot = lds[wid];
     // pass 1: end value and level-sum share of the chunk from a zero base;
     // pass 3: the prefix from the chunk's base, written out
     double* Pout = WRITE ? lm_slot(a, job, job, 0, j) + s0 * M + e : nullptr;
-    const double cw = a.dt * a.w[0];
+    const double cw = a.dt * (RAW ? 1.0 : a.w[0]);
     const double at = WRITE ? a.cscan[(long long)c * JM + je] : 0.0;
     PrefixState<I> st;
     double Pv = 0.0, acc = 0.0;
@@ -695,33 +699,43 @@ __global__ void __launch_bounds__(256, 4) lm_fwd_chunk_fused(LmArgs a) {
         bv[u] = 0.0;
         if (u <= len) {
             av[u] = Ab[u * M];
-            double v = Pb[u * M];
-            v += Te * shift;
-            bv[u] = v;
+            if (!RAW) {
+                double v = Pb[u * M];
+                v += Te * shift;
+                bv[u] = v;
+            }
         }
     }
     for (int o0 = 0; o0 <= len; o0 += kFuseU) {
-        __builtin_amdgcn_wave_barrier();  // the previous batch's reads stay before these writes
+        double raw[kFuseU];
+        if (RAW) {
 #pragma unroll
-        for (int u = 0; u < kFuseU; ++u) {
-            slot[u * kFuseSlot + r * (D + 1) + cc] = av[u];
-            slot[u * kFuseSlot + D * (D + 1) + e] = bv[u];
+            for (int u = 0; u < kFuseU; ++u) raw[u] = av[u];
+        } else {
+            __builtin_amdgcn_wave_barrier();  // the previous batch's reads stay before these writes
+#pragma unroll
+            for (int u = 0; u < kFuseU; ++u) {
+                slot[u * kFuseSlot + r * (D + 1) + cc] = av[u];
+                slot[u * kFuseSlot + D * (D + 1) + e] = bv[u];
+            }
+            __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the writes have landed
+            __builtin_amdgcn_wave_barrier();
         }
-        __builtin_amdgcn_s_waitcnt(0xC07F);  // lgkmcnt(0): the writes have landed
-        __builtin_amdgcn_wave_barrier();
         const int o1 = o0 + kFuseU;
         if (o1 <= len) {
             const double* an = Ab + (long long)o1 * M;
-            const double* pn = Pb + (long long)o1 * M;
+            const double* pn = RAW ? nullptr : Pb + (long long)o1 * M;
 #pragma unroll
             for (int u = 0; u < kFuseU; ++u) {
                 av[u] = 0.0;
                 bv[u] = 0.0;
                 if (o1 + u <= len) {
                     av[u] = an[u * M];
-                    double v = pn[u * M];
-                    v += Te * shift;
-                    bv[u] = v;
+                    if (!RAW) {
+                        double v = pn[u * M];
+                        v += Te * shift;
+                        bv[u] = v;
+                    }
                 }
             }
         }
@@ -731,11 +745,18 @@ __global__ void __launch_bounds__(256, 4) lm_fwd_chunk_fused(LmArgs a) {
         for (int u = 0; u < kFuseU; ++u) {
             const int o = o0 + u;
             if (o > len) break;
-            const double* Ar = slot + u * kFuseSlot + r * (D + 1);
-            const double* Bc = slot + u * kFuseSlot + D * (D + 1) + cc;
             double y = 0.0;
+            if (RAW) {
+                // a runtime-indexed register array would live in scratch: pick by unrolled compare
 #pragma unroll
-            for (int l = 0; l < D; ++l) y += Ar[l] * Bc[l * D];
+                for (int t = 0; t < kFuseU; ++t)
+                    if (t == u) y = raw[t];
+            } else {
+                const double* Ar = slot + u * kFuseSlot + r * (D + 1);
+                const double* Bc = slot + u * kFuseSlot + D * (D + 1) + cc;
+#pragma unroll
+                for (int l = 0; l < D; ++l) y += Ar[l] * Bc[l * D];
+            }
             const long long i = s0 + o;
             if (o == 0) {
                 if (c == 0) {
@@ -1191,18 +1212,32 @@ inline LevelCounts level_counts(const LmArgs& a, bool vjp) {
 template <int I>
 hipError_t lm_forward_level_i(const LmArgs& a, hipStream_t st) {
     const LevelCounts cnt = level_counts(a, false);
-    bool fused = false;
-    if constexpr (I != TRAPEZOID) fused = chunked_scan<I>(a) && fused_level_ok(a);
-    if (fused) {
-        // the products are formed inside both chunk passes: the level reads the
-        // samples and the previous level and writes this one
-        void* h = timer_begin("lm_fwd_fused", st, cnt.prod_bytes, cnt.prod_flops + cnt.scan_flops);
+    bool fused = false, raw = false;
+    if constexpr (I != TRAPEZOID) {
+        const bool chunked = chunked_scan<I>(a);
+        fused = chunked && fused_level_ok(a);
+        raw = chunked && fused_raw_ok(a);
+    }
+    if (fused || raw) {
+        // levels >= 2: the products are formed inside both chunk passes (the
+        // level reads the samples and the previous level and writes this one);
+        // level 1: the same wavefront-per-chunk passes over the samples
+        void* h = fused ? timer_begin("lm_fwd_fused", st, cnt.prod_bytes, cnt.prod_flops + cnt.scan_flops)
+                        : timer_begin("lm_fwd_scan", st, cnt.scan_bytes, cnt.scan_flops);
         const long long JM = a.jobs * a.m, waves = fused_items(a);  // one chunk per wavefront
         const dim3 grid((unsigned)((waves + 3) / 4));
         if constexpr (I != TRAPEZOID) {
-            hipLaunchKernelGGL((lm_fwd_chunk_fused<I, false>), grid, dim3(256), 0, st, a);
-            hipLaunchKernelGGL((lm_fwd_scan_c2<I>), dim3((JM + 63) / 64), dim3(64), 0, st, a);
-            if (a.level < a.n) hipLaunchKernelGGL((lm_fwd_chunk_fused<I, true>), grid, dim3(256), 0, st, a);
+            if (fused) {
+                hipLaunchKernelGGL((lm_fwd_chunk_fused<I, false, false>), grid, dim3(256), 0, st, a);
+                hipLaunchKernelGGL((lm_fwd_scan_c2<I>), dim3((JM + 63) / 64), dim3(64), 0, st, a);
+                if (a.level < a.n)
+                    hipLaunchKernelGGL((lm_fwd_chunk_fused<I, true, false>), grid, dim3(256), 0, st, a);
+            } else {
+                hipLaunchKernelGGL((lm_fwd_chunk_fused<I, false, true>), grid, dim3(256), 0, st, a);
+                hipLaunchKernelGGL((lm_fwd_scan_c2<I>), dim3((JM + 63) / 64), dim3(64), 0, st, a);
+                if (a.level < a.n)
+                    hipLaunchKernelGGL((lm_fwd_chunk_fused<I, true, true>), grid, dim3(256), 0, st, a);
+            }
         }
         timer_end(h);
         return hipGetLastError();

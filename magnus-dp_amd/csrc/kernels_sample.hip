@@ -87,8 +87,10 @@ __global__ void __launch_bounds__(256) sample_sc_fourier_kernel(SampleArgs a) {
 int sample_max_modes() { return kSampleMaxModes; }
 
 hipError_t launch_sample_fourier(SampleArgs a, hipStream_t s) {
-    // samples per workgroup: fill 256 threads with entries, keep the table within 256 (sample, mode) pairs
-    int sb = a.m >= kSampleThreads ? 1 : kSampleThreads / a.m;
+    // samples per workgroup: about eight entries per thread (a workgroup of only
+    // 256 entries spent most of its time in the sin/cos table, built by a few
+    // lanes), keeping the table within 256 (sample, mode) pairs
+    int sb = a.m >= kSampleThreads ? 1 : (8 * kSampleThreads) / a.m;
     const int sb_tab = kSampleThreads / (a.modes > 0 ? a.modes : 1);
     if (sb > sb_tab) sb = sb_tab;
     if (sb < 1) sb = 1;

@@ -570,6 +570,42 @@ def test_timing_counts_state_the_level_kernels_algorithmic_work():
     assert capi.timing_counts("lm_fwd_prod") == (0.0, 0.0)
 
 
+def test_d8_wbasis_levels_run_fused_and_state_their_work():
+    """A carry-free d = 8 forward takes the W-basis sweep (one job per instance;
+    S = 129 gives four 32-sample chunks): levels 2..n form their products inside
+    the chunk passes (no product launch), level 1 runs the chunk passes over the
+    samples; a fused launch states that its level reads the samples and the
+    previous level once and writes the level.  The results match the oracle."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    B, S, d, n = 2, 129, 8, 5
+    m = d * d
+    A = np.stack([fourier_matrices(np.random.default_rng([41, b]), S, d, modes=3) for b in range(B)])
+    capi.timing_reset()
+    capi.timing_enable(True)
+    try:
+        out = torch_ops.forward(n, torch.from_numpy(A).cuda(), 0.0, 1.0, op="many", integrator="Boole")
+        torch.cuda.synchronize()
+    finally:
+        capi.timing_enable(False)
+    assert capi.last_kernel() == "wbd"
+    assert capi.timing_read("lm_fwd_prod")[0] == 0
+    launches, _ = capi.timing_read("lm_fwd_fused")
+    by, fl = capi.timing_counts("lm_fwd_fused")
+    assert launches == n - 1
+    assert by == (n - 1) * 8 * (B * S * m + 2 * B * S * m)
+    assert fl == (n - 1) * B * S * (2 * d ** 3 + 8 * m)
+    launches, _ = capi.timing_read("lm_fwd_scan")
+    by, fl = capi.timing_counts("lm_fwd_scan")
+    assert launches == 1 and by == 16 * B * S * m and fl == 8 * B * S * m
+    capi.timing_reset()
+    out = out.cpu().numpy()
+    for b in range(B):
+        assert rel_fro(out[b], O.forward(n, A[b], 0.0, 1.0, op="many", integrator="Boole")) <= TOL
+
+
 def test_batched_matrix_api_matches_per_instance():
     import torch
 

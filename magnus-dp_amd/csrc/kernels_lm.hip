@@ -59,11 +59,11 @@ __global__ void __launch_bounds__(256) lm_fwd_prod(LmArgs a) {
 // ---------------------------------------------------------------------------
 constexpr int kRowMaxD = 16;
 
-// Where a workgroup's samples live: one thread per sample splits the flat
-// sample index p = job * S + i (two 64-bit divisions) once, so the staging
-// loops, which touch every entry, do no division by S or Q.  The table sits
-// behind the staged matrices in the dynamic LDS, sized by the workgroup's
-// sample count, so it costs the VJP no workgroup per CU.
+// Where a workgroup's samples live (forward row products): one thread per
+// sample splits the flat sample index p = job * S + i (two 64-bit divisions)
+// once, so the staging loop, which touches every entry, does no division by S
+// or Q.  The table sits behind the staged matrices in the dynamic LDS, sized by
+// the workgroup's sample count.
 struct RowSample {
     const double* P;  // previous level (or carry slot) of the sample's job, row 0
     const double* A;  // the sample's input matrix

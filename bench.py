@@ -19,6 +19,8 @@ Other workloads (--workload):
   dense-d32         256 x d = 32, S = 4097, Boole, n = 4, sum, record carry + VJP
   plain-d4          one d = 4 instance, S = 1025, Boole, n = 4, sum, forward only
   uneven            2048 instances of d 2..64, S 257..8193, n 2..6 in one graph launch
+  wbasis-d8         one resolution group of the controller's solve: 1000 d = 8 instances,
+                    S = 2049, Boole, n = 8, sum, forward + VJP without a carry
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 """

@@ -27,13 +27,23 @@ __device__ __forceinline__ double grid_t(long long i, long long S, double t0, do
 constexpr int kSampleMaxModes = 32;
 constexpr int kSampleThreads = 256;
 
+// STAGE: the instance's (2M+1) coefficient matrices fit kSampleStage doubles
+// and are copied to LDS once per workgroup (every entry reads 2M+1 of them per
+// sample); otherwise they are read from global memory.
+constexpr int kSampleStage = 2048;
+
+template <bool STAGE>
 __global__ void __launch_bounds__(kSampleThreads) sample_fourier_kernel(SampleArgs a) {
     __shared__ double tab[kSampleThreads * 2];  // [sample][mode][sin, cos], SB * modes <= 256
+    __shared__ double cst[STAGE ? kSampleStage : 1];
     const int m = a.m, modes = a.modes;
     const int SB = a.sb;
     const long long blocks_per_inst = (a.S + SB - 1) / SB;
     const long long b = blockIdx.x / blocks_per_inst;
     const long long i0 = (blockIdx.x - b * blocks_per_inst) * SB;
+    const double* Cg = a.coeffs + b * (long long)(2 * modes + 1) * m;
+    if (STAGE)
+        for (int x = threadIdx.x; x < (2 * modes + 1) * m; x += blockDim.x) cst[x] = Cg[x];
     for (int x = threadIdx.x; x < SB * modes; x += blockDim.x) {
         const int si = x / modes, k = x - si * modes + 1;
         const long long i = i0 + si;
@@ -48,7 +58,7 @@ __global__ void __launch_bounds__(kSampleThreads) sample_fourier_kernel(SampleAr
         tab[2 * x + 1] = cv;
     }
     __syncthreads();
-    const double* C = a.coeffs + b * (long long)(2 * modes + 1) * m;
+    const double* C = STAGE ? cst : Cg;
     for (int x = threadIdx.x; x < SB * m; x += blockDim.x) {
         const int si = x / m, e = x - si * m;
         const long long i = i0 + si;
@@ -97,7 +107,10 @@ hipError_t launch_sample_fourier(SampleArgs a, hipStream_t s) {
     a.sb = sb;
     const long long blocks = a.B * ((a.S + sb - 1) / sb);
     void* h = timer_begin("sample_fourier", s);
-    hipLaunchKernelGGL(sample_fourier_kernel, dim3(blocks), dim3(kSampleThreads), 0, s, a);
+    if ((long long)(2 * a.modes + 1) * a.m <= kSampleStage)
+        hipLaunchKernelGGL(sample_fourier_kernel<true>, dim3(blocks), dim3(kSampleThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(sample_fourier_kernel<false>, dim3(blocks), dim3(kSampleThreads), 0, s, a);
     timer_end(h);
     return hipGetLastError();
 }

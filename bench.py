@@ -155,7 +155,6 @@ def _cpu_baseline_sc(a_host, cot_host, n, S, budget_s):
     import oracle as O
 
     cores = len(os.sched_getaffinity(0))
-    cores = min(cores, 16)
     Q, K = (n + 3) // 2, n - 1
     results = {}
     for mode in ("reference-faithful", "all-cores"):
@@ -263,7 +262,7 @@ def _cpu_baseline_uneven(probs, A, cot, budget_s):
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
 
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = len(os.sched_getaffinity(0))
     cost = [_uneven_flops(p) for p in probs]
     total_cost = sum(cost)
     order = sorted(range(len(probs)), key=lambda i: cost[i])  # cheapest first, so the sample is bounded
@@ -420,34 +419,50 @@ def _bench_uneven(args, torch, dev):
 
 def _cpu_baseline_controller(coeffs, cot, budget_s):
     """The reference CPU path at the plain resolution (S = 16385, n = 8,
-    forward op='sum' + VJP) on host cores, one instance per core, over a
-    time-bounded sample of the evaluation set."""
+    forward op='sum' + VJP) on the host cores, over a time-bounded sample of the
+    evaluation set, in two modes: reference-faithful (instances in sequence,
+    the reference's Auto GL-lane rule for the forward, serial VJP) and
+    all-cores (one instance per core); the faster is reported."""
     import concurrent.futures as cf
-
-    import numpy as np
 
     sys.path.insert(0, str(ROOT / "oracle"))
     import oracle as O
     from magnus import controller as C, torch_ops
 
     S, n = C.PLAIN
-    cores = min(len(os.sched_getaffinity(0)), 16)
+    cores = len(os.sched_getaffinity(0))
+    lanes = O.auto_lanes(n, S, cores)
+    # the sample's inputs are drawn on the device before the clock starts
+    k = min(coeffs.shape[0], 4 * cores)
+    A = torch_ops.sample_fourier(coeffs[:k], S).cpu().numpy()
+    G = cot[:k].cpu().numpy()
 
-    def one(i):
-        a = torch_ops.sample_fourier(coeffs[i:i + 1], S)[0].cpu().numpy()
-        c = cot[i].cpu().numpy()
-        O.forward(n, a, 0.0, 1.0, op="sum", integrator="Boole")
-        O.vjp(n, a, c, 0.0, 1.0, op="sum", integrator="Boole")
+    def one(i, ln):
+        a = A[i % k]
+        O.forward(n, a, 0.0, 1.0, op="sum", integrator="Boole", lanes=ln)
+        O.vjp(n, a, G[i % k], 0.0, 1.0, op="sum", integrator="Boole")
 
-    done, t0 = 0, time.perf_counter()
+    results = {}
+    t0 = time.perf_counter()
+    done = 0
+    while time.perf_counter() - t0 < budget_s / 2 and done < coeffs.shape[0]:
+        one(done, lanes)
+        done += 1
+    results["reference-faithful"] = (done / (time.perf_counter() - t0), done, time.perf_counter() - t0, lanes)
+    t0 = time.perf_counter()
+    done = 0
     with cf.ThreadPoolExecutor(cores) as ex:
-        while time.perf_counter() - t0 < budget_s and done + cores <= coeffs.shape[0]:
-            list(ex.map(one, range(done, done + cores)))
+        while time.perf_counter() - t0 < budget_s / 2 and done + cores <= coeffs.shape[0]:
+            list(ex.map(lambda i: one(i, 1), range(done, done + cores)))
             done += cores
-    el = time.perf_counter() - t0
-    return {"value": done / el, "unit": "solves/s", "cores": cores, "kind": "port",
-            "sample": f"{done} plain-resolution (S={S}, n={n}) forward+VJP solves of the evaluation set, one "
-                      f"instance per core, in {el:.1f} s"}
+    results["all-cores"] = (done / (time.perf_counter() - t0), done, time.perf_counter() - t0, cores)
+    best = max(results, key=lambda m: results[m][0])
+    v, done, el, used = results[best]
+    return {"value": v, "unit": "solves/s", "cores": used, "kind": "port",
+            "sample": f"{done} plain-resolution (S={S}, n={n}) forward+VJP solves of the evaluation set "
+                      f"(cycling its first {k} instances) in {el:.1f} s, mode={best}; other mode "
+                      f"{[(m, round(r[0], 3), r[3]) for m, r in results.items() if m != best]} "
+                      f"(mode, solves/s, threads); host cores {cores}"}
 
 
 def _roctx():

@@ -87,12 +87,21 @@ ScratchCache* scratch_cache(hipStream_t s) {
 }
 
 void release_scratch_caches() {
-    std::lock_guard<std::mutex> lock(g_cache_mu);
+    // Snapshot the caches under g_cache_mu and release it before taking any
+    // cache's own lock: a call in flight holds its cache's lock and may take
+    // g_cache_mu (a nested DeviceScratch), so holding both here in the
+    // opposite order could deadlock.  Caches are never destroyed, so the
+    // pointers stay valid.
+    std::vector<std::pair<int, ScratchCache*>> caches;
+    {
+        std::lock_guard<std::mutex> lock(g_cache_mu);
+        for (auto& c : g_caches) caches.push_back({c.first.first, c.second.get()});
+    }
     int cur = 0;
     (void)hipGetDevice(&cur);
-    for (auto& c : g_caches) {
+    for (auto& c : caches) {
         std::lock_guard<std::recursive_mutex> l2(c.second->mu);
-        (void)hipSetDevice(c.first.first);
+        (void)hipSetDevice(c.first);
         (void)hipDeviceSynchronize();
         for (auto& b : c.second->blocks) (void)hipFree(b.p);
         c.second->blocks.clear();
@@ -457,14 +466,16 @@ size_t scan_chunk_bytes(const Plan& pl, uint64_t B) {
     return scratch_round(3 * C * B * pl.m * 8);
 }
 
-double* alloc_scan_chunks(DeviceScratch& scratch, const Plan& pl, uint64_t B, magnus::LmArgs& a) {
+// The chunked scan is part of the arithmetic (its chunk bases are summed in
+// chunk order, and the d = 8 fused levels need it), so a failed allocation is
+// an error, not a silent switch to the sequential scan.
+bool alloc_scan_chunks(DeviceScratch& scratch, const Plan& pl, uint64_t B, magnus::LmArgs& a) {
     a.clen = scan_chunk_len(pl, B);
     a.cscan = nullptr;
-    if (!a.clen) return nullptr;
+    if (!a.clen) return true;
     const uint64_t C = (pl.S - 1 + a.clen - 1) / a.clen;
     a.cscan = scratch.alloc<double>(3 * C * B * pl.m);
-    if (!a.cscan) a.clen = 0;
-    return a.cscan;
+    return a.cscan != nullptr;
 }
 
 size_t wbd_fold_bytes(const Plan& pl, uint64_t B) {  // z, totals, omega1, part, part2, work, scan chunks
@@ -544,7 +555,7 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
         return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
-    alloc_scan_chunks(scratch, pl, B, a);
+    if (!alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
     a.pingpong = pingpong;
     a.Y = Y;
@@ -587,7 +598,7 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     if (!W || !z || !z0 || !totals || !part || !part2 || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
-    alloc_scan_chunks(scratch, pl, B, a);
+    if (!alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
     {
         DeviceScratch fs(stream, arena);
@@ -678,7 +689,7 @@ int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hi
         return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
-    alloc_scan_chunks(scratch, pl, B, a);
+    if (!alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
     a.pingpong = pingpong;
     a.Y = Y;

@@ -14,8 +14,6 @@ namespace magnus {
 enum Integ : int { BOOLE = 0, SIMPSON = 1, TRAPEZOID = 2, RIEMANN = 3 };
 enum Op : int { OP_ONE = 0, OP_MANY = 1, OP_SUM = 2 };
 
-constexpr int kMaxQ = 64;  // GL orders of the table (max_order 64 => n <= 125)
-
 // ---------------------------------------------------------------------------
 // integrator.sum weights: out += f_i * c_i(scale)
 //   boole.hpp:76-106 (block of 4; interior boundaries are added twice, once

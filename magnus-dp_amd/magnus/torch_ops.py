@@ -42,6 +42,22 @@ def carry_shape(n: int, samples: int, dim: int):
     return ((n + 3) // 2, n - 1, samples, dim, dim)
 
 
+def _check_operand(t: torch.Tensor, A: torch.Tensor, shape, what: str) -> torch.Tensor:
+    """A cotangent or carry handed to the C ABI next to A: a float64 tensor on
+    A's device of exactly `shape` (the kernels read it by A's problem shape), as
+    the reference checks validate_matrix_output_shape / validate_vjp_data_shape
+    (src/composer/backends.hpp:124-180).  Returns it contiguous."""
+    if not isinstance(t, torch.Tensor):
+        raise TypeError(f"{what} must be a tensor")
+    if not t.is_cuda or t.device != A.device:
+        raise ValueError(f"{what} must be a GPU tensor on the input's device ({A.device})")
+    if t.dtype != torch.float64:
+        raise TypeError(f"{what} must be float64")
+    if tuple(t.shape) != tuple(shape):
+        raise ValueError(f"{what} has shape {tuple(t.shape)}; expected {tuple(shape)}")
+    return t.contiguous()
+
+
 def forward(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
             matrix_backend: str = "Auto", integrator: str = "Auto", record_vjp: bool = False):
     """Batched matrix Magnus: A (B, S, d, d) float64 on the GPU."""
@@ -80,11 +96,13 @@ def vjp(n: int, A: torch.Tensor, cot: torch.Tensor, carry: torch.Tensor | None =
         tf: float = 1.0, *, op: str = "sum", matrix_backend: str = "Auto", integrator: str = "Auto"):
     _check_input(A, 4, "A")
     A = A.contiguous()
-    cot = cot.contiguous().to(torch.float64)
-    B, S, d, _ = A.shape
+    B, S, d, d2 = A.shape
+    if d != d2:
+        raise ValueError("data must contain square matrices")
     p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    cot = _check_operand(cot, A, (B, n, d, d) if op == "many" else (B, d, d), "cotangent")
+    c = _check_operand(carry, A, (B,) + carry_shape(n, S, d), "carry") if (carry is not None and n > 1) else None
     dA = torch.empty_like(A)
-    c = carry.contiguous() if (carry is not None and n > 1) else None
     capi.check(capi.lib().magnus_vjp_f64(ctypes.byref(p), B, _ptr(A), _ptr(cot), _ptr(c), _ptr(dA), _stream()))
     return dA
 
@@ -112,11 +130,13 @@ def sc_vjp(n: int, a: torch.Tensor, cot: torch.Tensor, carry: torch.Tensor | Non
            tf: float = 1.0, *, op: str = "sum", integrator: str = "Auto"):
     _check_input(a, 3, "a")
     a = a.contiguous()
-    cot = cot.contiguous().to(torch.float64)
-    B, S, _ = a.shape
+    B, S, w = a.shape
+    if w != 3:
+        raise ValueError("data must have shape (samples, 3)")
     p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
+    cot = _check_operand(cot, a, (B, n, 3) if op == "many" else (B, 3), "cotangent")
+    c = _check_operand(carry, a, (B,) + carry_shape(n, S, 2), "carry") if (carry is not None and n > 1) else None
     da = torch.empty_like(a)
-    c = carry.contiguous() if (carry is not None and n > 1) else None
     capi.check(capi.lib().magnus_sc_vjp_f64(ctypes.byref(p), B, _ptr(a), _ptr(cot), _ptr(c), _ptr(da), _stream()))
     return da
 
@@ -142,7 +162,9 @@ def sc_forward_ckpt(n: int, a: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *
     p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
     elems = int(capi.lib().magnus_sc_checkpoint_elems(ctypes.byref(p), B))
     out = torch.empty((B, n, 3) if op == "many" else (B, 3), dtype=a.dtype, device=a.device)
-    ckpt = torch.empty((max(elems, 1),), dtype=a.dtype, device=a.device)
+    if elems == 0:
+        raise ValueError("this problem has no compact checkpoint (it does not take the single-pass path)")
+    ckpt = torch.empty((elems,), dtype=a.dtype, device=a.device)
     capi.check(capi.lib().magnus_sc_forward_ckpt_f64(ctypes.byref(p), B, _ptr(a), _ptr(out), _ptr(ckpt), _stream()))
     return out, ckpt
 
@@ -152,12 +174,20 @@ def sc_vjp_ckpt(n: int, a: torch.Tensor, cot: torch.Tensor, ckpt: torch.Tensor, 
     """sc_vjp from the summaries sc_forward_ckpt returned for the same a."""
     _check_input(a, 3, "a")
     a = a.contiguous()
-    cot = cot.contiguous().to(torch.float64)
-    B, S, _ = a.shape
+    B, S, w = a.shape
+    if w != 3:
+        raise ValueError("data must have shape (samples, 3)")
     p = capi.problem(n, 2, S, t0, tf, op, "SpaceCurve", integrator)
+    cot = _check_operand(cot, a, (B, n, 3) if op == "many" else (B, 3), "cotangent")
     elems = int(capi.lib().magnus_sc_checkpoint_elems(ctypes.byref(p), B))
-    if not ckpt.is_cuda or ckpt.dtype != torch.float64 or ckpt.numel() < elems or not ckpt.is_contiguous():
-        raise ValueError(f"ckpt must be a contiguous float64 GPU tensor of at least {elems} elements")
+    # the chunk layout depends on the batch size, so a checkpoint of another
+    # batch (even a larger one) would be read with the wrong layout
+    if elems == 0:
+        raise ValueError("this problem has no compact checkpoint (it does not take the single-pass path)")
+    if (not ckpt.is_cuda or ckpt.device != a.device or ckpt.dtype != torch.float64 or ckpt.numel() != elems
+            or not ckpt.is_contiguous()):
+        raise ValueError(f"ckpt must be the contiguous float64 GPU tensor of exactly {elems} elements that "
+                         f"sc_forward_ckpt returned for this batch")
     da = torch.empty_like(a)
     capi.check(capi.lib().magnus_sc_vjp_ckpt_f64(ctypes.byref(p), B, _ptr(a), _ptr(cot), _ptr(ckpt), _ptr(da),
                                                  _stream()))

@@ -77,6 +77,33 @@ def test_validation_rules_and_messages():
         capi.problem(4, 2, 9, 0.0, 1.0, op="missing")
 
 
+def test_gl_table_replacement_moves_the_order_bound():
+    """magnus_replace_gl_table (pybind.cpp:102-116, gausslegendre.cpp:74-111):
+    max_order follows the installed table and the order check uses it; numpy's
+    64-order table (the default, bit for bit) is restored afterwards."""
+    import magnus
+
+    try:
+        magnus.replace_gl_table(5)
+        assert magnus.max_order() == 5
+        assert _validate(n=8)[0] == capi.OK  # Q = 5
+        rc, msg = _validate(n=9)  # Q = 6 > 5
+        assert rc == capi.EINVAL and msg.startswith("requested magnus order exceeds GL table")
+        with pytest.raises(ValueError):
+            magnus.replace_gl_table(0)
+        w = np.empty(5)
+        x = np.empty(5)
+        dp = ctypes.POINTER(ctypes.c_double)
+        assert capi.lib().magnus_gl_order(5, w.ctypes.data_as(dp), x.ctypes.data_as(dp)) == 0
+        xr, wr = np.polynomial.legendre.leggauss(5)
+        assert np.array_equal(w, wr / 2) and np.array_equal(x, (xr + 1) / 2)
+        assert capi.lib().magnus_gl_order(6, None, None) == capi.ERANGE
+    finally:
+        magnus.replace_gl_table(64)
+    assert magnus.max_order() == 64
+    assert _validate(n=126)[0] == capi.OK and _validate(n=127)[0] == capi.EINVAL
+
+
 def test_sizes():
     p = capi.problem(4, 3, 9, 0.0, 1.0, op="many")
     assert capi.lib().magnus_carry_elems(ctypes.byref(p)) == 3 * 3 * 9 * 9

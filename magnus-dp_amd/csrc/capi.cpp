@@ -537,6 +537,25 @@ void fill_wbd_fold(const Plan& pl, long long B, magnus::WbdArgs& f) {
     fill_gl(pl, f.w, f.x);
 }
 
+// d = 8 Boole problems of order 2..8: the basis sweep is one single-pass
+// kernel (kernels_wbd8.hip) instead of a loop of level launches.
+bool use_wbd8(const Plan& pl) { return magnus::wbd8_sweep_ok((int)pl.d, pl.integ, (int)pl.n); }
+
+// The basis sweep of the W-basis path: sums, totals and (carry set) the levels.
+int wbd_sweep(const Plan& pl, magnus::LmArgs& a, hipStream_t stream) {
+    if (use_wbd8(pl)) {
+        hipError_t e = magnus::launch_wbd8_sweep(a, stream);
+        if (e != hipSuccess) return device_error(e, "sweep kernel launch");
+        return MAGNUS_OK;
+    }
+    for (int j = 1; j <= (int)pl.n; ++j) {
+        a.level = j;
+        hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
+        if (e != hipSuccess) return device_error(e, "forward kernel launch");
+    }
+    return MAGNUS_OK;
+}
+
 int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* out, hipStream_t stream,
                     Arena* arena) {
     DeviceScratch scratch(stream, arena);
@@ -549,13 +568,14 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
     double* part = scratch.alloc<double>(B * Q * n * m);
     double* part2 = scratch.alloc<double>(B * Q * n * m);
     double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
-    double* pingpong = scratch.alloc<double>(B * 2 * S * m);
-    double* Y = scratch.alloc<double>(B * S * m);
-    if (!z || !z0 || !totals || !part || !part2 || !work || !pingpong || !Y)
+    const bool levels = !use_wbd8(pl);  // the level loop keeps two level arrays and the products
+    double* pingpong = levels ? scratch.alloc<double>(B * 2 * S * m) : nullptr;
+    double* Y = levels ? scratch.alloc<double>(B * S * m) : nullptr;
+    if (!z || !z0 || !totals || !part || !part2 || !work || (levels && (!pingpong || !Y)))
         return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
-    if (!alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    if (levels && !alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
     a.pingpong = pingpong;
     a.Y = Y;
@@ -563,11 +583,7 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
     a.omega1 = z0;
     a.totals = totals;
     a.outputs = 1;
-    for (int j = 1; j <= (int)n; ++j) {
-        a.level = j;
-        hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
-        if (e != hipSuccess) return device_error(e, "forward kernel launch");
-    }
+    if (int rc = wbd_sweep(pl, a, stream)) return rc;
     magnus::WbdArgs f{};
     fill_wbd_fold(pl, B, f);
     f.z = z;
@@ -602,19 +618,15 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     a.in = in;
     {
         DeviceScratch fs(stream, arena);
-        double* Y = fs.alloc<double>(B * S * m);
-        if (!Y) return fail(MAGNUS_ENOMEM, "device allocation failed");
+        double* Y = use_wbd8(pl) ? nullptr : fs.alloc<double>(B * S * m);
+        if (!use_wbd8(pl) && !Y) return fail(MAGNUS_ENOMEM, "device allocation failed");
         a.carry = W;
         a.Y = Y;
         a.partial = z;
         a.omega1 = z0;
         a.totals = totals;
         a.outputs = 1;
-        for (int j = 1; j <= (int)n; ++j) {
-            a.level = j;
-            hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
-            if (e != hipSuccess) return device_error(e, "forward kernel launch");
-        }
+        if (int rc = wbd_sweep(pl, a, stream)) return rc;
     }
     double* zbar = scratch.alloc<double>(B * n * m);
     double* wbar = scratch.alloc<double>(B * n * m);
@@ -683,13 +695,14 @@ int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hi
     double* totals = scratch.alloc<double>(B * n * m);
     double* part = scratch.alloc<double>(B * Q * n * m);
     double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
-    double* pingpong = n >= 2 ? scratch.alloc<double>(B * 2 * S * m) : nullptr;
-    double* Y = n >= 2 ? scratch.alloc<double>(B * S * m) : nullptr;
-    if (!z || !z0 || !totals || !part || !work || (n >= 2 && (!pingpong || !Y)))
+    const bool levels = n >= 2 && !use_wbd8(pl);
+    double* pingpong = levels ? scratch.alloc<double>(B * 2 * S * m) : nullptr;
+    double* Y = levels ? scratch.alloc<double>(B * S * m) : nullptr;
+    if (!z || !z0 || !totals || !part || !work || (levels && (!pingpong || !Y)))
         return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
-    if (!alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    if (levels && !alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
     a.pingpong = pingpong;
     a.Y = Y;
@@ -697,11 +710,7 @@ int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hi
     a.omega1 = z0;
     a.totals = totals;
     a.outputs = 1;
-    for (int j = 1; j <= (int)n; ++j) {
-        a.level = j;
-        hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
-        if (e != hipSuccess) return device_error(e, "forward kernel launch");
-    }
+    if (int rc = wbd_sweep(pl, a, stream)) return rc;
     hipError_t e = hipMemcpy2DAsync(out, n * m * sizeof(double), z0, m * sizeof(double), m * sizeof(double), B,
                                     hipMemcpyDeviceToDevice, stream);
     if (e != hipSuccess) return device_error(e, "order-1 copy");

@@ -75,6 +75,17 @@ hipError_t launch_lm_vjp_init(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_level(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_final(const LmArgs& a, hipStream_t s);
 
+// Single-pass basis sweep of the dense W-basis path at d = 8 under the Boole
+// rule (kernels_wbd8.hip): one wavefront per two instances walks the samples
+// once and writes the sums Z_0 (omega1), Z_1..Z_K (partial rows 1..K, row
+// stride n) and the totals What_1..What_K (totals rows 0..K-1), K = n - 1 <=
+// kWbd8MaxK; with carry != nullptr it also writes the levels W_1..W_K
+// ([B][K][S][64]) for the reverse sweep.  Uses in, carry, partial, omega1,
+// totals, S, jobs (= instances), n, d, dt of LmArgs.
+constexpr int kWbd8MaxK = 7;
+bool wbd8_sweep_ok(int d, int integ, int n);
+hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s);
+
 // Dense W-basis GL fold (kernels_wbd.hip).  The level kernels run one job per
 // instance with shift 0 and weight 1, giving the basis streams W_j = Prefix(A W_{j-1})
 // (W_1 = Prefix(A)), their totals What_j and the sums Z_l = Sum(A W_l) (Z_0 = Sum(A)).

@@ -571,11 +571,12 @@ def test_timing_counts_state_the_level_kernels_algorithmic_work():
 
 
 def test_d8_wbasis_levels_run_fused_and_state_their_work():
-    """A carry-free d = 8 forward takes the W-basis sweep (one job per instance;
-    S = 129 gives four 32-sample chunks): levels 2..n form their products inside
-    the chunk passes (no product launch), level 1 runs the chunk passes over the
-    samples; a fused launch states that its level reads the samples and the
-    previous level once and writes the level.  The results match the oracle."""
+    """A carry-free d = 8 Simpson forward takes the W-basis sweep (one job per
+    instance; S = 129 gives four 32-sample chunks): levels 2..n form their
+    products inside the chunk passes (no product launch), level 1 runs the chunk
+    passes over the samples; a fused launch states that its level reads the
+    samples and the previous level once and writes the level.  The results match
+    the oracle."""
     import torch
 
     from magnus import capi, torch_ops
@@ -586,7 +587,7 @@ def test_d8_wbasis_levels_run_fused_and_state_their_work():
     capi.timing_reset()
     capi.timing_enable(True)
     try:
-        out = torch_ops.forward(n, torch.from_numpy(A).cuda(), 0.0, 1.0, op="many", integrator="Boole")
+        out = torch_ops.forward(n, torch.from_numpy(A).cuda(), 0.0, 1.0, op="many", integrator="Simpson")
         torch.cuda.synchronize()
     finally:
         capi.timing_enable(False)
@@ -603,7 +604,54 @@ def test_d8_wbasis_levels_run_fused_and_state_their_work():
     capi.timing_reset()
     out = out.cpu().numpy()
     for b in range(B):
+        assert rel_fro(out[b], O.forward(n, A[b], 0.0, 1.0, op="many", integrator="Simpson")) <= TOL
+
+
+@pytest.mark.parametrize("n", [2, 4, 8])
+@pytest.mark.parametrize("B", [1, 4, 5])
+def test_d8_boole_runs_the_single_pass_sweep(B, n):
+    """Carry-free d = 8 Boole problems (the controller's) take the single-pass
+    sweep: one launch per call, no level launches; it states that it reads the
+    samples once (and, for the VJP's recompute, writes the n - 1 levels).  The
+    forward (many, sum), the all-orders forward and the VJP match the oracle;
+    an odd batch leaves the last wavefront's partner lanes empty, and each
+    instance's results do not depend on the batch (bit for bit)."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    S, d = 257, 8
+    m, K = d * d, n - 1
+    A = np.stack([fourier_matrices(np.random.default_rng([43, b, n]), S, d, modes=4) for b in range(B)])
+    At = torch.from_numpy(A).cuda()
+    capi.timing_reset()
+    capi.timing_enable(True)
+    try:
+        out = torch_ops.forward(n, At, 0.0, 1.0, op="many", integrator="Boole")
+        torch.cuda.synchronize()
+    finally:
+        capi.timing_enable(False)
+    assert capi.last_kernel() == "wbd"
+    for name in ("lm_fwd_prod", "lm_fwd_fused", "lm_fwd_scan"):
+        assert capi.timing_read(name)[0] == 0, name
+    launches, _ = capi.timing_read("wbd8_sweep")
+    by, fl = capi.timing_counts("wbd8_sweep")
+    assert launches == 1 and by == 8 * B * S * m and fl == B * S * (K * (2 * d ** 3 + 8 * m) + 8 * m)
+    capi.timing_reset()
+    cot = np.stack([np.random.default_rng([43, b, n, 1]).standard_normal((d, d)) for b in range(B)])
+    out_sum = torch_ops.forward(n, At, 0.0, 1.0, op="sum", integrator="Boole").cpu().numpy()
+    orders = torch_ops.forward_orders(n, At, 0.0, 1.0, integrator="Boole").cpu().numpy()
+    dA = torch_ops.vjp(n, At, torch.from_numpy(cot).cuda(), None, 0.0, 1.0, op="sum", integrator="Boole")
+    dA = dA.cpu().numpy()
+    out = out.cpu().numpy()
+    for b in range(B):
         assert rel_fro(out[b], O.forward(n, A[b], 0.0, 1.0, op="many", integrator="Boole")) <= TOL
+        assert rel_fro(out_sum[b], O.forward(n, A[b], 0.0, 1.0, op="sum", integrator="Boole")) <= TOL
+        for k in range(1, n + 1):
+            assert rel_fro(orders[b, k - 1], O.forward(k, A[b], 0.0, 1.0, op="sum", integrator="Boole")) <= TOL
+        assert rel_fro(dA[b], O.vjp(n, A[b], cot[b], 0.0, 1.0, op="sum", integrator="Boole")) <= TOL
+    one = torch_ops.forward(n, At[B - 1:], 0.0, 1.0, op="many", integrator="Boole").cpu().numpy()
+    assert np.array_equal(one[0], out[B - 1])
 
 
 def test_batched_matrix_api_matches_per_instance():

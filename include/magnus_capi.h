@@ -159,6 +159,18 @@ int magnus_sc_forward_ckpt_f64(const magnus_problem* p, uint64_t count, const do
 int magnus_sc_vjp_ckpt_f64(const magnus_problem* p, uint64_t count, const double* in3, const double* cot3,
                            const double* ckpt, double* dA3, void* stream);
 
+/* ---- SpaceCurve VJP that allocates nothing.  No counterpart in the reference:
+ *      for callers that cannot allocate device memory (code under HIP stream
+ *      capture, tight serving loops).  Same arguments, kernels and results as
+ *      magnus_sc_vjp_f64 with carry == NULL on the single-pass path (su(2)
+ *      input, Boole/Simpson/Riemann, 2 <= n <= 8; other problems return
+ *      EINVAL).  Its scratch is a fixed 128 MiB device array that each GPU the
+ *      library runs on holds until the process ends (the 4096-trajectory,
+ *      16385-sample, n = 6 batch needs 120 MB); a batch needing more returns
+ *      ERANGE.  Calls that overlap on one GPU share that array and race. ---- */
+int magnus_sc_vjp_noalloc_f64(const magnus_problem* p, uint64_t count, const double* in3, const double* cot3,
+                              double* dA3, void* stream);
+
 /* ---- sampling A(t) on the device, on the reference's grid
  *      t_i = np.linspace(t0, tf, samples) (replaces evaluating a Python callable
  *      there, src/api/magnus/__init__.py:58-85, for these two families).

@@ -379,43 +379,87 @@ uint64_t wb_summary_elems(const Plan& pl, uint64_t count) {
     return count * C * (2 * (pl.n - 1) + 1) * 4;
 }
 
-// Single-pass path.  `ckpt` (optional) holds the chunk summaries: the forward
-// writes them there instead of scratch; a VJP given them skips the chunk pass.
-int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, double* out, double* dA,
-           double* ckpt, hipStream_t stream) {
-    // the chunk passes pack element offsets of the input into 48 bits
-    if (count > 0 && pl.S > ((uint64_t)1 << 47) / 3 / count)
-        return fail(MAGNUS_EINVAL, "SpaceCurve batch exceeds 2^47 input elements");
-    DeviceScratch scratch(stream);
-    const bool vjp = dA != nullptr;
-    magnus::WbArgs a{};
+// Chunk layout of the single-pass path for `count` instances.
+void wb_layout(const Plan& pl, uint64_t count, magnus::WbArgs& a) {
     a.B = (long long)count;
     a.S = (long long)pl.S;
     a.LC = wb_chunk_len(pl, count);
     a.C = (int)((pl.S - 1 + a.LC - 1) / a.LC);
     a.K = (int)pl.n - 1;
+}
+
+// Scratch of the single-pass VJP without a checkpoint: chunk summaries, chunk
+// cotangents, chunk-start stream values and boundary halves, each 256-byte aligned.
+size_t wb_vjp_scratch_bytes(const Plan& pl, uint64_t count) {
+    magnus::WbArgs a{};
+    wb_layout(pl, count, a);
+    const size_t BC = (size_t)count * a.C, K = (size_t)a.K;
+    return 2 * scratch_round(BC * (2 * K + 1) * 4 * 8) + scratch_round(BC * K * 4 * 8) + scratch_round(BC * 6 * 8);
+}
+
+// Single-pass path on given buffers (fsum always; src, bases, edge for the VJP).
+int run_wb_on(const Plan& pl, uint64_t count, const double* in, const double* cot, double* out, double* dA,
+              bool summaries_given, double* fsum, double* src, double* bases, double* edge, hipStream_t stream) {
+    // the chunk passes pack element offsets of the input into 48 bits
+    if (count > 0 && pl.S > ((uint64_t)1 << 47) / 3 / count)
+        return fail(MAGNUS_EINVAL, "SpaceCurve batch exceeds 2^47 input elements");
+    const bool vjp = dA != nullptr;
+    magnus::WbArgs a{};
+    wb_layout(pl, count, a);
     a.n_out = (int)pl.n_out;
     a.op = pl.op;
     a.Q = (int)pl.Q;
     a.dt = pl.dt;
     fill_gl(pl, a.w, a.x);
-    const size_t BC = (size_t)count * a.C, K = (size_t)a.K;
     a.in = in;
     a.cot = cot;
     a.out = out;
     a.dA = dA;
-    a.fsum = ckpt ? ckpt : scratch.alloc<double>(BC * (2 * K + 1) * 4);
-    if (!a.fsum) return fail(MAGNUS_ENOMEM, "device allocation failed");
-    if (vjp) {
-        a.src = scratch.alloc<double>(BC * (2 * K + 1) * 4);
-        a.bases = scratch.alloc<double>(BC * K * 4);
-        a.edge = scratch.alloc<double>(BC * 6);
-        if (!a.src || !a.bases || !a.edge) return fail(MAGNUS_ENOMEM, "device allocation failed");
-    }
-    hipError_t e = magnus::launch_wb(pl.integ, a, vjp, /*summaries_given=*/vjp && ckpt != nullptr, stream);
+    a.fsum = fsum;
+    a.src = src;
+    a.bases = bases;
+    a.edge = edge;
+    hipError_t e = magnus::launch_wb(pl.integ, a, vjp, vjp && summaries_given, stream);
     if (e != hipSuccess) return device_error(e, "wb kernel launch");
     g_kernel = "wb";
     return MAGNUS_OK;
+}
+
+int run_wb_vjp_on(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA, char* scratch,
+                  size_t bytes, hipStream_t stream) {
+    magnus::WbArgs a{};
+    wb_layout(pl, count, a);
+    const size_t BC = (size_t)count * a.C, K = (size_t)a.K;
+    if (bytes < wb_vjp_scratch_bytes(pl, count)) return fail(MAGNUS_ENOMEM, "scratch too small for this batch");
+    double* fsum = reinterpret_cast<double*>(scratch);
+    scratch += scratch_round(BC * (2 * K + 1) * 4 * 8);
+    double* src = reinterpret_cast<double*>(scratch);
+    scratch += scratch_round(BC * (2 * K + 1) * 4 * 8);
+    double* bases = reinterpret_cast<double*>(scratch);
+    scratch += scratch_round(BC * K * 4 * 8);
+    double* edge = reinterpret_cast<double*>(scratch);
+    return run_wb_on(pl, count, in, cot, nullptr, dA, false, fsum, src, bases, edge, stream);
+}
+
+// Single-pass path.  `ckpt` (optional) holds the chunk summaries: the forward
+// writes them there instead of scratch; a VJP given them skips the chunk pass.
+int run_wb(const Plan& pl, uint64_t count, const double* in, const double* cot, double* out, double* dA,
+           double* ckpt, hipStream_t stream) {
+    DeviceScratch scratch(stream);
+    const bool vjp = dA != nullptr;
+    magnus::WbArgs a{};
+    wb_layout(pl, count, a);
+    const size_t BC = (size_t)count * a.C, K = (size_t)a.K;
+    double* fsum = ckpt ? ckpt : scratch.alloc<double>(BC * (2 * K + 1) * 4);
+    if (!fsum) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    double *src = nullptr, *bases = nullptr, *edge = nullptr;
+    if (vjp) {
+        src = scratch.alloc<double>(BC * (2 * K + 1) * 4);
+        bases = scratch.alloc<double>(BC * K * 4);
+        edge = scratch.alloc<double>(BC * 6);
+        if (!src || !bases || !edge) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    }
+    return run_wb_on(pl, count, in, cot, out, dA, ckpt != nullptr, fsum, src, bases, edge, stream);
 }
 
 void fill_lm(const Plan& pl, long long jobs, magnus::LmArgs& a) {

@@ -122,6 +122,15 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
 int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, const double* carry, double* dA,
             hipStream_t stream, Arena* arena = nullptr);
 
+// Single-pass SpaceCurve path (su(2) input, block integrators, 2 <= n <= 8, no
+// reference-layout carry): whether a problem takes it, the scratch bytes of its
+// VJP without a checkpoint for `count` instances, and that VJP on caller-given
+// scratch (no allocation call).
+bool use_wb(const Plan& pl, bool has_carry);
+size_t wb_vjp_scratch_bytes(const Plan& pl, uint64_t count);
+int run_wb_vjp_on(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA, char* scratch,
+                  size_t bytes, hipStream_t stream);
+
 // Peak arena bytes of run_forward (outputs, no carry) and of run_vjp without a
 // carry (it recomputes the carry), for `count` instances.
 size_t forward_scratch_bytes(const Plan& pl, uint64_t count);

@@ -159,6 +159,21 @@ int magnus_sc_forward_ckpt_f64(const magnus_problem* p, uint64_t count, const do
 int magnus_sc_vjp_ckpt_f64(const magnus_problem* p, uint64_t count, const double* in3, const double* cot3,
                            const double* ckpt, double* dA3, void* stream);
 
+/* ---- kept basis state of the dense W-basis path (no counterpart in the
+ *      reference, whose saved state is the full carry above).  Matrix problems
+ *      with op "many" or "sum" and n >= 2 (op "one" and n = 1 return 0 /
+ *      EINVAL) solve through one basis sweep -- levels W_1..W_{n-1}, their
+ *      totals and the sums -- and a GL fold.  The forward can keep that
+ *      state, (n-1)*S*d*d + (2n+1)*d*d doubles per instance against the
+ *      carry's Q*(n-1)*S*d*d, and a VJP given it skips its own sweep; its dA is
+ *      the same as magnus_vjp_f64 with carry == NULL (same kernels on the same
+ *      state).  The state is valid only with the same problem, count and input. ---- */
+uint64_t magnus_basis_state_elems(const magnus_problem* p, uint64_t count);
+int magnus_forward_basis_f64(const magnus_problem* p, uint64_t count, const double* in, double* out, double* state,
+                             void* stream);
+int magnus_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot,
+                         const double* state, double* dA, void* stream);
+
 /* ---- SpaceCurve VJP that allocates nothing.  No counterpart in the reference:
  *      for callers that cannot allocate device memory (code under HIP stream
  *      capture, tight serving loops).  Same arguments, kernels and results as

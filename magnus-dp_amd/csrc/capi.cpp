@@ -600,39 +600,75 @@ int wbd_sweep(const Plan& pl, magnus::LmArgs& a, hipStream_t stream) {
     return MAGNUS_OK;
 }
 
-int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* out, hipStream_t stream,
+// The W-basis state of `count` instances, as one buffer: the levels W_1..W_K
+// [B][K][S][m], then the sums [B][n][m] (rows 1..K: Z_1..Z_K), Z_0 [B][m] and
+// the totals [B][n][m] (rows 0..K-1).  It is what the reverse sweep needs
+// besides the input and the cotangent.
+struct WbdState {
+    double *W, *z, *z0, *totals;
+};
+uint64_t wbd_state_elems(const Plan& pl, uint64_t count) {
+    const uint64_t n = pl.n, m = pl.m;
+    return count * ((n - 1) * pl.S * m + (2 * n + 1) * m);
+}
+WbdState wbd_state_view(const Plan& pl, uint64_t count, double* base) {
+    const uint64_t B = count, n = pl.n, m = pl.m;
+    WbdState st;
+    st.W = base;
+    st.z = st.W + B * (n - 1) * pl.S * m;
+    st.z0 = st.z + B * n * m;
+    st.totals = st.z0 + B * m;
+    return st;
+}
+
+// The basis sweep into a state: levels (when st.W is set), sums and totals.
+int wbd_sweep_state(const Plan& pl, uint64_t count, const double* in, const WbdState& st, hipStream_t stream,
                     Arena* arena) {
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
-    const size_t S = pl.S, m = pl.m, n = pl.n;
-    const size_t Q = pl.Q;
-    double* z = scratch.alloc<double>(B * n * m);
-    double* z0 = scratch.alloc<double>(B * m);
-    double* totals = scratch.alloc<double>(B * n * m);
-    double* part = scratch.alloc<double>(B * Q * n * m);
-    double* part2 = scratch.alloc<double>(B * Q * n * m);
-    double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
-    const bool levels = !use_wbd8(pl);  // the level loop keeps two level arrays and the products
-    double* pingpong = levels ? scratch.alloc<double>(B * 2 * S * m) : nullptr;
+    const size_t S = pl.S, m = pl.m;
+    const bool levels = !use_wbd8(pl);  // the level loop needs its products (and, without W, two level arrays)
     double* Y = levels ? scratch.alloc<double>(B * S * m) : nullptr;
-    if (!z || !z0 || !totals || !part || !part2 || !work || (levels && (!pingpong || !Y)))
-        return fail(MAGNUS_ENOMEM, "device allocation failed");
+    double* pingpong = (levels && !st.W) ? scratch.alloc<double>(B * 2 * S * m) : nullptr;
+    if (levels && (!Y || (!st.W && !pingpong))) return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::LmArgs a{};
     fill_wbd_lm(pl, B, a);
     if (levels && !alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
+    a.carry = st.W;
     a.pingpong = pingpong;
     a.Y = Y;
-    a.partial = z;
-    a.omega1 = z0;
-    a.totals = totals;
+    a.partial = st.z;
+    a.omega1 = st.z0;
+    a.totals = st.totals;
     a.outputs = 1;
-    if (int rc = wbd_sweep(pl, a, stream)) return rc;
+    return wbd_sweep(pl, a, stream);
+}
+
+// forward (op sum / many); `state` (optional) keeps the basis state for a later VJP
+int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* out, hipStream_t stream, Arena* arena,
+                    double* state = nullptr) {
+    DeviceScratch scratch(stream, arena);
+    const long long B = (long long)count;
+    const size_t m = pl.m, n = pl.n;
+    const size_t Q = pl.Q;
+    WbdState st{};
+    if (state) {
+        st = wbd_state_view(pl, count, state);
+    } else {
+        st.z = scratch.alloc<double>(B * n * m);
+        st.z0 = scratch.alloc<double>(B * m);
+        st.totals = scratch.alloc<double>(B * n * m);
+    }
+    double* part = scratch.alloc<double>(B * Q * n * m);
+    double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
+    if (!st.z || !st.z0 || !st.totals || !part || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    if (int rc = wbd_sweep_state(pl, count, in, st, stream, arena)) return rc;
     magnus::WbdArgs f{};
     fill_wbd_fold(pl, B, f);
-    f.z = z;
-    f.z0 = z0;
-    f.totals = totals;
+    f.z = st.z;
+    f.z0 = st.z0;
+    f.totals = st.totals;
     f.out = out;
     f.part = part;
     f.work = work;
@@ -642,46 +678,40 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
     return MAGNUS_OK;
 }
 
+// VJP; `state` (optional) is the basis state a forward kept for the same input,
+// otherwise the sweep is rerun into scratch (the reference's VJP::none)
 int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA, hipStream_t stream,
-                Arena* arena) {
+                Arena* arena, const double* state = nullptr) {
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const size_t S = pl.S, m = pl.m, n = pl.n;
     const size_t Q = pl.Q;
-    double* W = scratch.alloc<double>(B * (n - 1) * S * m);
-    double* z = scratch.alloc<double>(B * n * m);
-    double* z0 = scratch.alloc<double>(B * m);
-    double* totals = scratch.alloc<double>(B * n * m);
+    WbdState st{};
+    if (state) {
+        st = wbd_state_view(pl, count, const_cast<double*>(state));  // read only below
+    } else {
+        st.W = scratch.alloc<double>(B * (n - 1) * S * m);
+        st.z = scratch.alloc<double>(B * n * m);
+        st.z0 = scratch.alloc<double>(B * m);
+        st.totals = scratch.alloc<double>(B * n * m);
+        if (!st.W || !st.z || !st.z0 || !st.totals) return fail(MAGNUS_ENOMEM, "device allocation failed");
+        if (int rc = wbd_sweep_state(pl, count, in, st, stream, arena)) return rc;
+    }
     double* part = scratch.alloc<double>(B * Q * n * m);
     double* part2 = scratch.alloc<double>(B * Q * n * m);
     double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
-    if (!W || !z || !z0 || !totals || !part || !part2 || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
-    magnus::LmArgs a{};
-    fill_wbd_lm(pl, B, a);
-    if (!alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
-    a.in = in;
-    {
-        DeviceScratch fs(stream, arena);
-        double* Y = use_wbd8(pl) ? nullptr : fs.alloc<double>(B * S * m);
-        if (!use_wbd8(pl) && !Y) return fail(MAGNUS_ENOMEM, "device allocation failed");
-        a.carry = W;
-        a.Y = Y;
-        a.partial = z;
-        a.omega1 = z0;
-        a.totals = totals;
-        a.outputs = 1;
-        if (int rc = wbd_sweep(pl, a, stream)) return rc;
-    }
+    if (!part || !part2 || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    const bool rev8 = use_wbd8(pl);  // the single-pass reverse sweep writes dA itself
     double* zbar = scratch.alloc<double>(B * n * m);
     double* wbar = scratch.alloc<double>(B * n * m);
-    double* gbuf = scratch.alloc<double>(B * 2 * S * m);
-    double* dAjob = scratch.alloc<double>(B * S * m);
-    if (!zbar || !wbar || !gbuf || !dAjob) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    double* gbuf = rev8 ? nullptr : scratch.alloc<double>(B * 2 * S * m);
+    double* dAjob = rev8 ? nullptr : scratch.alloc<double>(B * S * m);
+    if (!zbar || !wbar || (!rev8 && (!gbuf || !dAjob))) return fail(MAGNUS_ENOMEM, "device allocation failed");
     magnus::WbdArgs f{};
     fill_wbd_fold(pl, B, f);
-    f.z = z;
-    f.z0 = z0;
-    f.totals = totals;
+    f.z = st.z;
+    f.z0 = st.z0;
+    f.totals = st.totals;
     f.cot = cot;
     f.zbar = zbar;
     f.wbar = wbar;
@@ -690,10 +720,27 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     f.work = work;
     hipError_t e = magnus::launch_wbd_fold_vjp(f, stream);
     if (e != hipSuccess) return device_error(e, "fold kernel launch");
-    a.Y = nullptr;
-    a.partial = nullptr;
-    a.omega1 = nullptr;
-    a.totals = nullptr;
+    if (rev8) {
+        magnus::Wbd8RevArgs r{};
+        r.in = in;
+        r.W = st.W;
+        r.zbar = zbar;
+        r.wbar = wbar;
+        r.dA = dA;
+        r.B = B;
+        r.S = (long long)S;
+        r.n = (int)n;
+        r.dt = pl.dt;
+        e = magnus::launch_wbd8_rev(r, stream);
+        if (e != hipSuccess) return device_error(e, "reverse sweep launch");
+        g_kernel = "wbd";
+        return MAGNUS_OK;
+    }
+    magnus::LmArgs a{};
+    fill_wbd_lm(pl, B, a);
+    if (!alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    a.in = in;
+    a.carry = st.W;
     a.cot = zbar;
     a.gbuf = gbuf;
     a.dAjob = dAjob;
@@ -1016,6 +1063,37 @@ int magnus_forward_orders_f64(const magnus_problem* p, uint64_t count, const dou
     if (count == 0) return MAGNUS_OK;
     if (!in || !out) return fail(MAGNUS_EINVAL, "null argument");
     return run_orders(pl, count, in, out, (hipStream_t)stream);
+}
+
+uint64_t magnus_basis_state_elems(const magnus_problem* p, uint64_t count) {
+    Plan pl;
+    if (make_plan(p, false, true, &pl)) return 0;
+    if (pl.sc || !use_wbd(pl)) return 0;
+    return wbd_state_elems(pl, count);
+}
+
+int magnus_forward_basis_f64(const magnus_problem* p, uint64_t count, const double* in, double* out, double* state,
+                             void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, true, &pl)) return rc;
+    if (pl.sc || !use_wbd(pl))
+        return fail(MAGNUS_EINVAL, "basis-state forward needs the dense W-basis path (matrix problem, op many or sum, "
+                                   "n >= 2)");
+    if (count == 0) return MAGNUS_OK;
+    if (!in || !out || !state) return fail(MAGNUS_EINVAL, "null argument");
+    return run_wbd_forward(pl, count, in, out, (hipStream_t)stream, nullptr, state);
+}
+
+int magnus_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot,
+                         const double* state, double* dA, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, false, &pl)) return rc;
+    if (pl.sc || !use_wbd(pl))
+        return fail(MAGNUS_EINVAL, "basis-state VJP needs the dense W-basis path (matrix problem, op many or sum, "
+                                   "n >= 2)");
+    if (count == 0) return MAGNUS_OK;
+    if (!in || !cot || !state || !dA) return fail(MAGNUS_EINVAL, "null argument");
+    return run_wbd_vjp(pl, count, in, cot, dA, (hipStream_t)stream, nullptr, state);
 }
 
 uint64_t magnus_sc_checkpoint_elems(const magnus_problem* p, uint64_t count) {

@@ -86,6 +86,23 @@ constexpr int kWbd8MaxK = 7;
 bool wbd8_sweep_ok(int d, int integ, int n);
 hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s);
 
+// Reverse single-pass sweep of the same problems (kernels_wbd8.hip): from the
+// kept levels W_1..W_K and the fold's cotangents of the sums (zbar rows 0..K)
+// and totals (wbar rows 0..K-1), one wavefront per two instances walks the
+// samples from the last to the first and writes the whole dA (no per-level
+// arrays).
+struct Wbd8RevArgs {
+    const double* in;    // [B][S][64]
+    const double* W;     // [B][K][S][64]
+    const double* zbar;  // [B][n][64]
+    const double* wbar;  // [B][n][64]
+    double* dA;          // [B][S][64]
+    long long B, S;
+    int n;
+    double dt;
+};
+hipError_t launch_wbd8_rev(const Wbd8RevArgs& a, hipStream_t s);
+
 // Dense W-basis GL fold (kernels_wbd.hip).  The level kernels run one job per
 // instance with shift 0 and weight 1, giving the basis streams W_j = Prefix(A W_{j-1})
 // (W_1 = Prefix(A)), their totals What_j and the sums Z_l = Sum(A W_l) (Z_0 = Sum(A)).

@@ -5,12 +5,14 @@
 // K = n - 1), their totals What_j = W_j[S-1] and the sums Z_0 = Sum(A),
 // Z_l = Sum(A W_l) (l = 1..K) are what the GL fold (kernels_wbd.hip) needs;
 // the level kernels build them one level at a time, each level a pass over
-// HBM.  Here one wavefront walks the samples of two instances once and keeps
-// every level's state in registers: per sample it forms the K products
-// A_i W_j[i] in sequence (each level's prefix at sample i needs that level's
-// product at i, src/magnus.hpp:241-251) and steps every level's prefix and
-// sum.  A sample is read once; the levels are written only when a VJP needs
-// them (LEVELS).
+// HBM.  Here one workgroup of K wavefronts walks the samples of two instances
+// once, one wavefront per level: level j's prefix at sample i needs that
+// level's product A_i W_{j-1}[i] (src/magnus.hpp:241-251), so the waves run as
+// a pipeline of Boole blocks -- at step t wave w takes block t - w, its input
+// block formed by wave w - 1 in the step before and handed over in LDS -- and
+// every level's state stays in its wave's registers.  A sample comes from HBM
+// once (the later waves find it in the caches); the levels are written only
+// when a VJP needs them (LEVELS).
 //
 // Products on the matrix cores.  v_mfma_f64_16x16x4f64 multiplies a 16 x 4
 // slice by a 4 x 16 slice (lane l holds A[l%16][l/16] and B[l/16][l%16];
@@ -21,8 +23,8 @@
 // the instances never mix (the off-diagonal blocks are discarded).  Lane l
 // owns column c = l%8 of instance l/8%2, rows g = l/16 and g + 4: exactly the
 // two W entries it feeds as the B operand (k = g and k = g + 4) and the two
-// product entries it gets back, so a level's result feeds the next level's
-// product with no data movement between lanes.  Half of each instruction's
+// product entries it gets back, so a level block handed from one wave to the
+// next goes from each lane to the same lane.  Half of each instruction's
 // work is the discarded blocks, so the sweep tops out at half the FP64 matrix
 // peak; it reads 1 KB per sample pair where the level kernels move about five
 // level-sized arrays per level.
@@ -84,147 +86,496 @@ __device__ __forceinline__ double brule(Acc& s, double f, const BooleW& c) {
     }
 }
 
-// the four values a lane reads of one sample: its A operand entries A[c][g],
-// A[c][g+4] and its own entries A[g][c], A[g+4][c]
-struct Samp {
-    double o0, o1, y0, y1;
-};
+// ---------------------------------------------------------------------------
+// Sample ring.  The samples of a workgroup's instance pair reach LDS through
+// direct global->LDS loads (global_load_lds_dwordx4: lane l's 16 bytes land
+// at the instruction's LDS base + 16 l) issued by one extra wavefront, the
+// loader, kRingAhead Boole blocks ahead of the first level's wave; a block
+// (4 samples of both instances, 4 KB) is four instructions.  The loader is the
+// only wave with vector memory loads in flight, so it can wait for a block
+// with a vmcnt that counts only those loads (the blocks issued after it), and
+// the workgroup barrier that follows publishes the block to every wave.  The
+// waves of the pipeline read their blocks from the ring up to K - 1 steps
+// later, so the ring holds K + kRingAhead blocks.
+// ---------------------------------------------------------------------------
+constexpr int kRingAhead = 6;
 
-__device__ __forceinline__ Samp load_samp(const double* __restrict__ p, int c, int g) {
-    Samp s;
-    s.o0 = p[c * 8 + g];
-    s.o1 = p[c * 8 + 4 + g];
-    s.y0 = p[g * 8 + c];
-    s.y1 = p[(g + 4) * 8 + c];
-    return s;
+template <int N>
+__device__ __forceinline__ void wait_vm() {  // vmcnt <= N (expcnt, lgkmcnt unconstrained)
+    static_assert(N < 64, "vmcnt has 6 bits");
+    __builtin_amdgcn_s_waitcnt((N & 15) | 0x0F70 | ((N >> 4) << 14));
 }
 
-template <int K, bool LEVELS>
-struct Sweep {
-    Acc st[K][2];      // level j+1's prefix state, entries (g, c) and (g + 4, c)
-    double z[K + 1][2];  // Z_0..Z_K
-    double* W;         // level output base of the instance (LEVELS), W_j at W + (j-1) S 64
-    long long S;
-    int c, g, inst;
-    bool live;
+// vmcnt <= 4 * blocks: the loader's loads of the `blocks` later blocks may still be in flight
+__device__ __forceinline__ void wait_blocks(int blocks) {
+    switch (blocks) {
+        case 0: wait_vm<0>(); break;
+        case 1: wait_vm<4>(); break;
+        case 2: wait_vm<8>(); break;
+        case 3: wait_vm<12>(); break;
+        case 4: wait_vm<16>(); break;
+        case 5: wait_vm<20>(); break;
+        default: wait_vm<24>(); break;
+    }
+    static_assert(kRingAhead - 1 <= 6, "wait_blocks covers kRingAhead - 1 later blocks");
+}
 
-    // sample i at block position P; cz its Boole sum weight
-    template <int P>
-    __device__ __forceinline__ void step(const Samp& s, long long i, double cz, const BooleW& bw) {
-        double y0 = s.y0, y1 = s.y1;
-        z[0][0] = fma(cz, y0, z[0][0]);
-        z[0][1] = fma(cz, y1, z[0][1]);
-        double w0 = brule<P>(st[0][0], y0, bw);
-        double w1 = brule<P>(st[0][1], y1, bw);
-        if constexpr (LEVELS) store(0, i, w0, w1);
+// Workgroup barrier of the pipelines: this wave's LDS writes complete, then
+// s_barrier.  __syncthreads() would also wait for every vector memory access
+// in flight (its release fence), which would drain the loader's ring loads
+// at every step; the "memory" clobber keeps the compiler from moving memory
+// accesses across the barrier.
+__device__ __forceinline__ void step_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+// one ring slot: [2 instances][4 samples x 64 entries]
+using RingSlot = double[2][4 * kM];
+
+// the loader's loads of Boole block blk (samples 4 blk + 1 .. 4 blk + 4) of
+// instances A0 and A1 into `slot` (every lane of the loader calls)
+__device__ __forceinline__ void ring_issue(RingSlot& slot, const double* A0, const double* A1, long long blk, int lane) {
 #pragma unroll
-        for (int j = 1; j <= K; ++j) {
-            dbl4 d = __builtin_amdgcn_mfma_f64_16x16x4f64(s.o0, w0, dbl4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
-            d = __builtin_amdgcn_mfma_f64_16x16x4f64(s.o1, w1, d, 0, 0, 0);
-            y0 = inst ? d[2] : d[0];
-            y1 = inst ? d[3] : d[1];
-            z[j][0] = fma(cz, y0, z[j][0]);
-            z[j][1] = fma(cz, y1, z[j][1]);
-            if (j < K) {
-                w0 = brule<P>(st[j][0], y0, bw);
-                w1 = brule<P>(st[j][1], y1, bw);
-                if constexpr (LEVELS) store(j, i, w0, w1);
-            }
-        }
+    for (int k = 0; k < 4; ++k) {
+        const double* src = (k < 2 ? A0 : A1) + (4 * blk + 1) * (long long)kM + (k & 1) * 128 + 2 * lane;
+        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&slot[k >> 1][(k & 1) * 128],
+                                         16, 0, 0);
     }
+}
 
-    __device__ __forceinline__ void store(int j, long long i, double w0, double w1) {
-        if (!live) return;
-        double* p = W + ((long long)j * S + i) * kM;
-        p[g * 8 + c] = w0;
-        p[(g + 4) * 8 + c] = w1;
-    }
+// Per-instance-pair context shared by the waves of a workgroup.
+struct Pair {
+    long long b, bl, S, nb;  // this lane's instance (bl: the live one it reads), samples, blocks
+    int c, g, inst;          // lane = column c of instance inst, rows g and g + 4
+    bool live;               // the lane's instance exists (an odd batch leaves a partner missing)
 };
 
-// one wavefront per pair of instances (2w, 2w + 1); a missing partner's lanes
-// read the live instance and store nothing
-template <int K, bool LEVELS>
-__global__ void __launch_bounds__(64) wbd8_sweep(LmArgs a) {
-    const int lane = threadIdx.x;
-    const int c = lane & 7, inst = (lane >> 3) & 1, g = lane >> 4;
-    const long long b = 2LL * blockIdx.x + inst;
-    const bool live = b < a.jobs;
-    const long long bl = live ? b : b - 1;
-    const long long S = a.S;
-    const double* __restrict__ A = a.in + bl * S * kM;
+// One wave of the forward pipeline.  FIRST: the wave of level 1's prefix
+// (W_1 = Prefix(A), Z_0 = Sum(A)), which also forms Y_2 = A W_1; otherwise it
+// forms Y_{j+1} = A W_j from the block of W_j the previous wave handed over.
+// MAKES: it steps level j + 1's prefix (j < K) and hands that block on.  The
+// role is a template parameter so that a step is straight-line code: the
+// four samples' ring reads and products are independent and overlap; only the
+// prefix recurrences run sample after sample.
+template <int K, bool LEVELS, bool FIRST, bool MAKES>
+__device__ __forceinline__ void sweep_wave(const LmArgs& a, const Pair& q, int w, const RingSlot* ring,
+                                           double (*hand)[4][2 * kM]) {
+    constexpr int R = K + kRingAhead;
+    const long long S = q.S, nb = q.nb;
+    const int j = w + 1;
+    const int c = q.c, g = q.g, inst = q.inst;
+    const int o0 = g * 8 + c, o1 = (g + 4) * 8 + c;  // own entries (g, c), (g + 4, c)
+    const int h0 = inst * kM + o0, h1 = inst * kM + o1;
+    const double* __restrict__ A = a.in + q.bl * S * kM;
+    double* __restrict__ W = LEVELS ? a.carry + q.bl * (long long)K * S * kM : nullptr;
     const BooleW bw(a.dt);
-
-    Sweep<K, LEVELS> sw;
-    sw.S = S;
-    sw.c = c;
-    sw.g = g;
-    sw.inst = inst;
-    sw.live = live;
-    sw.W = LEVELS ? a.carry + bl * (long long)K * S * kM : nullptr;
+    Acc s1[2], sn[2];           // prefix states of W_1 (FIRST) and W_{j+1} (MAKES)
+    double z1[2] = {}, zn[2] = {};  // Z_0 (FIRST) and Z_j
 
     // sample 0: every prefix is 0 there, so the products of levels 2..K+1 are 0
-    {
-        const Samp s0 = load_samp(A, c, g);
-        sw.z[0][0] = bw.b14 * s0.y0;
-        sw.z[0][1] = bw.b14 * s0.y1;
-        sw.st[0][0] = Acc{0.0, s0.y0, 0.0};
-        sw.st[0][1] = Acc{0.0, s0.y1, 0.0};
+    if constexpr (FIRST) {
+        const double y0 = A[o0], y1 = A[o1];
+        s1[0] = Acc{0.0, y0, 0.0};
+        s1[1] = Acc{0.0, y1, 0.0};
+        z1[0] = bw.b14 * y0;
+        z1[1] = bw.b14 * y1;
+    }
+    sn[0] = sn[1] = Acc{0.0, 0.0, 0.0};
+    if (LEVELS && q.live) {
+        if (FIRST) W[o0] = W[o1] = 0.0;
+        if (MAKES) W[(long long)j * S * kM + o0] = W[(long long)j * S * kM + o1] = 0.0;
+    }
+    double win[4][2] = {};
+    for (long long t = 0; t < nb + K - 1; ++t) {
+        const long long blk = t - w;
+        const bool act = blk >= 0 && blk < nb;
+        if (!FIRST && act) {  // the previous wave formed this block in the previous step
 #pragma unroll
-        for (int j = 1; j <= K; ++j) sw.z[j][0] = sw.z[j][1] = 0.0;
-#pragma unroll
-        for (int j = 1; j < K; ++j) sw.st[j][0] = sw.st[j][1] = Acc{0.0, 0.0, 0.0};
-        if constexpr (LEVELS) {
-#pragma unroll
-            for (int j = 0; j < K; ++j) sw.store(j, 0, 0.0, 0.0);
+            for (int p = 0; p < 4; ++p) {
+                win[p][0] = hand[w - 1][p][h0];
+                win[p][1] = hand[w - 1][p][h1];
+            }
         }
-    }
-    const long long nb = (S - 1) / 4;
-    Samp cur[4], nxt[4] = {};
+        step_barrier();  // the block is in the ring; every read of the hand blocks precedes their rewrite
+        if (act) {
+            const double* sl = &ring[blk % R][inst][0];
+            const long long i0 = 4 * blk;
+            double op0[4], op1[4], y[4][2], wv[4][2];
 #pragma unroll
-    for (int p = 0; p < 4; ++p) cur[p] = load_samp(A + (1 + p) * (long long)kM, c, g);
-    for (long long blk = 0; blk < nb; ++blk) {
-        const long long i0 = 4 * blk;
-        if (blk + 1 < nb) {  // the next block's samples are in flight during this block
+            for (int p = 0; p < 4; ++p) {  // A operand entries A[c][g], A[c][g + 4]
+                op0[p] = sl[p * kM + c * 8 + g];
+                op1[p] = sl[p * kM + c * 8 + 4 + g];
+            }
+            const double cz[4] = {bw.b64, bw.b24, bw.b64, blk + 1 < nb ? bw.b28 : bw.b14};
+            if constexpr (FIRST) {  // W_1 = Prefix(A) and Z_0 over the block
+                double a0[4], a1[4];
 #pragma unroll
-            for (int p = 0; p < 4; ++p) nxt[p] = load_samp(A + (i0 + 5 + p) * (long long)kM, c, g);
+                for (int p = 0; p < 4; ++p) {
+                    a0[p] = sl[p * kM + o0];
+                    a1[p] = sl[p * kM + o1];
+                    z1[0] = fma(cz[p], a0[p], z1[0]);
+                    z1[1] = fma(cz[p], a1[p], z1[1]);
+                }
+                wv[0][0] = brule<1>(s1[0], a0[0], bw);
+                wv[0][1] = brule<1>(s1[1], a1[0], bw);
+                wv[1][0] = brule<2>(s1[0], a0[1], bw);
+                wv[1][1] = brule<2>(s1[1], a1[1], bw);
+                wv[2][0] = brule<3>(s1[0], a0[2], bw);
+                wv[2][1] = brule<3>(s1[1], a1[2], bw);
+                wv[3][0] = brule<4>(s1[0], a0[3], bw);
+                wv[3][1] = brule<4>(s1[1], a1[3], bw);
+            } else {
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    wv[p][0] = win[p][0];
+                    wv[p][1] = win[p][1];
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {  // Y_{j+1} = A W_j at the four samples
+                dbl4 d = __builtin_amdgcn_mfma_f64_16x16x4f64(op0[p], wv[p][0], dbl4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(op1[p], wv[p][1], d, 0, 0, 0);
+                y[p][0] = inst ? d[2] : d[0];
+                y[p][1] = inst ? d[3] : d[1];
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                zn[0] = fma(cz[p], y[p][0], zn[0]);
+                zn[1] = fma(cz[p], y[p][1], zn[1]);
+            }
+            double v[4][2] = {};
+            if constexpr (MAKES) {  // W_{j+1} = Prefix(Y_{j+1}) over the block, handed on
+                v[0][0] = brule<1>(sn[0], y[0][0], bw);
+                v[0][1] = brule<1>(sn[1], y[0][1], bw);
+                v[1][0] = brule<2>(sn[0], y[1][0], bw);
+                v[1][1] = brule<2>(sn[1], y[1][1], bw);
+                v[2][0] = brule<3>(sn[0], y[2][0], bw);
+                v[2][1] = brule<3>(sn[1], y[2][1], bw);
+                v[3][0] = brule<4>(sn[0], y[3][0], bw);
+                v[3][1] = brule<4>(sn[1], y[3][1], bw);
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    hand[w][p][h0] = v[p][0];
+                    hand[w][p][h1] = v[p][1];
+                }
+            }
+            if (LEVELS && q.live) {  // the levels this wave forms, kept for the reverse sweep
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    const long long o = (i0 + 1 + p) * kM;
+                    if (FIRST) {
+                        W[o + o0] = wv[p][0];
+                        W[o + o1] = wv[p][1];
+                    }
+                    if (MAKES) {
+                        W[(long long)j * S * kM + o + o0] = v[p][0];
+                        W[(long long)j * S * kM + o + o1] = v[p][1];
+                    }
+                }
+            }
         }
-        sw.template step<1>(cur[0], i0 + 1, bw.b64, bw);
-        sw.template step<2>(cur[1], i0 + 2, bw.b24, bw);
-        sw.template step<3>(cur[2], i0 + 3, bw.b64, bw);
-        sw.template step<4>(cur[3], i0 + 4, blk + 1 < nb ? bw.b28 : bw.b14, bw);
-#pragma unroll
-        for (int p = 0; p < 4; ++p) cur[p] = nxt[p];
+        step_barrier();  // the blocks handed over in this step are visible to the next
     }
-    if (!live) return;
-    const long long M = kM, n = a.n;
-    const int e0 = g * 8 + c, e1 = (g + 4) * 8 + c;
-    a.omega1[b * M + e0] = sw.z[0][0];
-    a.omega1[b * M + e1] = sw.z[0][1];
-#pragma unroll
-    for (int l = 1; l <= K; ++l) {
-        a.partial[(b * n + l) * M + e0] = sw.z[l][0];
-        a.partial[(b * n + l) * M + e1] = sw.z[l][1];
+    if (!q.live) return;
+    const long long M = kM, n = a.n, b = q.b;
+    if constexpr (FIRST) {
+        a.omega1[b * M + o0] = z1[0];
+        a.omega1[b * M + o1] = z1[1];
+        a.totals[(b * n + 0) * M + o0] = s1[0].u;  // What_1 = W_1[S-1]
+        a.totals[(b * n + 0) * M + o1] = s1[1].u;
     }
-#pragma unroll
-    for (int j = 1; j <= K; ++j) {  // What_j = W_j[S-1]: the last block's position 4
-        a.totals[(b * n + (j - 1)) * M + e0] = sw.st[j - 1][0].u;
-        a.totals[(b * n + (j - 1)) * M + e1] = sw.st[j - 1][1].u;
+    a.partial[(b * n + j) * M + o0] = zn[0];
+    a.partial[(b * n + j) * M + o1] = zn[1];
+    if constexpr (MAKES) {  // What_{j+1}: the last block's position 4
+        a.totals[(b * n + j) * M + o0] = sn[0].u;
+        a.totals[(b * n + j) * M + o1] = sn[1].u;
+    }
+}
+
+// The loader of a pipeline: ring index r holds block blk_of(r); it keeps
+// kRingAhead - 1 blocks in flight ahead of wave 0 and, at step t, waits for
+// ring index t (wave 0's block) before the step's first barrier.
+template <int R, typename BlkOf>
+__device__ __forceinline__ void ring_loader(RingSlot* ring, const double* A0, const double* A1, long long nb,
+                                            long long steps, int lane, BlkOf blk_of) {
+    for (long long r = 0; r < kRingAhead - 1 && r < nb; ++r) ring_issue(ring[r % R], A0, A1, blk_of(r), lane);
+    for (long long t = 0; t < steps; ++t) {
+        const long long r = t + kRingAhead - 1;
+        if (r < nb) ring_issue(ring[r % R], A0, A1, blk_of(r), lane);
+        if (t < nb) wait_blocks((int)((r < nb ? r : nb - 1) - t));
+        step_barrier();
+        step_barrier();
+    }
+}
+
+// one workgroup per pair of instances (2x, 2x + 1): K level wavefronts and the
+// loader; a missing partner's lanes read the live instance and store nothing
+template <int K, bool LEVELS>
+__global__ void __launch_bounds__(64 * (K + 1)) wbd8_sweep(LmArgs a) {
+    constexpr int R = K + kRingAhead;  // ring slots
+    __shared__ RingSlot ring[R];
+    // level blocks handed from wave w to wave w + 1, [4 samples][2 instances x 64 entries]
+    __shared__ double hand[K > 1 ? K - 1 : 1][4][2 * kM];
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    Pair q;
+    q.c = lane & 7;
+    q.inst = (lane >> 3) & 1;
+    q.g = lane >> 4;
+    q.b = 2LL * blockIdx.x + q.inst;
+    q.live = q.b < a.jobs;
+    q.bl = q.live ? q.b : q.b - 1;
+    q.S = a.S;
+    q.nb = (a.S - 1) / 4;
+    if (w == K) {
+        const long long b0 = 2LL * blockIdx.x, b1 = b0 + 1 < a.jobs ? b0 + 1 : b0;
+        ring_loader<R>(ring, a.in + b0 * q.S * kM, a.in + b1 * q.S * kM, q.nb, q.nb + K - 1, lane,
+                       [](long long r) { return r; });
+    } else if (w == 0) {
+        sweep_wave<K, LEVELS, true, (K > 1)>(a, q, w, ring, hand);
+    } else if (w < K - 1) {
+        sweep_wave<K, LEVELS, false, true>(a, q, w, ring, hand);
+    } else {
+        sweep_wave<K, LEVELS, false, false>(a, q, w, ring, hand);
     }
 }
 
 template <bool LEVELS>
 hipError_t launch_k(const LmArgs& a, hipStream_t s) {
-    const dim3 grid((unsigned)((a.jobs + 1) / 2)), blk(64);
+    const dim3 grid((unsigned)((a.jobs + 1) / 2));
     switch (a.n - 1) {
-        case 1: hipLaunchKernelGGL((wbd8_sweep<1, LEVELS>), grid, blk, 0, s, a); break;
-        case 2: hipLaunchKernelGGL((wbd8_sweep<2, LEVELS>), grid, blk, 0, s, a); break;
-        case 3: hipLaunchKernelGGL((wbd8_sweep<3, LEVELS>), grid, blk, 0, s, a); break;
-        case 4: hipLaunchKernelGGL((wbd8_sweep<4, LEVELS>), grid, blk, 0, s, a); break;
-        case 5: hipLaunchKernelGGL((wbd8_sweep<5, LEVELS>), grid, blk, 0, s, a); break;
-        case 6: hipLaunchKernelGGL((wbd8_sweep<6, LEVELS>), grid, blk, 0, s, a); break;
-        case 7: hipLaunchKernelGGL((wbd8_sweep<7, LEVELS>), grid, blk, 0, s, a); break;
+        case 1: hipLaunchKernelGGL((wbd8_sweep<1, LEVELS>), grid, dim3(128), 0, s, a); break;
+        case 2: hipLaunchKernelGGL((wbd8_sweep<2, LEVELS>), grid, dim3(192), 0, s, a); break;
+        case 3: hipLaunchKernelGGL((wbd8_sweep<3, LEVELS>), grid, dim3(256), 0, s, a); break;
+        case 4: hipLaunchKernelGGL((wbd8_sweep<4, LEVELS>), grid, dim3(320), 0, s, a); break;
+        case 5: hipLaunchKernelGGL((wbd8_sweep<5, LEVELS>), grid, dim3(384), 0, s, a); break;
+        case 6: hipLaunchKernelGGL((wbd8_sweep<6, LEVELS>), grid, dim3(448), 0, s, a); break;
+        case 7: hipLaunchKernelGGL((wbd8_sweep<7, LEVELS>), grid, dim3(512), 0, s, a); break;
         default: return hipErrorInvalidValue;
     }
+    return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Reverse sweep.  With cotangents Zbar_l of the sums and What_bar_j of the
+// totals (the GL fold's adjoint), the adjoint of the sweep is, per sample i,
+//   Ybar_{K+1}[i] = c_i Zbar_K,
+//   Wbar_j[i]     = A_i^T Ybar_{j+1}[i]  (+ What_bar_j at i = S-1),
+//   Ybar_j        = Prefix^T(Wbar_j) + c Zbar_{j-1},            j = K..1,
+//   dA_i          = Ybar_1[i] + sum_j Ybar_{j+1}[i] W_j[i]^T,
+// the reverse of src/grad.hpp:311-348 with the basis in place of the nodes
+// (c_i the Boole sum weights).  Prefix^T runs backwards through each block
+// (boole.hpp:108-151): a block [s, s+4] needs its four Wbar values, the sum
+// of the block above's values (which reaches its top sample through the
+// block base) and that block's pending contribution to its bottom sample, so
+// a level carries two values per entry from block to block.
+//
+// One wavefront per level again, level K in wave 0, plus the sample ring's
+// loader: the waves walk the blocks from the last down as a pipeline (the
+// ring is filled in that order), each handing the next wave its block of
+// Ybar_j (every entry, so that the next wave can read it transposed for the
+// product Ybar W^T, whose contraction runs over Ybar's columns) and the dA
+// sum so far; the wave of level 1 adds Ybar_1 and writes dA.  The levels are
+// added to dA in a fixed order (K first), so results are reproducible.
+// ---------------------------------------------------------------------------
+
+// the Boole sum weight of sample i (sum_vjp_coef, boole.hpp:153-199)
+__device__ __forceinline__ double boole_c(long long i, long long S, const BooleW& c) {
+    if (i == 0 || i == S - 1) return c.b14;
+    const int r = (int)(i & 3);
+    return r == 2 ? c.b24 : (r == 0 ? c.b28 : c.b64);
+}
+
+__device__ __forceinline__ double pick0(const dbl4& d, int inst) { return inst ? d[2] : d[0]; }
+__device__ __forceinline__ double pick1(const dbl4& d, int inst) { return inst ? d[3] : d[1]; }
+
+// One wave of the reverse pipeline, level j = K - w.  TOP: level K, whose
+// input Ybar_{K+1} = c Zbar_K needs no hand-over; LAST: level 1, which adds
+// Ybar_1 and writes dA (the others hand Ybar_j and the dA sum on).
+template <int K, bool TOP, bool LAST>
+__device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, int w, int lane, const RingSlot* ring,
+                                         double (*ybh)[4][2 * kM], double (*dah)[4][2][64]) {
+    constexpr int R = K + kRingAhead;
+    const long long S = q.S, nb = q.nb, n = a.n, bl = q.bl;
+    const int j = K - w;
+    const int c = q.c, g = q.g, inst = q.inst;
+    const int o0 = g * 8 + c, o1 = (g + 4) * 8 + c;  // own entries (g, c), (g + 4, c)
+    const int t0 = c * 8 + g, t1 = c * 8 + g + 4;    // transposed entries (c, g), (c, g + 4)
+    const int i0e = inst * kM;
+    const double* __restrict__ Wj = a.W + (bl * (long long)K + (j - 1)) * S * kM;
+    double* __restrict__ dA = a.dA + bl * S * kM;
+    const BooleW bw(a.dt);
+
+    // Zbar_{j-1} (own entries) and, for level K, Zbar_K (own and transposed)
+    const double* zr = a.zbar + bl * n * kM;
+    const double zb0 = zr[(j - 1) * kM + o0], zb1 = zr[(j - 1) * kM + o1];
+    double zk0 = 0.0, zk1 = 0.0, zkt0 = 0.0, zkt1 = 0.0;
+    if constexpr (TOP) {
+        zk0 = zr[K * kM + o0];
+        zk1 = zr[K * kM + o1];
+        zkt0 = zr[K * kM + t0];
+        zkt1 = zr[K * kM + t1];
+    }
+    double up0 = 0.0, up1 = 0.0, pend0 = 0.0, pend1 = 0.0;
+
+    // W_j's transposed entries of a block, two blocks in flight ahead of the one in use
+    auto loadw = [&](double (*y)[2], long long blk) {
+#pragma unroll
+        for (int p = 0; p < 4; ++p) {
+            const long long o = (4 * blk + 1 + p) * (long long)kM;
+            y[p][0] = Wj[o + t0];
+            y[p][1] = Wj[o + t1];
+        }
+    };
+    double wt[4][2] = {}, wn[4][2] = {}, wn2[4][2] = {};
+    loadw(wt, nb - 1);
+    if (nb > 1) loadw(wn, nb - 2);
+    double yo[4][2] = {}, yt[4][2] = {}, part[4][2] = {};
+    for (long long t = 0; t < nb + K - 1; ++t) {
+        const long long r = t - w;  // ring index of this wave's block
+        const long long blk = nb - 1 - r;
+        const bool act = r >= 0 && blk >= 0;
+        if (!TOP && act) {  // the previous wave handed this block over in the previous step
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                yo[p][0] = ybh[w - 1][p][i0e + o0];
+                yo[p][1] = ybh[w - 1][p][i0e + o1];
+                yt[p][0] = ybh[w - 1][p][i0e + t0];
+                yt[p][1] = ybh[w - 1][p][i0e + t1];
+                part[p][0] = dah[w - 1][p][0][lane];
+                part[p][1] = dah[w - 1][p][1][lane];
+            }
+        }
+        step_barrier();  // the block is in the ring; every read of the hand blocks precedes their rewrite
+        if (act) {
+            const long long s = 4 * blk;
+            if (blk > 1) loadw(wn2, blk - 2);
+            double cz[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) cz[p] = boole_c(s + 1 + p, S, bw);
+            if constexpr (TOP) {  // Ybar_{K+1} = c Zbar_K
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    yo[p][0] = cz[p] * zk0;
+                    yo[p][1] = cz[p] * zk1;
+                    yt[p][0] = cz[p] * zkt0;
+                    yt[p][1] = cz[p] * zkt1;
+                }
+            }
+            const double* sl = &ring[r % R][inst][0];
+            double at0[4], at1[4];  // the A^T operand: A's own entries
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                at0[p] = sl[p * kM + o0];
+                at1[p] = sl[p * kM + o1];
+            }
+            double wb[4][2];
+            dbl4 acc[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                // Wbar_j = A^T Ybar_{j+1}
+                dbl4 d = __builtin_amdgcn_mfma_f64_16x16x4f64(at0[p], yo[p][0], dbl4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
+                d = __builtin_amdgcn_mfma_f64_16x16x4f64(at1[p], yo[p][1], d, 0, 0, 0);
+                wb[p][0] = pick0(d, inst);
+                wb[p][1] = pick1(d, inst);
+                // dA so far + Ybar_{j+1} W_j^T
+                if constexpr (TOP) acc[p] = dbl4{0.0, 0.0, 0.0, 0.0};
+                else acc[p] = inst ? dbl4{0.0, 0.0, part[p][0], part[p][1]} : dbl4{part[p][0], part[p][1], 0.0, 0.0};
+                acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(yt[p][0], wt[p][0], acc[p], 0, 0, 0);
+                acc[p] = __builtin_amdgcn_mfma_f64_16x16x4f64(yt[p][1], wt[p][1], acc[p], 0, 0, 0);
+            }
+            if (blk == nb - 1) {  // the total's cotangent enters at the last sample
+                const double* wr = a.wbar + (bl * n + (j - 1)) * kM;
+                wb[3][0] += wr[o0];
+                wb[3][1] += wr[o1];
+            }
+            // Ybar_j = Prefix^T(Wbar_j) + c Zbar_{j-1} over the block
+            double yn[4][2];
+#pragma unroll
+            for (int qq = 0; qq < 2; ++qq) {
+                double& up = qq ? up1 : up0;
+                double& pend = qq ? pend1 : pend0;
+                const double zq = qq ? zb1 : zb0;
+                const double g1 = wb[0][qq], g2 = wb[1][qq], g3 = wb[2][qq];
+                const double g4 = wb[3][qq] + up;
+                const double f4 = fma(bw.b14, g4, pend);
+                const double f3 = fma(bw.e38, g3, bw.b64 * g4);
+                const double f2 = fma(bw.t13, g2, fma(bw.e98, g3, bw.b24 * g4));
+                const double f1 = fma(bw.h2, g1, fma(bw.t43, g2, fma(bw.e98, g3, bw.b64 * g4)));
+                up = g1 + g2 + g3 + g4;
+                pend = fma(bw.h2, g1, fma(bw.t13, g2, fma(bw.e38, g3, bw.b14 * g4)));
+                yn[0][qq] = fma(cz[0], zq, f1);
+                yn[1][qq] = fma(cz[1], zq, f2);
+                yn[2][qq] = fma(cz[2], zq, f3);
+                yn[3][qq] = fma(cz[3], zq, f4);
+            }
+            if constexpr (!LAST) {
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    ybh[w][p][i0e + o0] = yn[p][0];
+                    ybh[w][p][i0e + o1] = yn[p][1];
+                    dah[w][p][0][lane] = pick0(acc[p], inst);
+                    dah[w][p][1][lane] = pick1(acc[p], inst);
+                }
+            } else if (q.live) {  // dA = Ybar_1 + sum_j Ybar_{j+1} W_j^T
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    double* o = dA + (s + 1 + p) * kM;
+                    o[o0] = pick0(acc[p], inst) + yn[p][0];
+                    o[o1] = pick1(acc[p], inst) + yn[p][1];
+                }
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p) {
+                wt[p][0] = wn[p][0];
+                wt[p][1] = wn[p][1];
+                wn[p][0] = wn2[p][0];
+                wn[p][1] = wn2[p][1];
+            }
+        }
+        step_barrier();  // the blocks handed over in this step are visible to the next
+    }
+    // sample 0: the prefixes are constant there (W_j[0] = 0), so dA_0 = Ybar_1[0]
+    // = level 1's pending from the lowest block + c_0 Zbar_0
+    if (LAST && q.live) {
+        dA[o0] = fma(bw.b14, zb0, pend0);
+        dA[o1] = fma(bw.b14, zb1, pend1);
+    }
+}
+
+template <int K>
+__global__ void __launch_bounds__(64 * (K + 1)) wbd8_rev(Wbd8RevArgs a) {
+    constexpr int H = K > 1 ? K - 1 : 1;
+    constexpr int R = K + kRingAhead;     // sample ring slots (the loader is wave K)
+    __shared__ RingSlot ring[R];
+    __shared__ double ybh[H][4][2 * kM];  // Ybar_j blocks handed from wave w to wave w + 1, by entry
+    __shared__ double dah[H][4][2][64];   // dA sums so far, by lane
+    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
+    Pair q;
+    q.c = lane & 7;
+    q.inst = (lane >> 3) & 1;
+    q.g = lane >> 4;
+    q.b = 2LL * blockIdx.x + q.inst;
+    q.live = q.b < a.B;
+    q.bl = q.live ? q.b : q.b - 1;
+    q.S = a.S;
+    q.nb = (a.S - 1) / 4;
+    if (w == K) {  // ring index r holds block nb - 1 - r (the blocks are walked top-down)
+        const long long b0 = 2LL * blockIdx.x, b1 = b0 + 1 < a.B ? b0 + 1 : b0;
+        const long long nb = q.nb;
+        ring_loader<R>(ring, a.in + b0 * q.S * kM, a.in + b1 * q.S * kM, nb, nb + K - 1, lane,
+                       [nb](long long r) { return nb - 1 - r; });
+    } else if (K == 1) {
+        rev_wave<K, true, true>(a, q, w, lane, ring, ybh, dah);
+    } else if (w == 0) {
+        rev_wave<K, true, false>(a, q, w, lane, ring, ybh, dah);
+    } else if (w < K - 1) {
+        rev_wave<K, false, false>(a, q, w, lane, ring, ybh, dah);
+    } else {
+        rev_wave<K, false, true>(a, q, w, lane, ring, ybh, dah);
+    }
+}
+
+template <int K>
+hipError_t launch_rev_k(const Wbd8RevArgs& a, hipStream_t s) {
+    hipLaunchKernelGGL((wbd8_rev<K>), dim3((unsigned)((a.B + 1) / 2)), dim3(64 * (K + 1)), 0, s, a);
     return hipGetLastError();
 }
 
@@ -243,6 +594,35 @@ hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s) {
     const double flops = B * S * (K * (2.0 * 512.0 + 8.0 * kM) + 8.0 * kM);
     void* h = timer_begin("wbd8_sweep", s, bytes, flops);
     hipError_t e = a.carry ? launch_k<true>(a, s) : launch_k<false>(a, s);
+    timer_end(h);
+    return e;
+}
+
+}  // namespace magnus
+
+namespace magnus {
+
+hipError_t launch_wbd8_rev(const Wbd8RevArgs& a, hipStream_t s) {
+    if (a.B == 0) return hipSuccess;
+    if (a.S < 5 || (a.S - 1) % 4 != 0 || a.n < 2 || a.n - 1 > kWbd8MaxK) return hipErrorInvalidValue;
+    // algorithmic work: the samples and the K levels read once, dA written once;
+    // per sample and level two products (A^T Ybar and Ybar W^T) and the
+    // transposed prefix step and sum cotangent, 10 flops per entry
+    const double K = a.n - 1, S = (double)a.S, B = (double)a.B;
+    const double bytes = 8.0 * B * S * kM * (K + 2.0);
+    const double flops = B * S * K * (2.0 * 2.0 * 512.0 + 10.0 * kM);
+    void* h = timer_begin("wbd8_rev", s, bytes, flops);
+    hipError_t e;
+    switch (a.n - 1) {
+        case 1: e = launch_rev_k<1>(a, s); break;
+        case 2: e = launch_rev_k<2>(a, s); break;
+        case 3: e = launch_rev_k<3>(a, s); break;
+        case 4: e = launch_rev_k<4>(a, s); break;
+        case 5: e = launch_rev_k<5>(a, s); break;
+        case 6: e = launch_rev_k<6>(a, s); break;
+        case 7: e = launch_rev_k<7>(a, s); break;
+        default: e = hipErrorInvalidValue;
+    }
     timer_end(h);
     return e;
 }

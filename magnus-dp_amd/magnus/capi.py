@@ -57,6 +57,10 @@ def lib() -> ctypes.CDLL:
         L.magnus_sc_forward_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp]
         L.magnus_sc_vjp_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp, vp]
         L.magnus_sc_vjp_noalloc_f64.argtypes = [P, u64, vp, vp, vp, vp]
+        L.magnus_basis_state_elems.argtypes = [P, u64]
+        L.magnus_basis_state_elems.restype = u64
+        L.magnus_forward_basis_f64.argtypes = [P, u64, vp, vp, vp, vp]
+        L.magnus_vjp_basis_f64.argtypes = [P, u64, vp, vp, vp, vp, vp]
         L.magnus_sc_checkpoint_elems.argtypes = [P, u64]
         L.magnus_sc_checkpoint_elems.restype = u64
         for name in ("magnus_carry_elems", "magnus_output_elems", "magnus_sc_carry_elems", "magnus_sc_output_elems"):

@@ -1,10 +1,10 @@
 """Learned resolution controller: pick (samples, order) per instance, verify, escalate.
 
-Problem.  For an instance A(t) (dim d, Fourier form) and a tolerance tol, the
-cheapest resolution (S, n) -- S in {65, ..., 16385} (2^k + 1), n in {2..8} --
-whose Magnus sum is within tol (relative Frobenius) of the S = 32769, n = 8
-sum.  The cost of a resolution is (S - 1)(n - 1), the number of sample
-products of the basis sweep.
+Problem.  For an instance A(t) (dim d, Fourier form) and a tolerance tol, a
+resolution whose Magnus sum is within tol (relative Frobenius) of the
+S = 32769, n = 8 sum, as cheaply as possible.  A class (S, n) -- S in
+{65, ..., 16385} (2^k + 1), n in {2..8} -- is judged by the doubling pair
+(S, 2S - 1) and delivered at its finer member (2S - 1, n).
 
 Pieces, all on the device:
   * labels: one all-orders sweep (magnus_forward_orders_f64) per S gives the
@@ -12,15 +12,20 @@ Pieces, all on the device:
   * features: magnus_ctrl_features_f64 on a 65-point coarse sampling;
   * network: magnus_mlp_* (3 layers, width 128, bfloat16 weights, FP32
     accumulation, Adam), classifying the 63 (S, n) classes;
-  * acceptance (the judge, also the fallback): a resolution (S, n) is accepted
-    when the sum at (S, n) agrees within JUDGE_MARGIN * tol with the sum at
-    doubled samples (2S - 1, n) and, for n < 8, with the sum at (S, n + 1)
-    (the margin covers the error left in the finer of each pair); a failed check
-    doubles S and/or raises n, and repeats.  The accepted (S, n) is delivered
-    (forward output and VJP at that resolution).  The learned path
-    starts the judge at the network's class; the same judge started at
-    (65, 2) is the method without the network; the plain method runs every
-    instance at (16385, 8).
+  * acceptance (the judge, also the fallback): class (S, n) is accepted when
+    the sums at (S, n) and (2S - 1, n) agree within JUDGE_FINE * tol and, for
+    n < 8, the sums at (S, n) and (S, n + 1) within JUDGE_MARGIN * tol.  The
+    finer sum of the doubling pair is delivered: with the rule's error falling
+    at least fourfold per doubling (second order; Boole's is higher), its
+    error is at most a third of the pair's difference.  A failed check doubles
+    S and/or raises n, and repeats.  The learned path starts the judge at the
+    network's class and runs the VJP inside the same round from the finer
+    sweep's kept basis state (torch_ops.forward_basis / vjp_basis), so an
+    accepted instance costs the coarse sweep, the finer sweep and one reverse
+    sweep; the same judge started at (65, 2) is the method without the
+    network (judge rounds, then the VJP at the accepted resolution); the plain
+    method runs every instance at (16385, 8), forward with its kept basis
+    state and the VJP from it.
 """
 from __future__ import annotations
 
@@ -40,7 +45,8 @@ PLAIN = (16385, 8)
 CLASSES = tuple((S, n) for S in S_GRID for n in N_GRID)
 FEATURE_GRID = 65
 MAX_MODES = 8
-JUDGE_MARGIN = 0.5
+JUDGE_MARGIN = 0.5  # next-order check: |sum(S, n + 1) - sum(S, n)| <= JUDGE_MARGIN tol
+JUDGE_FINE = 2.0    # doubling check: |sum(2S - 1, n) - sum(S, n)| <= JUDGE_FINE tol (the finer is delivered)
 
 
 def cost(S: int, n: int) -> int:
@@ -92,12 +98,14 @@ def rel_err(x: torch.Tensor, ref: torch.Tensor) -> torch.Tensor:
     return torch.linalg.vector_norm((x - ref).flatten(1), dim=1) / torch.linalg.vector_norm(ref.flatten(1), dim=1)
 
 
-def label(coeffs: torch.Tensor, tol: float, batch: int = 512, progress=None):
+def label(coeffs: torch.Tensor, tol: float, batch: int = 2048, progress=None):
     """Per instance: (labels, minimal, reachable, errors [B][9][7], reference
-    sums [B][d][d]); progress(done, total) is called after every batch.  `minimal` is the cheapest class within tol of the
-    reference; `labels` -- what the network learns -- is the cheapest class
-    the judge accepts in one round with a result within tol (its doubled-samples
-    and next-order checks pass), so a correct prediction costs one round."""
+    sums [B][d][d]); progress(done, total) is called after every batch.
+    errors[s][k] is the error of class (S_GRID[s], N_GRID[k])'s delivered sum
+    (2S - 1 samples); `minimal` is the cheapest class whose delivered sum is
+    within tol; `labels` -- what the network learns -- is the cheapest class
+    the judge accepts in one round with its delivered sum within tol, so a
+    correct prediction costs one round."""
     B = coeffs.shape[0]
     nmax = max(N_GRID)
     labels = np.empty(B, np.int32)
@@ -117,23 +125,23 @@ def label(coeffs: torch.Tensor, tol: float, batch: int = 512, progress=None):
         e = torch.empty((nb, len(S_GRID), len(N_GRID)), dtype=torch.float64, device=c.device)
         ok = torch.empty((nb, len(S_GRID), len(N_GRID)), dtype=torch.bool, device=c.device)
         for si, S in enumerate(S_GRID):
-            o, up = allS[si], allS[si + 1]  # up: 2S - 1 samples
+            o, up = allS[si], allS[si + 1]  # up: 2S - 1 samples, the delivered member
             for ni, n in enumerate(N_GRID):
                 cur = o[:, n - 1]
-                e[:, si, ni] = rel_err(cur, ref)
-                s_ok = rel_err(up[:, n - 1], cur) <= JUDGE_MARGIN * tol
+                e[:, si, ni] = rel_err(up[:, n - 1], ref)
+                s_ok = rel_err(up[:, n - 1], cur) <= JUDGE_FINE * tol
+                if S >= max(S_GRID):
+                    s_ok = torch.ones_like(s_ok)
                 n_ok = rel_err(o[:, n], cur) <= JUDGE_MARGIN * tol if n < nmax else torch.ones_like(s_ok)
                 ok[:, si, ni] = s_ok & n_ok & (e[:, si, ni] <= tol)
         e = e.cpu().numpy()
         ok = ok.cpu().numpy().reshape(nb, -1)
         errs[b0:b0 + nb] = e
-        flat = e.reshape(nb, -1)
-        for r in range(nb):
-            suff = np.nonzero(flat[r][order] <= tol)[0]
-            acc = np.nonzero(ok[r][order])[0]
-            reach[b0 + r] = suff.size > 0
-            minimal[b0 + r] = order[suff[0]] if suff.size else class_index(*PLAIN)
-            labels[b0 + r] = order[acc[0]] if acc.size else class_index(*PLAIN)
+        suff = e.reshape(nb, -1)[:, order] <= tol  # classes in order of cost
+        acc = ok[:, order]
+        reach[b0:b0 + nb] = suff.any(axis=1)
+        minimal[b0:b0 + nb] = np.where(suff.any(axis=1), order[suff.argmax(axis=1)], class_index(*PLAIN))
+        labels[b0:b0 + nb] = np.where(acc.any(axis=1), order[acc.argmax(axis=1)], class_index(*PLAIN))
         if progress is not None:
             progress(b0 + nb, B)
     return labels, minimal, reach, errs, torch.cat(refs)
@@ -233,53 +241,85 @@ class Controller:
 
 
 # --------------------------------------------------------------------- acceptance
-def _chunks(ids, S, d, budget=4e9):
-    """Split an id list so that a call's samples stay within ~budget bytes."""
-    per = max(1, int(budget // (S * d * d * 8 * 4)))
+def _chunks(ids, bytes_per_instance, budget=24e9):
+    """Split an id array so that a call's device buffers stay within ~budget bytes
+    (large enough that a call's sweep launch fills the GPU: a workgroup per
+    instance pair, several hundred pairs per launch)."""
+    per = max(1, int(budget // max(1, bytes_per_instance)))
     return [ids[k:k + per] for k in range(0, len(ids), per)]
 
 
 _STREAMS: list = []
+_POOL = [4]
 
 
-def _streams(k: int = 4):
+def set_streams(k: int) -> None:
+    """Number of side streams the groups of a round are spread over (1: serial)."""
+    _POOL[0] = max(1, int(k))
+
+
+def _streams():
     """A small pool of side streams (default priority) for independent groups."""
+    k = _POOL[0]
     while len(_STREAMS) < k:
         _STREAMS.append(torch.cuda.Stream())
     return _STREAMS[:k]
 
 
 def _index(ids, device):
-    return torch.tensor(ids, dtype=torch.long).pin_memory().to(device, non_blocking=True)
+    return torch.from_numpy(np.ascontiguousarray(ids, dtype=np.int64)).pin_memory().to(device, non_blocking=True)
 
 
-def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24):
-    """The judge: from per-instance starting resolutions (list of (S, n)),
-    verify and escalate until every instance passes.  Returns (accepted
-    resolutions, accepted sums [B][d][d], stats) with stats = {work: sum of
-    (S-1)(n-1) over the sweeps run, rounds: per-instance rounds, calls}.
-    The groups of a round run on a pool of streams; the host waits once per
-    round."""
+def _groups(S_arr, n_arr, ids):
+    """[(S, n, ids of that resolution)] for the given instance ids, in a fixed order."""
+    key = S_arr[ids] * 64 + n_arr[ids]
+    order = np.argsort(key, kind="stable")
+    key_sorted = key[order]
+    cuts = np.flatnonzero(np.diff(key_sorted)) + 1
+    return [(int(k // 64), int(k % 64), ids[part])
+            for k, part in zip(key_sorted[np.r_[0, cuts]], np.split(order, cuts))]
+
+
+def delivered(res):
+    """The resolution a judged class (S, n) delivers: the finer member of its doubling pair."""
+    return [(2 * S - 1, n) for S, n in res]
+
+
+def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: torch.Tensor | None = None):
+    """The judge: from per-instance starting classes (list of (S, n)), verify and
+    escalate until every instance passes.  Returns (accepted classes, delivered
+    sums [B][d][d] at (2S - 1, n), stats) with stats = {work: sum of (S-1)(n-1)
+    over the sweeps run (a reverse sweep counts twice), rounds: per-instance
+    rounds, calls, dA: per-instance dA through dA_of(stats, i), or None}.  With cot, every round
+    also runs the VJP of its finer sweep from that sweep's kept basis state,
+    and an instance's dA is the one of the round that accepted it.  The groups
+    of a round run on a pool of streams; the host waits once per round and
+    updates every instance with array operations."""
     B, d = coeffs.shape[0], coeffs.shape[-1]
-    res = list(start)
-    done = [False] * B
+    S_arr = np.array([r[0] for r in start], dtype=np.int64)
+    n_arr = np.array([r[1] for r in start], dtype=np.int64)
+    done = np.zeros(B, bool)
     rounds = np.zeros(B, np.int32)
     out = torch.empty_like(coeffs[:, 0])
+    # dA of each instance: chunk tensor and position of the round that accepted it
+    dA = {"chunks": [], "chunk": np.full(B, -1, np.int64), "pos": np.zeros(B, np.int64)} if cot is not None else None
     work = 0
     calls = 0
     main = torch.cuda.current_stream()
     pool = _streams()
+    S_top = max(S_GRID)
     for _ in range(max_rounds):
-        groups = {}
-        for i in range(B):
-            if not done[i]:
-                groups.setdefault(res[i], []).append(i)
-        if not groups:
+        active = np.flatnonzero(~done)
+        if active.size == 0:
             break
-        pending = []
+        ids_l, s_l, n_l, g_l = [], [], [], []
         k = 0
-        for (S, n), all_ids in groups.items():
-            for ids in _chunks(all_ids, S, d):
+        for S, n, all_ids in _groups(S_arr, n_arr, active):
+            S2 = 2 * S - 1
+            # per instance: both sweeps' samples; with the VJP the finer sweep's
+            # kept levels, its dA and the reverse sweep's buffers
+            per = (S + S2) * d * d * 8 * (2 if cot is None else n + 4)
+            for ids in _chunks(all_ids, per):
                 st = pool[k % len(pool)]
                 k += 1
                 st.wait_stream(main)
@@ -287,57 +327,77 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24):
                     idx = _index(ids, coeffs.device)
                     c = coeffs[idx]
                     nm = min(n + 1, REF_N)
-                    hi = sums(c, S, nm)
-                    cur = hi[:, n - 1]
-                    if S < REF_S:
-                        up = sums(c, 2 * S - 1, n)[:, n - 1]
-                        s_ok = rel_err(up, cur) <= JUDGE_MARGIN * tol
-                        calls += 1
-                        work += len(ids) * cost(2 * S - 1, n)
+                    lo = sums(c, S, nm)  # orders 1..nm at S samples
+                    cur = lo[:, n - 1]
+                    A2 = torch_ops.sample_fourier(c, S2)
+                    if cot is not None:
+                        fine, state = torch_ops.forward_basis(n, A2, 0.0, 1.0, op="sum", integrator="Boole")
+                        g = torch_ops.vjp_basis(n, A2, cot[idx], state, 0.0, 1.0, op="sum", integrator="Boole")
+                        del state
+                        work += len(ids) * 2 * cost(S2, n)
                     else:
+                        fine = torch_ops.forward(n, A2, 0.0, 1.0, op="sum", integrator="Boole")
+                        g = None
+                    del A2
+                    work += len(ids) * (cost(S, nm) + cost(S2, n))
+                    calls += 2
+                    if S < S_top:
+                        s_ok = rel_err(fine, cur) <= JUDGE_FINE * tol
+                    else:  # the finer member is the reference resolution
                         s_ok = torch.ones(len(ids), dtype=torch.bool, device=coeffs.device)
-                    n_ok = (rel_err(hi[:, nm - 1], cur) <= JUDGE_MARGIN * tol if nm > n
+                    n_ok = (rel_err(lo[:, nm - 1], cur) <= JUDGE_MARGIN * tol if nm > n
                             else torch.ones(len(ids), dtype=torch.bool, device=coeffs.device))
-                    out[idx] = cur
-                    calls += 1
-                    work += len(ids) * cost(S, nm)
-                    pending.append(((S, n), ids, s_ok, n_ok))
+                    out[idx] = fine
+                    ids_l.append(ids)
+                    s_l.append(s_ok)
+                    n_l.append(n_ok)
+                    g_l.append(g)
         for st in pool:
             main.wait_stream(st)
-        torch.cuda.synchronize()
-        for (S, n), ids, s_ok, n_ok in pending:
-            s_ok = s_ok.cpu().numpy()
-            n_ok = n_ok.cpu().numpy()
-            for kk, i in enumerate(ids):
-                rounds[i] += 1
-                if (s_ok[kk] and n_ok[kk]) or (S >= REF_S and n >= REF_N):
-                    done[i] = True
-                    res[i] = (S, n)
-                else:
-                    S2 = S if s_ok[kk] or S >= REF_S else 2 * S - 1
-                    n2 = n if n_ok[kk] or n >= REF_N else n + 1
-                    res[i] = (S2, n2)
-    return res, out, {"work": work, "rounds": rounds, "calls": calls}
+        s_ok = torch.cat(s_l).cpu().numpy()
+        n_ok = torch.cat(n_l).cpu().numpy()
+        if dA is not None:  # this round's dA (a later round overwrites the rejected instances')
+            for ids, g in zip(ids_l, g_l):
+                dA["chunk"][ids] = len(dA["chunks"])
+                dA["pos"][ids] = np.arange(len(ids))
+                dA["chunks"].append(g)
+        ids = np.concatenate(ids_l)
+        S_cur, n_cur = S_arr[ids], n_arr[ids]
+        rounds[ids] += 1
+        ok = (s_ok & n_ok) | ((S_cur >= S_top) & (n_cur >= REF_N))
+        done[ids[ok]] = True
+        rej = ~ok
+        S_arr[ids[rej]] = np.where(s_ok[rej] | (S_cur[rej] >= S_top), S_cur[rej], 2 * S_cur[rej] - 1)
+        n_arr[ids[rej]] = np.where(n_ok[rej] | (n_cur[rej] >= REF_N), n_cur[rej], n_cur[rej] + 1)
+    res = list(zip(S_arr.tolist(), n_arr.tolist()))
+    return res, out, {"work": work, "rounds": rounds, "calls": calls, "dA": dA}
+
+
+def dA_of(stats, i: int) -> torch.Tensor:
+    """Instance i's dA from accept(..., cot=...)'s stats."""
+    d = stats["dA"]
+    return d["chunks"][d["chunk"][i]][d["pos"][i]]
 
 
 def solve(coeffs: torch.Tensor, res, cot: torch.Tensor, keep_dA: bool = False, forward_out=None):
-    """Forward (op sum) + VJP of every instance at its resolution, grouped by
-    resolution (groups spread over a pool of streams); returns (outputs
-    [B][d][d], dA list or None, work in sweep units).  forward_out: outputs
-    the judge already computed at these resolutions (the forward is then not
-    repeated)."""
+    """Forward (op sum) + VJP of every instance at the resolution it is given
+    (res: (S, n) per instance), grouped by resolution (groups spread over a
+    pool of streams); returns (outputs [B][d][d], dA list or None, work in sweep
+    units, a reverse sweep counting twice).  Without forward_out the forward
+    keeps its basis state and the VJP starts from it; forward_out: outputs
+    computed already at these resolutions (the VJP then reruns the sweep)."""
     d = coeffs.shape[-1]
-    groups = {}
-    for i, r in enumerate(res):
-        groups.setdefault(tuple(r), []).append(i)
+    S_arr = np.array([r[0] for r in res], dtype=np.int64)
+    n_arr = np.array([r[1] for r in res], dtype=np.int64)
     out = torch.empty_like(coeffs[:, 0]) if forward_out is None else forward_out
     dA = [None] * len(res) if keep_dA else None
     work = 0
     main = torch.cuda.current_stream()
     pool = _streams()
     k = 0
-    for (S, n), all_ids in groups.items():
-        for ids in _chunks(all_ids, S, d):
+    for S, n, all_ids in _groups(S_arr, n_arr, np.arange(len(res))):
+        # per instance: samples, dA, the n - 1 kept levels and the reverse sweep's buffers
+        for ids in _chunks(all_ids, S * d * d * 8 * (n + 4)):
             st = pool[k % len(pool)]
             k += 1
             st.wait_stream(main)
@@ -345,12 +405,16 @@ def solve(coeffs: torch.Tensor, res, cot: torch.Tensor, keep_dA: bool = False, f
                 idx = _index(ids, coeffs.device)
                 A = torch_ops.sample_fourier(coeffs[idx], S)
                 if forward_out is None:
-                    out[idx] = torch_ops.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole")
-                    work += len(ids) * cost(S, n)
-                g = torch_ops.vjp(n, A, cot[idx], None, 0.0, 1.0, op="sum", integrator="Boole")
-                work += len(ids) * 3 * cost(S, n)  # forward recompute + reverse sweep
+                    o, state = torch_ops.forward_basis(n, A, 0.0, 1.0, op="sum", integrator="Boole")
+                    out[idx] = o
+                    g = torch_ops.vjp_basis(n, A, cot[idx], state, 0.0, 1.0, op="sum", integrator="Boole")
+                    del state
+                    work += len(ids) * 3 * cost(S, n)  # sweep + reverse sweep
+                else:
+                    g = torch_ops.vjp(n, A, cot[idx], None, 0.0, 1.0, op="sum", integrator="Boole")
+                    work += len(ids) * 3 * cost(S, n)  # sweep rerun + reverse sweep
                 if keep_dA:
-                    for kk, i in enumerate(ids):
+                    for kk, i in enumerate(ids.tolist()):
                         dA[i] = g[kk]
     for st in pool:
         main.wait_stream(st)

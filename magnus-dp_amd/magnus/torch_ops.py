@@ -107,6 +107,55 @@ def vjp(n: int, A: torch.Tensor, cot: torch.Tensor, carry: torch.Tensor | None =
     return dA
 
 
+def basis_state_elems(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
+                      matrix_backend: str = "Auto", integrator: str = "Auto") -> int:
+    """Doubles of the basis state forward_basis keeps for input A (B, S, d, d); 0
+    when the problem does not take the dense W-basis path (op 'one', n = 1)."""
+    B, S, d, _ = A.shape
+    p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    return int(capi.lib().magnus_basis_state_elems(ctypes.byref(p), B))
+
+
+def forward_basis(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
+                  matrix_backend: str = "Auto", integrator: str = "Auto"):
+    """forward() that also returns its basis state (a flat float64 tensor: the
+    sweep's levels, sums and totals), from which vjp_basis starts without
+    rerunning the sweep."""
+    _check_input(A, 4, "A")
+    A = A.contiguous()
+    B, S, d, d2 = A.shape
+    if d != d2:
+        raise ValueError("data must contain square matrices")
+    p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    elems = int(capi.lib().magnus_basis_state_elems(ctypes.byref(p), B))
+    if elems == 0:
+        raise ValueError("this problem has no basis state (it does not take the dense W-basis path)")
+    out = torch.empty((B, n, d, d) if op == "many" else (B, d, d), dtype=A.dtype, device=A.device)
+    state = torch.empty((elems,), dtype=A.dtype, device=A.device)
+    capi.check(capi.lib().magnus_forward_basis_f64(ctypes.byref(p), B, _ptr(A), _ptr(out), _ptr(state), _stream()))
+    return out, state
+
+
+def vjp_basis(n: int, A: torch.Tensor, cot: torch.Tensor, state: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *,
+              op: str = "sum", matrix_backend: str = "Auto", integrator: str = "Auto"):
+    """vjp() from the basis state forward_basis returned for the same A."""
+    _check_input(A, 4, "A")
+    A = A.contiguous()
+    B, S, d, d2 = A.shape
+    if d != d2:
+        raise ValueError("data must contain square matrices")
+    p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    cot = _check_operand(cot, A, (B, n, d, d) if op == "many" else (B, d, d), "cotangent")
+    elems = int(capi.lib().magnus_basis_state_elems(ctypes.byref(p), B))
+    if elems == 0:
+        raise ValueError("this problem has no basis state (it does not take the dense W-basis path)")
+    state = _check_operand(state, A, (elems,), "basis state")
+    dA = torch.empty_like(A)
+    capi.check(capi.lib().magnus_vjp_basis_f64(ctypes.byref(p), B, _ptr(A), _ptr(cot), _ptr(state), _ptr(dA),
+                                               _stream()))
+    return dA
+
+
 def sc_forward(n: int, a: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
                integrator: str = "Auto", record_vjp: bool = False):
     """Batched SpaceCurve Magnus: a (B, S, 3) float64 on the GPU."""
@@ -195,25 +244,37 @@ def sc_vjp_ckpt(n: int, a: torch.Tensor, cot: torch.Tensor, ckpt: torch.Tensor, 
 
 
 class MagnusFunction(torch.autograd.Function):
-    """out = Magnus(A); backward = HIP reverse sweep.  record_vjp keeps the carry
-    as the residual; otherwise the backward recomputes it (VJP::none)."""
+    """out = Magnus(A); backward = HIP reverse sweep.  record_vjp keeps the
+    reference's carry as the residual; otherwise, where the dense W-basis path
+    applies (op many or sum, n >= 2), the residual is the basis state of the
+    forward's sweep (n - 1 levels instead of the carry's Q (n - 1)) and the
+    backward skips its sweep; elsewhere the backward recomputes (VJP::none)."""
 
     @staticmethod
     def forward(ctx, A, n, t0, tf, op, matrix_backend, integrator, record_vjp):
+        kw = dict(op=op, matrix_backend=matrix_backend, integrator=integrator)
+        mode = "none"
         if record_vjp:
-            out, carry = forward(n, A, t0, tf, op=op, matrix_backend=matrix_backend, integrator=integrator,
-                                 record_vjp=True)
+            out, res = forward(n, A, t0, tf, record_vjp=True, **kw)
+            mode = "carry" if n > 1 else "none"
+        elif A.is_cuda and A.dim() == 4 and basis_state_elems(n, A, t0, tf, **kw) > 0:
+            out, res = forward_basis(n, A, t0, tf, **kw)
+            mode = "basis"
         else:
-            out, carry = forward(n, A, t0, tf, op=op, matrix_backend=matrix_backend, integrator=integrator), None
-        ctx.save_for_backward(A, carry if carry is not None else torch.empty(0, device=A.device))
-        ctx.cfg = (n, t0, tf, op, matrix_backend, integrator, carry is not None)
+            out, res = forward(n, A, t0, tf, **kw), None
+        ctx.save_for_backward(A, res if mode != "none" else torch.empty(0, device=A.device))
+        ctx.cfg = (n, t0, tf, op, matrix_backend, integrator, mode)
         return out
 
     @staticmethod
     def backward(ctx, g):
-        A, carry = ctx.saved_tensors
-        n, t0, tf, op, mb, integ, has_carry = ctx.cfg
-        dA = vjp(n, A, g, carry if has_carry else None, t0, tf, op=op, matrix_backend=mb, integrator=integ)
+        A, res = ctx.saved_tensors
+        n, t0, tf, op, mb, integ, mode = ctx.cfg
+        kw = dict(op=op, matrix_backend=mb, integrator=integ)
+        if mode == "basis":
+            dA = vjp_basis(n, A, g, res, t0, tf, **kw)
+        else:
+            dA = vjp(n, A, g, res if mode == "carry" else None, t0, tf, **kw)
         return dA, None, None, None, None, None, None, None
 
 

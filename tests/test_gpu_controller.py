@@ -180,3 +180,28 @@ def test_controller_gains_over_the_judge_without_it():
     assert work_l < 0.7 * work_0, (work_l, work_0)
     assert st_l["rounds"].mean() < 0.5 * st_0["rounds"].mean()
     assert hit_l >= 0.97 and hit_l >= hit_0 - 0.02, (hit_l, hit_0)
+
+
+def test_judge_delivers_the_finer_sum_and_its_vjp_from_the_kept_state():
+    """accept(..., cot=...) delivers, per instance, the sum of the finer member
+    (2S - 1, n) of its accepted doubling pair and the VJP of that sweep from
+    its kept basis state: both equal the one-problem forward and VJP at the
+    delivered resolution bit for bit (the VJP of a rejected round is
+    replaced by the accepting round's)."""
+    import torch
+
+    from magnus import controller as C
+    from magnus import torch_ops
+
+    tol = 1e-8
+    B = 24
+    coeffs = C.make_coefficients(B, seed=7)
+    cot = torch.from_numpy(np.random.default_rng(7).standard_normal((B, 8, 8))).cuda()
+    start = [(129, 3)] * (B // 2) + [(1025, 8)] * (B - B // 2)  # low starts escalate
+    res, out, st = C.accept(coeffs, start, tol, cot=cot)
+    assert st["rounds"].max() > 1
+    for i, (S, n) in enumerate(res):
+        A = torch_ops.sample_fourier(coeffs[i:i + 1], 2 * S - 1)
+        assert torch.equal(out[i], torch_ops.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole")[0]), i
+        ref = torch_ops.vjp(n, A, cot[i:i + 1], None, 0.0, 1.0, op="sum", integrator="Boole")[0]
+        assert torch.equal(C.dA_of(st, i), ref), i

@@ -612,7 +612,9 @@ def test_d8_wbasis_levels_run_fused_and_state_their_work():
 def test_d8_boole_runs_the_single_pass_sweep(B, n):
     """Carry-free d = 8 Boole problems (the controller's) take the single-pass
     sweep: one launch per call, no level launches; it states that it reads the
-    samples once (and, for the VJP's recompute, writes the n - 1 levels).  The
+    samples once (and, for the VJP's recompute, writes the n - 1 levels), and
+    the VJP's reverse sweep is one more launch that reads the samples and the
+    levels and writes dA.  The
     forward (many, sum), the all-orders forward and the VJP match the oracle;
     an odd batch leaves the last wavefront's partner lanes empty, and each
     instance's results do not depend on the batch (bit for bit)."""
@@ -641,7 +643,20 @@ def test_d8_boole_runs_the_single_pass_sweep(B, n):
     cot = np.stack([np.random.default_rng([43, b, n, 1]).standard_normal((d, d)) for b in range(B)])
     out_sum = torch_ops.forward(n, At, 0.0, 1.0, op="sum", integrator="Boole").cpu().numpy()
     orders = torch_ops.forward_orders(n, At, 0.0, 1.0, integrator="Boole").cpu().numpy()
-    dA = torch_ops.vjp(n, At, torch.from_numpy(cot).cuda(), None, 0.0, 1.0, op="sum", integrator="Boole")
+    capi.timing_enable(True)
+    try:
+        dA = torch_ops.vjp(n, At, torch.from_numpy(cot).cuda(), None, 0.0, 1.0, op="sum", integrator="Boole")
+        torch.cuda.synchronize()
+    finally:
+        capi.timing_enable(False)
+    # the VJP keeps the levels in its sweep and walks them back in one launch
+    assert capi.timing_read("wbd8_sweep")[0] == 1 and capi.timing_counts("wbd8_sweep")[0] == 8 * B * S * m * n
+    assert capi.timing_read("wbd8_rev")[0] == 1
+    by, fl = capi.timing_counts("wbd8_rev")
+    assert by == 8 * B * S * m * (K + 2) and fl == B * S * K * (4 * d ** 3 + 10 * m)
+    for name in ("lm_vjp_init", "lm_vjp_prod", "lm_vjp_scan", "lm_vjp_final"):
+        assert capi.timing_read(name)[0] == 0, name
+    capi.timing_reset()
     dA = dA.cpu().numpy()
     out = out.cpu().numpy()
     for b in range(B):
@@ -832,3 +847,63 @@ def test_dense_d32_vjp_central_differences():
         an = np.array([g[i, r, c] for i, r, c in picks])
         assert np.max(np.abs(an - fd)) <= 1e-6 * max(1.0, np.max(np.abs(fd))), np.max(np.abs(an - fd))
     assert rel_fro(g_free, g_carry) <= TOL
+
+
+@pytest.mark.parametrize("d,integrator,op,n", [(8, "Boole", "sum", 8), (8, "Boole", "many", 3), (4, "Simpson", "sum", 5),
+                                               (16, "Boole", "many", 4), (3, "Riemann", "sum", 2)])
+def test_basis_state_vjp_matches_recompute_bitwise(d, integrator, op, n):
+    """A forward that keeps its basis state (levels, sums, totals) and a VJP that
+    starts from it run the same kernels on the same state as the carry-free
+    forward and VJP, so outputs and dA are bit-identical; both match the oracle;
+    the state is n - 1 levels, against the carry's Q (n - 1)."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, S = 3, _samples_for(integrator, 129)
+    A = np.stack([fourier_matrices(np.random.default_rng([61, b, d]), S, d, modes=3) for b in range(B)])
+    At = torch.from_numpy(A).cuda()
+    out_shape = (B, n, d, d) if op == "many" else (B, d, d)
+    cot = torch.from_numpy(np.random.default_rng([61, d, 1]).standard_normal(out_shape)).cuda()
+    kw = dict(op=op, integrator=integrator)
+    ref_out = torch_ops.forward(n, At, 0.0, 1.0, **kw)
+    ref_dA = torch_ops.vjp(n, At, cot, None, 0.0, 1.0, **kw)
+    out, state = torch_ops.forward_basis(n, At, 0.0, 1.0, **kw)
+    assert state.numel() == B * ((n - 1) * S * d * d + (2 * n + 1) * d * d)
+    dA1 = torch_ops.vjp_basis(n, At, cot, state, 0.0, 1.0, **kw)
+    dA2 = torch_ops.vjp_basis(n, At, cot, state, 0.0, 1.0, **kw)  # the VJP does not write the state
+    assert torch.equal(out, ref_out)
+    assert torch.equal(dA1, ref_dA) and torch.equal(dA2, ref_dA)
+    c = cot.cpu().numpy()
+    for b in range(B):
+        assert rel_fro(out[b].cpu().numpy(), O.forward(n, A[b], 0.0, 1.0, **kw)) <= TOL
+        assert rel_fro(dA1[b].cpu().numpy(), O.vjp(n, A[b], c[b], 0.0, 1.0, **kw)) <= TOL
+
+
+def test_basis_state_autograd_and_argument_checks():
+    """magnus(record_vjp=False) keeps the basis state as its residual; gradients
+    match the oracle.  op 'one' has no basis state; a state of another batch is
+    refused before any launch."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, d, n = 2, 65, 8, 6
+    A = torch.from_numpy(np.stack([fourier_matrices(np.random.default_rng([63, b]), S, d, modes=3)
+                                   for b in range(B)])).cuda()
+    A.requires_grad_(True)
+    out = torch_ops.magnus(A, n, 0.0, 1.0, op="sum", integrator="Boole", record_vjp=False)
+    w = torch.from_numpy(gradient_values(tuple(out.shape), (0.2, 1.1))).cuda()
+    (out * w).sum().backward()
+    for b in range(B):
+        ref = O.vjp(n, A[b].detach().cpu().numpy(), w[b].cpu().numpy(), 0.0, 1.0, op="sum", integrator="Boole")
+        assert rel_fro(A.grad[b].cpu().numpy(), ref) <= TOL
+    Ad = A.detach()
+    assert torch_ops.basis_state_elems(n, Ad, op="one") == 0
+    with pytest.raises(ValueError):
+        torch_ops.forward_basis(n, Ad, op="one")
+    _, state = torch_ops.forward_basis(n, Ad, op="sum", integrator="Boole")
+    cot = torch.ones((B, d, d), dtype=torch.float64, device=Ad.device)
+    with pytest.raises(ValueError, match="basis state"):
+        torch_ops.vjp_basis(n, Ad[:1], cot[:1], state, op="sum", integrator="Boole")
+    torch.cuda.synchronize()

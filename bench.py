@@ -546,8 +546,8 @@ def _bench_controller(args, torch, dev):
         # the network's class starts the judge; each round runs the VJP of its
         # finer sweep from the kept basis state, so the accepted round delivers
         # the forward and the VJP
-        start, t1 = C.timed(lambda: [C.CLASSES[k] for k in ctl.predict(
-            C.features(torch_ops.sample_fourier(c_ev, C.FEATURE_GRID), 0.0, 1.0)).cpu().numpy()])
+        start, t1 = C.timed(lambda: ctl.predict(
+            C.features(torch_ops.sample_fourier(c_ev, C.FEATURE_GRID), 0.0, 1.0)).cpu().numpy())
         (res, out, st), t2 = C.timed(lambda: C.accept(c_ev, start, tol, cot=cot))
         stages["learned"] = {"predict_s": round(t1, 3), "judge_and_vjp_s": round(t2, 3), "judge_calls": st["calls"]}
         return res, out, st, 0
@@ -568,7 +568,12 @@ def _bench_controller(args, torch, dev):
     def measure(fn):
         """W untimed warm-up steps, then exactly K timed steps between device
         synchronisations, with the library's per-launch HIP events on; returns the
-        last step's result and the seconds per step."""
+        last step's result and the seconds per step.  Device memory cached by the
+        previous mode (torch's allocator, the library's scratch) is returned first,
+        outside the timed region, so the modes do not stack their working sets."""
+        torch.cuda.synchronize()
+        torch.cuda.empty_cache()
+        capi.lib().magnus_release_scratch()
         for _ in range(args.warmup):
             fn()
         torch.cuda.synchronize()

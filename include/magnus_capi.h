@@ -144,6 +144,12 @@ int magnus_sc_vjp_f64(const magnus_problem* p, uint64_t count, const double* in3
  * problems only.  For resolution control (choosing n per instance). */
 int magnus_forward_orders_f64(const magnus_problem* p, uint64_t count, const double* in, double* out, void* stream);
 
+/* The same for orders first..p->n only (1 <= first <= n): out [count][n-first+1][dim][dim],
+ * row k-first = order k.  A resolution judge that compares orders n and n+1 folds two
+ * orders instead of all of them. */
+int magnus_forward_orders_from_f64(const magnus_problem* p, uint64_t count, uint64_t first, const double* in,
+                                   double* out, void* stream);
+
 /* ---- compact checkpoints of the single-pass SpaceCurve path (no counterpart in
  *      the reference, whose saved state is the full carry above).  The forward
  *      keeps its per-chunk summaries (2(n-1)+1 4-vectors per chunk of samples,

@@ -777,7 +777,9 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
 
 // Magnus sums of every order 1..n from one basis sweep: row k-1 of out is the
 // op="sum" output of order k (its own Q(k) GL nodes).
-int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hipStream_t stream) {
+// Magnus sums of orders first..n from one basis sweep: row k - first of out is
+// the op="sum" output of order k (its own Q(k) GL nodes).
+int run_orders(const Plan& pl, uint64_t count, uint64_t first, const double* in, double* out, hipStream_t stream) {
     DeviceScratch scratch(stream);
     const long long B = (long long)count;
     const size_t S = pl.S, m = pl.m, n = pl.n, Q = pl.Q;
@@ -802,10 +804,50 @@ int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hi
     a.totals = totals;
     a.outputs = 1;
     if (int rc = wbd_sweep(pl, a, stream)) return rc;
-    hipError_t e = hipMemcpy2DAsync(out, n * m * sizeof(double), z0, m * sizeof(double), m * sizeof(double), B,
-                                    hipMemcpyDeviceToDevice, stream);
-    if (e != hipSuccess) return device_error(e, "order-1 copy");
-    for (size_t k = 2; k <= n; ++k) {
+    const size_t rows_out = n - first + 1;
+    hipError_t e = hipSuccess;
+    if (first == 1) {
+        e = hipMemcpy2DAsync(out, rows_out * m * sizeof(double), z0, m * sizeof(double), m * sizeof(double), B,
+                             hipMemcpyDeviceToDevice, stream);
+        if (e != hipSuccess) return device_error(e, "order-1 copy");
+    }
+    const size_t lo = first > 2 ? first : 2;
+    // orders lo..n in one fold launch: every order's GL nodes, flattened
+    size_t nflat = 0;
+    for (size_t k = lo; k <= n; ++k) nflat += (k + 3) / 2;
+    if (n >= lo && nflat <= (size_t)magnus::kArgMaxQ) {
+        double* fpart = scratch.alloc<double>(B * nflat * n * m);
+        double* fwork = scratch.alloc<double>(B * nflat * (2 * (n - 1) + 1) * m);
+        if (!fpart || !fwork) return fail(MAGNUS_ENOMEM, "device allocation failed");
+        magnus::WbdArgs f{};
+        fill_wbd_fold(pl, B, f);
+        f.op = MAGNUS_OP_SUM;
+        f.rows = (int)n;
+        f.z = z;
+        f.z0 = z0;
+        f.totals = totals;
+        f.out = out;
+        f.ostride = (long long)(rows_out * m);
+        f.part = fpart;
+        f.work = fwork;
+        f.lo = (int)lo;
+        f.kout = (int)first;
+        f.nflat = (int)nflat;
+        int fi = 0;
+        for (size_t k = lo; k <= n; ++k) {
+            const size_t Qk = (k + 3) / 2, off = gl_offset(Qk);
+            for (size_t qq = 0; qq < Qk; ++qq, ++fi) {
+                f.w[fi] = pl.gl->w[off + qq];
+                f.x[fi] = pl.gl->x[off + qq];
+                f.korder[fi] = (int)k;
+            }
+        }
+        e = magnus::launch_wbd_fold_orders(f, stream);
+        if (e != hipSuccess) return device_error(e, "fold kernel launch");
+        g_kernel = "wbd";
+        return MAGNUS_OK;
+    }
+    for (size_t k = lo; k <= n; ++k) {
         Plan pk = pl;
         pk.n = k;
         pk.Q = (k + 3) / 2;
@@ -818,8 +860,8 @@ int run_orders(const Plan& pl, uint64_t count, const double* in, double* out, hi
         f.z = z;
         f.z0 = z0;
         f.totals = totals;
-        f.out = out + (k - 1) * m;
-        f.ostride = (long long)(n * m);
+        f.out = out + (k - first) * m;
+        f.ostride = (long long)(rows_out * m);
         f.part = part;
         f.work = work;
         e = magnus::launch_wbd_fold(f, stream);
@@ -1062,7 +1104,18 @@ int magnus_forward_orders_f64(const magnus_problem* p, uint64_t count, const dou
     if (pl.sc) return fail(MAGNUS_EINVAL, "all-orders forward takes matrix problems (not the SpaceCurve backend)");
     if (count == 0) return MAGNUS_OK;
     if (!in || !out) return fail(MAGNUS_EINVAL, "null argument");
-    return run_orders(pl, count, in, out, (hipStream_t)stream);
+    return run_orders(pl, count, 1, in, out, (hipStream_t)stream);
+}
+
+int magnus_forward_orders_from_f64(const magnus_problem* p, uint64_t count, uint64_t first, const double* in,
+                                   double* out, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, true, &pl)) return rc;
+    if (pl.sc) return fail(MAGNUS_EINVAL, "all-orders forward takes matrix problems (not the SpaceCurve backend)");
+    if (first < 1 || first > pl.n) return fail(MAGNUS_EINVAL, "first order must lie in 1..n");
+    if (count == 0) return MAGNUS_OK;
+    if (!in || !out) return fail(MAGNUS_EINVAL, "null argument");
+    return run_orders(pl, count, first, in, out, (hipStream_t)stream);
 }
 
 uint64_t magnus_basis_state_elems(const magnus_problem* p, uint64_t count) {

@@ -125,8 +125,14 @@ struct WbdArgs {
     int rows;              // row stride of z / totals / zbar / wbar (0: n); rows >= n
     double w[kArgMaxQ];
     double x[kArgMaxQ];
+    // all-orders fold (launch_wbd_fold_orders): node f of the flattened GL
+    // tables of orders lo..n has weight w[f], node x[f] and order korder[f];
+    // part / work rows are per (instance, f); out row k-1 gets order k's sum
+    int nflat, lo, kout;  // kout: the order of out's row 0
+    int korder[kArgMaxQ];
 };
 hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s);      // -> out
+hipError_t launch_wbd_fold_orders(const WbdArgs& a, hipStream_t s);  // -> out rows lo-kout..n-kout (op sum)
 hipError_t launch_wbd_fold_vjp(const WbdArgs& a, hipStream_t s);  // -> zbar, wbar
 
 }  // namespace magnus

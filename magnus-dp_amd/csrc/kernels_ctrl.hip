@@ -26,6 +26,45 @@ constexpr int kThreads = 256;
 constexpr int kMaxG = 129;
 constexpr int kHarm = 16;
 
+// The same iteration with the dimension known at compile time: the matrix and
+// both vectors stay in registers (a runtime dimension indexes them in scratch).
+template <int D>
+__device__ double spectral_norm_d(const double* __restrict__ Ag) {
+    double A[D * D], v[D], w[D];
+#pragma unroll
+    for (int e = 0; e < D * D; ++e) A[e] = Ag[e];
+#pragma unroll
+    for (int i = 0; i < D; ++i) v[i] = 1.0 + 0.01 * i;
+    double lam = 0.0;
+    for (int it = 0; it < 60; ++it) {
+        double nv = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) {
+            double acc = 0.0;
+#pragma unroll
+            for (int k = 0; k < D; ++k) acc = fma(A[i * D + k], v[k], acc);
+            w[i] = acc;
+        }
+#pragma unroll
+        for (int k = 0; k < D; ++k) {
+            double acc = 0.0;
+#pragma unroll
+            for (int i = 0; i < D; ++i) acc = fma(A[i * D + k], w[i], acc);
+            v[k] = acc;
+            nv = fma(acc, acc, nv);
+        }
+        nv = sqrt(nv);
+        if (nv == 0.0) return 0.0;
+        double wn = 0.0;
+#pragma unroll
+        for (int i = 0; i < D; ++i) wn = fma(w[i], w[i], wn);
+        lam = sqrt(wn);
+#pragma unroll
+        for (int k = 0; k < D; ++k) v[k] /= nv;
+    }
+    return lam;
+}
+
 __device__ double spectral_norm(const double* A, int d) {
     // power iteration on A^T A from a fixed start with all components nonzero
     double v[16], w[16];
@@ -54,6 +93,8 @@ __device__ double spectral_norm(const double* A, int d) {
     return lam;
 }
 
+// D > 0: the dimension as a compile-time constant (the controller's d = 8)
+template <int D>
 __global__ void __launch_bounds__(kThreads) ctrl_features(CtrlArgs a) {
     const long long b = blockIdx.x;
     const int d = a.d, m = d * d, G = a.G, tid = threadIdx.x;
@@ -64,25 +105,27 @@ __global__ void __launch_bounds__(kThreads) ctrl_features(CtrlArgs a) {
     __shared__ double s_total;
     __shared__ double s_comm[4];
     __shared__ double s_mean[256];
+    __shared__ double s_cs[kMaxG], s_sn[kMaxG];  // twiddles of the periodic DFT
+    const int N = G - 1;
     // per-sample norms
     for (int g = tid; g < G; g += kThreads) {
         const double* Ag = A + (long long)g * m;
         double f = 0.0;
         for (int e = 0; e < m; ++e) f = fma(Ag[e], Ag[e], f);
         s_fro[g] = sqrt(f);
-        s_spec[g] = spectral_norm(Ag, d);
+        s_spec[g] = D > 0 ? spectral_norm_d<(D > 0 ? D : 1)>(Ag) : spectral_norm(Ag, d);
     }
+    for (int j = tid; j < N; j += kThreads) sincospi(2.0 * (double)j / N, &s_sn[j], &s_cs[j]);
     // harmonic energies of the periodic DFT over N = G - 1 points
     if (tid <= kHarm) s_energy[tid] = 0.0;
     if (tid == 0) s_total = 0.0;
     __syncthreads();
-    const int N = G - 1;
     for (int p = tid; p < kHarm * m; p += kThreads) {
         const int k = 1 + p / m, e = p - (k - 1) * m;
         double re = 0.0, im = 0.0;
         for (int i = 0; i < N; ++i) {
-            double sn, cs;
-            sincospi(2.0 * (double)((long long)k * i % N) / N, &sn, &cs);
+            const int j = (int)((long long)k * i % N);
+            const double sn = s_sn[j], cs = s_cs[j];
             const double x = A[(long long)i * m + e];
             re = fma(x, cs, re);
             im = fma(-x, sn, im);
@@ -156,7 +199,10 @@ __global__ void __launch_bounds__(kThreads) ctrl_features(CtrlArgs a) {
 hipError_t launch_ctrl_features(const CtrlArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     if (a.d < 1 || a.d > 16 || a.G < 9 || a.G > kMaxG || ((a.G - 1) & (a.G - 2)) != 0) return hipErrorInvalidValue;
-    hipLaunchKernelGGL(ctrl_features, dim3((unsigned)a.B), dim3(kThreads), 0, s, a);
+    if (a.d == 8)
+        hipLaunchKernelGGL(ctrl_features<8>, dim3((unsigned)a.B), dim3(kThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(ctrl_features<0>, dim3((unsigned)a.B), dim3(kThreads), 0, s, a);
     return hipGetLastError();
 }
 

@@ -185,6 +185,53 @@ __global__ void __launch_bounds__(kFoldThreads) wbd_fold(WbdArgs a) {
     }
 }
 
+// All orders lo..n at once: one workgroup per (instance, flattened node f),
+// node f of order korder[f] folding that order's Omega_2..Omega_k shares;
+// the same arithmetic as wbd_fold for each order
+template <int CAP>
+__global__ void __launch_bounds__(kFoldThreads) wbd_fold_orders(WbdArgs a) {
+    __shared__ FoldLds<CAP> L;
+    const long long b = blockIdx.x / a.nflat;
+    const int f = (int)(blockIdx.x - b * a.nflat);
+    const int k = a.korder[f], nmax = a.n, Kmax = nmax - 1, d = a.d;
+    const long long M = a.m;
+    double* T = a.work + (long long)blockIdx.x * (2 * Kmax + 1) * M;  // T_j at T + j M
+    double* Om = a.part + ((long long)blockIdx.x * nmax) * M;          // Omega_j at Om + (j-1) M, j = 2..k
+    const double* What = a.totals + b * (long long)row_stride(a) * M;
+    const double s = a.x[f] - 1.0, wq = a.w[f];
+    WbdArgs ak = a;
+    ak.n = k;  // fold_totals runs T_1..T_{k-1}
+    fold_totals(L, ak, What, T, s);
+    for (int j = 2; j <= k; ++j) {
+        double* O = Om + (j - 1) * M;
+        mat_set<CAP>(O, wq, zrow(a, b, j - 1), d);
+        for (int l = 0; l <= j - 2; ++l) mat_acc(L, O, wq * s, zrow(a, b, l), false, T + (j - 1 - l) * M, false, d);
+    }
+}
+
+// out row k - kout (op sum of order k, lo <= k <= n) from Z_0 and the node partials of
+// order k, Omega_j's nodes added in q order, then the Omega_j in j order
+__global__ void __launch_bounds__(256) wbd_fold_orders_reduce(WbdArgs a) {
+    const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    const long long M = a.m;
+    const int nk = a.n - a.lo + 1;
+    if (idx >= a.B * nk * M) return;
+    const long long b = idx / (nk * M);
+    const long long r = idx - b * nk * M;
+    const int k = a.lo + (int)(r / M);
+    const long long e = r - (long long)(k - a.lo) * M;
+    int f0 = 0;
+    while (f0 < a.nflat && a.korder[f0] != k) ++f0;
+    const int Q = (k + 3) / 2;
+    double v = a.z0[b * M + e];
+    for (int j = 2; j <= k; ++j) {
+        double om = 0.0;
+        for (int q = 0; q < Q; ++q) om += a.part[(((b * a.nflat) + f0 + q) * a.n + (j - 1)) * M + e];
+        v += om;
+    }
+    a.out[b * a.ostride + (long long)(k - a.kout) * M + e] = v;
+}
+
 // out from Omega_1 = Z_0 and the node partials, nodes added in q order
 __global__ void __launch_bounds__(256) wbd_fold_reduce(WbdArgs a) {
     const long long idx = (long long)blockIdx.x * blockDim.x + threadIdx.x;
@@ -285,6 +332,20 @@ hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s) {
     else
         hipLaunchKernelGGL(wbd_fold<kFoldLds>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
     hipLaunchKernelGGL(wbd_fold_reduce, dim3((unsigned)((a.B * a.m + 255) / 256)), dim3(256), 0, s, a);
+    timer_end(h);
+    return hipGetLastError();
+}
+
+hipError_t launch_wbd_fold_orders(const WbdArgs& a, hipStream_t s) {
+    if (a.B == 0) return hipSuccess;
+    void* h = timer_begin("wbd_fold", s);
+    const unsigned grid = (unsigned)(a.B * a.nflat);
+    if (a.d <= kFoldLdsSmall)
+        hipLaunchKernelGGL(wbd_fold_orders<kFoldLdsSmall>, dim3(grid), dim3(kFoldThreads), 0, s, a);
+    else
+        hipLaunchKernelGGL(wbd_fold_orders<kFoldLds>, dim3(grid), dim3(kFoldThreads), 0, s, a);
+    const long long outs = a.B * (a.n - a.lo + 1) * a.m;
+    hipLaunchKernelGGL(wbd_fold_orders_reduce, dim3((unsigned)((outs + 255) / 256)), dim3(256), 0, s, a);
     timer_end(h);
     return hipGetLastError();
 }

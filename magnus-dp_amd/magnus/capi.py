@@ -54,6 +54,7 @@ def lib() -> ctypes.CDLL:
         for name in ("magnus_vjp_f64", "magnus_sc_vjp_f64"):
             getattr(L, name).argtypes = [P, u64, vp, vp, vp, vp, vp]
         L.magnus_forward_orders_f64.argtypes = [P, u64, vp, vp, vp]
+        L.magnus_forward_orders_from_f64.argtypes = [P, u64, u64, vp, vp, vp]
         L.magnus_sc_forward_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp]
         L.magnus_sc_vjp_ckpt_f64.argtypes = [P, u64, vp, vp, vp, vp, vp]
         L.magnus_sc_vjp_noalloc_f64.argtypes = [P, u64, vp, vp, vp, vp]

@@ -89,9 +89,10 @@ def features(A_coarse: torch.Tensor, t0: float, tf: float) -> torch.Tensor:
     return out
 
 
-def sums(coeffs: torch.Tensor, S: int, n: int, integrator: str = "Boole") -> torch.Tensor:
-    """Magnus sums of orders 1..n at S samples: [B][n][d][d]."""
-    return torch_ops.forward_orders(n, torch_ops.sample_fourier(coeffs, S), 0.0, 1.0, integrator=integrator)
+def sums(coeffs: torch.Tensor, S: int, n: int, integrator: str = "Boole", first: int = 1) -> torch.Tensor:
+    """Magnus sums of orders first..n at S samples: [B][n - first + 1][d][d]."""
+    return torch_ops.forward_orders(n, torch_ops.sample_fourier(coeffs, S), 0.0, 1.0, integrator=integrator,
+                                    first=first)
 
 
 def rel_err(x: torch.Tensor, ref: torch.Tensor) -> torch.Tensor:
@@ -211,7 +212,7 @@ class MLP:
 class Controller:
     """Feature standardisation + MLP + the verified escalation."""
 
-    def __init__(self, in_dim: int, hidden: int = 128, seed: int = 0, max_batch: int = 1024):
+    def __init__(self, in_dim: int, hidden: int = 128, seed: int = 0, max_batch: int = 16384):
         self.net = MLP(in_dim, hidden, len(CLASSES), max_batch, seed)
         self.mu = None
         self.sd = None
@@ -241,10 +242,16 @@ class Controller:
 
 
 # --------------------------------------------------------------------- acceptance
-def _chunks(ids, bytes_per_instance, budget=24e9):
-    """Split an id array so that a call's device buffers stay within ~budget bytes
-    (large enough that a call's sweep launch fills the GPU: a workgroup per
-    instance pair, several hundred pairs per launch)."""
+def _budget() -> float:
+    """Device bytes one call of a round may use: a share of the free memory per
+    stream of the pool, at most 32 GB (large calls fill the GPU: a sweep launch
+    has a workgroup per instance pair)."""
+    free, _ = torch.cuda.mem_get_info()
+    return max(2e9, min(32e9, 0.5 * free / _POOL[0]))
+
+
+def _chunks(ids, bytes_per_instance, budget):
+    """Split an id array so that a call's device buffers stay within ~budget bytes."""
     per = max(1, int(budget // max(1, bytes_per_instance)))
     return [ids[k:k + per] for k in range(0, len(ids), per)]
 
@@ -280,6 +287,14 @@ def _groups(S_arr, n_arr, ids):
             for k, part in zip(key_sorted[np.r_[0, cuts]], np.split(order, cuts))]
 
 
+def _resolutions(start):
+    """(S, n) arrays from a list of (S, n) or an array of class indices."""
+    if isinstance(start, np.ndarray) and start.ndim == 1 and np.issubdtype(start.dtype, np.integer):
+        k = start.astype(np.int64)
+        return (np.asarray(S_GRID, np.int64)[k // len(N_GRID)], np.asarray(N_GRID, np.int64)[k % len(N_GRID)])
+    return (np.array([r[0] for r in start], dtype=np.int64), np.array([r[1] for r in start], dtype=np.int64))
+
+
 def delivered(res):
     """The resolution a judged class (S, n) delivers: the finer member of its doubling pair."""
     return [(2 * S - 1, n) for S, n in res]
@@ -296,8 +311,7 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
     of a round run on a pool of streams; the host waits once per round and
     updates every instance with array operations."""
     B, d = coeffs.shape[0], coeffs.shape[-1]
-    S_arr = np.array([r[0] for r in start], dtype=np.int64)
-    n_arr = np.array([r[1] for r in start], dtype=np.int64)
+    S_arr, n_arr = _resolutions(start)
     done = np.zeros(B, bool)
     rounds = np.zeros(B, np.int32)
     out = torch.empty_like(coeffs[:, 0])
@@ -308,6 +322,7 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
     main = torch.cuda.current_stream()
     pool = _streams()
     S_top = max(S_GRID)
+    budget = _budget()
     for _ in range(max_rounds):
         active = np.flatnonzero(~done)
         if active.size == 0:
@@ -319,7 +334,7 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
             # per instance: both sweeps' samples; with the VJP the finer sweep's
             # kept levels, its dA and the reverse sweep's buffers
             per = (S + S2) * d * d * 8 * (2 if cot is None else n + 4)
-            for ids in _chunks(all_ids, per):
+            for ids in _chunks(all_ids, per, budget):
                 st = pool[k % len(pool)]
                 k += 1
                 st.wait_stream(main)
@@ -327,8 +342,8 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
                     idx = _index(ids, coeffs.device)
                     c = coeffs[idx]
                     nm = min(n + 1, REF_N)
-                    lo = sums(c, S, nm)  # orders 1..nm at S samples
-                    cur = lo[:, n - 1]
+                    lo = sums(c, S, nm, first=n)  # orders n..nm at S samples
+                    cur = lo[:, 0]
                     A2 = torch_ops.sample_fourier(c, S2)
                     if cot is not None:
                         fine, state = torch_ops.forward_basis(n, A2, 0.0, 1.0, op="sum", integrator="Boole")
@@ -345,7 +360,7 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
                         s_ok = rel_err(fine, cur) <= JUDGE_FINE * tol
                     else:  # the finer member is the reference resolution
                         s_ok = torch.ones(len(ids), dtype=torch.bool, device=coeffs.device)
-                    n_ok = (rel_err(lo[:, nm - 1], cur) <= JUDGE_MARGIN * tol if nm > n
+                    n_ok = (rel_err(lo[:, nm - n], cur) <= JUDGE_MARGIN * tol if nm > n
                             else torch.ones(len(ids), dtype=torch.bool, device=coeffs.device))
                     out[idx] = fine
                     ids_l.append(ids)
@@ -394,10 +409,11 @@ def solve(coeffs: torch.Tensor, res, cot: torch.Tensor, keep_dA: bool = False, f
     work = 0
     main = torch.cuda.current_stream()
     pool = _streams()
+    budget = _budget()
     k = 0
     for S, n, all_ids in _groups(S_arr, n_arr, np.arange(len(res))):
         # per instance: samples, dA, the n - 1 kept levels and the reverse sweep's buffers
-        for ids in _chunks(all_ids, S * d * d * 8 * (n + 4)):
+        for ids in _chunks(all_ids, S * d * d * 8 * (n + 4), budget):
             st = pool[k % len(pool)]
             k += 1
             st.wait_stream(main)

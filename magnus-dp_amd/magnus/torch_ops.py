@@ -78,17 +78,21 @@ def forward(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: st
 
 
 def forward_orders(n: int, A: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, matrix_backend: str = "Auto",
-                   integrator: str = "Auto"):
-    """Magnus sums of orders 1..n from one basis sweep: (B, n, d, d), row k-1 =
-    forward(k, A, op="sum")."""
+                   integrator: str = "Auto", first: int = 1):
+    """Magnus sums of orders first..n from one basis sweep: (B, n - first + 1, d, d),
+    row k - first = forward(k, A, op="sum")."""
     _check_input(A, 4, "A")
     A = A.contiguous()
     B, S, d, d2 = A.shape
     if d != d2:
         raise ValueError("data must contain square matrices")
     p = capi.problem(n, d, S, t0, tf, "sum", matrix_backend, integrator)
-    out = torch.empty((B, n, d, d), dtype=A.dtype, device=A.device)
-    capi.check(capi.lib().magnus_forward_orders_f64(ctypes.byref(p), B, _ptr(A), _ptr(out), _stream()))
+    out = torch.empty((B, n - first + 1, d, d), dtype=A.dtype, device=A.device)
+    if first == 1:
+        capi.check(capi.lib().magnus_forward_orders_f64(ctypes.byref(p), B, _ptr(A), _ptr(out), _stream()))
+    else:
+        capi.check(capi.lib().magnus_forward_orders_from_f64(ctypes.byref(p), B, first, _ptr(A), _ptr(out),
+                                                             _stream()))
     return out
 
 

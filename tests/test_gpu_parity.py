@@ -907,3 +907,24 @@ def test_basis_state_autograd_and_argument_checks():
     with pytest.raises(ValueError, match="basis state"):
         torch_ops.vjp_basis(n, Ad[:1], cot[:1], state, op="sum", integrator="Boole")
     torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("d,integrator", [(8, "Boole"), (4, "Simpson")])
+def test_forward_orders_from_equals_the_all_orders_rows(d, integrator):
+    """Orders first..n from one sweep are the same values (bit for bit) as those
+    rows of the all-orders forward; first outside 1..n is refused."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, n = 3, _samples_for(integrator, 129), 8
+    A = torch.from_numpy(np.stack([fourier_matrices(np.random.default_rng([67, b]), S, d, modes=3)
+                                   for b in range(B)])).cuda()
+    full = torch_ops.forward_orders(n, A, integrator=integrator)
+    for first in (1, 2, 5, 8):
+        part = torch_ops.forward_orders(n, A, integrator=integrator, first=first)
+        assert part.shape == (B, n - first + 1, d, d)
+        assert torch.equal(part, full[:, first - 1:]), first
+    with pytest.raises(ValueError):
+        torch_ops.forward_orders(n, A, integrator=integrator, first=9)
+    torch.cuda.synchronize()

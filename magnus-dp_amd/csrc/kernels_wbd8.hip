@@ -415,7 +415,10 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
     }
     double up0 = 0.0, up1 = 0.0, pend0 = 0.0, pend1 = 0.0;
 
-    // W_j's transposed entries of a block, two blocks in flight ahead of the one in use
+    // W_j's transposed entries of a block, two blocks in flight ahead of the one in
+    // use.  The three buffers change roles by renaming (the active steps are
+    // unrolled by three), never by copying: a copy of a register whose load is in
+    // flight would wait for that load.
     auto loadw = [&](double (*y)[2], long long blk) {
 #pragma unroll
         for (int p = 0; p < 4; ++p) {
@@ -424,12 +427,15 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
             y[p][1] = Wj[o + t1];
         }
     };
-    double wt[4][2] = {}, wn[4][2] = {}, wn2[4][2] = {};
-    loadw(wt, nb - 1);
-    if (nb > 1) loadw(wn, nb - 2);
+    double wa[4][2] = {}, wbuf[4][2] = {}, wc[4][2] = {};
+    loadw(wa, nb - 1);
+    if (nb > 1) loadw(wbuf, nb - 2);
     double yo[4][2] = {}, yt[4][2] = {}, part[4][2] = {};
-    for (long long t = 0; t < nb + K - 1; ++t) {
-        const long long r = t - w;  // ring index of this wave's block
+
+    // one pipeline step; r = ring index of this wave's block (or out of range:
+    // the wave only takes part in the barriers), wt: W_j entries of the block,
+    // wpf: the buffer for the block two below
+    auto step = [&](long long r, double (*wt)[2], double (*wpf)[2]) {
         const long long blk = nb - 1 - r;
         const bool act = r >= 0 && blk >= 0;
         if (!TOP && act) {  // the previous wave handed this block over in the previous step
@@ -446,7 +452,7 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
         step_barrier();  // the block is in the ring; every read of the hand blocks precedes their rewrite
         if (act) {
             const long long s = 4 * blk;
-            if (blk > 1) loadw(wn2, blk - 2);
+            if (blk > 1) loadw(wpf, blk - 2);
             double cz[4];
 #pragma unroll
             for (int p = 0; p < 4; ++p) cz[p] = boole_c(s + 1 + p, S, bw);
@@ -522,16 +528,21 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
                     o[o1] = pick1(acc[p], inst) + yn[p][1];
                 }
             }
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                wt[p][0] = wn[p][0];
-                wt[p][1] = wn[p][1];
-                wn[p][0] = wn2[p][0];
-                wn[p][1] = wn2[p][1];
-            }
         }
         step_barrier();  // the blocks handed over in this step are visible to the next
+    };
+    // steps before this wave's first block, its nb blocks (buffers rotating by
+    // three), then the steps after its last block
+    for (long long t = 0; t < w; ++t) step(-1, wa, wc);
+    long long r = 0;
+    for (; r + 3 <= nb; r += 3) {
+        step(r, wa, wc);
+        step(r + 1, wbuf, wa);
+        step(r + 2, wc, wbuf);
     }
+    if (r < nb) step(r, wa, wc);
+    if (r + 1 < nb) step(r + 1, wbuf, wa);
+    for (long long t = w + nb; t < nb + K - 1; ++t) step(-1, wa, wc);
     // sample 0: the prefixes are constant there (W_j[0] = 0), so dA_0 = Ybar_1[0]
     // = level 1's pending from the lowest block + c_0 Zbar_0
     if (LAST && q.live) {

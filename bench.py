@@ -847,6 +847,11 @@ def main():
     for k in wbf:
         kernel_flops[k] = B * wbf[k]
         kernel_bytes[k] = B * wbb[k]
+    for k in per_kernel:
+        if k not in kernel_bytes:  # families whose launch sites state their own work
+            by, fl = capi.timing_counts(k)
+            kernel_bytes[k] = by / per_kernel[k]["launches"]
+            kernel_flops[k] = fl / per_kernel[k]["launches"]
     dom = max(per_kernel, key=lambda k: per_kernel[k]["avg_ms"]) if per_kernel else None
     roofline = None
     if dom in kernel_bytes:
@@ -862,6 +867,9 @@ def main():
             roofline = {"bound": "fp64", "achieved": round(flops_ach, 3), "peak": MI355X_FP64_TFLOPS,
                         "unit": "TFLOP/s", "frac": round(flops_ach / MI355X_FP64_TFLOPS, 4), "traffic": None}
         roofline["traffic"] = _measured_traffic(dom, config)
+        if roofline["traffic"] is None:
+            ratio = _measured_traffic_ratio(dom)
+            roofline["traffic"] = round(ratio * kernel_bytes[dom]) if ratio else None
         roofline["kernel"] = dom
         roofline["kernel_avg_ms"] = round(per_kernel[dom]["avg_ms"], 4)
         roofline["algorithmic_bytes_per_launch"] = kernel_bytes[dom]

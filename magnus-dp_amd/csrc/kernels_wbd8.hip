@@ -91,7 +91,15 @@ __device__ __forceinline__ double brule(Acc& s, double f, const BooleW& c) {
 // direct global->LDS loads (global_load_lds_dwordx4: lane l's 16 bytes land
 // at the instruction's LDS base + 16 l) issued by one extra wavefront, the
 // loader, kRingAhead Boole blocks ahead of the first level's wave; a block
-// (4 samples of both instances, 4 KB) is four instructions.  The loader is the
+// (4 samples of both instances, 4 KB) is four instructions, one per sample.
+// Within a sample's 1 KB the 16-byte chunk (A[r][2h], A[r][2h+1]) of instance
+// inst sits at chunk position chunk_pos(inst, r, h), a GF(2)-linear bijection
+// of the bits (inst, r, h) chosen so that the waves' reads -- A[c][g] and
+// A[c][g + 4] (the A operand), A[g][c] and A[g + 4][c] (own entries) -- hit
+// distinct bank pairs in every half-wave (64 banks) and in every 16-lane
+// quarter (32 banks) except for the A operand's read, which is then two-way
+// (its 16 lanes read one half of 8 chunks); the loader lane that fills
+// position l reads chunk chunk_src(l).  The loader is the
 // only wave with vector memory loads in flight, so it can wait for a block
 // with a vmcnt that counts only those loads (the blocks issued after it), and
 // the workgroup barrier that follows publishes the block to every wave.  The
@@ -99,6 +107,22 @@ __device__ __forceinline__ double brule(Acc& s, double f, const BooleW& c) {
 // later, so the ring holds K + kRingAhead blocks.
 // ---------------------------------------------------------------------------
 constexpr int kRingAhead = 6;
+
+// the bits (inst, r0, r1, r2, h0, h1) of a chunk, low first, map to these
+// position masks (XORed); chunk_src applies the inverse map
+__device__ __forceinline__ int chunk_pos(int inst, int r, int h) {
+    const int x = inst | (r << 1) | (h << 4);
+    return (-(x & 1) & 15) ^ (-((x >> 1) & 1) & 42) ^ (-((x >> 2) & 1) & 54) ^ (-((x >> 3) & 1) & 24) ^
+           (-((x >> 4) & 1) & 38) ^ (-((x >> 5) & 1) & 36);
+}
+__device__ __forceinline__ int chunk_src(int pos) {
+    return (-(pos & 1) & 35) ^ (-((pos >> 1) & 1) & 48) ^ (-((pos >> 2) & 1) & 14) ^ (-((pos >> 3) & 1) & 28) ^
+           (-((pos >> 4) & 1) & 20) ^ (-((pos >> 5) & 1) & 46);
+}
+// the LDS word of entry (r, col) of instance inst within a ring sample
+__device__ __forceinline__ int ring_word(int inst, int r, int col) {
+    return 2 * chunk_pos(inst, r, col >> 1) + (col & 1);
+}
 
 template <int N>
 __device__ __forceinline__ void wait_vm() {  // vmcnt <= N (expcnt, lgkmcnt unconstrained)
@@ -127,18 +151,17 @@ __device__ __forceinline__ void wait_blocks(int blocks) {
 // accesses across the barrier.
 __device__ __forceinline__ void step_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
-// one ring slot: [2 instances][4 samples x 64 entries]
-using RingSlot = double[2][4 * kM];
+// one ring slot: [4 samples][64 chunks of both instances]
+using RingSlot = double[4][2 * kM];
 
-// the loader's loads of Boole block blk (samples 4 blk + 1 .. 4 blk + 4) of
-// instances A0 and A1 into `slot` (every lane of the loader calls)
-__device__ __forceinline__ void ring_issue(RingSlot& slot, const double* A0, const double* A1, long long blk, int lane) {
+// the loader's loads of Boole block blk (samples 4 blk + 1 .. 4 blk + 4) into
+// `slot`; src: this lane's chunk in sample 0 of its instance (every lane of
+// the loader calls)
+__device__ __forceinline__ void ring_issue(RingSlot& slot, const double* src, long long blk) {
 #pragma unroll
-    for (int k = 0; k < 4; ++k) {
-        const double* src = (k < 2 ? A0 : A1) + (4 * blk + 1) * (long long)kM + (k & 1) * 128 + 2 * lane;
-        __builtin_amdgcn_global_load_lds(src, (__attribute__((address_space(3))) void*)&slot[k >> 1][(k & 1) * 128],
-                                         16, 0, 0);
-    }
+    for (int p = 0; p < 4; ++p)
+        __builtin_amdgcn_global_load_lds(src + (4 * blk + 1 + p) * (long long)kM,
+                                         (__attribute__((address_space(3))) void*)&slot[p][0], 16, 0, 0);
 }
 
 // Per-instance-pair context shared by the waves of a workgroup.
@@ -156,14 +179,16 @@ struct Pair {
 // four samples' ring reads and products are independent and overlap; only the
 // prefix recurrences run sample after sample.
 template <int K, bool LEVELS, bool FIRST, bool MAKES>
-__device__ __forceinline__ void sweep_wave(const LmArgs& a, const Pair& q, int w, const RingSlot* ring,
-                                           double (*hand)[4][2 * kM]) {
+__device__ __forceinline__ void sweep_wave(const LmArgs& a, const Pair& q, int w, int lane, const RingSlot* ring,
+                                           double (*hand)[4][2][64]) {
     constexpr int R = K + kRingAhead;
     const long long S = q.S, nb = q.nb;
     const int j = w + 1;
     const int c = q.c, g = q.g, inst = q.inst;
     const int o0 = g * 8 + c, o1 = (g + 4) * 8 + c;  // own entries (g, c), (g + 4, c)
-    const int h0 = inst * kM + o0, h1 = inst * kM + o1;
+    // ring words of the A operand entries (c, g), (c, g + 4) and the own entries
+    const int ra0 = ring_word(inst, c, g), ra1 = ring_word(inst, c, g + 4);
+    const int ro0 = ring_word(inst, g, c), ro1 = ring_word(inst, g + 4, c);
     const double* __restrict__ A = a.in + q.bl * S * kM;
     double* __restrict__ W = LEVELS ? a.carry + q.bl * (long long)K * S * kM : nullptr;
     const BooleW bw(a.dt);
@@ -190,27 +215,27 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, const Pair& q, int w
         if (!FIRST && act) {  // the previous wave formed this block in the previous step
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
-                win[p][0] = hand[w - 1][p][h0];
-                win[p][1] = hand[w - 1][p][h1];
+                win[p][0] = hand[w - 1][p][0][lane];
+                win[p][1] = hand[w - 1][p][1][lane];
             }
         }
         step_barrier();  // the block is in the ring; every read of the hand blocks precedes their rewrite
         if (act) {
-            const double* sl = &ring[blk % R][inst][0];
+            const RingSlot& sl = ring[blk % R];
             const long long i0 = 4 * blk;
             double op0[4], op1[4], y[4][2], wv[4][2];
 #pragma unroll
             for (int p = 0; p < 4; ++p) {  // A operand entries A[c][g], A[c][g + 4]
-                op0[p] = sl[p * kM + c * 8 + g];
-                op1[p] = sl[p * kM + c * 8 + 4 + g];
+                op0[p] = sl[p][ra0];
+                op1[p] = sl[p][ra1];
             }
             const double cz[4] = {bw.b64, bw.b24, bw.b64, blk + 1 < nb ? bw.b28 : bw.b14};
             if constexpr (FIRST) {  // W_1 = Prefix(A) and Z_0 over the block
                 double a0[4], a1[4];
 #pragma unroll
                 for (int p = 0; p < 4; ++p) {
-                    a0[p] = sl[p * kM + o0];
-                    a1[p] = sl[p * kM + o1];
+                    a0[p] = sl[p][ro0];
+                    a1[p] = sl[p][ro1];
                     z1[0] = fma(cz[p], a0[p], z1[0]);
                     z1[1] = fma(cz[p], a1[p], z1[1]);
                 }
@@ -253,8 +278,8 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, const Pair& q, int w
                 v[3][1] = brule<4>(sn[1], y[3][1], bw);
 #pragma unroll
                 for (int p = 0; p < 4; ++p) {
-                    hand[w][p][h0] = v[p][0];
-                    hand[w][p][h1] = v[p][1];
+                    hand[w][p][0][lane] = v[p][0];
+                    hand[w][p][1][lane] = v[p][1];
                 }
             }
             if (LEVELS && q.live) {  // the levels this wave forms, kept for the reverse sweep
@@ -296,10 +321,12 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, const Pair& q, int w
 template <int R, typename BlkOf>
 __device__ __forceinline__ void ring_loader(RingSlot* ring, const double* A0, const double* A1, long long nb,
                                             long long steps, int lane, BlkOf blk_of) {
-    for (long long r = 0; r < kRingAhead - 1 && r < nb; ++r) ring_issue(ring[r % R], A0, A1, blk_of(r), lane);
+    const int x = chunk_src(lane);  // (inst, r, h) of the chunk this lane places
+    const double* src = ((x & 1) ? A1 : A0) + ((x >> 1) & 7) * 8 + 2 * (x >> 4);
+    for (long long r = 0; r < kRingAhead - 1 && r < nb; ++r) ring_issue(ring[r % R], src, blk_of(r));
     for (long long t = 0; t < steps; ++t) {
         const long long r = t + kRingAhead - 1;
-        if (r < nb) ring_issue(ring[r % R], A0, A1, blk_of(r), lane);
+        if (r < nb) ring_issue(ring[r % R], src, blk_of(r));
         if (t < nb) wait_blocks((int)((r < nb ? r : nb - 1) - t));
         step_barrier();
         step_barrier();
@@ -309,11 +336,11 @@ __device__ __forceinline__ void ring_loader(RingSlot* ring, const double* A0, co
 // one workgroup per pair of instances (2x, 2x + 1): K level wavefronts and the
 // loader; a missing partner's lanes read the live instance and store nothing
 template <int K, bool LEVELS>
-__global__ void __launch_bounds__(64 * (K + 1)) wbd8_sweep(LmArgs a) {
+__global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_eu(4))) wbd8_sweep(LmArgs a) {
     constexpr int R = K + kRingAhead;  // ring slots
     __shared__ RingSlot ring[R];
-    // level blocks handed from wave w to wave w + 1, [4 samples][2 instances x 64 entries]
-    __shared__ double hand[K > 1 ? K - 1 : 1][4][2 * kM];
+    // level blocks handed from wave w to wave w + 1, [4 samples][2 entries][lane]
+    __shared__ double hand[K > 1 ? K - 1 : 1][4][2][64];
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     Pair q;
     q.c = lane & 7;
@@ -329,11 +356,11 @@ __global__ void __launch_bounds__(64 * (K + 1)) wbd8_sweep(LmArgs a) {
         ring_loader<R>(ring, a.in + b0 * q.S * kM, a.in + b1 * q.S * kM, q.nb, q.nb + K - 1, lane,
                        [](long long r) { return r; });
     } else if (w == 0) {
-        sweep_wave<K, LEVELS, true, (K > 1)>(a, q, w, ring, hand);
+        sweep_wave<K, LEVELS, true, (K > 1)>(a, q, w, lane, ring, hand);
     } else if (w < K - 1) {
-        sweep_wave<K, LEVELS, false, true>(a, q, w, ring, hand);
+        sweep_wave<K, LEVELS, false, true>(a, q, w, lane, ring, hand);
     } else {
-        sweep_wave<K, LEVELS, false, false>(a, q, w, ring, hand);
+        sweep_wave<K, LEVELS, false, false>(a, q, w, lane, ring, hand);
     }
 }
 
@@ -384,6 +411,13 @@ __device__ __forceinline__ double boole_c(long long i, long long S, const BooleW
 }
 
 __device__ __forceinline__ double pick0(const dbl4& d, int inst) { return inst ? d[2] : d[0]; }
+
+// LDS word of entry (r, col) of instance inst in a handed Ybar block: each
+// row's 16 words hold both instances, the instance half chosen by the row's
+// parity and the column XOR-swizzled by the row, so that the own entries
+// (g, c), (g + 4, c) and the transposed (c, g), (c, g + 4) hit distinct bank
+// pairs in every 16-lane quarter (32 banks) and half-wave (64 banks)
+__device__ __forceinline__ int ybar_word(int inst, int r, int col) { return 16 * r + 8 * ((r ^ inst) & 1) + (col ^ r); }
 __device__ __forceinline__ double pick1(const dbl4& d, int inst) { return inst ? d[3] : d[1]; }
 
 // One wave of the reverse pipeline, level j = K - w.  TOP: level K, whose
@@ -398,7 +432,10 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
     const int c = q.c, g = q.g, inst = q.inst;
     const int o0 = g * 8 + c, o1 = (g + 4) * 8 + c;  // own entries (g, c), (g + 4, c)
     const int t0 = c * 8 + g, t1 = c * 8 + g + 4;    // transposed entries (c, g), (c, g + 4)
-    const int i0e = inst * kM;
+    const int ro0 = ring_word(inst, g, c), ro1 = ring_word(inst, g + 4, c);
+    // words of the own and the transposed entries in a handed Ybar block
+    const int yo0 = ybar_word(inst, g, c), yo1 = ybar_word(inst, g + 4, c);
+    const int yt0 = ybar_word(inst, c, g), yt1 = ybar_word(inst, c, g + 4);
     const double* __restrict__ Wj = a.W + (bl * (long long)K + (j - 1)) * S * kM;
     double* __restrict__ dA = a.dA + bl * S * kM;
     const BooleW bw(a.dt);
@@ -441,10 +478,10 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
         if (!TOP && act) {  // the previous wave handed this block over in the previous step
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
-                yo[p][0] = ybh[w - 1][p][i0e + o0];
-                yo[p][1] = ybh[w - 1][p][i0e + o1];
-                yt[p][0] = ybh[w - 1][p][i0e + t0];
-                yt[p][1] = ybh[w - 1][p][i0e + t1];
+                yo[p][0] = ybh[w - 1][p][yo0];
+                yo[p][1] = ybh[w - 1][p][yo1];
+                yt[p][0] = ybh[w - 1][p][yt0];
+                yt[p][1] = ybh[w - 1][p][yt1];
                 part[p][0] = dah[w - 1][p][0][lane];
                 part[p][1] = dah[w - 1][p][1][lane];
             }
@@ -465,12 +502,12 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
                     yt[p][1] = cz[p] * zkt1;
                 }
             }
-            const double* sl = &ring[r % R][inst][0];
+            const RingSlot& sl = ring[r % R];
             double at0[4], at1[4];  // the A^T operand: A's own entries
 #pragma unroll
             for (int p = 0; p < 4; ++p) {
-                at0[p] = sl[p * kM + o0];
-                at1[p] = sl[p * kM + o1];
+                at0[p] = sl[p][ro0];
+                at1[p] = sl[p][ro1];
             }
             double wb[4][2];
             dbl4 acc[4];
@@ -515,8 +552,8 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, const Pair& q, in
             if constexpr (!LAST) {
 #pragma unroll
                 for (int p = 0; p < 4; ++p) {
-                    ybh[w][p][i0e + o0] = yn[p][0];
-                    ybh[w][p][i0e + o1] = yn[p][1];
+                    ybh[w][p][yo0] = yn[p][0];
+                    ybh[w][p][yo1] = yn[p][1];
                     dah[w][p][0][lane] = pick0(acc[p], inst);
                     dah[w][p][1][lane] = pick1(acc[p], inst);
                 }
@@ -556,7 +593,7 @@ __global__ void __launch_bounds__(64 * (K + 1)) wbd8_rev(Wbd8RevArgs a) {
     constexpr int H = K > 1 ? K - 1 : 1;
     constexpr int R = K + kRingAhead;     // sample ring slots (the loader is wave K)
     __shared__ RingSlot ring[R];
-    __shared__ double ybh[H][4][2 * kM];  // Ybar_j blocks handed from wave w to wave w + 1, by entry
+    __shared__ double ybh[H][4][2 * kM];  // Ybar_j blocks handed from wave w to wave w + 1 (ybar_word)
     __shared__ double dah[H][4][2][64];   // dA sums so far, by lane
     const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
     Pair q;

@@ -171,150 +171,6 @@ struct Pair {
     bool live;               // the lane's instance exists (an odd batch leaves a partner missing)
 };
 
-// One wave of the forward pipeline.  FIRST: the wave of level 1's prefix
-// (W_1 = Prefix(A), Z_0 = Sum(A)), which also forms Y_2 = A W_1; otherwise it
-// forms Y_{j+1} = A W_j from the block of W_j the previous wave handed over.
-// MAKES: it steps level j + 1's prefix (j < K) and hands that block on.  The
-// role is a template parameter so that a step is straight-line code: the
-// four samples' ring reads and products are independent and overlap; only the
-// prefix recurrences run sample after sample.
-template <int K, bool LEVELS, bool FIRST, bool MAKES>
-__device__ __forceinline__ void sweep_wave(const LmArgs& a, const Pair& q, int w, int lane, const RingSlot* ring,
-                                           double (*hand)[4][2][64]) {
-    constexpr int R = K + kRingAhead;
-    const long long S = q.S, nb = q.nb;
-    const int j = w + 1;
-    const int c = q.c, g = q.g, inst = q.inst;
-    const int o0 = g * 8 + c, o1 = (g + 4) * 8 + c;  // own entries (g, c), (g + 4, c)
-    // ring words of the A operand entries (c, g), (c, g + 4) and the own entries
-    const int ra0 = ring_word(inst, c, g), ra1 = ring_word(inst, c, g + 4);
-    const int ro0 = ring_word(inst, g, c), ro1 = ring_word(inst, g + 4, c);
-    const double* __restrict__ A = a.in + q.bl * S * kM;
-    double* __restrict__ W = LEVELS ? a.carry + q.bl * (long long)K * S * kM : nullptr;
-    const BooleW bw(a.dt);
-    Acc s1[2], sn[2];           // prefix states of W_1 (FIRST) and W_{j+1} (MAKES)
-    double z1[2] = {}, zn[2] = {};  // Z_0 (FIRST) and Z_j
-
-    // sample 0: every prefix is 0 there, so the products of levels 2..K+1 are 0
-    if constexpr (FIRST) {
-        const double y0 = A[o0], y1 = A[o1];
-        s1[0] = Acc{0.0, y0, 0.0};
-        s1[1] = Acc{0.0, y1, 0.0};
-        z1[0] = bw.b14 * y0;
-        z1[1] = bw.b14 * y1;
-    }
-    sn[0] = sn[1] = Acc{0.0, 0.0, 0.0};
-    if (LEVELS && q.live) {
-        if (FIRST) W[o0] = W[o1] = 0.0;
-        if (MAKES) W[(long long)j * S * kM + o0] = W[(long long)j * S * kM + o1] = 0.0;
-    }
-    double win[4][2] = {};
-    for (long long t = 0; t < nb + K - 1; ++t) {
-        const long long blk = t - w;
-        const bool act = blk >= 0 && blk < nb;
-        if (!FIRST && act) {  // the previous wave formed this block in the previous step
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                win[p][0] = hand[w - 1][p][0][lane];
-                win[p][1] = hand[w - 1][p][1][lane];
-            }
-        }
-        step_barrier();  // the block is in the ring; every read of the hand blocks precedes their rewrite
-        if (act) {
-            const RingSlot& sl = ring[blk % R];
-            const long long i0 = 4 * blk;
-            double op0[4], op1[4], y[4][2], wv[4][2];
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {  // A operand entries A[c][g], A[c][g + 4]
-                op0[p] = sl[p][ra0];
-                op1[p] = sl[p][ra1];
-            }
-            const double cz[4] = {bw.b64, bw.b24, bw.b64, blk + 1 < nb ? bw.b28 : bw.b14};
-            if constexpr (FIRST) {  // W_1 = Prefix(A) and Z_0 over the block
-                double a0[4], a1[4];
-#pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    a0[p] = sl[p][ro0];
-                    a1[p] = sl[p][ro1];
-                    z1[0] = fma(cz[p], a0[p], z1[0]);
-                    z1[1] = fma(cz[p], a1[p], z1[1]);
-                }
-                wv[0][0] = brule<1>(s1[0], a0[0], bw);
-                wv[0][1] = brule<1>(s1[1], a1[0], bw);
-                wv[1][0] = brule<2>(s1[0], a0[1], bw);
-                wv[1][1] = brule<2>(s1[1], a1[1], bw);
-                wv[2][0] = brule<3>(s1[0], a0[2], bw);
-                wv[2][1] = brule<3>(s1[1], a1[2], bw);
-                wv[3][0] = brule<4>(s1[0], a0[3], bw);
-                wv[3][1] = brule<4>(s1[1], a1[3], bw);
-            } else {
-#pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    wv[p][0] = win[p][0];
-                    wv[p][1] = win[p][1];
-                }
-            }
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {  // Y_{j+1} = A W_j at the four samples
-                dbl4 d = __builtin_amdgcn_mfma_f64_16x16x4f64(op0[p], wv[p][0], dbl4{0.0, 0.0, 0.0, 0.0}, 0, 0, 0);
-                d = __builtin_amdgcn_mfma_f64_16x16x4f64(op1[p], wv[p][1], d, 0, 0, 0);
-                y[p][0] = inst ? d[2] : d[0];
-                y[p][1] = inst ? d[3] : d[1];
-            }
-#pragma unroll
-            for (int p = 0; p < 4; ++p) {
-                zn[0] = fma(cz[p], y[p][0], zn[0]);
-                zn[1] = fma(cz[p], y[p][1], zn[1]);
-            }
-            double v[4][2] = {};
-            if constexpr (MAKES) {  // W_{j+1} = Prefix(Y_{j+1}) over the block, handed on
-                v[0][0] = brule<1>(sn[0], y[0][0], bw);
-                v[0][1] = brule<1>(sn[1], y[0][1], bw);
-                v[1][0] = brule<2>(sn[0], y[1][0], bw);
-                v[1][1] = brule<2>(sn[1], y[1][1], bw);
-                v[2][0] = brule<3>(sn[0], y[2][0], bw);
-                v[2][1] = brule<3>(sn[1], y[2][1], bw);
-                v[3][0] = brule<4>(sn[0], y[3][0], bw);
-                v[3][1] = brule<4>(sn[1], y[3][1], bw);
-#pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    hand[w][p][0][lane] = v[p][0];
-                    hand[w][p][1][lane] = v[p][1];
-                }
-            }
-            if (LEVELS && q.live) {  // the levels this wave forms, kept for the reverse sweep
-#pragma unroll
-                for (int p = 0; p < 4; ++p) {
-                    const long long o = (i0 + 1 + p) * kM;
-                    if (FIRST) {
-                        W[o + o0] = wv[p][0];
-                        W[o + o1] = wv[p][1];
-                    }
-                    if (MAKES) {
-                        W[(long long)j * S * kM + o + o0] = v[p][0];
-                        W[(long long)j * S * kM + o + o1] = v[p][1];
-                    }
-                }
-            }
-        }
-        step_barrier();  // the blocks handed over in this step are visible to the next
-    }
-    if (!q.live) return;
-    const long long M = kM, n = a.n, b = q.b;
-    if constexpr (FIRST) {
-        a.omega1[b * M + o0] = z1[0];
-        a.omega1[b * M + o1] = z1[1];
-        a.totals[(b * n + 0) * M + o0] = s1[0].u;  // What_1 = W_1[S-1]
-        a.totals[(b * n + 0) * M + o1] = s1[1].u;
-    }
-    a.partial[(b * n + j) * M + o0] = zn[0];
-    a.partial[(b * n + j) * M + o1] = zn[1];
-    if constexpr (MAKES) {  // What_{j+1}: the last block's position 4
-        a.totals[(b * n + j) * M + o0] = sn[0].u;
-        a.totals[(b * n + j) * M + o1] = sn[1].u;
-    }
-}
-
 // The loader of a pipeline: ring index r holds block blk_of(r); it keeps
 // kRingAhead - 1 blocks in flight ahead of wave 0 and, at step t, waits for
 // ring index t (wave 0's block) before the step's first barrier.
@@ -333,40 +189,238 @@ __device__ __forceinline__ void ring_loader(RingSlot* ring, const double* A0, co
     }
 }
 
-// one workgroup per pair of instances (2x, 2x + 1): K level wavefronts and the
-// loader; a missing partner's lanes read the live instance and store nothing
+// ---------------------------------------------------------------------------
+// Forward sweep on v_mfma_f64_4x4x4f64.  That instruction runs four
+// independent 4 x 4 x 4 blocks: block b's A[i][k] is lane 16k + 4b + i, its
+// B[k][j] lane 16k + 4b + j and its D[i][j] lane 16i + 4b + j.  An 8 x 8
+// product Y = A W is four output blocks (I, J) over two halves kappa of the
+// contraction: block b = 2I + J of instruction kappa multiplies rows
+// 2i + I of A, columns 2k + kappa, by rows 2k + kappa of W, columns 4J + j.
+// Lane L = 16k + 8I + 4J + j then owns Y[2k + I][4J + j]; the B operands
+// of the two instructions are rows 2k and 2k + 1 of column 4J + j, one of
+// them the lane's own entry, the other owned by lane L ^ 8 (a DPP rotation
+// by 8 within each row of 16 lanes fetches it), and the A operands of lane
+// L (i = L % 4) are one 16-byte chunk, A[2i + I][2k .. 2k + 1].  No lane
+// computes anything discarded, so one instance fills a wavefront and a
+// product is two instructions.
+//
+// One workgroup per instance: K level wavefronts and a loader.  The ring
+// holds the instance's samples (a block of 4 samples is 2 KB, two loads of
+// 1 KB: lane l places chunk position l % 32 of sample l / 32); within a
+// sample, chunk (r, h) = (A[r][2h], A[r][2h + 1]) sits at position
+// 4r + (h ^ ((r >> 1) & 3)), so that the A operand chunks of every 8 lanes
+// and the own entries of every 16 lanes fall in distinct banks.  The waves
+// hand their level blocks on in LDS, indexed by lane, in two buffers used in
+// alternate steps, so one barrier per step orders every hand-over.
+// ---------------------------------------------------------------------------
+constexpr int kAhead4 = 6;  // blocks the loader keeps in flight ahead of wave 0
+
+__device__ __forceinline__ int chunk4_pos(int r, int h) { return 4 * r + (h ^ ((r >> 1) & 3)); }
+__device__ __forceinline__ int ring4_word(int r, int col) { return 2 * chunk4_pos(r, col >> 1) + (col & 1); }
+
+// one ring slot: [4 samples][64 entries], 16-byte aligned for the chunk reads
+struct alignas(16) Ring4Slot {
+    double v[4][kM];
+};
+
+// vmcnt <= 2 * blocks: the loader's loads of the `blocks` later blocks may still be in flight
+__device__ __forceinline__ void wait_blocks4(int blocks) {
+    switch (blocks) {
+        case 0: wait_vm<0>(); break;
+        case 1: wait_vm<2>(); break;
+        case 2: wait_vm<4>(); break;
+        case 3: wait_vm<6>(); break;
+        case 4: wait_vm<8>(); break;
+        default: wait_vm<10>(); break;
+    }
+    static_assert(kAhead4 - 1 <= 5, "wait_blocks4 covers kAhead4 - 1 later blocks");
+}
+
+// the loader wave of a one-instance pipeline: at step t it issues block
+// t + kAhead4 - 1 and waits for block t + 1 (wave 0's next block) before the
+// step's barrier; the prologue issues the first blocks and waits for block 0
+// before the barrier that opens the pipeline.  Ring index r holds block
+// blk_of(r).
+template <int R, typename BlkOf>
+__device__ __forceinline__ void ring4_loader(Ring4Slot* ring, const double* A, long long nb, long long steps,
+                                             int lane, BlkOf blk_of) {
+    const int pos = lane & 31, r = pos >> 2, h = (pos & 3) ^ ((r >> 1) & 3);
+    const double* src = A + (1 + (lane >> 5)) * (long long)kM + r * 8 + 2 * h;  // sample 4 blk + 1 + lane / 32
+    auto issue = [&](long long ri) {
+        const long long blk = blk_of(ri);
+        Ring4Slot& slot = ring[ri % R];
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+            __builtin_amdgcn_global_load_lds(src + (4 * blk + 2 * q) * (long long)kM,
+                                             (__attribute__((address_space(3))) void*)&slot.v[2 * q][0], 16, 0, 0);
+    };
+    const long long pre = nb < kAhead4 - 1 ? nb : kAhead4 - 1;
+    for (long long ri = 0; ri < pre; ++ri) issue(ri);
+    wait_blocks4((int)(pre - 1));
+    step_barrier();
+    for (long long t = 0; t < steps; ++t) {
+        const long long ri = t + kAhead4 - 1;
+        if (ri < nb) issue(ri);
+        if (t + 1 < nb) wait_blocks4((int)((ri < nb ? ri : nb - 1) - (t + 1)));
+        step_barrier();
+    }
+}
+
+// The B operand pair (W[2k][c], W[2k + 1][c]) of lane L from its own entry
+// W[2k + I][c]: a DPP row rotation by 8 brings lane L ^ 8's entry, and the
+// bank mask keeps the own value in the lanes (I = 0 for the first, I = 1 for
+// the second) that own it.
+__device__ __forceinline__ int dpp_ror8(int old, int src, int banks) {
+    return banks == 0xC ? __builtin_amdgcn_update_dpp(old, src, 0x128, 0xF, 0xC, false)
+                        : __builtin_amdgcn_update_dpp(old, src, 0x128, 0xF, 0x3, false);
+}
+__device__ __forceinline__ double2 bpair_dpp(double own) {
+    const unsigned long long v = __builtin_bit_cast(unsigned long long, own);
+    const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+    auto join = [](int l, int h) {
+        return __builtin_bit_cast(double, ((unsigned long long)(unsigned)h << 32) | (unsigned)l);
+    };
+    return double2{join(dpp_ror8(lo, lo, 0xC), dpp_ror8(hi, hi, 0xC)),   // rows 2k: I = 1 lanes take L ^ 8
+                   join(dpp_ror8(lo, lo, 0x3), dpp_ror8(hi, hi, 0x3))};  // rows 2k + 1: I = 0 lanes take L ^ 8
+}
+
+// Y[2k + I][4J + j] of Y = A W from the A chunk a (A[2i + I][2k], A[2i + I][2k + 1])
+// and the B pair wp (W[2k][4J + j], W[2k + 1][4J + j])
+__device__ __forceinline__ double prod4(double2 a, double2 wp) {
+    const double d = __builtin_amdgcn_mfma_f64_4x4x4f64(a.x, wp.x, 0.0, 0, 0, 0);
+    return __builtin_amdgcn_mfma_f64_4x4x4f64(a.y, wp.y, d, 0, 0, 0);
+}
+
+// Hand-over layout of a level block: per sample, lane L's own entry sits at
+// word 2 q + I with q = L without its bit 3, so that lane L reads its B pair
+// (the entries of lanes L & ~8 and L | 8) as one 16-byte chunk.
+__device__ __forceinline__ int hand_word(int lane) { return 2 * ((lane & 7) | ((lane >> 4) << 3)) + ((lane >> 3) & 1); }
+
+// One wave of the forward pipeline, level j = w + 1.  FIRST: the wave of level
+// 1's prefix (W_1 = Prefix(A), Z_0 = Sum(A)), which also forms Y_2 = A W_1;
+// otherwise it forms Y_{j+1} = A W_j from the block of W_j the previous wave
+// handed over.  MAKES: it steps level j + 1's prefix (j < K) and hands that
+// block on.  The role is a template parameter and the step indices are
+// wave-uniform (scalar registers), so that a step is straight-line code: the
+// four samples' reads and products are independent and overlap; only the
+// prefix recurrences run sample after sample.
+template <int K, bool LEVELS, bool FIRST, bool MAKES>
+__device__ __forceinline__ void sweep4_wave(const LmArgs& a, int b, int w, int lane, const Ring4Slot* ring,
+                                            double (*hand)[K > 1 ? K - 1 : 1][4][64]) {
+    constexpr int R = K + kAhead4 - 1;
+    const long long S = a.S;
+    const int nb = (int)((S - 1) / 4);
+    const int j = w + 1;
+    const int k = lane >> 4, I = (lane >> 3) & 1, J = (lane >> 2) & 1, i = lane & 3;
+    const int own = (2 * k + I) * 8 + 4 * J + i;  // own entry (2k + I, 4J + j), j = lane % 4
+    const int ra = 2 * chunk4_pos(2 * i + I, k);  // the A operand chunk
+    const int ro = ring4_word(2 * k + I, 4 * J + i);
+    const int hw = hand_word(lane), hp = hw & ~1;  // own word and B pair chunk in a hand-over
+    const double* __restrict__ A = a.in + (long long)b * S * kM;
+    double* __restrict__ W = LEVELS ? a.carry + (long long)b * K * S * kM : nullptr;
+    const BooleW bw(a.dt);
+    Acc s1, sn = Acc{0.0, 0.0, 0.0};  // prefix states of W_1 (FIRST) and W_{j+1} (MAKES)
+    double z1 = 0.0, zn = 0.0;        // Z_0 (FIRST) and Z_j
+
+    // sample 0: every prefix is 0 there, so the products of levels 2..K+1 are 0
+    if constexpr (FIRST) {
+        const double y0 = A[own];
+        s1 = Acc{0.0, y0, 0.0};
+        z1 = bw.b14 * y0;
+    }
+    if constexpr (LEVELS) {
+        if (FIRST) W[own] = 0.0;
+        if (MAKES) W[(long long)j * S * kM + own] = 0.0;
+    }
+    step_barrier();  // block 0 is in the ring
+    int slot = 0;    // ring slot of this wave's next block
+    for (int t = 0; t < nb + K - 1; ++t) {
+        const int blk = t - w;
+        if (blk >= 0 && blk < nb) {
+            const Ring4Slot& sl = ring[slot];
+            slot = slot + 1 == R ? 0 : slot + 1;
+            double2 av[4], wp[4];
+            double y[4];
+#pragma unroll
+            for (int p = 0; p < 4; ++p) av[p] = *reinterpret_cast<const double2*>(&sl.v[p][ra]);
+            const double c4 = blk + 1 < nb ? bw.b28 : bw.b14;
+            const double cz[4] = {bw.b64, bw.b24, bw.b64, c4};
+            if constexpr (FIRST) {  // W_1 = Prefix(A) and Z_0 over the block
+                double ao[4], wv[4];
+#pragma unroll
+                for (int p = 0; p < 4; ++p) {
+                    ao[p] = sl.v[p][ro];
+                    z1 = fma(cz[p], ao[p], z1);
+                }
+                wv[0] = brule<1>(s1, ao[0], bw);
+                wv[1] = brule<2>(s1, ao[1], bw);
+                wv[2] = brule<3>(s1, ao[2], bw);
+                wv[3] = brule<4>(s1, ao[3], bw);
+#pragma unroll
+                for (int p = 0; p < 4; ++p) wp[p] = bpair_dpp(wv[p]);
+                if constexpr (LEVELS) {
+                    double* Wb = W + (4LL * blk + 1) * kM + own;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) Wb[p * kM] = wv[p];
+                }
+            } else {  // the previous wave formed this block in the previous step
+#pragma unroll
+                for (int p = 0; p < 4; ++p) wp[p] = *reinterpret_cast<const double2*>(&hand[(t - 1) & 1][w - 1][p][hp]);
+            }
+#pragma unroll
+            for (int p = 0; p < 4; ++p) y[p] = prod4(av[p], wp[p]);  // Y_{j+1} = A W_j
+#pragma unroll
+            for (int p = 0; p < 4; ++p) zn = fma(cz[p], y[p], zn);
+            if constexpr (MAKES) {  // W_{j+1} = Prefix(Y_{j+1}) over the block, handed on
+                double v[4];
+                v[0] = brule<1>(sn, y[0], bw);
+                v[1] = brule<2>(sn, y[1], bw);
+                v[2] = brule<3>(sn, y[2], bw);
+                v[3] = brule<4>(sn, y[3], bw);
+#pragma unroll
+                for (int p = 0; p < 4; ++p) hand[t & 1][w][p][hw] = v[p];
+                if constexpr (LEVELS) {  // the level this wave forms, kept for the reverse sweep
+                    double* Wb = W + ((long long)j * S + 4LL * blk + 1) * kM + own;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) Wb[p * kM] = v[p];
+                }
+            }
+        }
+        step_barrier();  // this step's hand-overs and the loader's next block are visible
+    }
+    const long long M = kM, n = a.n;
+    if constexpr (FIRST) {
+        a.omega1[b * M + own] = z1;
+        a.totals[(b * n + 0) * M + own] = s1.u;  // What_1 = W_1[S-1]
+    }
+    a.partial[(b * n + j) * M + own] = zn;
+    if constexpr (MAKES) a.totals[(b * n + j) * M + own] = sn.u;  // What_{j+1}: the last block's position 4
+}
+
+// one workgroup per instance: K level wavefronts and the loader (wave K)
 template <int K, bool LEVELS>
-__global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_eu(4))) wbd8_sweep(LmArgs a) {
-    constexpr int R = K + kRingAhead;  // ring slots
-    __shared__ RingSlot ring[R];
-    // level blocks handed from wave w to wave w + 1, [4 samples][2 entries][lane]
-    __shared__ double hand[K > 1 ? K - 1 : 1][4][2][64];
-    const int w = threadIdx.x >> 6, lane = threadIdx.x & 63;
-    Pair q;
-    q.c = lane & 7;
-    q.inst = (lane >> 3) & 1;
-    q.g = lane >> 4;
-    q.b = 2LL * blockIdx.x + q.inst;
-    q.live = q.b < a.jobs;
-    q.bl = q.live ? q.b : q.b - 1;
-    q.S = a.S;
-    q.nb = (a.S - 1) / 4;
+__global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_eu(6))) wbd8_sweep(LmArgs a) {
+    constexpr int R = K + kAhead4 - 1;  // ring slots
+    __shared__ Ring4Slot ring[R];
+    // level blocks handed from wave w to wave w + 1: [step parity][w][4 samples][lane]
+    __shared__ alignas(16) double hand[2][K > 1 ? K - 1 : 1][4][64];
+    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int b = blockIdx.x;
+    const long long nb = (a.S - 1) / 4;
     if (w == K) {
-        const long long b0 = 2LL * blockIdx.x, b1 = b0 + 1 < a.jobs ? b0 + 1 : b0;
-        ring_loader<R>(ring, a.in + b0 * q.S * kM, a.in + b1 * q.S * kM, q.nb, q.nb + K - 1, lane,
-                       [](long long r) { return r; });
+        ring4_loader<R>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane, [](long long r) { return r; });
     } else if (w == 0) {
-        sweep_wave<K, LEVELS, true, (K > 1)>(a, q, w, lane, ring, hand);
+        sweep4_wave<K, LEVELS, true, (K > 1)>(a, b, w, lane, ring, hand);
     } else if (w < K - 1) {
-        sweep_wave<K, LEVELS, false, true>(a, q, w, lane, ring, hand);
+        sweep4_wave<K, LEVELS, false, true>(a, b, w, lane, ring, hand);
     } else {
-        sweep_wave<K, LEVELS, false, false>(a, q, w, lane, ring, hand);
+        sweep4_wave<K, LEVELS, false, false>(a, b, w, lane, ring, hand);
     }
 }
 
 template <bool LEVELS>
 hipError_t launch_k(const LmArgs& a, hipStream_t s) {
-    const dim3 grid((unsigned)((a.jobs + 1) / 2));
+    const dim3 grid((unsigned)a.jobs);
     switch (a.n - 1) {
         case 1: hipLaunchKernelGGL((wbd8_sweep<1, LEVELS>), grid, dim3(128), 0, s, a); break;
         case 2: hipLaunchKernelGGL((wbd8_sweep<2, LEVELS>), grid, dim3(192), 0, s, a); break;
@@ -633,7 +687,9 @@ bool wbd8_sweep_ok(int d, int integ, int n) { return d == 8 && integ == BOOLE &&
 
 hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s) {
     if (a.jobs == 0) return hipSuccess;
-    if (a.d != 8 || a.S < 5 || (a.S - 1) % 4 != 0 || a.n < 2 || a.n - 1 > kWbd8MaxK) return hipErrorInvalidValue;
+    if (a.d != 8 || a.S < 5 || (a.S - 1) % 4 != 0 || a.S > (1LL << 30) || a.jobs > (1LL << 30) || a.n < 2 ||
+        a.n - 1 > kWbd8MaxK)
+        return hipErrorInvalidValue;
     // algorithmic work: the samples read once (and the K levels written when
     // kept); per sample K products of 2 d^3 flops plus a prefix step and a sum
     // of 8 flops per entry for each of the K + 1 levels

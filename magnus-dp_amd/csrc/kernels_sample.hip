@@ -22,13 +22,17 @@ __device__ __forceinline__ double grid_t(long long i, long long S, double t0, do
     return __dadd_rn(__dmul_rn((double)i, step), t0);
 }
 
-// one workgroup handles SB consecutive samples of one instance: the sin/cos
-// table of its samples is built once in LDS, then the entries are filled
+// One workgroup handles SB consecutive samples of IB consecutive instances:
+// the sin/cos table of its samples (which depends on the grid and the modes
+// only) is built once in LDS, then the entries of each instance are filled in
+// turn.  The table is the expensive part (two FP64 transcendentals per sample
+// and mode), so sharing it across instances leaves the kernel bound by its
+// output writes.
 constexpr int kSampleMaxModes = 32;
 constexpr int kSampleThreads = 256;
 
 // STAGE: the instance's (2M+1) coefficient matrices fit kSampleStage doubles
-// and are copied to LDS once per workgroup (every entry reads 2M+1 of them per
+// and are copied to LDS once per instance (every entry reads 2M+1 of them per
 // sample); otherwise they are read from global memory.
 constexpr int kSampleStage = 2048;
 
@@ -37,13 +41,62 @@ __global__ void __launch_bounds__(kSampleThreads) sample_fourier_kernel(SampleAr
     __shared__ double tab[kSampleThreads * 2];  // [sample][mode][sin, cos], SB * modes <= 256
     __shared__ double cst[STAGE ? kSampleStage : 1];
     const int m = a.m, modes = a.modes;
-    const int SB = a.sb;
-    const long long blocks_per_inst = (a.S + SB - 1) / SB;
-    const long long b = blockIdx.x / blocks_per_inst;
-    const long long i0 = (blockIdx.x - b * blocks_per_inst) * SB;
-    const double* Cg = a.coeffs + b * (long long)(2 * modes + 1) * m;
-    if (STAGE)
-        for (int x = threadIdx.x; x < (2 * modes + 1) * m; x += blockDim.x) cst[x] = Cg[x];
+    const int SB = a.sb, IB = a.ib;
+    const long long blocks_per_group = (a.S + SB - 1) / SB;
+    const long long g = blockIdx.x / blocks_per_group;
+    const long long i0 = (blockIdx.x - g * blocks_per_group) * SB;
+    for (int x = threadIdx.x; x < SB * modes; x += blockDim.x) {
+        const int si = x / modes, k = x - si * modes + 1;
+        const long long i = i0 + si;
+        double sv = 0.0, cv = 0.0;
+        if (i < a.S) {
+            const double t = grid_t(i, a.S, a.t0, a.tf, a.step);
+            const double arg = __dmul_rn(__dmul_rn(kTwoPi, (double)k), t);
+            sv = sin(arg);
+            cv = cos(arg);
+        }
+        tab[2 * x] = sv;
+        tab[2 * x + 1] = cv;
+    }
+    for (int bi = 0; bi < IB; ++bi) {
+        const long long b = g * IB + bi;
+        if (b >= a.B) break;
+        const double* Cg = a.coeffs + b * (long long)(2 * modes + 1) * m;
+        if (STAGE) {
+            __syncthreads();  // the previous instance's coefficients are read
+            for (int x = threadIdx.x; x < (2 * modes + 1) * m; x += blockDim.x) cst[x] = Cg[x];
+        }
+        __syncthreads();  // the table (and the staged coefficients) are in LDS
+        const double* C = STAGE ? cst : Cg;
+        for (int x = threadIdx.x; x < SB * m; x += blockDim.x) {
+            const int si = x / m, e = x - si * m;
+            const long long i = i0 + si;
+            if (i >= a.S) continue;
+            double v = C[e];
+            for (int k = 1; k <= modes; ++k) {
+                const double* tk = tab + 2 * (si * modes + k - 1);
+                v = __dadd_rn(v, __dmul_rn(tk[0], C[(2 * k - 1) * m + e]));
+                v = __dadd_rn(v, __dmul_rn(tk[1], C[(2 * k) * m + e]));
+            }
+            a.out[(b * a.S + i) * m + e] = v;
+        }
+    }
+}
+
+// The same for m dividing the workgroup size and at most kRegModes modes:
+// thread x keeps entry e = x % m of the instance's 2M + 1 coefficient
+// matrices in registers and fills that entry of samples x / m, x / m + 256 / m,
+// ..., so that only the table is read from LDS (a broadcast within each
+// wavefront); the per-entry arithmetic is the generic kernel's.
+constexpr int kRegModes = 8;
+
+__global__ void __launch_bounds__(kSampleThreads) sample_fourier_reg(SampleArgs a) {
+    __shared__ double tab[kSampleThreads * 2];
+    const int m = a.m, modes = a.modes;
+    const int SB = a.sb, IB = a.ib;
+    const long long blocks_per_group = (a.S + SB - 1) / SB;
+    const long long g = blockIdx.x / blocks_per_group;
+    const long long i0 = (blockIdx.x - g * blocks_per_group) * SB;
     for (int x = threadIdx.x; x < SB * modes; x += blockDim.x) {
         const int si = x / modes, k = x - si * modes + 1;
         const long long i = i0 + si;
@@ -58,18 +111,27 @@ __global__ void __launch_bounds__(kSampleThreads) sample_fourier_kernel(SampleAr
         tab[2 * x + 1] = cv;
     }
     __syncthreads();
-    const double* C = STAGE ? cst : Cg;
-    for (int x = threadIdx.x; x < SB * m; x += blockDim.x) {
-        const int si = x / m, e = x - si * m;
-        const long long i = i0 + si;
-        if (i >= a.S) continue;
-        double v = C[e];
-        for (int k = 1; k <= modes; ++k) {
-            const double* tk = tab + 2 * (si * modes + k - 1);
-            v = __dadd_rn(v, __dmul_rn(tk[0], C[(2 * k - 1) * m + e]));
-            v = __dadd_rn(v, __dmul_rn(tk[1], C[(2 * k) * m + e]));
+    const int e = threadIdx.x % m, s0 = threadIdx.x / m, sstride = kSampleThreads / m;
+    for (int bi = 0; bi < IB; ++bi) {
+        const long long b = g * IB + bi;
+        if (b >= a.B) break;
+        const double* Cg = a.coeffs + b * (long long)(2 * modes + 1) * m + e;
+        double c[2 * kRegModes + 1];
+#pragma unroll
+        for (int q = 0; q < 2 * kRegModes + 1; ++q) c[q] = q < 2 * modes + 1 ? Cg[(long long)q * m] : 0.0;
+        for (int si = s0; si < SB; si += sstride) {
+            const long long i = i0 + si;
+            if (i >= a.S) break;
+            double v = c[0];
+#pragma unroll
+            for (int k = 1; k <= kRegModes; ++k) {
+                if (k > modes) break;
+                const double* tk = tab + 2 * (si * modes + k - 1);
+                v = __dadd_rn(v, __dmul_rn(tk[0], c[2 * k - 1]));
+                v = __dadd_rn(v, __dmul_rn(tk[1], c[2 * k]));
+            }
+            a.out[(b * a.S + i) * m + e] = v;
         }
-        a.out[(b * a.S + i) * m + e] = v;
     }
 }
 
@@ -105,9 +167,15 @@ hipError_t launch_sample_fourier(SampleArgs a, hipStream_t s) {
     if (sb > sb_tab) sb = sb_tab;
     if (sb < 1) sb = 1;
     a.sb = sb;
-    const long long blocks = a.B * ((a.S + sb - 1) / sb);
+    // instances per workgroup: eight share a table once the batch still gives
+    // thousands of workgroups
+    const long long sblocks = (a.S + sb - 1) / sb;
+    a.ib = a.B * sblocks >= 8 * 4096 ? 8 : 1;
+    const long long blocks = ((a.B + a.ib - 1) / a.ib) * sblocks;
     void* h = timer_begin("sample_fourier", s);
-    if ((long long)(2 * a.modes + 1) * a.m <= kSampleStage)
+    if (a.m <= kSampleThreads && kSampleThreads % a.m == 0 && a.modes <= kRegModes)
+        hipLaunchKernelGGL(sample_fourier_reg, dim3(blocks), dim3(kSampleThreads), 0, s, a);
+    else if ((long long)(2 * a.modes + 1) * a.m <= kSampleStage)
         hipLaunchKernelGGL(sample_fourier_kernel<true>, dim3(blocks), dim3(kSampleThreads), 0, s, a);
     else
         hipLaunchKernelGGL(sample_fourier_kernel<false>, dim3(blocks), dim3(kSampleThreads), 0, s, a);

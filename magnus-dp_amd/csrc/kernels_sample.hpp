@@ -16,6 +16,7 @@ struct SampleArgs {
     long long S;
     int m, modes;
     int sb;                // samples per workgroup (set by the launcher)
+    int ib;                // instances per workgroup, sharing its sin/cos table (set by the launcher)
     double t0, tf, step;   // step = (tf - t0) / (S - 1), as numpy.linspace computes it
 };
 

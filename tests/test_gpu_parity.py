@@ -442,6 +442,31 @@ def test_device_sampling_matches_numpy_fourier_form(d, modes, S, t0, tf):
     assert np.max(np.abs(got - ref)) <= 1e-14 * max(1.0, np.max(np.abs(ref)))
 
 
+def test_device_sampling_large_batch_equals_per_instance_calls():
+    """A large batch (workgroups sharing one sin/cos table across eight
+    instances, the last group ragged) gives the same samples bit for bit as
+    one call per instance, and matches numpy on a few instances."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, S, d, modes = 1021, 2049, 8, 6
+    coeffs = torch.randn((B, 2 * modes + 1, d, d), dtype=torch.float64, device="cuda",
+                         generator=torch.Generator(device="cuda").manual_seed(5)) / np.sqrt(d)
+    got = torch_ops.sample_fourier(coeffs, S, -0.3, 1.4)
+    for b in (0, 7, 8, 513, B - 2, B - 1):
+        one = torch_ops.sample_fourier(coeffs[b:b + 1], S, -0.3, 1.4)
+        assert torch.equal(one[0], got[b]), b
+    c = coeffs.cpu().numpy()
+    t = np.linspace(-0.3, 1.4, S)
+    for b in (0, B - 1):
+        ref = np.broadcast_to(c[b, 0], (S, d, d)).copy()
+        for k in range(1, modes + 1):
+            ref = ref + np.sin(2 * np.pi * k * t)[:, None, None] * c[b, 2 * k - 1]
+            ref = ref + np.cos(2 * np.pi * k * t)[:, None, None] * c[b, 2 * k]
+        assert rel_fro(got[b].cpu().numpy(), ref) <= 1e-14
+
+
 @pytest.mark.parametrize("modes,S,t0,tf", [(4, 16385, 0.0, 1.0), (1, 2, 0.0, 1.0), (7, 101, -1.0, 2.5)])
 def test_device_sampling_matches_numpy_spacecurve_form(modes, S, t0, tf):
     import torch
@@ -667,6 +692,32 @@ def test_d8_boole_runs_the_single_pass_sweep(B, n):
         assert rel_fro(dA[b], O.vjp(n, A[b], cot[b], 0.0, 1.0, op="sum", integrator="Boole")) <= TOL
     one = torch_ops.forward(n, At[B - 1:], 0.0, 1.0, op="many", integrator="Boole").cpu().numpy()
     assert np.array_equal(one[0], out[B - 1])
+
+
+@pytest.mark.parametrize("S", [5, 9, 13, 21])
+@pytest.mark.parametrize("n", [2, 3, 8])
+def test_d8_boole_vjp_few_blocks(S, n):
+    """The d = 8 single-pass sweeps at one to five Boole blocks (pipelines
+    shorter than their depth): the VJP, with and without the kept basis state,
+    matches the oracle, and both are the same bits."""
+    import torch
+
+    from magnus import torch_ops
+
+    B, d = 3, 8
+    A = np.stack([fourier_matrices(np.random.default_rng([47, b, n, S]), S, d, modes=3) for b in range(B)])
+    At = torch.from_numpy(A).cuda()
+    cot = np.stack([np.random.default_rng([47, b, n, S, 1]).standard_normal((d, d)) for b in range(B)])
+    ct = torch.from_numpy(cot).cuda()
+    kw = dict(op="sum", integrator="Boole")
+    dA = torch_ops.vjp(n, At, ct, None, 0.0, 1.0, **kw)
+    out, state = torch_ops.forward_basis(n, At, 0.0, 1.0, **kw)
+    assert state.numel() == B * ((n - 1) * S * d * d + (2 * n + 1) * d * d)
+    dA2 = torch_ops.vjp_basis(n, At, ct, state, 0.0, 1.0, **kw)
+    assert torch.equal(dA, dA2)
+    dA = dA.cpu().numpy()
+    for b in range(B):
+        assert rel_fro(dA[b], O.vjp(n, A[b], cot[b], 0.0, 1.0, **kw)) <= TOL
 
 
 def test_batched_matrix_api_matches_per_instance():

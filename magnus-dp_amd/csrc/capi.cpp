@@ -601,21 +601,27 @@ int wbd_sweep(const Plan& pl, magnus::LmArgs& a, hipStream_t stream) {
 }
 
 // The W-basis state of `count` instances, as one buffer: the levels W_1..W_K
-// [B][K][S][m], then the sums [B][n][m] (rows 1..K: Z_1..Z_K), Z_0 [B][m] and
-// the totals [B][n][m] (rows 0..K-1).  It is what the reverse sweep needs
-// besides the input and the cotangent.
+// [B][K][S][m] (the single-pass d = 8 sweeps keep W_2..W_K only, or W_1 when
+// K = 1: their reverse sweep rebuilds W_1 from its total), then the sums
+// [B][n][m] (rows 1..K: Z_1..Z_K), Z_0 [B][m] and the totals [B][n][m] (rows
+// 0..K-1).  It is what the reverse sweep needs besides the input and the
+// cotangent.
 struct WbdState {
     double *W, *z, *z0, *totals;
 };
+uint64_t wbd_kept_levels(const Plan& pl) {
+    const uint64_t K = pl.n - 1;
+    return use_wbd8(pl) && K > 1 ? K - 1 : K;
+}
 uint64_t wbd_state_elems(const Plan& pl, uint64_t count) {
     const uint64_t n = pl.n, m = pl.m;
-    return count * ((n - 1) * pl.S * m + (2 * n + 1) * m);
+    return count * (wbd_kept_levels(pl) * pl.S * m + (2 * n + 1) * m);
 }
 WbdState wbd_state_view(const Plan& pl, uint64_t count, double* base) {
     const uint64_t B = count, n = pl.n, m = pl.m;
     WbdState st;
     st.W = base;
-    st.z = st.W + B * (n - 1) * pl.S * m;
+    st.z = st.W + B * wbd_kept_levels(pl) * pl.S * m;
     st.z0 = st.z + B * n * m;
     st.totals = st.z0 + B * m;
     return st;
@@ -690,7 +696,7 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     if (state) {
         st = wbd_state_view(pl, count, const_cast<double*>(state));  // read only below
     } else {
-        st.W = scratch.alloc<double>(B * (n - 1) * S * m);
+        st.W = scratch.alloc<double>(B * wbd_kept_levels(pl) * S * m);
         st.z = scratch.alloc<double>(B * n * m);
         st.z0 = scratch.alloc<double>(B * m);
         st.totals = scratch.alloc<double>(B * n * m);
@@ -724,6 +730,7 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
         magnus::Wbd8RevArgs r{};
         r.in = in;
         r.W = st.W;
+        r.totals = st.totals;
         r.zbar = zbar;
         r.wbar = wbar;
         r.dA = dA;

@@ -11,8 +11,10 @@
 // pipeline of Boole blocks -- at step t wave w takes block t - w, its input
 // block formed by wave w - 1 in the step before and handed over in LDS --
 // and every level's state stays in its wave's registers.  The levels are
-// written only when a VJP needs them (LEVELS); the reverse sweep walks the
-// blocks backwards the same way, one wave per level again.
+// written only when a VJP needs them (LEVELS), and then W_2..W_K only: the
+// reverse sweep walks the blocks backwards the same way, one wave per level
+// again, and its level-1 wave rebuilds W_1 = Prefix(A) backwards from the
+// total What_1 (for K = 1, W_1 is kept).
 //
 // Products on the matrix cores.  v_mfma_f64_4x4x4f64 runs four independent
 // 4 x 4 x 4 blocks: block b's A[i][k] is lane 16k + 4b + i, its B[k][j] lane
@@ -39,6 +41,10 @@ namespace magnus {
 namespace {
 
 constexpr int kM = 64;  // entries of a d = 8 sample
+
+// levels a VJP keeps in HBM: W_2..W_K (W_1 alone when K = 1), and the index of level j among them
+__host__ __device__ constexpr int kept_levels(int K) { return K > 1 ? K - 1 : 1; }
+__host__ __device__ constexpr int kept_index(int K, int j) { return K > 1 ? j - 2 : 0; }
 
 // Boole weights times dt: trapezoid, Simpson, 3/8 and Boole rules of a block
 struct BooleW {
@@ -225,7 +231,7 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
     const int ro = ring_word(2 * L.k + L.I, L.c);       // the own entry
     const int hw = 16 * L.k + 2 * L.c + L.I, hp = hw - L.I;
     const double* __restrict__ A = a.in + (long long)b * S * kM;
-    double* __restrict__ W = LEVELS ? a.carry + (long long)b * K * S * kM : nullptr;
+    double* __restrict__ W = LEVELS ? a.carry + (long long)b * kept_levels(K) * S * kM : nullptr;
     const BooleW bw(a.dt);
     Acc s1, sn = Acc{0.0, 0.0, 0.0};  // prefix states of W_1 (FIRST) and W_{j+1} (MAKES)
     double z1 = 0.0, zn = 0.0;        // Z_0 (FIRST) and Z_j
@@ -237,8 +243,8 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
         z1 = bw.b14 * y0;
     }
     if constexpr (LEVELS) {
-        if (FIRST) W[own] = 0.0;
-        if (MAKES) W[(long long)j * S * kM + own] = 0.0;
+        if (FIRST && K == 1) W[own] = 0.0;
+        if (MAKES) W[(long long)kept_index(K, j + 1) * S * kM + own] = 0.0;
     }
     step_barrier();  // block 0 is in the ring
     int slot = 0;    // ring slot of this wave's next block
@@ -266,7 +272,7 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
                 wv[3] = brule<4>(s1, ao[3], bw);
 #pragma unroll
                 for (int p = 0; p < 4; ++p) wp[p] = bpair_dpp(wv[p]);
-                if constexpr (LEVELS) {
+                if constexpr (LEVELS && K == 1) {  // W_1 is kept only when it is the only level
                     double* Wb = W + (4LL * blk + 1) * kM + own;
 #pragma unroll
                     for (int p = 0; p < 4; ++p) Wb[p * kM] = wv[p];
@@ -288,7 +294,7 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
 #pragma unroll
                 for (int p = 0; p < 4; ++p) hand[t & 1][w][p][hw] = v[p];
                 if constexpr (LEVELS) {  // the level this wave forms, kept for the reverse sweep
-                    double* Wb = W + ((long long)j * S + 4LL * blk + 1) * kM + own;
+                    double* Wb = W + ((long long)kept_index(K, j + 1) * S + 4LL * blk + 1) * kM + own;
 #pragma unroll
                     for (int p = 0; p < 4; ++p) Wb[p * kM] = v[p];
                 }
@@ -382,7 +388,14 @@ __device__ __forceinline__ double boole_c(long long i, long long S, const BooleW
 
 // One wave of the reverse pipeline, level j = K - w.  TOP: level K, whose
 // input Ybar_{K+1} = c Zbar_K needs no hand-over; LAST: level 1, which adds
-// Ybar_1 and writes dA (the others hand Ybar_j and the dA sum on).
+// Ybar_1 and writes dA (the others hand Ybar_j and the dA sum on).  For K > 1
+// the level-1 wave rebuilds W_1 = Prefix(A) backwards instead of reading it:
+// from a block's end value, the block's base is that value minus the block's
+// Boole sum, and its inner positions follow from the base with the forward's
+// rules; it starts from the total What_1 and works with the transposed
+// entries W_1[c][2k + I] that its W^T B pair needs (the ring holds the
+// block's samples and, one slot above, its first sample, the last of the
+// block below).
 template <int K, bool TOP, bool LAST>
 __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int lane, const Lane& L,
                                          const RingSlot* ring, double (*ybh)[K > 1 ? K - 1 : 1][4][64],
@@ -399,7 +412,15 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     const int ya0 = ybar_word(2 * i + I, 2 * k), ya1 = ybar_word(2 * i + I, 2 * k + 1);
     // this lane's half W_j[c][2k + I] of W_j^T's B pair (W_j[c][2k], W_j[c][2k + 1]);
     // lane L ^ 8 loads the other half
-    const double* __restrict__ Wj = a.W + ((long long)b * K + (j - 1)) * S * kM + c * 8 + 2 * k + I;
+    constexpr bool REBUILD = LAST && K > 1;
+    const double* __restrict__ Wj =
+        REBUILD ? nullptr : a.W + ((long long)b * kept_levels(K) + kept_index(K, j)) * S * kM + c * 8 + 2 * k + I;
+    const int rq = ring_word(c, 2 * k + I);  // REBUILD: A[c][2k + I] in a ring sample
+    double w1 = 0.0, a0t = 0.0;              // REBUILD: W_1[c][2k + I] at the current block's end; A_0 there
+    if constexpr (REBUILD) {
+        w1 = a.totals[(long long)b * n * kM + c * 8 + 2 * k + I];
+        a0t = a.in[(long long)b * S * kM + c * 8 + 2 * k + I];
+    }
     double* __restrict__ dA = a.dA + (long long)b * S * kM;
     const BooleW bw(a.dt);
 
@@ -417,10 +438,12 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     // unrolled by three), never by copying: a copy of a register whose load is
     // in flight would wait for that load.
     auto loadw = [&](double* y, int blk) {
+        if constexpr (!REBUILD) {
 #pragma unroll
-        for (int p = 0; p < 4; ++p) y[p] = Wj[(4LL * blk + 1 + p) * kM];
+            for (int p = 0; p < 4; ++p) y[p] = Wj[(4LL * blk + 1 + p) * kM];
+        }
     };
-    double wa[4], wbuf[4], wc[4];
+    double wa[4] = {}, wbuf[4] = {}, wc[4] = {};
     loadw(wa, nb - 1);
     if (nb > 1) loadw(wbuf, nb - 2);
     int slot = 0;  // ring slot of this wave's next block
@@ -435,6 +458,18 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
             const RingSlot& sl = ring[slot];
             slot = slot + 1 == R ? 0 : slot + 1;
             if (blk > 1) loadw(wpf, blk - 2);
+            if constexpr (REBUILD) {  // W_1 over the block, from its end value w1
+                const double a4[5] = {blk > 0 ? ring[slot].v[3][rq] : a0t, sl.v[0][rq], sl.v[1][rq], sl.v[2][rq],
+                                      sl.v[3][rq]};
+                const double base =
+                    w1 - fma(bw.b14, a4[4], fma(bw.b64, a4[3], fma(bw.b24, a4[2], fma(bw.b64, a4[1], bw.b14 * a4[0]))));
+                Acc st = Acc{base, a4[0], 0.0};
+                wt[0] = brule<1>(st, a4[1], bw);
+                wt[1] = brule<2>(st, a4[2], bw);
+                wt[2] = brule<3>(st, a4[3], bw);
+                wt[3] = w1;
+                w1 = base;
+            }
             const long long s = 4LL * blk;
             double cz[4];
 #pragma unroll
@@ -553,11 +588,11 @@ hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s) {
     if (a.d != 8 || a.S < 5 || (a.S - 1) % 4 != 0 || a.S > (1LL << 30) || a.jobs > (1LL << 30) || a.n < 2 ||
         a.n - 1 > kWbd8MaxK)
         return hipErrorInvalidValue;
-    // algorithmic work: the samples read once (and the K levels written when
-    // kept); per sample K products of 2 d^3 flops plus a prefix step and a sum
+    // algorithmic work: the samples read once (and the kept levels W_2..W_K
+    // written for a VJP); per sample K products of 2 d^3 flops plus a prefix step and a sum
     // of 8 flops per entry for each of the K + 1 levels
     const double K = a.n - 1, S = (double)a.S, B = (double)a.jobs;
-    const double bytes = 8.0 * B * S * kM * (1.0 + (a.carry ? K : 0.0));
+    const double bytes = 8.0 * B * S * kM * (1.0 + (a.carry ? kept_levels(a.n - 1) : 0.0));
     const double flops = B * S * (K * (2.0 * 512.0 + 8.0 * kM) + 8.0 * kM);
     void* h = timer_begin("wbd8_sweep", s, bytes, flops);
     hipError_t e = a.carry ? launch_k<true>(a, s) : launch_k<false>(a, s);
@@ -570,11 +605,11 @@ hipError_t launch_wbd8_rev(const Wbd8RevArgs& a, hipStream_t s) {
     if (a.S < 5 || (a.S - 1) % 4 != 0 || a.S > (1LL << 30) || a.B > (1LL << 30) || a.n < 2 ||
         a.n - 1 > kWbd8MaxK)
         return hipErrorInvalidValue;
-    // algorithmic work: the samples and the K levels read once, dA written once;
+    // algorithmic work: the samples and the kept levels (W_2..W_K) read once, dA written once;
     // per sample and level two products (A^T Ybar and Ybar W^T) and the
     // transposed prefix step and sum cotangent, 10 flops per entry
     const double K = a.n - 1, S = (double)a.S, B = (double)a.B;
-    const double bytes = 8.0 * B * S * kM * (K + 2.0);
+    const double bytes = 8.0 * B * S * kM * (kept_levels(a.n - 1) + 2.0);
     const double flops = B * S * K * (2.0 * 2.0 * 512.0 + 10.0 * kM);
     void* h = timer_begin("wbd8_rev", s, bytes, flops);
     hipError_t e;

@@ -171,9 +171,9 @@ int magnus_sc_vjp_ckpt_f64(const magnus_problem* p, uint64_t count, const double
  *      EINVAL) solve through one basis sweep -- levels W_1..W_{n-1}, their
  *      totals and the sums -- and a GL fold.  The forward can keep that
  *      state, (n-1)*S*d*d + (2n+1)*d*d doubles per instance against the
- *      carry's Q*(n-1)*S*d*d (d = 8 under the Boole rule with n = 3..8: n-2
- *      levels, W_1 being rebuilt from its total), and a VJP given it skips its
- *      own sweep; its dA is
+ *      carry's Q*(n-1)*S*d*d (d = 8 under the Boole rule with n <= 8: the
+ *      (n-1)/2 even levels, the odd ones being rebuilt from their totals),
+ *      and a VJP given it skips its own sweep; its dA is
  *      the same as magnus_vjp_f64 with carry == NULL (same kernels on the same
  *      state).  The state is valid only with the same problem, count and input. ---- */
 uint64_t magnus_basis_state_elems(const magnus_problem* p, uint64_t count);

@@ -601,8 +601,9 @@ int wbd_sweep(const Plan& pl, magnus::LmArgs& a, hipStream_t stream) {
 }
 
 // The W-basis state of `count` instances, as one buffer: the levels W_1..W_K
-// [B][K][S][m] (the single-pass d = 8 sweeps keep W_2..W_K only, or W_1 when
-// K = 1: their reverse sweep rebuilds W_1 from its total), then the sums
+// [B][K][S][m] (the single-pass d = 8 sweeps keep the even levels only,
+// [B][K/2][S][m]: their reverse sweep rebuilds each odd level from its total
+// and the kept level below it), then the sums
 // [B][n][m] (rows 1..K: Z_1..Z_K), Z_0 [B][m] and the totals [B][n][m] (rows
 // 0..K-1).  It is what the reverse sweep needs besides the input and the
 // cotangent.
@@ -611,7 +612,7 @@ struct WbdState {
 };
 uint64_t wbd_kept_levels(const Plan& pl) {
     const uint64_t K = pl.n - 1;
-    return use_wbd8(pl) && K > 1 ? K - 1 : K;
+    return use_wbd8(pl) ? K / 2 : K;
 }
 uint64_t wbd_state_elems(const Plan& pl, uint64_t count) {
     const uint64_t n = pl.n, m = pl.m;
@@ -696,11 +697,12 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     if (state) {
         st = wbd_state_view(pl, count, const_cast<double*>(state));  // read only below
     } else {
-        st.W = scratch.alloc<double>(B * wbd_kept_levels(pl) * S * m);
+        const uint64_t we = B * wbd_kept_levels(pl) * S * m;  // 0 for a d = 8 problem of order 2
+        st.W = we ? scratch.alloc<double>(we) : nullptr;
         st.z = scratch.alloc<double>(B * n * m);
         st.z0 = scratch.alloc<double>(B * m);
         st.totals = scratch.alloc<double>(B * n * m);
-        if (!st.W || !st.z || !st.z0 || !st.totals) return fail(MAGNUS_ENOMEM, "device allocation failed");
+        if ((we && !st.W) || !st.z || !st.z0 || !st.totals) return fail(MAGNUS_ENOMEM, "device allocation failed");
         if (int rc = wbd_sweep_state(pl, count, in, st, stream, arena)) return rc;
     }
     double* part = scratch.alloc<double>(B * Q * n * m);

@@ -93,8 +93,8 @@ hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s);
 // arrays).
 struct Wbd8RevArgs {
     const double* in;    // [B][S][64]
-    const double* W;       // [B][K - 1][S][64]: W_2..W_K (W_1 alone when K = 1)
-    const double* totals;  // [B][n][64]: row 0 is What_1, from which the reverse rebuilds W_1
+    const double* W;       // [B][K / 2][S][64]: the even levels W_2, W_4, ...
+    const double* totals;  // [B][n][64]: rows 0..K-1 are What_1..What_K, from which the reverse rebuilds the odd levels
     const double* zbar;  // [B][n][64]
     const double* wbar;  // [B][n][64]
     double* dA;          // [B][S][64]

@@ -11,10 +11,10 @@
 // pipeline of Boole blocks -- at step t wave w takes block t - w, its input
 // block formed by wave w - 1 in the step before and handed over in LDS --
 // and every level's state stays in its wave's registers.  The levels are
-// written only when a VJP needs them (LEVELS), and then W_2..W_K only: the
-// reverse sweep walks the blocks backwards the same way, one wave per level
-// again, and its level-1 wave rebuilds W_1 = Prefix(A) backwards from the
-// total What_1 (for K = 1, W_1 is kept).
+// written only when a VJP needs them (LEVELS), and then the even ones only:
+// the reverse sweep walks the blocks backwards the same way, one wave per
+// level again, and the wave of an odd level rebuilds it backwards from its
+// total What_j and the kept level below it (W_1 from the samples).
 //
 // Products on the matrix cores.  v_mfma_f64_4x4x4f64 runs four independent
 // 4 x 4 x 4 blocks: block b's A[i][k] is lane 16k + 4b + i, its B[k][j] lane
@@ -42,9 +42,9 @@ namespace {
 
 constexpr int kM = 64;  // entries of a d = 8 sample
 
-// levels a VJP keeps in HBM: W_2..W_K (W_1 alone when K = 1), and the index of level j among them
-__host__ __device__ constexpr int kept_levels(int K) { return K > 1 ? K - 1 : 1; }
-__host__ __device__ constexpr int kept_index(int K, int j) { return K > 1 ? j - 2 : 0; }
+// levels a VJP keeps in HBM: the even ones, W_2, W_4, ..., and the index of even level j among them
+__host__ __device__ constexpr int kept_levels(int K) { return K / 2; }
+__host__ __device__ constexpr int kept_index(int j) { return j / 2 - 1; }
 
 // Boole weights times dt: trapezoid, Simpson, 3/8 and Boole rules of a block
 struct BooleW {
@@ -140,15 +140,17 @@ __device__ __forceinline__ void wait_blocks(int blocks) {
 __device__ __forceinline__ void step_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
 
 // The loader wave of a pipeline: ring index r holds block blk_of(r).  At step
-// t it issues ring index t + kAhead - 1 and waits for index t + 1 (wave 0's
-// next block) before the step's barrier; the prologue issues the first
-// indices and waits for index 0 before the barrier that opens the pipeline.
-// The waves read an index up to K - 1 steps after wave 0, and the loader
-// rewrites a slot only after the barrier that ends the step of its last
-// reader, so the ring holds K + kAhead - 1 slots.
-template <int R, typename BlkOf>
+// t it issues ring index t + kAhead - 1 and waits for index t + LEAD before
+// the step's barrier (wave 0 reads index t at step t, and with LEAD = 2 index
+// t + 1 as well); the prologue issues the first indices and waits for indices
+// 0..LEAD-1 before the barrier that opens the pipeline.  The waves read an
+// index up to K - 1 steps after wave 0, and the loader rewrites a slot only
+// after the barrier that ends the step of its last reader, so the ring holds
+// K + kAhead - 1 slots.
+template <int R, int LEAD, typename BlkOf>
 __device__ __forceinline__ void ring_loader(RingSlot* ring, const double* A, long long nb, long long steps, int lane,
                                             BlkOf blk_of) {
+    static_assert(LEAD >= 1 && LEAD < kAhead, "the loader must issue ahead of what it waits for");
     const int pos = lane & 31, r = pos >> 2, h = (pos & 3) ^ ((r >> 1) & 3);
     const double* src = A + (1 + (lane >> 5)) * (long long)kM + r * 8 + 2 * h;  // sample 4 blk + 1 + lane / 32
     auto issue = [&](long long ri) {
@@ -159,14 +161,18 @@ __device__ __forceinline__ void ring_loader(RingSlot* ring, const double* A, lon
             __builtin_amdgcn_global_load_lds(src + (4 * blk + 2 * q) * (long long)kM,
                                              (__attribute__((address_space(3))) void*)&slot.v[2 * q][0], 16, 0, 0);
     };
+    // index `need` (and those before it) complete; the ones issued after it may be in flight
+    auto wait_for = [&](long long need, long long last_issued) {
+        wait_blocks((int)(last_issued - (need < last_issued ? need : last_issued)));
+    };
     const long long pre = nb < kAhead - 1 ? nb : kAhead - 1;
     for (long long ri = 0; ri < pre; ++ri) issue(ri);
-    wait_blocks((int)(pre - 1));
+    wait_for(LEAD - 1, pre - 1);
     step_barrier();
     for (long long t = 0; t < steps; ++t) {
         const long long ri = t + kAhead - 1;
         if (ri < nb) issue(ri);
-        if (t + 1 < nb) wait_blocks((int)((ri < nb ? ri : nb - 1) - (t + 1)));
+        if (t + LEAD < nb) wait_for(t + LEAD, ri < nb ? ri : nb - 1);
         step_barrier();
     }
 }
@@ -187,6 +193,20 @@ __device__ __forceinline__ double2 bpair_dpp(double own) {
     };
     return double2{join(dpp_ror8(lo, lo, 0xC), dpp_ror8(hi, hi, 0xC)),   // row 2k: I = 1 lanes take L ^ 8
                    join(dpp_ror8(lo, lo, 0x3), dpp_ror8(hi, hi, 0x3))};  // row 2k + 1: I = 0 lanes take L ^ 8
+}
+
+// (V[2k][col], V[2k + 1][col]) of lane L from its own V[2k + J][col] (lane
+// L ^ 4 holds the other: row rotations by 4 and 12 with bank masks)
+__device__ __forceinline__ double2 pair4_dpp(double own) {
+    const unsigned long long v = __builtin_bit_cast(unsigned long long, own);
+    const int lo = (int)(unsigned)v, hi = (int)(unsigned)(v >> 32);
+    auto join = [](int l, int h) {
+        return __builtin_bit_cast(double, ((unsigned long long)(unsigned)h << 32) | (unsigned)l);
+    };
+    return double2{join(__builtin_amdgcn_update_dpp(lo, lo, 0x124, 0xF, 0xA, false),   // J = 1 lanes take L - 4
+                        __builtin_amdgcn_update_dpp(hi, hi, 0x124, 0xF, 0xA, false)),
+                   join(__builtin_amdgcn_update_dpp(lo, lo, 0x12C, 0xF, 0x5, false),   // J = 0 lanes take L + 4
+                        __builtin_amdgcn_update_dpp(hi, hi, 0x12C, 0xF, 0x5, false))};
 }
 
 // Y[2k + I][4J + j] (+ c) of Y = X V from the A chunk x and the B pair v
@@ -242,10 +262,8 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
         s1 = Acc{0.0, y0, 0.0};
         z1 = bw.b14 * y0;
     }
-    if constexpr (LEVELS) {
-        if (FIRST && K == 1) W[own] = 0.0;
-        if (MAKES) W[(long long)kept_index(K, j + 1) * S * kM + own] = 0.0;
-    }
+    const bool keeps = LEVELS && MAKES && ((j + 1) & 1) == 0;  // W_{j+1} is an even level
+    if (keeps) W[(long long)kept_index(j + 1) * S * kM + own] = 0.0;
     step_barrier();  // block 0 is in the ring
     int slot = 0;    // ring slot of this wave's next block
     for (int t = 0; t < nb + K - 1; ++t) {
@@ -272,11 +290,6 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
                 wv[3] = brule<4>(s1, ao[3], bw);
 #pragma unroll
                 for (int p = 0; p < 4; ++p) wp[p] = bpair_dpp(wv[p]);
-                if constexpr (LEVELS && K == 1) {  // W_1 is kept only when it is the only level
-                    double* Wb = W + (4LL * blk + 1) * kM + own;
-#pragma unroll
-                    for (int p = 0; p < 4; ++p) Wb[p * kM] = wv[p];
-                }
             } else {  // the previous wave formed this block in the previous step
 #pragma unroll
                 for (int p = 0; p < 4; ++p) wp[p] = *reinterpret_cast<const double2*>(&hand[(t - 1) & 1][w - 1][p][hp]);
@@ -293,8 +306,8 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
                 v[3] = brule<4>(sn, y[3], bw);
 #pragma unroll
                 for (int p = 0; p < 4; ++p) hand[t & 1][w][p][hw] = v[p];
-                if constexpr (LEVELS) {  // the level this wave forms, kept for the reverse sweep
-                    double* Wb = W + ((long long)kept_index(K, j + 1) * S + 4LL * blk + 1) * kM + own;
+                if (keeps) {  // the level this wave forms, kept for the reverse sweep
+                    double* Wb = W + ((long long)kept_index(j + 1) * S + 4LL * blk + 1) * kM + own;
 #pragma unroll
                     for (int p = 0; p < 4; ++p) Wb[p * kM] = v[p];
                 }
@@ -323,7 +336,7 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     const long long nb = (a.S - 1) / 4;
     const Lane L(lane);
     if (w == K) {
-        ring_loader<R>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane, [](long long r) { return r; });
+        ring_loader<R, 1>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane, [](long long r) { return r; });
     } else if (w == 0) {
         sweep_wave<K, LEVELS, true, (K > 1)>(a, b, w, L, ring, hand);
     } else if (w < K - 1) {
@@ -388,15 +401,18 @@ __device__ __forceinline__ double boole_c(long long i, long long S, const BooleW
 
 // One wave of the reverse pipeline, level j = K - w.  TOP: level K, whose
 // input Ybar_{K+1} = c Zbar_K needs no hand-over; LAST: level 1, which adds
-// Ybar_1 and writes dA (the others hand Ybar_j and the dA sum on).  For K > 1
-// the level-1 wave rebuilds W_1 = Prefix(A) backwards instead of reading it:
-// from a block's end value, the block's base is that value minus the block's
-// Boole sum, and its inner positions follow from the base with the forward's
-// rules; it starts from the total What_1 and works with the transposed
-// entries W_1[c][2k + I] that its W^T B pair needs (the ring holds the
-// block's samples and, one slot above, its first sample, the last of the
-// block below).
-template <int K, bool TOP, bool LAST>
+// Ybar_1 and writes dA (the others hand Ybar_j and the dA sum on).  MODE: how
+// the wave gets W_j's entries W_j[c][2k + I] (its half of W_j^T's B pair):
+// 0 (even j) reads them from the kept level; 1 (j = 1) and 2 (odd j >= 3)
+// rebuild the level backwards: from a block's end value, the block's base is
+// that value minus the block's Boole sum of the integrand Y_j, and its inner
+// positions follow from the base with the forward's rules, starting from the
+// total What_j.  Y_1 = A comes from the ring; Y_j = A W_{j-1} is formed as
+// Y_j^T = W_{j-1}^T A^T, whose entries (2k + I, c) are the ones needed, from
+// the kept level W_{j-1} (each lane reads one entry, lane L ^ 4 the other of
+// its pair) and A's rows from the ring.  A block's first sample is the last of
+// the block below, one ring slot above and in the next buffer of W_{j-1}.
+template <int K, bool TOP, bool LAST, int MODE>
 __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int lane, const Lane& L,
                                          const RingSlot* ring, double (*ybh)[K > 1 ? K - 1 : 1][4][64],
                                          double (*dah)[K > 1 ? K - 1 : 1][4][64]) {
@@ -410,17 +426,19 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     // Ybar hand-over words: own entry, B pair, A chunk (2i + I, 2k), (2i + I, 2k + 1)
     const int yw = ybar_word(2 * k + I, c), yp = ybar_word(2 * k, c);
     const int ya0 = ybar_word(2 * i + I, 2 * k), ya1 = ybar_word(2 * i + I, 2 * k + 1);
-    // this lane's half W_j[c][2k + I] of W_j^T's B pair (W_j[c][2k], W_j[c][2k + 1]);
-    // lane L ^ 8 loads the other half
-    constexpr bool REBUILD = LAST && K > 1;
+    static_assert(MODE != 1 || LAST, "the samples rebuild level 1 only");
+    // MODE 0: this lane's half W_j[c][2k + I] of W_j^T's B pair (lane L ^ 8
+    // loads the other half); MODE 2: this lane's entry W_{j-1}[2k + J][2i + I]
     const double* __restrict__ Wj =
-        REBUILD ? nullptr : a.W + ((long long)b * kept_levels(K) + kept_index(K, j)) * S * kM + c * 8 + 2 * k + I;
-    const int rq = ring_word(c, 2 * k + I);  // REBUILD: A[c][2k + I] in a ring sample
-    double w1 = 0.0, a0t = 0.0;              // REBUILD: W_1[c][2k + I] at the current block's end; A_0 there
-    if constexpr (REBUILD) {
-        w1 = a.totals[(long long)b * n * kM + c * 8 + 2 * k + I];
-        a0t = a.in[(long long)b * S * kM + c * 8 + 2 * k + I];
-    }
+        MODE == 1 ? nullptr
+        : MODE == 0
+            ? a.W + ((long long)b * kept_levels(K) + kept_index(j)) * S * kM + c * 8 + 2 * k + I
+            : a.W + ((long long)b * kept_levels(K) + kept_index(j - 1)) * S * kM + (2 * k + L.J) * 8 + 2 * i + I;
+    const int rq = ring_word(c, 2 * k + I);  // MODE 1: A[c][2k + I] in a ring sample
+    const int rc = 2 * chunk_pos(c, k);      // MODE 2: A's row chunk A[c][2k .. 2k + 1]
+    double wend = 0.0, a0t = 0.0;  // MODE 1, 2: W_j[c][2k + I] at the current block's end; MODE 1: A_0 there
+    if constexpr (MODE != 0) wend = a.totals[((long long)b * n + (j - 1)) * kM + c * 8 + 2 * k + I];
+    if constexpr (MODE == 1) a0t = a.in[(long long)b * S * kM + c * 8 + 2 * k + I];
     double* __restrict__ dA = a.dA + (long long)b * S * kM;
     const BooleW bw(a.dt);
 
@@ -438,7 +456,7 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     // unrolled by three), never by copying: a copy of a register whose load is
     // in flight would wait for that load.
     auto loadw = [&](double* y, int blk) {
-        if constexpr (!REBUILD) {
+        if constexpr (MODE != 1) {
 #pragma unroll
             for (int p = 0; p < 4; ++p) y[p] = Wj[(4LL * blk + 1 + p) * kM];
         }
@@ -449,26 +467,39 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     int slot = 0;  // ring slot of this wave's next block
 
     // one pipeline step; r = this wave's reverse position (block nb - 1 - r;
-    // out of range: the wave only takes part in the barrier), wt: W_j's halves
-    // of the block, wpf: the buffer for the block two below
-    auto step = [&](int r, double* wt, double* wpf) {
+    // out of range: the wave only takes part in the barrier), wt: the loaded
+    // entries of the block (MODE 2: of W_{j-1}, which the step turns into W_j's
+    // halves), wn: those of the block below, wpf: the buffer for the block two
+    // below
+    auto step = [&](int r, double* wt, const double* wn, double* wpf) {
         const int blk = nb - 1 - r;
         if (r >= 0 && blk >= 0) {
             const int t = r + w;  // the pipeline step
             const RingSlot& sl = ring[slot];
             slot = slot + 1 == R ? 0 : slot + 1;
             if (blk > 1) loadw(wpf, blk - 2);
-            if constexpr (REBUILD) {  // W_1 over the block, from its end value w1
-                const double a4[5] = {blk > 0 ? ring[slot].v[3][rq] : a0t, sl.v[0][rq], sl.v[1][rq], sl.v[2][rq],
-                                      sl.v[3][rq]};
+            if constexpr (MODE != 0) {  // W_j over the block, from its end value wend
+                double f[5];  // the integrand Y_j[c][2k + I] at the block's samples 4 blk .. 4 blk + 4
+                if constexpr (MODE == 1) {
+                    f[0] = blk > 0 ? ring[slot].v[3][rq] : a0t;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p) f[p + 1] = sl.v[p][rq];
+                } else {
+                    // Y_j[0] = A_0 W_{j-1}[0] = 0; otherwise the block below's last sample
+                    f[0] = blk > 0 ? prod(pair4_dpp(wn[3]), *reinterpret_cast<const double2*>(&ring[slot].v[3][rc]))
+                                   : 0.0;
+#pragma unroll
+                    for (int p = 0; p < 4; ++p)
+                        f[p + 1] = prod(pair4_dpp(wt[p]), *reinterpret_cast<const double2*>(&sl.v[p][rc]));
+                }
                 const double base =
-                    w1 - fma(bw.b14, a4[4], fma(bw.b64, a4[3], fma(bw.b24, a4[2], fma(bw.b64, a4[1], bw.b14 * a4[0]))));
-                Acc st = Acc{base, a4[0], 0.0};
-                wt[0] = brule<1>(st, a4[1], bw);
-                wt[1] = brule<2>(st, a4[2], bw);
-                wt[2] = brule<3>(st, a4[3], bw);
-                wt[3] = w1;
-                w1 = base;
+                    wend - fma(bw.b14, f[4], fma(bw.b64, f[3], fma(bw.b24, f[2], fma(bw.b64, f[1], bw.b14 * f[0]))));
+                Acc st = Acc{base, f[0], 0.0};
+                wt[0] = brule<1>(st, f[1], bw);
+                wt[1] = brule<2>(st, f[2], bw);
+                wt[2] = brule<3>(st, f[3], bw);
+                wt[3] = wend;
+                wend = base;
             }
             const long long s = 4LL * blk;
             double cz[4];
@@ -531,16 +562,16 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     step_barrier();  // ring index 0 is in the ring
     // steps before this wave's first block, its nb blocks (buffers rotating by
     // three), then the steps after its last block
-    for (int t = 0; t < w; ++t) step(-1, wa, wc);
+    for (int t = 0; t < w; ++t) step(-1, wa, wbuf, wc);
     int r = 0;
     for (; r + 3 <= nb; r += 3) {
-        step(r, wa, wc);
-        step(r + 1, wbuf, wa);
-        step(r + 2, wc, wbuf);
+        step(r, wa, wbuf, wc);
+        step(r + 1, wbuf, wc, wa);
+        step(r + 2, wc, wa, wbuf);
     }
-    if (r < nb) step(r, wa, wc);
-    if (r + 1 < nb) step(r + 1, wbuf, wa);
-    for (int t = w + nb; t < nb + K - 1; ++t) step(-1, wa, wc);
+    if (r < nb) step(r, wa, wbuf, wc);
+    if (r + 1 < nb) step(r + 1, wbuf, wc, wa);
+    for (int t = w + nb; t < nb + K - 1; ++t) step(-1, wa, wbuf, wc);
     // sample 0: the prefixes are constant there (W_j[0] = 0), so dA_0 = Ybar_1[0]
     // = level 1's pending from the lowest block + c_0 Zbar_0
     if constexpr (LAST) dA[own] = fma(bw.b14, zb, pend);
@@ -560,16 +591,18 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     const long long nb = (a.S - 1) / 4;
     const Lane L(lane);
     if (w == K) {  // ring index r holds block nb - 1 - r (the blocks are walked top-down)
-        ring_loader<R>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane,
-                       [nb](long long r) { return nb - 1 - r; });
+        ring_loader<R, 2>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane,
+                          [nb](long long r) { return nb - 1 - r; });
     } else if (K == 1) {
-        rev_wave<K, true, true>(a, b, w, lane, L, ring, ybh, dah);
+        rev_wave<K, true, true, 1>(a, b, w, lane, L, ring, ybh, dah);
     } else if (w == 0) {
-        rev_wave<K, true, false>(a, b, w, lane, L, ring, ybh, dah);
-    } else if (w < K - 1) {
-        rev_wave<K, false, false>(a, b, w, lane, L, ring, ybh, dah);
+        rev_wave<K, true, false, (K & 1) ? 2 : 0>(a, b, w, lane, L, ring, ybh, dah);
+    } else if (w == K - 1) {
+        rev_wave<K, false, true, 1>(a, b, w, lane, L, ring, ybh, dah);
+    } else if ((K - w) & 1) {
+        rev_wave<K, false, false, 2>(a, b, w, lane, L, ring, ybh, dah);
     } else {
-        rev_wave<K, false, true>(a, b, w, lane, L, ring, ybh, dah);
+        rev_wave<K, false, false, 0>(a, b, w, lane, L, ring, ybh, dah);
     }
 }
 

@@ -674,9 +674,9 @@ def test_d8_boole_runs_the_single_pass_sweep(B, n):
         torch.cuda.synchronize()
     finally:
         capi.timing_enable(False)
-    # the VJP keeps the levels W_2..W_K in its sweep (W_1 when K = 1) and walks
-    # them back in one launch, rebuilding W_1 from its total
-    kept = K - 1 if K > 1 else 1
+    # the VJP keeps the even levels in its sweep and walks them back in one
+    # launch, rebuilding the odd levels from their totals
+    kept = K // 2
     assert capi.timing_read("wbd8_sweep")[0] == 1
     assert capi.timing_counts("wbd8_sweep")[0] == 8 * B * S * m * (1 + kept)
     assert capi.timing_read("wbd8_rev")[0] == 1
@@ -715,7 +715,7 @@ def test_d8_boole_vjp_few_blocks(S, n):
     kw = dict(op="sum", integrator="Boole")
     dA = torch_ops.vjp(n, At, ct, None, 0.0, 1.0, **kw)
     out, state = torch_ops.forward_basis(n, At, 0.0, 1.0, **kw)
-    assert state.numel() == B * (max(n - 2, 1) * S * d * d + (2 * n + 1) * d * d)
+    assert state.numel() == B * ((n - 1) // 2 * S * d * d + (2 * n + 1) * d * d)
     dA2 = torch_ops.vjp_basis(n, At, ct, state, 0.0, 1.0, **kw)
     assert torch.equal(dA, dA2)
     dA = dA.cpu().numpy()
@@ -909,8 +909,8 @@ def test_basis_state_vjp_matches_recompute_bitwise(d, integrator, op, n):
     """A forward that keeps its basis state (levels, sums, totals) and a VJP that
     starts from it run the same kernels on the same state as the carry-free
     forward and VJP, so outputs and dA are bit-identical; both match the oracle;
-    the state is n - 1 levels, against the carry's Q (n - 1) (n - 2 for the
-    d = 8 Boole sweeps with n >= 3, which rebuild W_1 from its total)."""
+    the state is n - 1 levels, against the carry's Q (n - 1) ((n - 1) // 2
+    for the d = 8 Boole sweeps, which rebuild the odd levels from their totals)."""
     import torch
 
     from magnus import torch_ops
@@ -924,7 +924,7 @@ def test_basis_state_vjp_matches_recompute_bitwise(d, integrator, op, n):
     ref_out = torch_ops.forward(n, At, 0.0, 1.0, **kw)
     ref_dA = torch_ops.vjp(n, At, cot, None, 0.0, 1.0, **kw)
     out, state = torch_ops.forward_basis(n, At, 0.0, 1.0, **kw)
-    kept = n - 2 if (d == 8 and integrator == "Boole" and n >= 3) else n - 1
+    kept = (n - 1) // 2 if (d == 8 and integrator == "Boole") else n - 1
     assert state.numel() == B * (kept * S * d * d + (2 * n + 1) * d * d)
     dA1 = torch_ops.vjp_basis(n, At, cot, state, 0.0, 1.0, **kw)
     dA2 = torch_ops.vjp_basis(n, At, cot, state, 0.0, 1.0, **kw)  # the VJP does not write the state

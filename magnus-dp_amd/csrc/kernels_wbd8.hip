@@ -577,7 +577,33 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     if constexpr (LAST) dA[own] = fma(bw.b14, zb, pend);
 }
 
-// one workgroup per instance: K level wavefronts and the loader (wave K)
+// Which pipeline role (0..K-1: the level wave at that lag, K: the loader)
+// each hardware wavefront of a reverse workgroup takes.  Wavefront h runs on
+// SIMD h % 4, so h and h + 4 share one.  The waves of odd levels >= 3 rebuild
+// their level on top of the adjoint and are the busiest, the loader computes
+// nothing: roles are paired heaviest with lightest so that the busy ones
+// spread over the SIMDs.  (The forward sweep keeps wave h at lag h: its
+// level waves carry near equal work, and a remapping measured slower.)
+template <int K>
+struct RevRoles {
+    int role[K + 1];
+    static constexpr int weight(int p) { return p == K ? 0 : (((K - p) & 1) && K - p >= 3 ? 2 : 1); }
+    constexpr RevRoles() : role() {
+        constexpr int N = K + 1;
+        int order[N] = {};  // roles by weight, heaviest first (stable)
+        int cnt = 0;
+        for (int wgt = 2; wgt >= 0; --wgt)
+            for (int p = 0; p <= K; ++p)
+                if (weight(p) == wgt) order[cnt++] = p;
+        int lo = 0, hi = N - 1;
+        for (int sm = 0; sm < 4 && sm < N; ++sm) {
+            role[sm] = order[lo++];
+            if (sm + 4 < N) role[sm + 4] = order[hi--];  // a SIMD with two wavefronts: the lightest left
+        }
+    }
+};
+
+// one workgroup per instance: K level wavefronts and the loader (role K)
 template <int K>
 __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_eu(4))) wbd8_rev(Wbd8RevArgs a) {
     constexpr int H = K > 1 ? K - 1 : 1;
@@ -586,7 +612,9 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     // Ybar_j blocks (ybar_word) and dA sums (by lane) handed from wave w to wave w + 1
     __shared__ alignas(16) double ybh[2][H][4][64];
     __shared__ double dah[2][H][4][64];
-    const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    constexpr RevRoles<K> roles;
+    const int hw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+    const int w = roles.role[hw];
     const int b = blockIdx.x;
     const long long nb = (a.S - 1) / 4;
     const Lane L(lane);

@@ -544,7 +544,7 @@ def _bench_controller(args, torch, dev):
 
     def learned():
         # the network's class starts the judge; each round runs the VJP of its
-        # finer sweep from the kept basis state, so the accepted round delivers
+        # (S, n) sweep from the kept basis state, so the accepted round delivers
         # the forward and the VJP
         start, t1 = C.timed(lambda: ctl.predict(
             C.features(torch_ops.sample_fourier(c_ev, C.FEATURE_GRID), 0.0, 1.0)).cpu().numpy())

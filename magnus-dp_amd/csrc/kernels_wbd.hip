@@ -322,12 +322,166 @@ __global__ void __launch_bounds__(256) wbd_fold_vjp_reduce(WbdArgs a) {
     a.wbar[idx] = w;
 }
 
+// ---------------------------------------------------------------------------
+// Small problems (d <= 8, n <= NM): one wavefront per (instance, node).  Lane e
+// owns entry e (row e / d, column e % d) of every matrix; the instance's totals
+// and sums, the node's T_j and (VJP) its cotangents sit in LDS, so a product is
+// 2d LDS reads and d FMAs per lane with no global-memory round trip between the
+// dependent products.  Every entry is formed by the operations of mat_set /
+// mat_axpy / mat_acc above (their d <= 8 branch), in the same order.  The
+// workgroup is one wavefront, so its barriers cost nothing.
+// ---------------------------------------------------------------------------
+constexpr int kSmallD = 8;
+constexpr int kSmallM = kSmallD * kSmallD;
+
+// (X' Y')[r][c] summed over k ascending from 0.0, X' = X or X^T, Y' = Y or Y^T
+__device__ __forceinline__ double sdot(const double* X, bool tx, const double* Y, bool ty, int d, int r, int c) {
+    double acc = 0.0;
+    for (int k = 0; k < d; ++k) acc = fma(tx ? X[k * d + r] : X[r * d + k], ty ? Y[c * d + k] : Y[k * d + c], acc);
+    return acc;
+}
+
+template <int NM>
+struct SmallFold {
+    double wh[NM][kSmallM];  // What_1..What_K at rows 0..K-1
+    double z[NM][kSmallM];   // Z_0..Z_{n-1}
+    double t[NM][kSmallM];   // T_1..T_K at rows 1..K
+};
+
+// load What_1..What_K and Z_0..Z_{n-1} of instance b (orders up to n), then T_1..T_K
+template <int NM>
+__device__ void small_totals(SmallFold<NM>& L, const WbdArgs& a, long long b, int n, double s) {
+    const int K = n - 1, d = a.d, e = threadIdx.x, r = e / d, c = e - r * d;
+    const long long M = a.m;
+    const bool live = e < a.m;
+    const double* What = a.totals + b * (long long)row_stride(a) * M;
+    if (live) {
+        for (int j = 0; j < K; ++j) L.wh[j][e] = What[j * M + e];
+        for (int l = 0; l < n; ++l) L.z[l][e] = zrow(a, b, l)[e];
+    }
+    __syncthreads();
+    for (int j = 1; j <= K; ++j) {
+        if (live) {
+            double v = 1.0 * L.wh[j - 1][e];
+            for (int l = 1; l < j; ++l) v = fma(s, sdot(L.wh[l - 1], false, L.t[j - l], false, d, r, c), v);
+            L.t[j][e] = v;
+        }
+        __syncthreads();
+    }
+}
+
+// node (weight wq, shift s) shares of Omega_2..Omega_n into part rows Om + (j-1) M
+template <int NM>
+__device__ void small_omegas(SmallFold<NM>& L, const WbdArgs& a, int n, double s, double wq, double* Om) {
+    const int d = a.d, e = threadIdx.x, r = e / d, c = e - r * d;
+    if (e >= a.m) return;
+    const long long M = a.m;
+    for (int j = 2; j <= n; ++j) {
+        double v = wq * L.z[j - 1][e];
+        for (int l = 0; l <= j - 2; ++l) v = fma(wq * s, sdot(L.z[l], false, L.t[j - 1 - l], false, d, r, c), v);
+        Om[(j - 1) * M + e] = v;
+    }
+}
+
+template <int NM>
+__global__ void __launch_bounds__(64) wbd_fold_small(WbdArgs a) {
+    __shared__ SmallFold<NM> L;
+    const long long b = blockIdx.x / a.Q;
+    const int q = (int)(blockIdx.x - b * a.Q);
+    const double s = a.x[q] - 1.0;
+    small_totals(L, a, b, a.n, s);
+    small_omegas(L, a, a.n, s, a.w[q], a.part + ((long long)blockIdx.x * a.n) * a.m);
+}
+
+template <int NM>
+__global__ void __launch_bounds__(64) wbd_fold_orders_small(WbdArgs a) {
+    __shared__ SmallFold<NM> L;
+    const long long b = blockIdx.x / a.nflat;
+    const int f = (int)(blockIdx.x - b * a.nflat);
+    const int k = a.korder[f];
+    const double s = a.x[f] - 1.0;
+    small_totals(L, a, b, k, s);
+    small_omegas(L, a, k, s, a.w[f], a.part + ((long long)blockIdx.x * a.n) * a.m);
+}
+
+template <int NM>
+struct SmallFoldVjp {
+    SmallFold<NM> f;
+    double tb[NM][kSmallM];  // T_bar_1..T_bar_K at rows 1..K
+    double zb[NM][kSmallM];  // Zbar_0..Zbar_{n-1}
+    double wb[NM][kSmallM];  // What_bar_1..What_bar_n at rows 0..n-1
+    double ob[NM][kSmallM];  // cotangent of Omega_j at row j-1 (rows with obar(j) set)
+};
+
+template <int NM>
+__global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
+    __shared__ SmallFoldVjp<NM> L;
+    const long long b = blockIdx.x / a.Q;
+    const int q = (int)(blockIdx.x - b * a.Q);
+    const int n = a.n, K = n - 1, d = a.d, e = threadIdx.x, r = e / d, c = e - r * d;
+    const long long M = a.m;
+    const bool live = e < a.m;
+    const double s = a.x[q] - 1.0, wq = a.w[q];
+    if (live) {
+        for (int l = 0; l < n; ++l) {
+            L.zb[l][e] = 0.0;
+            L.wb[l][e] = 0.0;
+        }
+        for (int j = 1; j <= K; ++j) L.tb[j][e] = 0.0;
+        for (int j = 2; j <= n; ++j) {
+            const double* Ob = obar(a, b, j);
+            if (Ob) L.ob[j - 1][e] = Ob[e];
+        }
+    }
+    small_totals(L.f, a, b, n, s);  // ends with a barrier: the loads above are visible
+    auto& T = L.f.t;
+    auto& Z = L.f.z;
+    if (live) {
+        for (int j = 2; j <= n; ++j) {
+            if (!obar(a, b, j)) continue;
+            const double* Ob = L.ob[j - 1];
+            L.zb[j - 1][e] = fma(wq, Ob[e], L.zb[j - 1][e]);
+            for (int l = 0; l <= j - 2; ++l) {
+                L.zb[l][e] = fma(wq * s, sdot(Ob, false, T[j - 1 - l], true, d, r, c), L.zb[l][e]);
+                L.tb[j - 1 - l][e] = fma(wq * s, sdot(Z[l], true, Ob, false, d, r, c), L.tb[j - 1 - l][e]);
+            }
+        }
+    }
+    __syncthreads();
+    for (int j = K; j >= 1; --j) {
+        if (live) {
+            L.wb[j - 1][e] = fma(1.0, L.tb[j][e], L.wb[j - 1][e]);
+            for (int l = 1; l < j; ++l) {
+                L.wb[l - 1][e] = fma(s, sdot(L.tb[j], false, T[j - l], true, d, r, c), L.wb[l - 1][e]);
+                L.tb[j - l][e] = fma(s, sdot(L.f.wh[l - 1], true, L.tb[j], false, d, r, c), L.tb[j - l][e]);
+            }
+        }
+        __syncthreads();
+    }
+    if (live) {
+        double* Zb = a.part + (long long)blockIdx.x * n * M;
+        double* Wb = a.part2 + (long long)blockIdx.x * n * M;
+        for (int l = 0; l < n; ++l) {
+            Zb[l * M + e] = L.zb[l][e];
+            Wb[l * M + e] = L.wb[l][e];
+        }
+    }
+}
+
+// the small kernels apply at d <= 8 and n <= 16 (NM = 8 or 16)
+__device__ __host__ inline int small_fold_nm(int d, int n) { return d > kSmallD ? 0 : n <= 8 ? 8 : n <= 16 ? 16 : 0; }
+
 }  // namespace
 
 hipError_t launch_wbd_fold(const WbdArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     void* h = timer_begin("wbd_fold", s);
-    if (a.d <= kFoldLdsSmall)
+    const int nm = small_fold_nm(a.d, a.n);
+    if (nm == 8)
+        hipLaunchKernelGGL(wbd_fold_small<8>, dim3((unsigned)(a.B * a.Q)), dim3(64), 0, s, a);
+    else if (nm == 16)
+        hipLaunchKernelGGL(wbd_fold_small<16>, dim3((unsigned)(a.B * a.Q)), dim3(64), 0, s, a);
+    else if (a.d <= kFoldLdsSmall)
         hipLaunchKernelGGL(wbd_fold<kFoldLdsSmall>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
     else
         hipLaunchKernelGGL(wbd_fold<kFoldLds>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
@@ -340,7 +494,12 @@ hipError_t launch_wbd_fold_orders(const WbdArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     void* h = timer_begin("wbd_fold", s);
     const unsigned grid = (unsigned)(a.B * a.nflat);
-    if (a.d <= kFoldLdsSmall)
+    const int nm = small_fold_nm(a.d, a.n);
+    if (nm == 8)
+        hipLaunchKernelGGL(wbd_fold_orders_small<8>, dim3(grid), dim3(64), 0, s, a);
+    else if (nm == 16)
+        hipLaunchKernelGGL(wbd_fold_orders_small<16>, dim3(grid), dim3(64), 0, s, a);
+    else if (a.d <= kFoldLdsSmall)
         hipLaunchKernelGGL(wbd_fold_orders<kFoldLdsSmall>, dim3(grid), dim3(kFoldThreads), 0, s, a);
     else
         hipLaunchKernelGGL(wbd_fold_orders<kFoldLds>, dim3(grid), dim3(kFoldThreads), 0, s, a);
@@ -353,7 +512,12 @@ hipError_t launch_wbd_fold_orders(const WbdArgs& a, hipStream_t s) {
 hipError_t launch_wbd_fold_vjp(const WbdArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
     void* h = timer_begin("wbd_fold_vjp", s);
-    if (a.d <= kFoldLdsSmall)
+    const int nm = small_fold_nm(a.d, a.n);
+    if (nm == 8)
+        hipLaunchKernelGGL(wbd_fold_vjp_small<8>, dim3((unsigned)(a.B * a.Q)), dim3(64), 0, s, a);
+    else if (nm == 16)
+        hipLaunchKernelGGL(wbd_fold_vjp_small<16>, dim3((unsigned)(a.B * a.Q)), dim3(64), 0, s, a);
+    else if (a.d <= kFoldLdsSmall)
         hipLaunchKernelGGL(wbd_fold_vjp<kFoldLdsSmall>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);
     else
         hipLaunchKernelGGL(wbd_fold_vjp<kFoldLds>, dim3((unsigned)(a.B * a.Q)), dim3(kFoldThreads), 0, s, a);

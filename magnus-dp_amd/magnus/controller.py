@@ -3,8 +3,8 @@
 Problem.  For an instance A(t) (dim d, Fourier form) and a tolerance tol, a
 resolution whose Magnus sum is within tol (relative Frobenius) of the
 S = 32769, n = 8 sum, as cheaply as possible.  A class (S, n) -- S in
-{65, ..., 16385} (2^k + 1), n in {2..8} -- is judged by the doubling pair
-(S, 2S - 1) and delivered at its finer member (2S - 1, n).
+{65, ..., 16385} (2^k + 1), n in {2..8} -- is delivered at (S, n) and judged
+by the doubling pair (S, 2S - 1).
 
 Pieces, all on the device:
   * labels: one all-orders sweep (magnus_forward_orders_f64) per S gives the
@@ -14,18 +14,22 @@ Pieces, all on the device:
     accumulation, Adam), classifying the 63 (S, n) classes;
   * acceptance (the judge, also the fallback): class (S, n) is accepted when
     the sums at (S, n) and (2S - 1, n) agree within JUDGE_FINE * tol and, for
-    n < 8, the sums at (S, n) and (S, n + 1) within JUDGE_MARGIN * tol.  The
-    finer sum of the doubling pair is delivered: with the rule's error falling
-    at least fourfold per doubling (second order; Boole's is higher), its
-    error is at most a third of the pair's difference.  A failed check doubles
-    S and/or raises n, and repeats.  The learned path starts the judge at the
-    network's class and runs the VJP inside the same round from the finer
-    sweep's kept basis state (torch_ops.forward_basis / vjp_basis), so an
-    accepted instance costs the coarse sweep, the finer sweep and one reverse
-    sweep; the same judge started at (65, 2) is the method without the
-    network (judge rounds, then the VJP at the accepted resolution); the plain
-    method runs every instance at (16385, 8), forward with its kept basis
-    state and the VJP from it.
+    n < 8, the sums at (2S - 1, n) and (2S - 1, n + 1) within JUDGE_MARGIN *
+    tol.  The pair's difference estimates the delivered (S, n) sum's
+    discretisation error (the finer member's error is a small fraction of the
+    coarser's: at least fourfold smaller per doubling, 64-fold for Boole in
+    its asymptotic range), and the next order's term estimates its truncation
+    error, so the two bounds add up to under tol.  A failed check doubles S
+    and/or raises n, and repeats.  The learned path starts the judge at the
+    network's class; a round keeps the basis state of its (S, n) sweep
+    (torch_ops.forward_basis) and runs the VJP from it (torch_ops.vjp_basis),
+    and checks with a sweep at 2S - 1 that keeps no levels, so an accepted
+    instance costs one sweep keeping its levels and one reverse sweep at the
+    delivered resolution plus one level-free sweep at twice the samples; the
+    same judge started at (65, 2) is the method without the network (judge
+    rounds, then the VJP at the accepted resolution); the plain method runs
+    every instance at (16385, 8), forward with its kept basis state and the
+    VJP from it.
 """
 from __future__ import annotations
 
@@ -45,8 +49,8 @@ PLAIN = (16385, 8)
 CLASSES = tuple((S, n) for S in S_GRID for n in N_GRID)
 FEATURE_GRID = 65
 MAX_MODES = 8
-JUDGE_MARGIN = 0.5  # next-order check: |sum(S, n + 1) - sum(S, n)| <= JUDGE_MARGIN tol
-JUDGE_FINE = 2.0    # doubling check: |sum(2S - 1, n) - sum(S, n)| <= JUDGE_FINE tol (the finer is delivered)
+JUDGE_MARGIN = 0.25  # next-order check: |sum(2S - 1, n + 1) - sum(2S - 1, n)| <= JUDGE_MARGIN tol
+JUDGE_FINE = 0.75    # doubling check: |sum(2S - 1, n) - sum(S, n)| <= JUDGE_FINE tol (sum(S, n) is delivered)
 
 
 def cost(S: int, n: int) -> int:
@@ -103,7 +107,7 @@ def label(coeffs: torch.Tensor, tol: float, batch: int = 2048, progress=None):
     """Per instance: (labels, minimal, reachable, errors [B][9][7], reference
     sums [B][d][d]); progress(done, total) is called after every batch.
     errors[s][k] is the error of class (S_GRID[s], N_GRID[k])'s delivered sum
-    (2S - 1 samples); `minimal` is the cheapest class whose delivered sum is
+    (S samples, order n); `minimal` is the cheapest class whose delivered sum is
     within tol; `labels` -- what the network learns -- is the cheapest class
     the judge accepts in one round with its delivered sum within tol, so a
     correct prediction costs one round."""
@@ -126,14 +130,15 @@ def label(coeffs: torch.Tensor, tol: float, batch: int = 2048, progress=None):
         e = torch.empty((nb, len(S_GRID), len(N_GRID)), dtype=torch.float64, device=c.device)
         ok = torch.empty((nb, len(S_GRID), len(N_GRID)), dtype=torch.bool, device=c.device)
         for si, S in enumerate(S_GRID):
-            o, up = allS[si], allS[si + 1]  # up: 2S - 1 samples, the delivered member
+            o, up = allS[si], allS[si + 1]  # o: the delivered member, up: 2S - 1 samples
             for ni, n in enumerate(N_GRID):
                 cur = o[:, n - 1]
-                e[:, si, ni] = rel_err(up[:, n - 1], ref)
+                e[:, si, ni] = rel_err(cur, ref)
                 s_ok = rel_err(up[:, n - 1], cur) <= JUDGE_FINE * tol
                 if S >= max(S_GRID):
                     s_ok = torch.ones_like(s_ok)
-                n_ok = rel_err(o[:, n], cur) <= JUDGE_MARGIN * tol if n < nmax else torch.ones_like(s_ok)
+                n_ok = (rel_err(up[:, n], up[:, n - 1]) <= JUDGE_MARGIN * tol if n < nmax
+                        else torch.ones_like(s_ok))
                 ok[:, si, ni] = s_ok & n_ok & (e[:, si, ni] <= tol)
         e = e.cpu().numpy()
         ok = ok.cpu().numpy().reshape(nb, -1)
@@ -296,20 +301,20 @@ def _resolutions(start):
 
 
 def delivered(res):
-    """The resolution a judged class (S, n) delivers: the finer member of its doubling pair."""
-    return [(2 * S - 1, n) for S, n in res]
+    """The resolution a judged class (S, n) delivers: (S, n) itself, the coarser member of its doubling pair."""
+    return [(S, n) for S, n in res]
 
 
 def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: torch.Tensor | None = None):
     """The judge: from per-instance starting classes (list of (S, n)), verify and
     escalate until every instance passes.  Returns (accepted classes, delivered
-    sums [B][d][d] at (2S - 1, n), stats) with stats = {work: sum of (S-1)(n-1)
-    over the sweeps run (a reverse sweep counts twice), rounds: per-instance
-    rounds, calls, dA: per-instance dA through dA_of(stats, i), or None}.  With cot, every round
-    also runs the VJP of its finer sweep from that sweep's kept basis state,
-    and an instance's dA is the one of the round that accepted it.  The groups
-    of a round run on a pool of streams; the host waits once per round and
-    updates every instance with array operations."""
+    sums [B][d][d] at (S, n), stats) with stats = {work: sum of (S-1)(n-1) over
+    the sweeps run (a reverse sweep counts twice), rounds: per-instance rounds,
+    calls, dA: per-instance dA through dA_of(stats, i), or None}.  With cot,
+    every round also runs the VJP of its (S, n) sweep from that sweep's kept
+    basis state, and an instance's dA is the one of the round that accepted
+    it.  The groups of a round run on a pool of streams; the host waits once
+    per round and updates every instance with array operations."""
     B, d = coeffs.shape[0], coeffs.shape[-1]
     S_arr, n_arr = _resolutions(start)
     done = np.zeros(B, bool)
@@ -331,9 +336,9 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
         k = 0
         for S, n, all_ids in _groups(S_arr, n_arr, active):
             S2 = 2 * S - 1
-            # per instance: both sweeps' samples; with the VJP the finer sweep's
-            # kept levels, its dA and the reverse sweep's buffers
-            per = (S + S2) * d * d * 8 * (2 if cot is None else n + 4)
+            # per instance: both sweeps' samples; with the VJP the delivered
+            # sweep's kept levels, its dA and the reverse sweep's buffers
+            per = (S * (2 if cot is None else n + 4) + S2 * 2) * d * d * 8
             for ids in _chunks(all_ids, per, budget):
                 st = pool[k % len(pool)]
                 k += 1
@@ -341,28 +346,28 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
                 with torch.cuda.stream(st):
                     idx = _index(ids, coeffs.device)
                     c = coeffs[idx]
-                    nm = min(n + 1, REF_N)
-                    lo = sums(c, S, nm, first=n)  # orders n..nm at S samples
-                    cur = lo[:, 0]
-                    A2 = torch_ops.sample_fourier(c, S2)
+                    A = torch_ops.sample_fourier(c, S)
                     if cot is not None:
-                        fine, state = torch_ops.forward_basis(n, A2, 0.0, 1.0, op="sum", integrator="Boole")
-                        g = torch_ops.vjp_basis(n, A2, cot[idx], state, 0.0, 1.0, op="sum", integrator="Boole")
+                        cur, state = torch_ops.forward_basis(n, A, 0.0, 1.0, op="sum", integrator="Boole")
+                        g = torch_ops.vjp_basis(n, A, cot[idx], state, 0.0, 1.0, op="sum", integrator="Boole")
                         del state
-                        work += len(ids) * 2 * cost(S2, n)
+                        work += len(ids) * 3 * cost(S, n)  # sweep + reverse sweep
                     else:
-                        fine = torch_ops.forward(n, A2, 0.0, 1.0, op="sum", integrator="Boole")
+                        cur = torch_ops.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole")
                         g = None
-                    del A2
-                    work += len(ids) * (cost(S, nm) + cost(S2, n))
+                        work += len(ids) * cost(S, n)
+                    del A
+                    nm = min(n + 1, REF_N)
+                    hi = sums(c, S2, nm, first=n)  # orders n..nm at 2S - 1 samples, no levels kept
+                    work += len(ids) * cost(S2, nm)
                     calls += 2
                     if S < S_top:
-                        s_ok = rel_err(fine, cur) <= JUDGE_FINE * tol
+                        s_ok = rel_err(hi[:, 0], cur) <= JUDGE_FINE * tol
                     else:  # the finer member is the reference resolution
                         s_ok = torch.ones(len(ids), dtype=torch.bool, device=coeffs.device)
-                    n_ok = (rel_err(lo[:, nm - n], cur) <= JUDGE_MARGIN * tol if nm > n
+                    n_ok = (rel_err(hi[:, nm - n], hi[:, 0]) <= JUDGE_MARGIN * tol if nm > n
                             else torch.ones(len(ids), dtype=torch.bool, device=coeffs.device))
-                    out[idx] = fine
+                    out[idx] = cur
                     ids_l.append(ids)
                     s_l.append(s_ok)
                     n_l.append(n_ok)

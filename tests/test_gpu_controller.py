@@ -182,12 +182,12 @@ def test_controller_gains_over_the_judge_without_it():
     assert hit_l >= 0.97 and hit_l >= hit_0 - 0.02, (hit_l, hit_0)
 
 
-def test_judge_delivers_the_finer_sum_and_its_vjp_from_the_kept_state():
-    """accept(..., cot=...) delivers, per instance, the sum of the finer member
-    (2S - 1, n) of its accepted doubling pair and the VJP of that sweep from
-    its kept basis state: both equal the one-problem forward and VJP at the
-    delivered resolution bit for bit (the VJP of a rejected round is
-    replaced by the accepting round's)."""
+def test_judge_delivers_its_class_sum_and_its_vjp_from_the_kept_state():
+    """accept(..., cot=...) delivers, per instance, the sum of its accepted
+    class (S, n) -- the coarser member of the doubling pair -- and the VJP of
+    that sweep from its kept basis state: both equal the one-problem forward
+    and VJP at the delivered resolution bit for bit (the VJP of a rejected
+    round is replaced by the accepting round's)."""
     import torch
 
     from magnus import controller as C
@@ -201,7 +201,7 @@ def test_judge_delivers_the_finer_sum_and_its_vjp_from_the_kept_state():
     res, out, st = C.accept(coeffs, start, tol, cot=cot)
     assert st["rounds"].max() > 1
     for i, (S, n) in enumerate(res):
-        A = torch_ops.sample_fourier(coeffs[i:i + 1], 2 * S - 1)
+        A = torch_ops.sample_fourier(coeffs[i:i + 1], S)
         assert torch.equal(out[i], torch_ops.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole")[0]), i
         ref = torch_ops.vjp(n, A, cot[i:i + 1], None, 0.0, 1.0, op="sum", integrator="Boole")[0]
         assert torch.equal(C.dA_of(st, i), ref), i

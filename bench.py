@@ -84,7 +84,8 @@ def _counts(Q, K, S, m, m_in, P, n_out, op_one=False):
 
 
 KERNELS = ("sample_fourier", "sample_sc_fourier", "lm_fwd_prod", "lm_fwd_scan", "lm_fwd_fused", "lm_vjp_init", "lm_vjp_prod", "lm_vjp_scan", "lm_vjp_final",
-           "wb_fwd_pass1", "wb_combine", "wb_fwd_pass1_vjp", "wb_combine_vjp", "wb_rev", "wb_fixup", "wbd8_sweep", "wbd8_rev")
+           "wb_fwd_pass1", "wb_combine", "wb_fwd_pass1_vjp", "wb_combine_vjp", "wb_rev", "wb_fixup", "wbd8_sweep", "wbd8_rev",
+           "walk_fwd", "walk_rev", "walk_reduce")
 
 # su(2) products on the single-pass path: pure sample x 4-vector = 4 mul + 8 FMA
 # (20 flops), 4-vector x 4-vector = 4 mul + 12 FMA (28 flops).

@@ -551,7 +551,8 @@ size_t vjp_scratch_bytes(const Plan& pl, uint64_t count) {
     const size_t jobs = count * pl.Q;
     const size_t carry = scratch_round(count * pl.Q * (pl.n - 1) * S * m * 8);
     const size_t fwd = scratch_round(jobs * S * m * 8);  // VJP::none forward: Y only
-    const size_t rev = scratch_round(jobs * 2 * S * m * 8) + scratch_round(jobs * S * m * 8);
+    const size_t rev = scratch_round(jobs * 2 * S * m * 8) + scratch_round(jobs * S * m * 8) +
+                       scratch_round(jobs * 2 * m * 8);  // gbuf, dAjob, dbar (level walkers)
     return carry + (fwd > rev ? fwd : rev);
 }
 
@@ -892,13 +893,14 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
     double* omega1 = outputs ? scratch.alloc<double>((size_t)B * m) : nullptr;
     double* pingpong = nullptr;
     double* Y = nullptr;
-    if (!carry && pl.n >= 2) pingpong = scratch.alloc<double>((size_t)jobs * 2 * S * m);
-    if (pl.n >= 2) Y = scratch.alloc<double>((size_t)jobs * S * m);
-    if ((outputs && (!partial || !omega1)) || (!carry && pl.n >= 2 && !pingpong) || (pl.n >= 2 && !Y))
-        return fail(MAGNUS_ENOMEM, "device allocation failed");
-
     magnus::LmArgs a{};
     fill_lm(pl, jobs, a);
+    const bool walk_y = magnus::walk_ok(pl.integ, a);  // the walkers form their products in registers
+    if (!carry && pl.n >= 2) pingpong = scratch.alloc<double>((size_t)jobs * 2 * S * m);
+    if (pl.n >= 2 && !walk_y) Y = scratch.alloc<double>((size_t)jobs * S * m);
+    if ((outputs && (!partial || !omega1)) || (!carry && pl.n >= 2 && !pingpong) || (pl.n >= 2 && !walk_y && !Y))
+        return fail(MAGNUS_ENOMEM, "device allocation failed");
+
     a.in = in;
     a.carry = carry;
     a.pingpong = pingpong;
@@ -906,9 +908,13 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
     a.partial = partial;
     a.omega1 = omega1;
     a.outputs = outputs ? 1 : 0;
-    for (int j = 1; j <= (int)pl.n; ++j) {
+    const bool walk = magnus::walk_ok(pl.integ, a);  // fused level walkers (d = 16, 32, Boole)
+    // without outputs the last level (sums only) has nothing to write
+    const int top = outputs ? (int)pl.n : (int)pl.n - 1;
+    for (int j = 1; j <= top; ++j) {
         a.level = j;
-        hipError_t e = magnus::launch_lm_forward_level(pl.integ, a, stream);
+        hipError_t e = walk ? magnus::launch_walk_forward_level(a, stream)
+                            : magnus::launch_lm_forward_level(pl.integ, a, stream);
         if (e != hipSuccess) return device_error(e, "forward kernel launch");
     }
     if (outputs) {
@@ -928,6 +934,40 @@ int run_forward(const Plan& pl, uint64_t count, const double* in, double* out, d
         if (e != hipSuccess) return device_error(e, "reduce kernel launch");
     }
     g_kernel = "lm";
+    return MAGNUS_OK;
+}
+
+// VJP from the carry on the level walkers: levels n..2, then the reduce that
+// adds each job's last adjoint (with its deferred total cotangent) and the nodes
+int run_walk_vjp(const Plan& pl, long long B, magnus::LmArgs& a, const double* cot, double* dA,
+                 DeviceScratch& scratch, hipStream_t stream) {
+    a.dbar = scratch.alloc<double>((size_t)a.jobs * 2 * pl.m);
+    if (!a.dbar) return fail(MAGNUS_ENOMEM, "device allocation failed");
+    hipError_t e = hipSuccess;
+    for (int k = (int)pl.n; k >= 2 && e == hipSuccess; --k) {
+        a.level = k;
+        e = magnus::launch_walk_vjp_level(a, stream);
+        a.ga ^= 1;
+    }
+    if (e != hipSuccess) return device_error(e, "vjp kernel launch");
+    magnus::VjpReduceArgs r{};
+    r.dAjob = a.dAjob;
+    r.cot = cot;
+    r.dA = dA;
+    r.B = B;
+    r.S = (long long)pl.S;
+    r.n = (int)pl.n;
+    r.m = (int)pl.m;
+    r.m_out = (int)pl.m;
+    r.m_cot = (int)pl.m_cot;
+    r.n_out = (int)pl.n_out;
+    r.Qj = (int)pl.Q;
+    r.op = pl.op;
+    r.sc = 0;
+    r.dt = pl.dt;
+    e = magnus::launch_walk_vjp_reduce(a, r, stream);
+    if (e != hipSuccess) return device_error(e, "vjp reduce launch");
+    g_kernel = "lm";  // the per-node level recursion (here on its fused walkers)
     return MAGNUS_OK;
 }
 
@@ -958,6 +998,7 @@ int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot,
         a.gbuf = gbuf;
         a.dAjob = dAjob;
         a.ga = 0;
+        if (magnus::walk_ok(pl.integ, a)) return run_walk_vjp(pl, B, a, cot, dA, scratch, stream);
         hipError_t e = magnus::launch_lm_vjp_init(pl.integ, a, stream);
         for (int k = (int)pl.n; k >= 2 && e == hipSuccess; --k) {
             a.level = k;

@@ -42,6 +42,7 @@ struct LmArgs {
     double* totals;      // [jobs][n][m] last prefix value of each level, or nullptr (forward)
     const double* wbar;  // [jobs][n][m] cotangents of the level totals, or nullptr (VJP)
     double* cscan;       // [3][nchunks][jobs*m] chunk scratch of the sample-parallel scans, or nullptr
+    double* dbar;        // [jobs][2][m] deferred cotangents of the level totals (level walkers' VJP)
     int clen;            // samples per chunk of the sample-parallel scan (multiple of 4)
     long long S;
     long long jobs;
@@ -74,6 +75,22 @@ hipError_t launch_lm_forward_level(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_init(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_level(int integ, const LmArgs& a, hipStream_t s);
 hipError_t launch_lm_vjp_final(const LmArgs& a, hipStream_t s);
+
+// Level walkers of the per-node recursion (kernels_walk.hip) for d in {16,
+// 32} under the Boole rule: one workgroup per job walks one level's samples
+// once, products on the matrix cores fed straight into the prefix recurrence.
+// walk_ok: the problem and LmArgs they apply to (generic matrices, per-node
+// jobs, default product path).  The forward level replaces
+// launch_lm_forward_level (same outputs).  The VJP runs launch_walk_vjp_level
+// for k = n..2 (no init or final launch; ga alternates as for the level
+// kernels) and then launch_walk_vjp_reduce, which replaces launch_vjp_reduce:
+// a level leaves the next adjoint in gbuf without the total's cotangent
+// (grad.hpp:305-313), which is deferred in dbar and added where the adjoint is
+// read.
+bool walk_ok(int integ, const LmArgs& a);
+hipError_t launch_walk_forward_level(const LmArgs& a, hipStream_t s);
+hipError_t launch_walk_vjp_level(const LmArgs& a, hipStream_t s);
+hipError_t launch_walk_vjp_reduce(const LmArgs& a, const VjpReduceArgs& r, hipStream_t s);
 
 // Single-pass basis sweep of the dense W-basis path at d = 8 under the Boole
 // rule (kernels_wbd8.hip): one wavefront per two instances walks the samples

@@ -97,6 +97,50 @@ struct FwdOps {
     double p[WalkShape<D>::KS];  // P^(j-1)_i column operand
 };
 
+// Operand buffers in flight: the loads of samples i + 1 .. i + NB - 1 are issued
+// while sample i is formed.  Register-bound at d = 32 with products (a
+// tighter register budget for more resident workgroups spilled and ran
+// slower); level 1 holds four values per sample.
+template <int D, bool RAW>
+constexpr int fwd_buffers() { return RAW ? 4 : (D == 16 ? 4 : 2); }
+
+// samples 0 .. S-1 in order, NB - 1 samples' loads ahead of the one formed
+// (the unrolled rotation keeps every buffer in named registers)
+template <int NB, typename Ops, typename Load, typename Step>
+__device__ __forceinline__ void walk_up(long long S, Ops* o, Load load, Step step) {
+#pragma unroll
+    for (int p = 0; p < NB - 1; ++p)
+        if (p < S) load(p, o[p]);
+    for (long long i0 = 0; i0 < S; i0 += NB) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const long long i = i0 + u;
+            if (i < S) {
+                if (i + NB - 1 < S) load(i + NB - 1, o[(u + NB - 1) % NB]);
+                step(i, o[u]);
+            }
+        }
+    }
+}
+
+// samples S-1 .. 0, NB - 1 samples' loads ahead
+template <int NB, typename Ops, typename Load, typename Step>
+__device__ __forceinline__ void walk_down(long long S, Ops* o, Load load, Step step) {
+#pragma unroll
+    for (int p = 0; p < NB - 1; ++p)
+        if (S - 1 - p >= 0) load(S - 1 - p, o[p]);
+    for (long long i0 = S - 1; i0 >= 0; i0 -= NB) {
+#pragma unroll
+        for (int u = 0; u < NB; ++u) {
+            const long long i = i0 - u;
+            if (i >= 0) {
+                if (i - (NB - 1) >= 0) load(i - (NB - 1), o[(u + NB - 1) % NB]);
+                step(i, o[u]);
+            }
+        }
+    }
+}
+
 template <int D, bool RAW>
 __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_fwd(LmArgs a) {
     using W = WalkShape<D>;
@@ -166,17 +210,8 @@ __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_fwd(LmArgs a) {
             }
         }
     };
-    // two operand buffers: the loads of sample i + 1 are in flight while sample i is formed
-    FwdOps<D> o0, o1;
-    load(0, o0);
-    for (long long i = 0; i < S; i += 2) {
-        if (i + 1 < S) load(i + 1, o1);
-        step(i, o0);
-        if (i + 1 < S) {
-            if (i + 2 < S) load(i + 2, o0);
-            step(i + 1, o1);
-        }
-    }
+    FwdOps<D> o[fwd_buffers<D, RAW>()];
+    walk_up<fwd_buffers<D, RAW>()>(S, o, load, step);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         if (a.totals && Pout) a.totals[(job * n + (j - 1)) * M + e[r]] = last[r];
@@ -334,16 +369,8 @@ __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_rev(LmArgs a) {
             }
         }
     };
-    RevOps<D> o0, o1;
-    load(S - 1, o0);
-    for (long long i = S - 1; i >= 0; i -= 2) {
-        if (i >= 1) load(i - 1, o1);
-        step(i, o0);
-        if (i >= 1) {
-            if (i >= 2) load(i - 2, o0);
-            step(i - 1, o1);
-        }
-    }
+    RevOps<D> o[D == 16 ? 3 : 2];
+    walk_down<D == 16 ? 3 : 2>(S, o, load, step);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         Uo[e[r]] = fin(pend[r], 0, r);
@@ -405,7 +432,7 @@ hipError_t launch_walk_vjp_level(const LmArgs& a, hipStream_t s) {
     const double per = (double)a.S * a.m;
     const bool top = a.level == a.n;
     // reads A, P^(k-1) (and barY, dA_job below the top level); writes dA_job and barY^(k-1)
-    const double bytes = 8.0 * per * a.jobs * (top ? 4 : 7);
+    const double bytes = 8.0 * per * a.jobs * (top ? 4 : 6);
     const double flops = (double)a.jobs * a.S * (4.0 * a.d * a.d * a.d + 16.0 * a.m);
     void* h = timer_begin("walk_rev", s, bytes, flops);
     const dim3 grid((unsigned)a.jobs);

@@ -334,10 +334,21 @@ __global__ void __launch_bounds__(256) wbd_fold_vjp_reduce(WbdArgs a) {
 constexpr int kSmallD = 8;
 constexpr int kSmallM = kSmallD * kSmallD;
 
-// (X' Y')[r][c] summed over k ascending from 0.0, X' = X or X^T, Y' = Y or Y^T
-__device__ __forceinline__ double sdot(const double* X, bool tx, const double* Y, bool ty, int d, int r, int c) {
+// (X' Y')[r][c] summed over k ascending from 0.0, X' = X or X^T, Y' = Y or Y^T;
+// unrolled to kSmallD steps so that all operand reads are issued before the
+// first FMA waits on one (a loop over the runtime d waited on LDS every step)
+template <bool TX, bool TY>
+__device__ __forceinline__ double sdot(const double* X, const double* Y, int d, int r, int c) {
+    double x[kSmallD], y[kSmallD];
+#pragma unroll
+    for (int k = 0; k < kSmallD; ++k) {
+        x[k] = k < d ? (TX ? X[k * d + r] : X[r * d + k]) : 0.0;
+        y[k] = k < d ? (TY ? Y[c * d + k] : Y[k * d + c]) : 0.0;
+    }
     double acc = 0.0;
-    for (int k = 0; k < d; ++k) acc = fma(tx ? X[k * d + r] : X[r * d + k], ty ? Y[c * d + k] : Y[k * d + c], acc);
+#pragma unroll
+    for (int k = 0; k < kSmallD; ++k)
+        if (k < d) acc = fma(x[k], y[k], acc);
     return acc;
 }
 
@@ -363,7 +374,7 @@ __device__ void small_totals(SmallFold<NM>& L, const WbdArgs& a, long long b, in
     for (int j = 1; j <= K; ++j) {
         if (live) {
             double v = 1.0 * L.wh[j - 1][e];
-            for (int l = 1; l < j; ++l) v = fma(s, sdot(L.wh[l - 1], false, L.t[j - l], false, d, r, c), v);
+            for (int l = 1; l < j; ++l) v = fma(s, sdot<false, false>(L.wh[l - 1], L.t[j - l], d, r, c), v);
             L.t[j][e] = v;
         }
         __syncthreads();
@@ -378,7 +389,7 @@ __device__ void small_omegas(SmallFold<NM>& L, const WbdArgs& a, int n, double s
     const long long M = a.m;
     for (int j = 2; j <= n; ++j) {
         double v = wq * L.z[j - 1][e];
-        for (int l = 0; l <= j - 2; ++l) v = fma(wq * s, sdot(L.z[l], false, L.t[j - 1 - l], false, d, r, c), v);
+        for (int l = 0; l <= j - 2; ++l) v = fma(wq * s, sdot<false, false>(L.z[l], L.t[j - 1 - l], d, r, c), v);
         Om[(j - 1) * M + e] = v;
     }
 }
@@ -442,8 +453,8 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
             const double* Ob = L.ob[j - 1];
             L.zb[j - 1][e] = fma(wq, Ob[e], L.zb[j - 1][e]);
             for (int l = 0; l <= j - 2; ++l) {
-                L.zb[l][e] = fma(wq * s, sdot(Ob, false, T[j - 1 - l], true, d, r, c), L.zb[l][e]);
-                L.tb[j - 1 - l][e] = fma(wq * s, sdot(Z[l], true, Ob, false, d, r, c), L.tb[j - 1 - l][e]);
+                L.zb[l][e] = fma(wq * s, sdot<false, true>(Ob, T[j - 1 - l], d, r, c), L.zb[l][e]);
+                L.tb[j - 1 - l][e] = fma(wq * s, sdot<true, false>(Z[l], Ob, d, r, c), L.tb[j - 1 - l][e]);
             }
         }
     }
@@ -452,8 +463,8 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
         if (live) {
             L.wb[j - 1][e] = fma(1.0, L.tb[j][e], L.wb[j - 1][e]);
             for (int l = 1; l < j; ++l) {
-                L.wb[l - 1][e] = fma(s, sdot(L.tb[j], false, T[j - l], true, d, r, c), L.wb[l - 1][e]);
-                L.tb[j - l][e] = fma(s, sdot(L.f.wh[l - 1], true, L.tb[j], false, d, r, c), L.tb[j - l][e]);
+                L.wb[l - 1][e] = fma(s, sdot<false, true>(L.tb[j], T[j - l], d, r, c), L.wb[l - 1][e]);
+                L.tb[j - l][e] = fma(s, sdot<true, false>(L.f.wh[l - 1], L.tb[j], d, r, c), L.tb[j - l][e]);
             }
         }
         __syncthreads();

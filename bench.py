@@ -208,6 +208,50 @@ def _cpu_baseline_sc(a_host, cot_host, n, S, budget_s):
     }
 
 
+def _cpu_baseline_dense(a_host, cot_host, n, S, budget_s):
+    """Oracle on host cores for the dense record + VJP step (forward recording the
+    reference carry, VJP from it): the faster of (1) instances in sequence with the
+    reference Auto GL-lane rule (VJP serial) and (2) one instance per core, each
+    mode for about budget_s / 2 seconds on the batch's first instances."""
+    import numpy as np
+
+    sys.path.insert(0, str(ROOT / "oracle"))
+    import oracle as O
+
+    cores = len(os.sched_getaffinity(0))
+    Q, K = (n + 3) // 2, n - 1
+    d = a_host.shape[-1]
+    results = {}
+    for mode in ("reference-faithful", "all-cores"):
+        done, t0 = 0, time.perf_counter()
+        chunk = 1 if mode == "reference-faithful" else min(cores, a_host.shape[0])
+        pool = a_host.shape[0] - a_host.shape[0] % chunk
+        while time.perf_counter() - t0 < budget_s / 2:
+            lo = done % pool
+            a = np.ascontiguousarray(a_host[lo:lo + chunk])
+            c = np.ascontiguousarray(cot_host[lo:lo + chunk])
+            carry = np.zeros((chunk, Q, K, S, d, d))
+            lanes, threads = (O.auto_lanes(n, S, cores), 1) if mode == "reference-faithful" else (1, cores)
+            O.forward_batch(n, a, 0.0, 1.0, op="sum", integrator="Boole", lanes=lanes, threads=threads, carry=carry)
+            O.vjp_batch(n, a, c, 0.0, 1.0, op="sum", integrator="Boole", threads=threads, carry=carry)
+            done += chunk
+        el = time.perf_counter() - t0
+        results[mode] = (done / el if el > 0 else 0.0, done, el,
+                         O.auto_lanes(n, S, cores) if mode == "reference-faithful" else cores)
+    best = max(results, key=lambda k: results[k][0])
+    v, done, el, used = results[best]
+    return {
+        "value": v,
+        "unit": "solves/s",
+        "cores": used,
+        "kind": "port",
+        "sample": f"{done} forward (recording the carry) + VJP (from it) solves at full size (S={S}, d={d}) cycling "
+                  f"through the batch's first {a_host.shape[0]} instances in {el:.1f} s, mode={best}; other mode "
+                  f"{[(k, round(r[0], 3), r[3]) for k, r in results.items() if k != best]} (mode, solves/s, threads); "
+                  f"host cores {cores}",
+    }
+
+
 UNEVEN_DIMS = (2, 4, 8, 16, 32, 64)
 UNEVEN_SAMPLES = (257, 1025, 4097, 8193)
 
@@ -885,6 +929,9 @@ def main():
     if not args.no_cpu and wl == "spacecurve-batch":
         k = min(B, 1024)
         cpu = _cpu_baseline_sc(a[:k].cpu().numpy(), cot[:k].cpu().numpy(), n, S, args.cpu_budget)
+    elif not args.no_cpu and wl == "dense-d32":
+        k = min(B, 32)
+        cpu = _cpu_baseline_dense(a[:k].cpu().numpy(), cot[:k].cpu().numpy(), n, S, args.cpu_budget)
 
     value = B / (ms_step / 1e3)
     line = {

@@ -871,6 +871,38 @@ def test_dense_wbasis_large_norm(scale):
 
 
 
+def test_config_dense_d32_central_differences_full_length():
+    """The dense configuration's gradient check at its own shape: one instance of
+    d = 32, S = 4097, 5 Fourier modes, Boole, n = 4, op sum; the VJP from the
+    recorded carry (the level walkers) against central differences with h = 1e-6
+    on 64 random sample entries (perturbed copies built on the device, their
+    forwards in one batched call)."""
+    import torch
+
+    from magnus import torch_ops
+
+    S, d, n = 4097, 32, 4
+    A = fourier_matrices(np.random.default_rng([3, 0]), S, d, modes=5)
+    cot = np.random.default_rng([3, 0, 1]).standard_normal((d, d))
+    At = torch.from_numpy(A[None]).cuda()
+    ct = torch.from_numpy(cot[None]).cuda()
+    _, carry = torch_ops.forward(n, At, 0.0, 1.0, op="sum", integrator="Boole", record_vjp=True)
+    g = torch_ops.vjp(n, At, ct, carry, 0.0, 1.0, op="sum", integrator="Boole")[0].cpu().numpy()
+    del carry
+    rng = np.random.default_rng([3, 0, 2])
+    picks = [(int(rng.integers(S)), int(rng.integers(d)), int(rng.integers(d))) for _ in range(64)]
+    h = 1e-6
+    Ap = At.repeat(2 * len(picks), 1, 1, 1)
+    for k, (i, r, c) in enumerate(picks):
+        Ap[2 * k, i, r, c] += h
+        Ap[2 * k + 1, i, r, c] -= h
+    outs = torch_ops.forward(n, Ap, 0.0, 1.0, op="sum", integrator="Boole").cpu().numpy()
+    del Ap
+    fd = np.array([((outs[2 * k] - outs[2 * k + 1]) * cot).sum() / (2 * h) for k in range(len(picks))])
+    an = np.array([g[i, r, c] for i, r, c in picks])
+    assert np.max(np.abs(an - fd)) <= 1e-6 * max(1.0, np.max(np.abs(fd))), np.max(np.abs(an - fd))
+
+
 def test_dense_d32_vjp_central_differences():
     """Dense MFMA path (d=32, Boole, n=4, op sum): the VJP from the recorded
     carry and the carry-free (W-basis) VJP against central differences with

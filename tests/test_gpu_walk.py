@@ -62,7 +62,7 @@ def test_walkers_match_oracle(d, op, n, S, B):
     assert capi.timing_read("lm_fwd_prod")[0] == 0 and capi.timing_read("lm_vjp_prod")[0] == 0
     by, fl = capi.timing_counts("walk_rev")
     per = 8 * S * d * d * B * Q
-    assert by == per * (4 + 7 * (n - 2))
+    assert by == per * (4 + 6 * (n - 2))
     out, carry, dA = (x.cpu().numpy() for x in (out, carry, dA))
     for b in range(B):
         ref, ref_carry = O.forward(n, A[b], 0.0, 1.0, op=op, integrator="Boole", record_vjp=True)

@@ -182,6 +182,26 @@ int magnus_forward_basis_f64(const magnus_problem* p, uint64_t count, const doub
 int magnus_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot,
                          const double* state, double* dA, void* stream);
 
+/* ---- dense VJP with time-segment checkpoints (no counterpart in the
+ *      reference; north-star "checkpointed prefix carries in place of full
+ *      vjp_data when memory requires it").  The VJP of magnus_vjp_f64 with
+ *      carry == NULL (VJP::none, src/grad.hpp:95-110 with
+ *      src/composer/backends.hpp:222-251) for dense problems of dim 16 or 32
+ *      under the Boole rule with op "many" or "sum" and n >= 2 (EINVAL
+ *      otherwise), keeping the basis levels for one segment of samples at a
+ *      time: the first pass keeps each level's prefix state at every segment
+ *      start, the reverse rebuilds one segment's levels from it and walks them
+ *      back.  segments = 0 picks the segment length that balances the segment
+ *      buffers against the checkpoints (about 4-6 % of the n - 1 full levels at
+ *      S = 4097 and 8193); any count of segments gives the same dA bit for bit.
+ *      magnus_vjp_segmented_scratch_bytes: the device bytes one call allocates
+ *      (0 where the path does not apply); magnus_vjp_segment_length: the
+ *      segment length in samples that a segment count gives. ---- */
+uint64_t magnus_vjp_segmented_scratch_bytes(const magnus_problem* p, uint64_t count, uint64_t segments);
+uint64_t magnus_vjp_segment_length(const magnus_problem* p, uint64_t segments);
+int magnus_vjp_segmented_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot, double* dA,
+                             uint64_t segments, void* stream);
+
 /* ---- SpaceCurve VJP that allocates nothing.  No counterpart in the reference:
  *      for callers that cannot allocate device memory (code under HIP stream
  *      capture, tight serving loops).  Same arguments, kernels and results as

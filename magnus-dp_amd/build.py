@@ -37,7 +37,7 @@ COMMON = [
 
 DEVICE_SOURCES = [
     "kernels_lm.hip", "kernels_wb.hip", "kernels_sample.hip", "kernels_expm.hip", "kernels_uneven.hip",
-    "kernels_wbd.hip", "kernels_ctrl.hip", "kernels_mlp.hip", "kernels_wbd8.hip", "noalloc.hip", "kernels_walk.hip",
+    "kernels_wbd.hip", "kernels_ctrl.hip", "kernels_mlp.hip", "kernels_wbd8.hip", "noalloc.hip", "kernels_walk.hip", "kernels_walkseg.hip",
 ]
 HOST_SOURCES = ["capi.cpp", "uneven.cpp", "mlp.cpp"]
 

@@ -14,6 +14,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+#include <cmath>
 #include <cstring>
 #include <limits>
 #include <memory>
@@ -1029,6 +1031,122 @@ int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot,
     return MAGNUS_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Dense W-basis VJP with time-segment checkpoints (kernels_walkseg.hip): the
+// levels are kept for one segment of L samples at a time instead of all S.
+// ---------------------------------------------------------------------------
+bool use_walkseg(const Plan& pl) { return !pl.sc && use_wbd(pl) && magnus::walkseg_ok((int)pl.d, pl.integ, (int)pl.n); }
+
+// Segment length (a multiple of 4 samples).  segments = 0: the length that
+// balances the per-instance segment buffers ((K + 3)(L + 1) level rows) against
+// the checkpoints (2 K rows per segment), L ~ sqrt(2 K (S - 1) / (K + 3)).
+uint64_t seg_length(const Plan& pl, uint64_t segments) {
+    const uint64_t span = pl.S - 1, K = pl.n - 1;
+    uint64_t L;
+    if (segments) {
+        L = (span + segments - 1) / segments;
+    } else {
+        L = (uint64_t)std::ceil(std::sqrt(2.0 * (double)K * (double)span / (double)(K + 3)));
+    }
+    L = (L + 3) / 4 * 4;
+    if (L < 4) L = 4;
+    if (L > span) L = span;
+    return L;
+}
+
+size_t seg_scratch_bytes(const Plan& pl, uint64_t B, uint64_t L) {
+    const size_t m = pl.m, n = pl.n, K = n - 1, Q = pl.Q, G = (pl.S - 1 + L - 1) / L;
+    return scratch_round(B * K * (L + 1) * m * 8) + scratch_round(B * G * K * 2 * m * 8) +
+           2 * scratch_round(B * n * m * 8) + scratch_round(B * m * 8) + scratch_round(B * 2 * (L + 1) * m * 8) +
+           scratch_round(B * (L + 1) * m * 8) + scratch_round(B * K * 2 * m * 8) + 2 * scratch_round(B * n * m * 8) +
+           2 * scratch_round(B * Q * n * m * 8) + scratch_round(B * Q * (2 * K + 1) * m * 8);
+}
+
+int run_wbd_vjp_seg(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA,
+                    uint64_t segments, hipStream_t stream) {
+    DeviceScratch scratch(stream, nullptr);
+    const long long B = (long long)count;
+    const uint64_t S = pl.S, m = pl.m, n = pl.n, K = n - 1, Q = pl.Q;
+    const uint64_t L = seg_length(pl, segments), G = (S - 1 + L - 1) / L;
+    magnus::WalkSegArgs a{};
+    a.in = in;
+    a.wseg = scratch.alloc<double>(B * K * (L + 1) * m);
+    a.ckp = scratch.alloc<double>(B * G * K * 2 * m);
+    a.sums = scratch.alloc<double>(B * n * m);
+    a.z0 = scratch.alloc<double>(B * m);
+    a.totals = scratch.alloc<double>(B * n * m);
+    a.useg = scratch.alloc<double>(B * 2 * (L + 1) * m);
+    a.dAseg = scratch.alloc<double>(B * (L + 1) * m);
+    a.rstate = scratch.alloc<double>(B * K * 2 * m);
+    double* zbar = scratch.alloc<double>(B * n * m);
+    double* wbar = scratch.alloc<double>(B * n * m);
+    double* part = scratch.alloc<double>(B * Q * n * m);
+    double* part2 = scratch.alloc<double>(B * Q * n * m);
+    double* work = scratch.alloc<double>(B * Q * (2 * K + 1) * m);
+    if (!a.wseg || !a.ckp || !a.sums || !a.z0 || !a.totals || !a.useg || !a.dAseg || !a.rstate || !zbar || !wbar ||
+        !part || !part2 || !work)
+        return fail(MAGNUS_ENOMEM, "device allocation failed");
+    a.zbar = zbar;
+    a.wbar = wbar;
+    a.dA = dA;
+    a.B = B;
+    a.S = (long long)S;
+    a.L = (int)L;
+    a.G = (int)G;
+    a.n = (int)n;
+    a.d = (int)pl.d;
+    a.m = (int)m;
+    a.dt = pl.dt;
+    auto seg = [&](uint64_t g) {
+        a.g = (int)g;
+        a.s0 = (long long)(g * L);
+        a.s1 = (long long)std::min<uint64_t>((g + 1) * L, S - 1);
+    };
+    hipError_t e = hipSuccess;
+    // first pass: the segments in order, every level; checkpoints, sums, totals
+    a.mode = 0;
+    for (uint64_t g = 0; g < G && e == hipSuccess; ++g) {
+        seg(g);
+        for (uint64_t j = 1; j <= n && e == hipSuccess; ++j) {
+            a.level = (int)j;
+            e = magnus::launch_walkseg_forward(a, stream);
+        }
+    }
+    if (e != hipSuccess) return device_error(e, "segment walker launch");
+    magnus::WbdArgs f{};
+    fill_wbd_fold(pl, B, f);
+    f.z = a.sums;
+    f.z0 = a.z0;
+    f.totals = a.totals;
+    f.cot = cot;
+    f.zbar = zbar;
+    f.wbar = wbar;
+    f.part = part;
+    f.part2 = part2;
+    f.work = work;
+    e = magnus::launch_wbd_fold_vjp(f, stream);
+    if (e != hipSuccess) return device_error(e, "fold kernel launch");
+    // reverse: the segments from the last down; rebuild the levels, then the reverse levels
+    for (uint64_t gg = G; gg-- > 0 && e == hipSuccess;) {
+        seg(gg);
+        a.mode = 1;
+        for (uint64_t j = 1; j <= K && e == hipSuccess; ++j) {
+            a.level = (int)j;
+            e = magnus::launch_walkseg_forward(a, stream);
+        }
+        a.ga = 0;
+        for (uint64_t k = n; k >= 2 && e == hipSuccess; --k) {
+            a.level = (int)k;
+            e = magnus::launch_walkseg_reverse(a, stream);
+            a.ga ^= 1;
+        }
+        if (e == hipSuccess) e = magnus::launch_walkseg_reduce(a, stream);
+    }
+    if (e != hipSuccess) return device_error(e, "segment walker launch");
+    g_kernel = "walkseg";
+    return MAGNUS_OK;
+}
+
 uint64_t carry_elems(const magnus_problem* p, uint64_t dim) {
     if (!p || p->n < 2) return 0;
     return ((p->n + 3) / 2) * (p->n - 1) * p->samples * dim * dim;
@@ -1185,6 +1303,30 @@ int magnus_forward_basis_f64(const magnus_problem* p, uint64_t count, const doub
     if (count == 0) return MAGNUS_OK;
     if (!in || !out || !state) return fail(MAGNUS_EINVAL, "null argument");
     return run_wbd_forward(pl, count, in, out, (hipStream_t)stream, nullptr, state);
+}
+
+uint64_t magnus_vjp_segmented_scratch_bytes(const magnus_problem* p, uint64_t count, uint64_t segments) {
+    Plan pl;
+    if (make_plan(p, false, false, &pl) || !use_walkseg(pl)) return 0;
+    return seg_scratch_bytes(pl, count, seg_length(pl, segments));
+}
+
+uint64_t magnus_vjp_segment_length(const magnus_problem* p, uint64_t segments) {
+    Plan pl;
+    if (make_plan(p, false, false, &pl) || !use_walkseg(pl)) return 0;
+    return seg_length(pl, segments);
+}
+
+int magnus_vjp_segmented_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot, double* dA,
+                             uint64_t segments, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, false, &pl)) return rc;
+    if (!use_walkseg(pl))
+        return fail(MAGNUS_EINVAL, "segmented VJP needs a dense W-basis problem (op many or sum, n >= 2) with dim 16 "
+                                   "or 32 under the Boole rule");
+    if (count == 0) return MAGNUS_OK;
+    if (!in || !cot || !dA) return fail(MAGNUS_EINVAL, "null argument");
+    return run_wbd_vjp_seg(pl, count, in, cot, dA, segments, (hipStream_t)stream);
 }
 
 int magnus_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot,

@@ -92,6 +92,38 @@ hipError_t launch_walk_forward_level(const LmArgs& a, hipStream_t s);
 hipError_t launch_walk_vjp_level(const LmArgs& a, hipStream_t s);
 hipError_t launch_walk_vjp_reduce(const LmArgs& a, const VjpReduceArgs& r, hipStream_t s);
 
+// Segment walkers of the dense W-basis VJP with time-segment checkpoints
+// (kernels_walkseg.hip), d in {16, 32}, Boole: the samples are cut into G
+// segments [s_g, s_{g+1}] of block-aligned length; a workgroup per instance
+// walks one level of one segment.  The forward keeps, per segment start and
+// level, the prefix recurrence's state (P, f) and the running level sums, so
+// the reverse rebuilds one segment's levels at a time from its checkpoint and
+// walks them back with the reverse recurrence's state (g4, pending) handed
+// from segment to segment.  Same operations, in the same order, as one walk
+// over all samples, so the result does not depend on G.
+struct WalkSegArgs {
+    const double* in;      // [B][S][m]
+    double* wseg;          // [B][K][L + 1][m]: W_1..W_K at samples s_g + t, t = 0..L
+    double* ckp;           // [B][G][K][2][m]: (P, f) of levels 1..K at s_g (g >= 1)
+    double* sums;          // [B][n][m]: running level sums Z_0..Z_{n-1}
+    double* z0;            // [B][m]: Z_0 (the level-1 sum) once the last segment is done
+    double* totals;        // [B][n][m]: rows 0..K-1 = W_j[S - 1]
+    double* useg;          // [B][2][L + 1][m]: adjoint halves on the segment
+    double* dAseg;         // [B][L + 1][m]
+    double* rstate;        // [B][K][2][m]: (g4, pending) of reverse level k at row k - 2
+    const double* zbar;    // [B][n][m]
+    const double* wbar;    // [B][n][m]
+    double* dA;            // [B][S][m]
+    long long B, S;
+    long long s0, s1;      // the segment's first and last sample
+    int L, G, g, n, d, m, level, ga, mode;  // mode (forward): 0 first pass (sums, checkpoints, totals), 1 rebuild
+    double dt;
+};
+bool walkseg_ok(int d, int integ, int n);
+hipError_t launch_walkseg_forward(const WalkSegArgs& a, hipStream_t s);
+hipError_t launch_walkseg_reverse(const WalkSegArgs& a, hipStream_t s);
+hipError_t launch_walkseg_reduce(const WalkSegArgs& a, hipStream_t s);
+
 // Single-pass basis sweep of the dense W-basis path at d = 8 under the Boole
 // rule (kernels_wbd8.hip): one wavefront per two instances walks the samples
 // once and writes the sums Z_0 (omega1), Z_1..Z_K (partial rows 1..K, row

@@ -62,6 +62,11 @@ def lib() -> ctypes.CDLL:
         L.magnus_basis_state_elems.restype = u64
         L.magnus_forward_basis_f64.argtypes = [P, u64, vp, vp, vp, vp]
         L.magnus_vjp_basis_f64.argtypes = [P, u64, vp, vp, vp, vp, vp]
+        L.magnus_vjp_segmented_f64.argtypes = [P, u64, vp, vp, vp, u64, vp]
+        L.magnus_vjp_segmented_scratch_bytes.argtypes = [P, u64, u64]
+        L.magnus_vjp_segmented_scratch_bytes.restype = u64
+        L.magnus_vjp_segment_length.argtypes = [P, u64]
+        L.magnus_vjp_segment_length.restype = u64
         L.magnus_sc_checkpoint_elems.argtypes = [P, u64]
         L.magnus_sc_checkpoint_elems.restype = u64
         for name in ("magnus_carry_elems", "magnus_output_elems", "magnus_sc_carry_elems", "magnus_sc_output_elems"):

@@ -160,6 +160,32 @@ def vjp_basis(n: int, A: torch.Tensor, cot: torch.Tensor, state: torch.Tensor, t
     return dA
 
 
+def vjp_segmented(n: int, A: torch.Tensor, cot: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
+                  matrix_backend: str = "Auto", integrator: str = "Auto", segments: int = 0):
+    """vjp() without a carry that keeps the basis levels for one time segment at
+    a time (dense dim 16 or 32, Boole, op many or sum, n >= 2); segments = 0
+    picks the segment length.  The result does not depend on the segments."""
+    _check_input(A, 4, "A")
+    A = A.contiguous()
+    B, S, d, d2 = A.shape
+    if d != d2:
+        raise ValueError("data must contain square matrices")
+    p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    cot = _check_operand(cot, A, (B, n, d, d) if op == "many" else (B, d, d), "cotangent")
+    dA = torch.empty_like(A)
+    capi.check(capi.lib().magnus_vjp_segmented_f64(ctypes.byref(p), B, _ptr(A), _ptr(cot), _ptr(dA), int(segments),
+                                                   _stream()))
+    return dA
+
+
+def vjp_segmented_scratch_bytes(n: int, B: int, S: int, d: int, t0: float = 0.0, tf: float = 1.0, *,
+                                op: str = "sum", matrix_backend: str = "Auto", integrator: str = "Auto",
+                                segments: int = 0) -> int:
+    """Device bytes vjp_segmented allocates for B instances (0 where it does not apply)."""
+    p = capi.problem(n, d, S, t0, tf, op, matrix_backend, integrator)
+    return int(capi.lib().magnus_vjp_segmented_scratch_bytes(ctypes.byref(p), B, int(segments)))
+
+
 def sc_forward(n: int, a: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
                integrator: str = "Auto", record_vjp: bool = False):
     """Batched SpaceCurve Magnus: a (B, S, 3) float64 on the GPU."""

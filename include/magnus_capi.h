@@ -337,6 +337,16 @@ void magnus_timing_reset(void);
  * steady-state path).  This waits for the devices and frees every cache. */
 int magnus_release_scratch(void);
 
+/* Bound on the bytes a call may take from its stream's scratch cache (0: no
+ * bound, the default), for sharing the device with the caller's own memory.
+ * A call that would pass it fails with ENOMEM, except the carry-free dense VJP
+ * (magnus_vjp_f64 with carry == NULL), which then keeps its basis levels for
+ * one time segment at a time where magnus_vjp_segmented_f64 applies.  The
+ * bound counts the cache's cursor position, so memory the cache already holds
+ * beyond it stays unused.  No counterpart in the reference. */
+int magnus_set_scratch_limit(uint64_t bytes);
+uint64_t magnus_scratch_limit(void);
+
 /* thread-local message of the last failing call on this thread */
 const char* magnus_last_error(void);
 

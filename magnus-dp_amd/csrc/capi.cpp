@@ -43,16 +43,24 @@ int fail(int code, const std::string& msg) {
 
 void set_last_kernel(const char* name) { g_kernel = name; }
 
+std::atomic<uint64_t> g_scratch_limit{0};
+
 void* ScratchCache::bump(size_t bytes) {
-    for (; bi < blocks.size(); ++bi, off = 0) {
+    const size_t limit = (size_t)g_scratch_limit.load(std::memory_order_relaxed);
+    size_t ahead = 0;  // bytes of the blocks before the cursor's block
+    for (size_t i = 0; i < bi && i < blocks.size(); ++i) ahead += blocks[i].size;
+    for (; bi < blocks.size(); ahead += blocks[bi].size, ++bi, off = 0) {
         if (off + bytes <= blocks[bi].size) {
+            if (limit && ahead + off + bytes > limit) return nullptr;
             void* p = blocks[bi].p + off;
             off += bytes;
             return p;
         }
     }
+    if (limit && ahead + bytes > limit) return nullptr;
     size_t size = bytes < (size_t(256) << 20) ? (size_t(256) << 20) : bytes;
     if (!blocks.empty() && size < 2 * blocks.back().size) size = 2 * blocks.back().size;
+    if (limit && ahead + size > limit) size = limit - ahead;
     void* p = nullptr;
     if (hipMalloc(&p, size) != hipSuccess) {
         (void)hipGetLastError();
@@ -973,9 +981,19 @@ int run_walk_vjp(const Plan& pl, long long B, magnus::LmArgs& a, const double* c
     return MAGNUS_OK;
 }
 
+bool use_walkseg(const Plan& pl);
+int run_wbd_vjp_seg(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA,
+                    uint64_t segments, hipStream_t stream);
+
 int run_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, const double* carry,
             double* dA, hipStream_t stream, Arena* arena) {
-    if (!carry && use_wbd(pl)) return run_wbd_vjp(pl, count, in, cot, dA, stream, arena);
+    if (!carry && use_wbd(pl)) {
+        const int rc = run_wbd_vjp(pl, count, in, cot, dA, stream, arena);
+        // Where the n - 1 full basis levels do not fit, keep them for one time
+        // segment at a time (same dA up to rounding).
+        if (rc == MAGNUS_ENOMEM && !arena && use_walkseg(pl)) return run_wbd_vjp_seg(pl, count, in, cot, dA, 0, stream);
+        return rc;
+    }
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const uint64_t S = pl.S, m = pl.m;
@@ -1486,6 +1504,13 @@ void magnus_timing_reset(void) {
     timing_drain_locked();
     g_totals.clear();
 }
+
+int magnus_set_scratch_limit(uint64_t bytes) {
+    g_scratch_limit.store(bytes, std::memory_order_relaxed);
+    return MAGNUS_OK;
+}
+
+uint64_t magnus_scratch_limit(void) { return g_scratch_limit.load(std::memory_order_relaxed); }
 
 int magnus_release_scratch(void) {
     release_scratch_caches();

@@ -5,6 +5,7 @@
 
 #include <hip/hip_runtime.h>
 
+#include <atomic>
 #include <cstdint>
 #include <memory>
 #include <mutex>
@@ -66,6 +67,10 @@ struct ScratchCache {
     size_t bi = 0, off = 0;  // cursor: block index, offset
     void* bump(size_t bytes);
 };
+// Bound on a cache's cursor position (bytes of the blocks ahead of the
+// cursor's block plus its offset); 0: none.  An allocation that would pass it
+// fails as if the device were out of memory.
+extern std::atomic<uint64_t> g_scratch_limit;
 ScratchCache* scratch_cache(hipStream_t s);
 void release_scratch_caches();
 

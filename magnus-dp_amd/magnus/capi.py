@@ -114,6 +114,9 @@ def lib() -> ctypes.CDLL:
         L.magnus_mlp_forward.argtypes = [vp, u64, vp, vp, vp]
         L.magnus_mlp_train_step.argtypes = [vp, u64, vp, vp, ctypes.c_float, vp, vp]
         L.magnus_release_scratch.argtypes = []
+        L.magnus_set_scratch_limit.argtypes = [u64]
+        L.magnus_scratch_limit.argtypes = []
+        L.magnus_scratch_limit.restype = u64
         L.magnus_timing_enable.argtypes = [ctypes.c_int]
         L.magnus_timing_read.argtypes = [ctypes.c_char_p, ctypes.POINTER(u64), ctypes.POINTER(ctypes.c_double)]
         L.magnus_timing_counts.argtypes = [ctypes.c_char_p, ctypes.POINTER(ctypes.c_double),
@@ -186,3 +189,12 @@ def timing_counts(name: str) -> tuple[float, float]:
 
 def last_kernel() -> str:
     return lib().magnus_last_kernel().decode()
+
+
+def set_scratch_limit(nbytes: int) -> None:
+    """Bound the bytes a call may take from its stream's scratch cache (0: none)."""
+    check(lib().magnus_set_scratch_limit(int(nbytes)))
+
+
+def scratch_limit() -> int:
+    return int(lib().magnus_scratch_limit())

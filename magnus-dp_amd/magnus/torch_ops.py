@@ -273,12 +273,22 @@ def sc_vjp_ckpt(n: int, a: torch.Tensor, cot: torch.Tensor, ckpt: torch.Tensor, 
     return da
 
 
+def _residual_fits(nbytes: int, device) -> bool:
+    """A residual of nbytes leaves at least as much device memory free as it
+    takes (free device memory plus torch's cached, unallocated blocks)."""
+    free, _ = torch.cuda.mem_get_info(device)
+    cached = torch.cuda.memory_reserved(device) - torch.cuda.memory_allocated(device)
+    return 2 * nbytes <= free + cached
+
+
 class MagnusFunction(torch.autograd.Function):
     """out = Magnus(A); backward = HIP reverse sweep.  record_vjp keeps the
     reference's carry as the residual; otherwise, where the dense W-basis path
-    applies (op many or sum, n >= 2), the residual is the basis state of the
-    forward's sweep (n - 1 levels instead of the carry's Q (n - 1)) and the
-    backward skips its sweep; elsewhere the backward recomputes (VJP::none)."""
+    applies (op many or sum, n >= 2) and its state fits in half the free device
+    memory, the residual is the basis state of the forward's sweep (n - 1
+    levels instead of the carry's Q (n - 1)) and the backward skips its sweep;
+    elsewhere the backward recomputes (VJP::none), keeping the basis levels for
+    one time segment at a time where the full levels do not fit."""
 
     @staticmethod
     def forward(ctx, A, n, t0, tf, op, matrix_backend, integrator, record_vjp):
@@ -287,7 +297,8 @@ class MagnusFunction(torch.autograd.Function):
         if record_vjp:
             out, res = forward(n, A, t0, tf, record_vjp=True, **kw)
             mode = "carry" if n > 1 else "none"
-        elif A.is_cuda and A.dim() == 4 and basis_state_elems(n, A, t0, tf, **kw) > 0:
+        elif (A.is_cuda and A.dim() == 4 and (elems := basis_state_elems(n, A, t0, tf, **kw)) > 0
+              and _residual_fits(8 * elems, A.device)):
             out, res = forward_basis(n, A, t0, tf, **kw)
             mode = "basis"
         else:

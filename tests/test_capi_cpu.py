@@ -114,6 +114,30 @@ def test_sizes():
     assert capi.lib().magnus_carry_elems(ctypes.byref(p1)) == 0
 
 
+def test_segmented_vjp_sizes_and_scratch_limit():
+    """Segment lengths are block multiples of about sqrt(2 K (S - 1) / (K + 3))
+    samples (or (S - 1) / segments), the segmented VJP's scratch is 0 where it
+    does not apply, and the scratch bound reads back as set."""
+    L = capi.lib()
+    p = capi.problem(4, 16, 65, 0.0, 1.0, op="sum", integrator="Boole")
+    assert L.magnus_vjp_segment_length(ctypes.byref(p), 0) == 8
+    assert L.magnus_vjp_segment_length(ctypes.byref(p), 2) == 32
+    assert L.magnus_vjp_segment_length(ctypes.byref(p), 3) == 24
+    assert L.magnus_vjp_segment_length(ctypes.byref(p), 1000) == 4
+    assert L.magnus_vjp_segmented_scratch_bytes(ctypes.byref(p), 2, 0) > 0
+    for kw in (dict(d=8, op="sum", integ="Boole"), dict(d=16, op="one", integ="Boole"),
+               dict(d=16, op="sum", integ="Simpson"), dict(d=64, op="sum", integ="Boole")):
+        q = capi.problem(4, kw["d"], 65, 0.0, 1.0, op=kw["op"], integrator=kw["integ"])
+        assert L.magnus_vjp_segmented_scratch_bytes(ctypes.byref(q), 2, 0) == 0, kw
+    assert capi.scratch_limit() == 0
+    try:
+        capi.set_scratch_limit(1 << 20)
+        assert capi.scratch_limit() == 1 << 20
+    finally:
+        capi.set_scratch_limit(0)
+    assert capi.scratch_limit() == 0
+
+
 def test_sampling_argument_checks_before_device_work():
     L = capi.lib()
     dummy = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation

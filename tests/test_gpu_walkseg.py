@@ -84,6 +84,38 @@ def test_segmented_vjp_full_length_matches_the_unsegmented_vjp(d, S, n):
     assert 0 < used <= 0.07 * levels, (used, levels)
 
 
+def test_carry_free_vjp_keeps_segments_when_the_levels_do_not_fit():
+    """With the library's scratch bounded below the n - 1 full levels, the
+    carry-free VJP falls back to the segmented one (same dA bit for bit as a
+    direct segmented call, 1e-12 of the full-level VJP)."""
+    import torch
+
+    from magnus import capi, torch_ops
+
+    d, S, n, B = 16, 257, 5, 2
+    A = np.stack([fourier_matrices(np.random.default_rng([79, b]), S, d, modes=3) for b in range(B)])
+    cot = np.stack([np.random.default_rng([79, b, 1]).standard_normal((d, d)) for b in range(B)])
+    At, ct = torch.from_numpy(A).cuda(), torch.from_numpy(cot).cuda()
+    full = torch_ops.vjp(n, At, ct, None, 0.0, 1.0, op="sum", integrator="Boole")
+    assert capi.last_kernel() == "wbd"
+    seg = torch_ops.vjp_segmented(n, At, ct, 0.0, 1.0, op="sum", integrator="Boole")
+    need = torch_ops.vjp_segmented_scratch_bytes(n, B, S, d, op="sum", integrator="Boole")
+    levels = B * (n - 1) * S * d * d * 8
+    limit = need + need // 2
+    assert limit < levels
+    torch.cuda.synchronize()
+    capi.set_scratch_limit(limit)
+    try:
+        got = torch_ops.vjp(n, At, ct, None, 0.0, 1.0, op="sum", integrator="Boole")
+        assert capi.last_kernel() == "walkseg"
+    finally:
+        capi.set_scratch_limit(0)
+    assert torch.equal(got, seg)
+    g, f = got.cpu().numpy(), full.cpu().numpy()
+    for b in range(B):
+        assert rel_fro(g[b], f[b]) <= TOL, b
+
+
 def test_segmented_vjp_rejects_other_problems():
     import torch
 

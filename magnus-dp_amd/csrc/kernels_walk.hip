@@ -45,6 +45,7 @@
 // inline (lm_vjp_init's values) and writes dA_job instead of adding to it.
 #include "kernels.hpp"
 #include "rules.hpp"
+#include "walk.hpp"
 
 namespace magnus {
 
@@ -103,43 +104,8 @@ struct FwdOps {
 // slower); level 1 holds four values per sample.
 template <int D, bool RAW>
 constexpr int fwd_buffers() { return RAW ? 4 : (D == 16 ? 4 : 2); }
-
-// samples 0 .. S-1 in order, NB - 1 samples' loads ahead of the one formed
-// (the unrolled rotation keeps every buffer in named registers)
-template <int NB, typename Ops, typename Load, typename Step>
-__device__ __forceinline__ void walk_up(long long S, Ops* o, Load load, Step step) {
-#pragma unroll
-    for (int p = 0; p < NB - 1; ++p)
-        if (p < S) load(p, o[p]);
-    for (long long i0 = 0; i0 < S; i0 += NB) {
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-            const long long i = i0 + u;
-            if (i < S) {
-                if (i + NB - 1 < S) load(i + NB - 1, o[(u + NB - 1) % NB]);
-                step(i, o[u]);
-            }
-        }
-    }
-}
-
-// samples S-1 .. 0, NB - 1 samples' loads ahead
-template <int NB, typename Ops, typename Load, typename Step>
-__device__ __forceinline__ void walk_down(long long S, Ops* o, Load load, Step step) {
-#pragma unroll
-    for (int p = 0; p < NB - 1; ++p)
-        if (S - 1 - p >= 0) load(S - 1 - p, o[p]);
-    for (long long i0 = S - 1; i0 >= 0; i0 -= NB) {
-#pragma unroll
-        for (int u = 0; u < NB; ++u) {
-            const long long i = i0 - u;
-            if (i >= 0) {
-                if (i - (NB - 1) >= 0) load(i - (NB - 1), o[(u + NB - 1) % NB]);
-                step(i, o[u]);
-            }
-        }
-    }
-}
+template <int D>
+constexpr int rev_buffers() { return D == 16 ? 3 : 2; }
 
 template <int D, bool RAW>
 __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_fwd(LmArgs a) {
@@ -184,7 +150,7 @@ __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_fwd(LmArgs a) {
     };
     PrefixState<BOOLE> st[4];
     double sum[4] = {0.0, 0.0, 0.0, 0.0}, last[4] = {0.0, 0.0, 0.0, 0.0};
-    auto step = [&](long long i, const FwdOps<D>& o) {
+    auto step = [&](long long i, const FwdOps<D>& o, auto) {
         double f[4];
         if constexpr (RAW) {
 #pragma unroll
@@ -210,8 +176,9 @@ __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_fwd(LmArgs a) {
             }
         }
     };
-    FwdOps<D> o[fwd_buffers<D, RAW>()];
-    walk_up<fwd_buffers<D, RAW>()>(S, o, load, step);
+    constexpr int NB = fwd_buffers<D, RAW>();
+    FwdOps<D> o[NB];
+    walk_range<NB, NB, true>(0, S - 1, o, load, step);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         if (a.totals && Pout) a.totals[(job * n + (j - 1)) * M + e[r]] = last[r];
@@ -308,7 +275,10 @@ __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_rev(LmArgs a) {
     // g4 and pending (prefix_vjp_boole), and Delta
     double g1[4], g2[4], g3[4], g4[4], pend[4], dl[4];
     auto fin = [&](double v, long long i, int r) { return addo ? sum_vjp_add<BOOLE>(v, i, S, osum[r], dscale) : v; };
-    auto step = [&](long long i, const RevOps<D>& o) {
+    // POS: the sample's position in its Boole block (i & 3), 4 for the last
+    // sample S - 1 (bar_total with shift 0, lm_vjp_scan_b2)
+    auto step_at = [&](long long i, const RevOps<D>& o, auto posc) {
+        constexpr int POS = decltype(posc)::value;
         // barY_i at the operand positions
         const double cy = TOP ? sum_vjp_coef<BOOLE>(i, S, dscale) : sum_vjp_coef<BOOLE>(i, S, dt);
         dbl4 dav = {0.0, 0.0, 0.0, 0.0}, tmp = {0.0, 0.0, 0.0, 0.0};
@@ -324,21 +294,23 @@ __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_rev(LmArgs a) {
             const double yb = TOP ? ycol[s] * cy : o.ucol[s] + cy * ycol[s];
             tmp = __builtin_amdgcn_mfma_f64_16x16x4f64(o.acol[s], yb, tmp, 0, 0, 0);
         }
-        const int pos = (int)(i & 3);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             dAj[i * M + e[r]] = TOP ? dav[r] : o.da[r] + dav[r];
             const double t = tmp[r];
-            if (i == S - 1) {
+            if constexpr (POS == 4) {
                 g4[r] = t;
                 pend[r] = 0.0;
                 dl[r] = t * shift;
             } else {
                 dl[r] += t * shift;
-                if (pos == 3) g3[r] = t;
-                else if (pos == 2) g2[r] = t;
-                else if (pos == 1) g1[r] = t;
-                else {  // block [i, i + 4] (boole.hpp:108-151)
+                if constexpr (POS == 3) {
+                    g3[r] = t;
+                } else if constexpr (POS == 2) {
+                    g2[r] = t;
+                } else if constexpr (POS == 1) {
+                    g1[r] = t;
+                } else {  // block [i, i + 4] (boole.hpp:108-151)
                     double a0 = t;
                     a0 += g1[r];
                     a0 += g2[r];
@@ -369,8 +341,19 @@ __global__ void __launch_bounds__(WalkShape<D>::THREADS) walk_rev(LmArgs a) {
             }
         }
     };
-    RevOps<D> o[D == 16 ? 3 : 2];
-    walk_down<D == 16 ? 3 : 2>(S, o, load, step);
+    // The last sample on its own, then S - 2 .. 0: whole Boole blocks from a
+    // block's last inner sample, so with an unroll of a multiple of 4 the
+    // unrolled position u is at block position 3 - u.
+    constexpr int NB = rev_buffers<D>(), UNROLL = NB % 4 == 0 ? NB : (NB % 2 == 0 ? 4 : 4 * NB);
+    {
+        RevOps<D> ol;
+        load(S - 1, ol);
+        step_at(S - 1, ol, std::integral_constant<int, 4>{});
+    }
+    RevOps<D> o[NB];
+    walk_range<NB, UNROLL, false>(0, S - 2, o, load, [&](long long i, const RevOps<D>& op, auto uc) {
+        step_at(i, op, std::integral_constant<int, (3 - decltype(uc)::value) & 3>{});
+    });
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         Uo[e[r]] = fin(pend[r], 0, r);

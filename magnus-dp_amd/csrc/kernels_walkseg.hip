@@ -34,6 +34,7 @@
 
 #include "kernels.hpp"
 #include "rules.hpp"
+#include "walk.hpp"
 
 namespace magnus {
 
@@ -61,23 +62,14 @@ __device__ __forceinline__ void seg_load_row(double* v, const double* p) {
     }
 }
 
-// samples lo..hi in order (up) or hi..lo (down), the next sample's loads in
-// flight while the current one is formed (two named buffers, rotated by unrolling)
-template <bool UP, typename Ops, typename Load, typename Step>
-__device__ __forceinline__ void seg_walk(long long lo, long long hi, Ops* o, Load load, Step step) {
-    if (lo > hi) return;
-    const long long n = hi - lo + 1;
-    auto at = [&](long long k) { return UP ? lo + k : hi - k; };
-    load(at(0), o[0]);
-    for (long long k0 = 0; k0 < n; k0 += 2) {
-        if (k0 + 1 < n) load(at(k0 + 1), o[1]);
-        step(at(k0), o[0]);
-        if (k0 + 1 < n) {
-            if (k0 + 2 < n) load(at(k0 + 2), o[0]);
-            step(at(k0 + 1), o[1]);
-        }
-    }
-}
+// prefetch depths: forward d = 16 16 VGPRs per sample, d = 32 32; reverse
+// d = 16 40, d = 32 72 (one workgroup per CU at the batches this path
+// serves).  The reverse's depth is a multiple of the Boole block, so that a
+// sample's position in its block is known where the step is unrolled.
+template <int D>
+constexpr int seg_fwd_buffers() { return D == 16 ? 8 : 6; }
+template <int D>
+constexpr int seg_rev_buffers() { return 4; }
 
 template <int D>
 struct SegFwdOps {
@@ -136,7 +128,7 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
             for (int s = 0; s < KS; ++s) o.w[s] = wcol[(i - s0) * M + s * D];
         }
     };
-    auto step = [&](long long i, const SegFwdOps<D>& o) {
+    auto step = [&](long long i, const SegFwdOps<D>& o, auto) {
         double f[4];
         if constexpr (RAW) {
 #pragma unroll
@@ -154,8 +146,9 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
             if (pref) Wout[(i - s0) * M + e[r]] = st[r].step(i, f[r], dt);
         }
     };
-    SegFwdOps<D> o[2];
-    seg_walk<true>(i0, s1, o, load, step);
+    constexpr int NB = seg_fwd_buffers<D>();
+    SegFwdOps<D> o[NB];
+    walk_range<NB, NB, true>(i0, s1, o, load, step);
     if (!first) return;
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
@@ -236,6 +229,11 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
             pend[r] = rs[M + e[r]];
         }
     }
+    double wb[4] = {0.0, 0.0, 0.0, 0.0};
+    if (last_seg) {
+#pragma unroll
+        for (int r = 0; r < 4; ++r) wb[r] = a.wbar[(b * n + (k - 2)) * M + e[r]];
+    }
     auto load = [&](long long i, SegRevOps<D>& o) {
         const long long t = i - s0;
         seg_load_row<KS>(o.wrow, Wl + t * M + wrow0);
@@ -249,7 +247,10 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
             for (int r = 0; r < 4; ++r) o.da[r] = dAs[t * M + e[r]];
         }
     };
-    auto step = [&](long long i, const SegRevOps<D>& o) {
+    // POS: the sample's position in its Boole block (i & 3), 4 for the last
+    // sample S - 1, where the level's total cotangent enters
+    auto step_at = [&](long long i, const SegRevOps<D>& o, auto posc) {
+        constexpr int POS = decltype(posc)::value;
         const long long t = i - s0;
         const double cy = TOP ? sum_vjp_coef<BOOLE>(i, S, dt) : 0.0;
         dbl4 dav = {0.0, 0.0, 0.0, 0.0}, tmp = {0.0, 0.0, 0.0, 0.0};
@@ -259,19 +260,18 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
 #pragma unroll
         for (int s = 0; s < KS; ++s)
             tmp = __builtin_amdgcn_mfma_f64_16x16x4f64(o.acol[s], TOP ? ycol[s] * cy : o.ucol[s], tmp, 0, 0, 0);
-        const int pos = (int)(i & 3);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
             dAs[t * M + e[r]] = TOP ? dav[r] : o.da[r] + dav[r];
             const double tv = tmp[r];
-            if (i == S - 1) {  // bar_total with shift 0 plus the total's cotangent (lm_vjp_scan_b2)
-                g4[r] = tv + a.wbar[(b * n + (k - 2)) * M + e[r]];
+            if constexpr (POS == 4) {  // bar_total with shift 0 plus the total's cotangent (lm_vjp_scan_b2)
+                g4[r] = tv + wb[r];
                 pend[r] = 0.0;
-            } else if (pos == 3) {
+            } else if constexpr (POS == 3) {
                 g3[r] = tv;
-            } else if (pos == 2) {
+            } else if constexpr (POS == 2) {
                 g2[r] = tv;
-            } else if (pos == 1) {
+            } else if constexpr (POS == 1) {
                 g1[r] = tv;
             } else {  // block [i, i + 4] (boole_rev_block)
                 double a0 = tv;
@@ -302,8 +302,22 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
             }
         }
     };
-    SegRevOps<D> o[2];
-    seg_walk<false>(s0, itop, o, load, step);
+    // every walk below starts at a block's last inner sample (i % 4 == 3) and
+    // covers whole blocks, so unrolled position u is at block position 3 - u
+    constexpr int NB = seg_rev_buffers<D>();
+    static_assert(NB % 4 == 0, "reverse prefetch depth must be a multiple of the Boole block");
+    auto step = [&](long long i, const SegRevOps<D>& o, auto uc) {
+        step_at(i, o, std::integral_constant<int, (3 - decltype(uc)::value) & 3>{});
+    };
+    long long top = itop;
+    if (last_seg) {  // the last sample first, on its own
+        SegRevOps<D> ol;
+        load(S - 1, ol);
+        step_at(S - 1, ol, std::integral_constant<int, 4>{});
+        top = S - 2;
+    }
+    SegRevOps<D> o[NB];
+    walk_range<NB, NB, false>(s0, top, o, load, step);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
         if (g == 0) {  // prefix_vjp_boole's last value: G[0] = pending

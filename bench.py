@@ -413,16 +413,28 @@ def _bench_uneven(args, torch, dev):
     # per-instance sequential launches of the one-problem entry points
     torch.cuda.synchronize()
     t = time.perf_counter()
-    seq_out = []
+    seq_out, seq_dA = [], []
     for i, p in enumerate(probs):
         seq_out.append(torch_ops.forward(p["n"], A[i][None], 0.0, 1.0, op="sum", integrator="Boole")[0])
         dA_i = torch_ops.vjp(p["n"], A[i][None], cot[i][None], None, 0.0, 1.0, op="sum", integrator="Boole")[0]
         if i < 16:
-            assert torch.equal(dA_i, dAs[i]), f"uneven dA differs from the per-instance call at {i}"
+            seq_dA.append((i, dA_i))
     torch.cuda.synchronize()
     seq_ms = (time.perf_counter() - t) * 1e3
+
+    # bucketed instances equal their per-instance calls bit for bit, pipeline
+    # instances to rounding (another summation order of the same recurrences)
+    def same(i, ref, got, what):
+        if plan.in_pipeline(i):
+            err = float(torch.linalg.norm(got - ref) / max(float(torch.linalg.norm(ref)), 1e-300))
+            assert err <= 1e-12, f"uneven {what} differs from the per-instance call at {i}: {err:.2e}"
+        else:
+            assert torch.equal(got, ref), f"uneven {what} differs from the per-instance call at {i}"
+
+    for i, dA_i in seq_dA:
+        same(i, dA_i, dAs[i], "dA")
     for i in range(count):
-        assert torch.equal(seq_out[i], outs[i]), f"uneven output differs from the per-instance call at {i}"
+        same(i, seq_out[i], outs[i], "output")
     print(f"[bench] uneven: per-instance sequential {seq_ms:.1f} ms", file=sys.stderr, flush=True)
 
     # per-kernel totals of one eager step (the branches overlap, so these add

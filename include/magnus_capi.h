@@ -244,17 +244,26 @@ int magnus_sc_expm_f64(uint64_t count, const double* in3, double* out, void* str
 
 /* ---- uneven batches (no counterpart in the reference, which solves one problem
  *      per call).  A plan holds `count` matrix problems of any dim, samples,
- *      order, op and integrator.  It groups them into buckets of equal shape,
- *      splits each bucket into sub-batches that fit its workspace, spreads the
- *      sub-batches over `branches` HIP streams (longest first), and enqueues,
- *      per sub-batch, a gather of the instances' buffers, the level-parallel
- *      kernels and a scatter of the results.  With use_graph != 0 the whole
- *      sequence is captured once into a hipGraph and each call is one graph
- *      launch.  Buffers are per-instance DEVICE pointers passed as HOST arrays
- *      of `count` pointers (layouts as above with count = 1).  Results are the
- *      same as per-instance magnus_forward_f64 / magnus_vjp_f64 calls (carry ==
- *      NULL) with the plan's product path per bucket.  Calls on one plan must
- *      not overlap (one stream). ---- */
+ *      order, op and integrator.  Dense problems of dim 16 or 32 under the
+ *      Boole rule with op "many" or "sum" and n >= 2 run as one grouped
+ *      pipeline: each is cut into segments of samples, and one launch per
+ *      pipeline step and dim runs one level of one segment of every such
+ *      instance that the step reaches (grouped walkers; the GL fold once per
+ *      (dim, order, op) group); instances that do not fit the pipeline's share
+ *      of the workspace together run in consecutive waves.  The other problems
+ *      are grouped into buckets of equal shape, each bucket split into
+ *      sub-batches that fit its workspace and spread over `branches` HIP
+ *      streams (longest first), each sub-batch a gather of the instances'
+ *      buffers, the level-parallel kernels and a scatter of the results.  With
+ *      use_graph != 0 the whole sequence is captured once into a hipGraph and
+ *      each call is one graph launch.  Buffers are per-instance DEVICE pointers
+ *      passed as HOST arrays of `count` pointers (layouts as above with
+ *      count = 1).  Bucketed instances give the results of per-instance
+ *      magnus_forward_f64 / magnus_vjp_f64 calls (carry == NULL) with the
+ *      plan's product path per bucket, bit for bit; pipeline instances agree
+ *      with them to rounding (the same recurrences, products on the matrix
+ *      cores in another summation order) and do not depend on the rest of the
+ *      batch.  Calls on one plan must not overlap (one stream). ---- */
 typedef struct magnus_uneven_plan magnus_uneven_plan;
 
 /* workspace_bytes: device scratch for all branches (0: half of the free device
@@ -284,8 +293,11 @@ int magnus_uneven_autotune(magnus_uneven_plan* plan, const double* const* in, co
 uint64_t magnus_uneven_tuning(const magnus_uneven_plan* plan, char* buf, uint64_t len);
 int magnus_uneven_set_tuning(magnus_uneven_plan* plan, const char* text);
 
-/* plan facts: 0 buckets, 1 sub-batches, 2 branches in use, 3 workspace bytes */
+/* plan facts: 0 buckets, 1 sub-batches, 2 branches in use, 3 workspace bytes,
+ * 4 instances in the grouped pipeline, 5 pipeline waves */
 uint64_t magnus_uneven_info(const magnus_uneven_plan* plan, int what);
+/* 1 if instance i runs in the grouped pipeline, 0 if in a bucket (or out of range) */
+int magnus_uneven_in_pipeline(const magnus_uneven_plan* plan, uint64_t i);
 
 /* ---- resolution controller (no counterpart in the reference).  Features of a
  *      coarse sampling and a 3-layer MLP (in -> hidden -> hidden -> out, ReLU)

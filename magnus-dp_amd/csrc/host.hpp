@@ -14,6 +14,10 @@
 
 #include "magnus_capi.h"
 
+namespace magnus {
+struct WbdArgs;
+}
+
 namespace magnus_host {
 
 struct GLTable {
@@ -135,6 +139,10 @@ bool use_wb(const Plan& pl, bool has_carry);
 size_t wb_vjp_scratch_bytes(const Plan& pl, uint64_t count);
 int run_wb_vjp_on(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA, char* scratch,
                   size_t bytes, hipStream_t stream);
+
+// The dense W-basis GL fold's arguments for `B` instances of one problem
+// (B, n, d, m, Q, op, n_out and the GL nodes; buffers left to the caller).
+void fill_wbd_fold(const Plan& pl, long long B, magnus::WbdArgs& f);
 
 // Peak arena bytes of run_forward (outputs, no carry) and of run_vjp without a
 // carry (it recomputes the carry), for `count` instances.

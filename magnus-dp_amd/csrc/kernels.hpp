@@ -124,6 +124,47 @@ hipError_t launch_walkseg_forward(const WalkSegArgs& a, hipStream_t s);
 hipError_t launch_walkseg_reverse(const WalkSegArgs& a, hipStream_t s);
 hipError_t launch_walkseg_reduce(const WalkSegArgs& a, hipStream_t s);
 
+// Grouped walkers of the dense W-basis path (kernels_walkseg.hip), d in {16,
+// 32}, Boole, op many or sum: many instances of one dim, each with its own
+// samples and order, run as a pipeline of launches over task tables.  The
+// samples of an instance are cut into segments of L samples; a task walks one
+// level of one segment of one instance (one workgroup).  Forward task
+// (i, g, j) reads W_{j-1} on segment g and the state level j left at the end
+// of segment g - 1, so the launch of step t runs every task with g + j - 1 = t;
+// reverse task (i, g, k) reads barY^(k) on segment g and the state level k
+// left at the start of segment g + 1, so step t runs (G_i - 1 - g) + (n_i - k)
+// = t, and the dA of segment g (kind 2) runs one step after its level 2.  The
+// adjoints barY^(k) alternate between two segment slots by segment parity.
+struct GwInst {
+    double* W;           // [K][S][m] basis levels W_1..W_K
+    double* fst;         // [K][2][m] (P, f) of each level at the end of the last walked segment
+    double* U;           // [K][2][L + 1][m] barY^(1..K), two segment slots
+    double* rst;         // [K][2][m] (g4, pending) of each reverse level
+    double* sums;        // [n][m] running sums Z_0..Z_{n-1}
+    double* z0;          // [m] Z_0 once the last segment is walked
+    double* totals;      // [n][m] rows 0..K-1: W_j[S - 1]
+    const double* zbar;  // [n][m]
+    const double* wbar;  // [n][m]
+    long long S;
+    int n;
+    int id;              // the instance's row in the call's pointer table
+    double dt;
+};
+struct GwTask {
+    int inst, seg, level;
+    int kind;  // 0 forward level, 1 reverse level, 2 dA of the segment's own samples
+};
+struct GwArgs {
+    const GwTask* tasks;       // this launch's tasks
+    const GwInst* inst;
+    const double* const* tab;  // [3][count]: inputs, outputs or cotangents, dA
+    long long count;
+    int L;                     // segment length (a multiple of 4)
+};
+bool gw_ok(int d, int integ, int n);
+hipError_t launch_gw_forward(int d, const GwArgs& a, int ntasks, double bytes, double flops, hipStream_t s);
+hipError_t launch_gw_reverse(int d, const GwArgs& a, int ntasks, double bytes, double flops, hipStream_t s);
+
 // Single-pass basis sweep of the dense W-basis path at d = 8 under the Boole
 // rule (kernels_wbd8.hip): one wavefront per two instances walks the samples
 // once and writes the sums Z_0 (omega1), Z_1..Z_K (partial rows 1..K, row

@@ -77,47 +77,61 @@ struct SegFwdOps {
     double w[SegShape<D>::KS];  // W_{j-1} at sample i, column operand
 };
 
-// Level j of segment g: W_j = Prefix(A W_{j-1}) (level 1: Prefix(A)) on the
-// segment's samples from its start state, and the level sum (first pass).
+// What one forward walk reads and writes: level j of one instance over the
+// samples s0..s1 of one segment.  Rows of W are addressed relative to wbase
+// (a segment buffer starts at its segment's first sample, a full level at 0).
+struct FwdWalk {
+    const double* A;     // the instance's samples [S][M]
+    double* Wout;        // W_j (nullptr: the sum-only level n)
+    const double* Win;   // W_{j-1} (level >= 2)
+    long long wbase;
+    const double* ckin;  // (P, f) at s0 to resume from (segments after the first, levels < n)
+    double* ckout;       // (P, f) at s1 for the next segment, or nullptr
+    double* sum;         // the running level sum (read when resuming), or nullptr
+    double* z0;          // copy of the level-1 sum at the end, or nullptr
+    double* total;       // W_j[S - 1], or nullptr
+    long long S, s0, s1;
+    bool resume;         // a segment after the first: s0 was walked by the segment before
+    bool restart_row;    // resuming, write W_j[s0] from the start state (a fresh segment buffer)
+    double dt;
+};
+
+// W_j = Prefix(A W_{j-1}) (level 1: Prefix(A)) on the segment's samples from
+// its start state, and the level sum.
 template <int D, bool RAW>
-__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
+__device__ __forceinline__ void fwd_walk(const FwdWalk& w) {
     using W = SegShape<D>;
     constexpr int KS = W::KS, M = W::M;
-    const long long b = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int I = wave / W::TT, J = wave - (wave / W::TT) * W::TT, li = lane & 15, lk = lane >> 4;
-    const long long S = a.S, s0 = a.s0, s1 = a.s1, L1 = a.L + 1;
-    const int n = a.n, K = n - 1, j = a.level, g = a.g, G = a.G;
-    const double dt = a.dt;
-    const bool pref = j < n;         // levels 1..K keep a prefix; level n is a sum only
-    const bool first = a.mode == 0;  // first pass: sums, checkpoints, totals
-    const double* A = a.in + b * S * M;
-    double* Wout = pref ? a.wseg + ((b * K + (j - 1)) * L1) * M : nullptr;
-    const double* Win = RAW ? nullptr : a.wseg + ((b * K + (j - 2)) * L1) * M;
+    const long long S = w.S, s0 = w.s0, wb = w.wbase;
+    const double dt = w.dt;
+    const double* A = w.A;
+    double* Wout = w.Wout;
+    const bool keep_sum = w.sum != nullptr;
     int e[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) e[r] = (16 * I + 4 * r + lk) * D + 16 * J + li;
     PrefixState<BOOLE> st[4];
     double sum[4] = {0.0, 0.0, 0.0, 0.0};
-    long long i0 = 0;
-    if (g > 0) {  // resume after the segment start, which the previous segment processed
+    long long i0 = s0;
+    if (w.resume) {  // resume after the segment start, which the previous segment processed
         i0 = s0 + 1;
-        if (pref) {
-            const double* ck = a.ckp + (((b * G + g) * K + (j - 1)) * 2) * M;
+        if (Wout) {
 #pragma unroll
             for (int r = 0; r < 4; ++r) {
-                const double P = ck[e[r]], f = ck[M + e[r]];
+                const double P = w.ckin[e[r]], f = w.ckin[M + e[r]];
                 st[r].start(f, P);
-                Wout[e[r]] = P;
+                if (w.restart_row) Wout[(s0 - wb) * M + e[r]] = P;
             }
         }
-        if (first) {
+        if (keep_sum) {
 #pragma unroll
-            for (int r = 0; r < 4; ++r) sum[r] = a.sums[(b * n + (j - 1)) * M + e[r]];
+            for (int r = 0; r < 4; ++r) sum[r] = w.sum[e[r]];
         }
     }
     const double* arow = A + (16 * I + li) * D + lk * KS;
-    const double* wcol = RAW ? nullptr : Win + (lk * KS) * D + 16 * J + li;
+    const double* wcol = RAW ? nullptr : w.Win + (lk * KS) * D + 16 * J + li;
     auto load = [&](long long i, SegFwdOps<D>& o) {
         if constexpr (RAW) {
 #pragma unroll
@@ -125,7 +139,7 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
         } else {
             seg_load_row<KS>(o.a, arow + i * M);
 #pragma unroll
-            for (int s = 0; s < KS; ++s) o.w[s] = wcol[(i - s0) * M + s * D];
+            for (int s = 0; s < KS; ++s) o.w[s] = wcol[(i - wb) * M + s * D];
         }
     };
     auto step = [&](long long i, const SegFwdOps<D>& o, auto) {
@@ -142,25 +156,51 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
         }
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            if (first) sum_acc<BOOLE>(sum[r], i, S, f[r], dt);
-            if (pref) Wout[(i - s0) * M + e[r]] = st[r].step(i, f[r], dt);
+            if (keep_sum) sum_acc<BOOLE>(sum[r], i, S, f[r], dt);
+            if (Wout) Wout[(i - wb) * M + e[r]] = st[r].step(i, f[r], dt);
         }
     };
     constexpr int NB = seg_fwd_buffers<D>();
     SegFwdOps<D> o[NB];
-    walk_range<NB, NB, true>(i0, s1, o, load, step);
-    if (!first) return;
+    walk_range<NB, NB, true>(i0, w.s1, o, load, step);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        a.sums[(b * n + (j - 1)) * M + e[r]] = sum[r];
-        if (j == 1 && s1 == S - 1) a.z0[b * M + e[r]] = sum[r];
-        if (pref && g + 1 < G) {  // the next segment's start state: (P, f) at s1, a block end
-            double* ck = a.ckp + (((b * G + g + 1) * K + (j - 1)) * 2) * M;
-            ck[e[r]] = st[r].base;
-            ck[M + e[r]] = st[r].f0;
+        if (keep_sum) w.sum[e[r]] = sum[r];
+        if (w.z0) w.z0[e[r]] = sum[r];
+        if (w.ckout) {  // the next segment's start state: (P, f) at s1, a block end
+            w.ckout[e[r]] = st[r].base;
+            w.ckout[M + e[r]] = st[r].f0;
         }
-        if (pref && s1 == S - 1) a.totals[(b * n + (j - 1)) * M + e[r]] = st[r].base;
+        if (w.total) w.total[e[r]] = st[r].base;
     }
+}
+
+// Level j of segment g of every instance (one workgroup each): mode 0 is the
+// first pass (sums, checkpoints, totals), mode 1 rebuilds the segment's levels.
+template <int D, bool RAW>
+__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
+    constexpr int M = SegShape<D>::M;
+    const long long b = blockIdx.x;
+    const long long S = a.S, L1 = a.L + 1;
+    const int n = a.n, K = n - 1, j = a.level, g = a.g, G = a.G;
+    const bool pref = j < n, first = a.mode == 0, last = a.s1 == S - 1;
+    FwdWalk w;
+    w.A = a.in + b * S * M;
+    w.Wout = pref ? a.wseg + ((b * K + (j - 1)) * L1) * M : nullptr;
+    w.Win = j >= 2 ? a.wseg + ((b * K + (j - 2)) * L1) * M : nullptr;
+    w.wbase = a.s0;
+    w.ckin = (g > 0 && pref) ? a.ckp + (((b * G + g) * K + (j - 1)) * 2) * M : nullptr;
+    w.ckout = (first && pref && g + 1 < G) ? a.ckp + (((b * G + g + 1) * K + (j - 1)) * 2) * M : nullptr;
+    w.sum = first ? a.sums + (b * n + (j - 1)) * M : nullptr;
+    w.z0 = (first && j == 1 && last) ? a.z0 + b * M : nullptr;
+    w.total = (first && pref && last) ? a.totals + (b * n + (j - 1)) * M : nullptr;
+    w.S = S;
+    w.s0 = a.s0;
+    w.s1 = a.s1;
+    w.resume = g > 0;
+    w.restart_row = true;
+    w.dt = a.dt;
+    fwd_walk<D, RAW>(w);
 }
 
 // The top output of a reverse Boole block, g4 14 dt / 45 + pending: the block
@@ -181,32 +221,52 @@ struct SegRevOps {
     double da[4];                  // dA at the lane's entries (below the top level)
 };
 
-// Reverse level k of segment g (grad.hpp:311-335 in the W-basis form: shift 0,
-// weight 1, the totals' cotangents What_bar injected at the last sample).
+// What one reverse walk reads and writes: level k of one instance over the
+// own samples s0..itop of one segment (itop = S - 1 on the last segment, the
+// sample before the next segment's start otherwise).  Rows of W, of the
+// adjoints and of the dA accumulation are addressed relative to their bases.
+struct RevWalk {
+    const double* A;     // the instance's samples [S][M]
+    const double* Wl;    // W_{k-1}, row i at (i - wbase)
+    long long wbase;
+    const double* U;     // barY^(k) below the top level, row i at (i - ubase)
+    double* Uo;          // barY^(k-1) out, same rows
+    long long ubase;
+    double* dAs;         // dA accumulation, row i at (i - dbase)
+    long long dbase;
+    const double* zb;    // Zbar rows 0..n-1 of the instance
+    const double* wbar;  // What_bar_{k-1} (read on the last segment)
+    double* rs;          // (g4, pending) of this level: read from the segment above, written for the one below
+    long long S, s0, itop;
+    int n, k;
+    bool last_seg, first_seg;
+    double dt;
+};
+
+// Reverse level k over one segment (grad.hpp:311-335 in the W-basis form:
+// shift 0, weight 1, the totals' cotangents What_bar injected at the last sample).
 template <int D, bool TOP>
-__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
+__device__ __forceinline__ void rev_walk(const RevWalk& w) {
     using W = SegShape<D>;
     constexpr int KS = W::KS, M = W::M;
-    const long long b = blockIdx.x;
     const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
     const int I = wave / W::TT, J = wave - (wave / W::TT) * W::TT, li = lane & 15, lk = lane >> 4;
-    const long long S = a.S, s0 = a.s0, s1 = a.s1, L1 = a.L + 1;
-    const int n = a.n, K = n - 1, k = a.level, g = a.g, G = a.G;
-    const double dt = a.dt;
-    const bool last_seg = g == G - 1;
-    const long long itop = last_seg ? S - 1 : s1 - 1;  // the segment's own samples: s0 .. itop
+    const long long S = w.S, s0 = w.s0, itop = w.itop;
+    const int n = w.n, k = w.k;
+    const double dt = w.dt;
+    const bool last_seg = w.last_seg;
     int e[4];
 #pragma unroll
     for (int r = 0; r < 4; ++r) e[r] = (16 * I + 4 * r + lk) * D + 16 * J + li;
     const int erow = (16 * I + li) * D + lk * KS;
     const int ecol = (lk * KS) * D + 16 * J + li;
     const int wrow0 = (16 * J + li) * D + lk * KS;
-    const double* A = a.in + b * S * M;
-    const double* Wl = a.wseg + ((b * K + (k - 2)) * L1) * M;  // W_{k-1} on the segment
-    const double* U = a.useg + ((b * 2 + a.ga) * L1) * M;
-    double* Uo = a.useg + ((b * 2 + (1 - a.ga)) * L1) * M;
-    double* dAs = a.dAseg + b * L1 * M;
-    const double* zb = a.zbar + b * (long long)n * M;
+    const double* A = w.A;
+    const double* Wl = w.Wl - w.wbase * M;
+    const double* U = w.U - w.ubase * M;
+    double* Uo = w.Uo - w.ubase * M;
+    double* dAs = w.dAs - w.dbase * M;
+    const double* zb = w.zb;
     double yrow[KS], ycol[KS];
     if constexpr (TOP) {  // barY^(n) = Zbar_{n-1} c_i (lm_vjp_init with weight 1)
 #pragma unroll
@@ -222,36 +282,33 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
     auto fin = [&](double v, long long i, int r) { return addo ? sum_vjp_add<BOOLE>(v, i, S, osum[r], dt) : v; };
     double g1[4], g2[4], g3[4], g4[4], pend[4];
     if (!last_seg) {
-        const double* rs = a.rstate + ((b * K + (k - 2)) * 2) * M;
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            g4[r] = rs[e[r]];
-            pend[r] = rs[M + e[r]];
+            g4[r] = w.rs[e[r]];
+            pend[r] = w.rs[M + e[r]];
         }
     }
     double wb[4] = {0.0, 0.0, 0.0, 0.0};
     if (last_seg) {
 #pragma unroll
-        for (int r = 0; r < 4; ++r) wb[r] = a.wbar[(b * n + (k - 2)) * M + e[r]];
+        for (int r = 0; r < 4; ++r) wb[r] = w.wbar[e[r]];
     }
     auto load = [&](long long i, SegRevOps<D>& o) {
-        const long long t = i - s0;
-        seg_load_row<KS>(o.wrow, Wl + t * M + wrow0);
+        seg_load_row<KS>(o.wrow, Wl + i * M + wrow0);
 #pragma unroll
         for (int s = 0; s < KS; ++s) o.acol[s] = A[i * M + (lk * KS + s) * D + 16 * I + li];
         if constexpr (!TOP) {
-            seg_load_row<KS>(o.urow, U + t * M + erow);
+            seg_load_row<KS>(o.urow, U + i * M + erow);
 #pragma unroll
-            for (int s = 0; s < KS; ++s) o.ucol[s] = U[t * M + ecol + s * D];
+            for (int s = 0; s < KS; ++s) o.ucol[s] = U[i * M + ecol + s * D];
 #pragma unroll
-            for (int r = 0; r < 4; ++r) o.da[r] = dAs[t * M + e[r]];
+            for (int r = 0; r < 4; ++r) o.da[r] = dAs[i * M + e[r]];
         }
     };
     // POS: the sample's position in its Boole block (i & 3), 4 for the last
     // sample S - 1, where the level's total cotangent enters
     auto step_at = [&](long long i, const SegRevOps<D>& o, auto posc) {
         constexpr int POS = decltype(posc)::value;
-        const long long t = i - s0;
         const double cy = TOP ? sum_vjp_coef<BOOLE>(i, S, dt) : 0.0;
         dbl4 dav = {0.0, 0.0, 0.0, 0.0}, tmp = {0.0, 0.0, 0.0, 0.0};
 #pragma unroll
@@ -262,7 +319,7 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
             tmp = __builtin_amdgcn_mfma_f64_16x16x4f64(o.acol[s], TOP ? ycol[s] * cy : o.ucol[s], tmp, 0, 0, 0);
 #pragma unroll
         for (int r = 0; r < 4; ++r) {
-            dAs[t * M + e[r]] = TOP ? dav[r] : o.da[r] + dav[r];
+            dAs[i * M + e[r]] = TOP ? dav[r] : o.da[r] + dav[r];
             const double tv = tmp[r];
             if constexpr (POS == 4) {  // bar_total with shift 0 plus the total's cotangent (lm_vjp_scan_b2)
                 g4[r] = tv + wb[r];
@@ -295,10 +352,10 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
                 p += g4[r] * (14 * dt / 45);
                 pend[r] = p;
                 g4[r] = a0;
-                Uo[(t + 1) * M + e[r]] = fin(a1, i + 1, r);
-                Uo[(t + 2) * M + e[r]] = fin(a2, i + 2, r);
-                Uo[(t + 3) * M + e[r]] = fin(a3, i + 3, r);
-                Uo[(t + 4) * M + e[r]] = fin(a4, i + 4, r);
+                Uo[(i + 1) * M + e[r]] = fin(a1, i + 1, r);
+                Uo[(i + 2) * M + e[r]] = fin(a2, i + 2, r);
+                Uo[(i + 3) * M + e[r]] = fin(a3, i + 3, r);
+                Uo[(i + 4) * M + e[r]] = fin(a4, i + 4, r);
             }
         }
     };
@@ -320,15 +377,43 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
     walk_range<NB, NB, false>(s0, top, o, load, step);
 #pragma unroll
     for (int r = 0; r < 4; ++r) {
-        if (g == 0) {  // prefix_vjp_boole's last value: G[0] = pending
+        if (w.first_seg) {  // prefix_vjp_boole's last value: G[0] = pending
             Uo[e[r]] = fin(pend[r], 0, r);
         } else {  // the top output of block [s0 - 4, s0], from the state it would receive
-            Uo[e[r]] = fin(top_out(g4[r], pend[r], dt), s0, r);
-            double* rs = a.rstate + ((b * K + (k - 2)) * 2) * M;
-            rs[e[r]] = g4[r];
-            rs[M + e[r]] = pend[r];
+            Uo[s0 * M + e[r]] = fin(top_out(g4[r], pend[r], dt), s0, r);
+            w.rs[e[r]] = g4[r];
+            w.rs[M + e[r]] = pend[r];
         }
     }
+}
+
+template <int D, bool TOP>
+__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
+    constexpr int M = SegShape<D>::M;
+    const long long b = blockIdx.x;
+    const long long S = a.S, s0 = a.s0, L1 = a.L + 1;
+    const int n = a.n, K = n - 1, k = a.level, g = a.g, G = a.G;
+    RevWalk w;
+    w.A = a.in + b * S * M;
+    w.Wl = a.wseg + ((b * K + (k - 2)) * L1) * M;  // W_{k-1} on the segment
+    w.wbase = s0;
+    w.U = a.useg + ((b * 2 + a.ga) * L1) * M;
+    w.Uo = a.useg + ((b * 2 + (1 - a.ga)) * L1) * M;
+    w.ubase = s0;
+    w.dAs = a.dAseg + b * L1 * M;
+    w.dbase = s0;
+    w.zb = a.zbar + b * (long long)n * M;
+    w.wbar = a.wbar + (b * n + (k - 2)) * M;
+    w.rs = a.rstate + ((b * K + (k - 2)) * 2) * M;
+    w.S = S;
+    w.s0 = s0;
+    w.last_seg = g == G - 1;
+    w.itop = w.last_seg ? S - 1 : a.s1 - 1;
+    w.first_seg = g == 0;
+    w.n = n;
+    w.k = k;
+    w.dt = a.dt;
+    rev_walk<D, TOP>(w);
 }
 
 // dA on the segment's own samples: sum_vjp(Zbar_0, dt) + (dA_job + barY^(1))
@@ -350,7 +435,116 @@ __global__ void seg_reduce(WalkSegArgs a) {
     a.dA[(b * a.S + i) * a.m + e] = v;
 }
 
+// ---------------------------------------------------------------------------
+// Grouped walkers: one workgroup per task (instance, segment g, level) of a
+// task table, each instance with its own samples, order and full basis levels.
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ long long gw_seg_end(long long s0, long long L, long long S) {
+    return s0 + L < S - 1 ? s0 + L : S - 1;
+}
+
+template <int D>
+__global__ void __launch_bounds__(SegShape<D>::THREADS) gw_fwd(GwArgs a) {
+    constexpr int M = SegShape<D>::M;
+    const GwTask t = a.tasks[blockIdx.x];
+    const GwInst& in = a.inst[t.inst];
+    const long long S = in.S, L = a.L;
+    const int n = in.n, j = t.level;
+    const long long s0 = (long long)t.seg * L, s1 = gw_seg_end(s0, L, S);
+    const bool pref = j < n, last = s1 == S - 1;
+    FwdWalk w;
+    w.A = a.tab[in.id];
+    w.Wout = pref ? in.W + (long long)(j - 1) * S * M : nullptr;
+    w.Win = j >= 2 ? in.W + (long long)(j - 2) * S * M : nullptr;
+    w.wbase = 0;
+    w.ckin = pref ? in.fst + (long long)(j - 1) * 2 * M : nullptr;
+    w.ckout = (pref && !last) ? in.fst + (long long)(j - 1) * 2 * M : nullptr;  // a thread rewrites its own entries
+    w.sum = in.sums + (long long)(j - 1) * M;
+    w.z0 = (j == 1 && last) ? in.z0 : nullptr;
+    w.total = (pref && last) ? in.totals + (long long)(j - 1) * M : nullptr;
+    w.S = S;
+    w.s0 = s0;
+    w.s1 = s1;
+    w.resume = t.seg > 0;
+    w.restart_row = false;  // the level's full array already holds W_j[s0]
+    w.dt = in.dt;
+    if (j == 1) fwd_walk<D, true>(w);
+    else fwd_walk<D, false>(w);
+}
+
+template <int D>
+__global__ void __launch_bounds__(SegShape<D>::THREADS) gw_rev(GwArgs a) {
+    constexpr int M = SegShape<D>::M;
+    const GwTask t = a.tasks[blockIdx.x];
+    const GwInst& in = a.inst[t.inst];
+    const long long S = in.S, L = a.L, L1 = L + 1;
+    const int n = in.n, g = t.seg;
+    const long long s0 = (long long)g * L, s1 = gw_seg_end(s0, L, S);
+    const bool last_seg = s1 == S - 1;
+    const long long itop = last_seg ? S - 1 : s1 - 1;
+    double* dA = const_cast<double*>(a.tab[2 * a.count + in.id]);
+    const int slot = g & 1;
+    if (t.kind == 2) {  // dA on the segment's own samples: sum_vjp(Zbar_0, dt) + (dA + barY^(1))
+        const double* U1 = in.U + ((long long)slot * L1) * M;
+        const long long per = (itop - s0 + 1) * M;
+        for (long long x = threadIdx.x; x < per; x += SegShape<D>::THREADS) {
+            const long long r = x / M;
+            const int e = (int)(x - r * M);
+            const long long i = s0 + r;
+            double v = in.zbar[e] * sum_vjp_coef<BOOLE>(i, S, in.dt);
+            double d = dA[i * M + e];
+            d += U1[r * M + e];
+            v += d;
+            dA[i * M + e] = v;
+        }
+        return;
+    }
+    const int k = t.level;
+    RevWalk w;
+    w.A = a.tab[in.id];
+    w.Wl = in.W + (long long)(k - 2) * S * M;
+    w.wbase = 0;
+    w.Uo = in.U + (((long long)(k - 2) * 2 + slot) * L1) * M;  // barY^(k-1)
+    w.U = k < n ? in.U + (((long long)(k - 1) * 2 + slot) * L1) * M : w.Uo;  // barY^(k) (unread at the top)
+    w.ubase = s0;
+    w.dAs = dA;
+    w.dbase = 0;
+    w.zb = in.zbar;
+    w.wbar = in.wbar + (long long)(k - 2) * M;
+    w.rs = in.rst + (long long)(k - 2) * 2 * M;
+    w.S = S;
+    w.s0 = s0;
+    w.itop = itop;
+    w.n = n;
+    w.k = k;
+    w.last_seg = last_seg;
+    w.first_seg = g == 0;
+    w.dt = in.dt;
+    if (k == n) rev_walk<D, true>(w);
+    else rev_walk<D, false>(w);
+}
+
 }  // namespace
+
+bool gw_ok(int d, int integ, int n) { return integ == BOOLE && (d == 16 || d == 32) && n >= 2; }
+
+hipError_t launch_gw_forward(int d, const GwArgs& a, int ntasks, double bytes, double flops, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    void* h = timer_begin("gw_fwd", s, bytes, flops);
+    if (d == 16) hipLaunchKernelGGL((gw_fwd<16>), dim3(ntasks), dim3(SegShape<16>::THREADS), 0, s, a);
+    else hipLaunchKernelGGL((gw_fwd<32>), dim3(ntasks), dim3(SegShape<32>::THREADS), 0, s, a);
+    timer_end(h);
+    return hipGetLastError();
+}
+
+hipError_t launch_gw_reverse(int d, const GwArgs& a, int ntasks, double bytes, double flops, hipStream_t s) {
+    if (ntasks <= 0) return hipSuccess;
+    void* h = timer_begin("gw_rev", s, bytes, flops);
+    if (d == 16) hipLaunchKernelGGL((gw_rev<16>), dim3(ntasks), dim3(SegShape<16>::THREADS), 0, s, a);
+    else hipLaunchKernelGGL((gw_rev<32>), dim3(ntasks), dim3(SegShape<32>::THREADS), 0, s, a);
+    timer_end(h);
+    return hipGetLastError();
+}
 
 bool walkseg_ok(int d, int integ, int n) { return integ == BOOLE && (d == 16 || d == 32) && n >= 2; }
 

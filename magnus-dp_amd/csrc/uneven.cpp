@@ -1,15 +1,25 @@
 // uneven.cpp -- uneven-batch plans: many matrix problems of different dim,
 // samples and order solved by one enqueue (or one hipGraph launch).
 //
-// Schedule.  Instances are grouped into buckets of equal shape (n, samples,
+// Schedule.  Dense W-basis problems of dim 16 or 32 under the Boole rule (op
+// many or sum, n >= 2) run as one grouped pipeline: every such instance, of
+// any samples and order, is a set of tasks (instance, segment of samples,
+// level) of the grouped walkers, and one launch per pipeline step and dim
+// runs every task of that step (kernels_walkseg.hip); the GL fold between the
+// passes runs once per (dim, order, op) group.  Instances that do not fit the
+// pipeline's workspace together run in consecutive waves.
+//
+// The other instances are grouped into buckets of equal shape (n, samples,
 // dim, op, matrix backend, integrator, dt); each bucket runs the batched
-// level-parallel kernels of the one-problem path, so every instance's results
-// are those of a per-instance call.  A bucket is cut into sub-batches sized to
-// the per-branch workspace; sub-batches are assigned longest-first to the
-// branch (HIP stream) with the least estimated work, and each branch owns one
-// arena sized for its largest sub-batch, reused in stream order.  All device
-// memory (pointer tables, slot lists, arenas) is allocated when the plan is
-// prepared, so the enqueue makes no allocation call and can be captured.
+// level-parallel kernels of the one-problem path, so every such instance's
+// results are those of a per-instance call.  A bucket is cut into sub-batches
+// sized to the per-branch workspace; sub-batches are assigned longest-first to
+// the branch (HIP stream) with the least estimated work, and each branch owns
+// one arena sized for its largest sub-batch, reused in stream order.  The
+// pipeline runs on a stream of its own beside the branches.  All device memory
+// (pointer tables, slot lists, arenas, task tables, the pipeline's workspace)
+// is allocated when the plan is prepared, so the enqueue makes no allocation
+// call and can be captured.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -23,6 +33,7 @@
 #include <vector>
 
 #include "host.hpp"
+#include "kernels.hpp"
 #include "kernels_uneven.hpp"
 #include "magnus_capi.h"
 
@@ -74,6 +85,40 @@ std::vector<int> tile_candidates(const Plan& pl) {
     return {0};
 }
 
+// ---- grouped walker pipeline -------------------------------------------------
+constexpr int kGwDims = 2;  // dim 16, 32
+constexpr int kGwL = 128;   // segment length in samples (a multiple of the Boole block)
+
+int gw_dim_index(uint64_t d) { return d == 16 ? 0 : 1; }
+bool gw_eligible(const Plan& pl) {
+    return !pl.sc && pl.op != MAGNUS_OP_ONE && magnus::gw_ok((int)pl.d, pl.integ, (int)pl.n);
+}
+
+// instances of one (dim, order, op): one fold launch, packed per-slot arrays
+struct GwGroup {
+    Plan pl;
+    std::vector<uint32_t> ids;  // the instances' table rows, in slot order
+    size_t ids_off = 0;         // offset of ids in the device slot list
+    size_t z = 0, z0 = 0, totals = 0, zbar = 0, wbar = 0, cot = 0, out = 0, part = 0, part2 = 0, work = 0;
+};
+
+struct GwWave {
+    std::vector<GwGroup> groups;
+    size_t inst_off = 0, ninst = 0;  // in the device GwInst array
+    size_t bytes = 0;                // workspace bytes
+    // per dim: task offsets of each step (size steps + 1), into the device task table at *_base
+    std::vector<size_t> fwd_off[kGwDims], rev_off[kGwDims];
+    size_t fwd_base[kGwDims] = {}, rev_base[kGwDims] = {};
+    std::vector<double> fwd_bytes[kGwDims], fwd_flops[kGwDims], rev_bytes[kGwDims], rev_flops[kGwDims];
+};
+
+size_t gw_inst_bytes(const Plan& pl, int L) {
+    const size_t K = pl.n - 1, m = pl.m, n = pl.n, S = pl.S, Q = pl.Q, no = pl.n_out;
+    return scratch_round(K * S * m * 8) + 2 * scratch_round(K * 2 * m * 8) + scratch_round(K * 2 * (L + 1) * m * 8) +
+           scratch_round(n * m * 8) * 5 + scratch_round(m * 8) + 2 * scratch_round(no * m * 8) +
+           2 * scratch_round(Q * n * m * 8) + scratch_round(Q * (2 * K + 1) * m * 8);
+}
+
 }  // namespace
 
 struct magnus_uneven_plan {
@@ -99,6 +144,16 @@ struct magnus_uneven_plan {
     hipStream_t cap = nullptr;
     hipGraphExec_t exec[2] = {nullptr, nullptr};  // 0 forward, 1 VJP
     hipGraph_t graph[2] = {nullptr, nullptr};
+    // grouped walker pipeline
+    std::vector<Plan> gw_plans;  // per instance (empty Plan where the instance is not in the pipeline)
+    std::vector<uint32_t> gw_ids;
+    std::vector<GwWave> gw;
+    size_t gw_budget = 0;
+    magnus::GwInst* d_inst = nullptr;
+    magnus::GwTask* d_tasks = nullptr;
+    char* gw_ws = nullptr;
+    hipStream_t gw_stream = nullptr;
+    hipEvent_t gw_join = nullptr;
 };
 
 namespace {
@@ -109,6 +164,181 @@ void drop_graphs(magnus_uneven_plan* p) {
         if (p->graph[i]) (void)hipGraphDestroy(p->graph[i]);
         p->exec[i] = nullptr;
         p->graph[i] = nullptr;
+    }
+}
+
+// Waves of pipeline instances (largest first) within the pipeline's budget,
+// their (dim, order, op) groups, workspace offsets and the task tables of every
+// step.  Group slot lists are appended to the device slot list at ids_total.
+int gw_layout(magnus_uneven_plan* p, size_t& ids_total, std::vector<magnus::GwInst>& inst,
+              std::vector<magnus::GwTask>& tasks, std::vector<size_t>& slot_ws) {
+    p->gw.clear();
+    if (p->gw_ids.empty()) return MAGNUS_OK;
+    const int L = kGwL;
+    std::vector<uint32_t> order = p->gw_ids;
+    auto work = [&](uint32_t i) {
+        const Plan& pl = p->gw_plans[i];
+        return (double)pl.S * (double)(pl.n - 1) * (double)pl.d * pl.d * pl.d;
+    };
+    std::stable_sort(order.begin(), order.end(), [&](uint32_t a, uint32_t b) { return work(a) > work(b); });
+    std::vector<std::vector<uint32_t>> waves;
+    size_t cur = 0;
+    for (uint32_t i : order) {
+        const size_t b = gw_inst_bytes(p->gw_plans[i], L);
+        if (b > p->gw_budget)
+            return fail(MAGNUS_ENOMEM, "uneven plan: one instance of dim " + std::to_string(p->gw_plans[i].d) + ", " +
+                                           std::to_string(p->gw_plans[i].S) + " samples needs " + std::to_string(b) +
+                                           " pipeline workspace bytes");
+        if (waves.empty() || cur + b > p->gw_budget) {
+            waves.push_back({});
+            cur = 0;
+        }
+        waves.back().push_back(i);
+        cur += b;
+    }
+    for (auto& members : waves) {
+        GwWave wv;
+        std::map<std::tuple<uint64_t, uint64_t, int>, int> gkey;
+        for (uint32_t i : members) {
+            const Plan& pl = p->gw_plans[i];
+            auto key = std::make_tuple(pl.d, pl.n, pl.op);
+            auto it = gkey.find(key);
+            if (it == gkey.end()) {
+                it = gkey.emplace(key, (int)wv.groups.size()).first;
+                wv.groups.push_back(GwGroup{});
+                wv.groups.back().pl = pl;
+            }
+            wv.groups[it->second].ids.push_back(i);
+        }
+        size_t off = 0;
+        auto take = [&](size_t bytes) {
+            const size_t at = off;
+            off += scratch_round(bytes);
+            return at;
+        };
+        wv.inst_off = inst.size();
+        std::vector<int> steps_f(kGwDims, 0), steps_r(kGwDims, 0);
+        struct Pending {
+            int slot, G, n, dim;
+        };
+        std::vector<Pending> pend;
+        for (auto& grp : wv.groups) {
+            const Plan& pl = grp.pl;
+            const size_t B = grp.ids.size(), m = pl.m, n = pl.n, K = n - 1, Q = pl.Q, no = pl.n_out;
+            grp.ids_off = ids_total;
+            ids_total += B;
+            grp.z = take(B * n * m * 8);
+            grp.z0 = take(B * m * 8);
+            grp.totals = take(B * n * m * 8);
+            grp.zbar = take(B * n * m * 8);
+            grp.wbar = take(B * n * m * 8);
+            grp.cot = take(B * no * m * 8);
+            grp.out = take(B * no * m * 8);
+            grp.part = take(B * Q * n * m * 8);
+            grp.part2 = take(B * Q * n * m * 8);
+            grp.work = take(B * Q * (2 * K + 1) * m * 8);
+            for (size_t b = 0; b < B; ++b) {
+                const uint32_t i = grp.ids[b];
+                const Plan& pi = p->gw_plans[i];
+                magnus::GwInst gi{};
+                gi.S = (long long)pi.S;
+                gi.n = (int)pi.n;
+                gi.id = (int)i;
+                gi.dt = pi.dt;
+                // device pointers are bound once the workspace is allocated: keep offsets
+                // in the pointer fields' place through slot_ws (arrays below) and the group
+                const size_t own = take(K * pi.S * m * 8);
+                take(K * 2 * m * 8);
+                take(K * 2 * (L + 1) * m * 8);
+                take(K * 2 * m * 8);
+                slot_ws.push_back(own);
+                const int G = (int)((pi.S - 1 + L - 1) / L);
+                const int di = gw_dim_index(pi.d);
+                pend.push_back({(int)(inst.size() - wv.inst_off), G, (int)pi.n, di});
+                inst.push_back(gi);
+                steps_f[di] = std::max(steps_f[di], G + (int)pi.n - 1);
+                steps_r[di] = std::max(steps_r[di], G + (int)pi.n - 2 + 1);
+            }
+        }
+        wv.ninst = inst.size() - wv.inst_off;
+        wv.bytes = off;
+        // task tables: per dim, steps in order
+        for (int di = 0; di < kGwDims; ++di) {
+            const double d = di == 0 ? 16.0 : 32.0, m = d * d;
+            std::vector<std::vector<magnus::GwTask>> fs(steps_f[di]), rs(steps_r[di]);
+            std::vector<double> fb(steps_f[di], 0.0), ff(steps_f[di], 0.0), rb(steps_r[di], 0.0), rf(steps_r[di], 0.0);
+            for (auto& q : pend) {
+                if (q.dim != di) continue;
+                const long long S = (long long)inst[wv.inst_off + q.slot].S;
+                for (int g = 0; g < q.G; ++g) {
+                    const long long s0 = (long long)g * L, s1 = std::min<long long>(s0 + L, S - 1);
+                    const double len = (double)(s1 - s0 + (g == 0 ? 1 : 0));
+                    const double own = (double)((s1 == S - 1 ? S - 1 : s1 - 1) - s0 + 1);
+                    for (int j = 1; j <= q.n; ++j) {
+                        const int t = g + j - 1;
+                        fs[t].push_back({q.slot, g, j, 0});
+                        fb[t] += 8.0 * len * m * ((j >= 2 ? 2 : 1) + (j < q.n ? 1 : 0));
+                        ff[t] += len * ((j >= 2 ? 2.0 * d * d * d : 0.0) + 8.0 * m);
+                    }
+                    for (int k = q.n; k >= 2; --k) {
+                        const int t = (q.G - 1 - g) + (q.n - k);
+                        rs[t].push_back({q.slot, g, k, 1});
+                        rb[t] += 8.0 * own * m * (k == q.n ? 4 : 6);
+                        rf[t] += own * (4.0 * d * d * d + 16.0 * m);
+                    }
+                    const int tf = (q.G - 1 - g) + (q.n - 2) + 1;
+                    rs[tf].push_back({q.slot, g, 0, 2});
+                    rb[tf] += 8.0 * own * m * 3;
+                }
+            }
+            wv.fwd_base[di] = tasks.size();
+            wv.fwd_off[di].assign(1, 0);
+            for (int t = 0; t < steps_f[di]; ++t) {
+                tasks.insert(tasks.end(), fs[t].begin(), fs[t].end());
+                wv.fwd_off[di].push_back(tasks.size() - wv.fwd_base[di]);
+            }
+            wv.rev_base[di] = tasks.size();
+            wv.rev_off[di].assign(1, 0);
+            for (int t = 0; t < steps_r[di]; ++t) {
+                tasks.insert(tasks.end(), rs[t].begin(), rs[t].end());
+                wv.rev_off[di].push_back(tasks.size() - wv.rev_base[di]);
+            }
+            wv.fwd_bytes[di] = fb;
+            wv.fwd_flops[di] = ff;
+            wv.rev_bytes[di] = rb;
+            wv.rev_flops[di] = rf;
+        }
+        p->gw.push_back(std::move(wv));
+    }
+    return MAGNUS_OK;
+}
+
+// device pointers of every pipeline instance, from the workspace offsets
+void gw_bind(magnus_uneven_plan* p, std::vector<magnus::GwInst>& inst, const std::vector<size_t>& slot_ws) {
+    char* ws = p->gw_ws;
+    for (auto& wv : p->gw) {
+        size_t slot = wv.inst_off;
+        for (auto& grp : wv.groups) {
+            const Plan& pl = grp.pl;
+            const size_t m = pl.m, n = pl.n, K = n - 1;
+            for (size_t b = 0; b < grp.ids.size(); ++b, ++slot) {
+                magnus::GwInst& gi = inst[slot];
+                const size_t S = (size_t)gi.S;
+                size_t at = slot_ws[slot];
+                gi.W = reinterpret_cast<double*>(ws + at);
+                at += scratch_round(K * S * m * 8);
+                gi.fst = reinterpret_cast<double*>(ws + at);
+                at += scratch_round(K * 2 * m * 8);
+                gi.U = reinterpret_cast<double*>(ws + at);
+                at += scratch_round(K * 2 * (kGwL + 1) * m * 8);
+                gi.rst = reinterpret_cast<double*>(ws + at);
+                gi.sums = reinterpret_cast<double*>(ws + grp.z) + b * n * m;
+                gi.z0 = reinterpret_cast<double*>(ws + grp.z0) + b * m;
+                gi.totals = reinterpret_cast<double*>(ws + grp.totals) + b * n * m;
+                gi.zbar = reinterpret_cast<double*>(ws + grp.zbar) + b * n * m;
+                gi.wbar = reinterpret_cast<double*>(ws + grp.wbar) + b * n * m;
+            }
+        }
     }
 }
 
@@ -144,8 +374,17 @@ int prepare(magnus_uneven_plan* p) {
         bk.ids_off = ids_total;
         ids_total += bk.ids.size();
     }
+    // grouped pipeline: waves of instances within the budget, groups, tasks
+    std::vector<magnus::GwInst> gw_inst;
+    std::vector<magnus::GwTask> gw_tasks;
+    std::vector<size_t> gw_slot_ws;  // per GwInst: workspace offset of its own arrays
+    if (int rc = gw_layout(p, ids_total, gw_inst, gw_tasks, gw_slot_ws)) return rc;
     const size_t ids_bytes = scratch_round(std::max<size_t>(ids_total, 1) * sizeof(uint32_t));
-    size_t total = tab_bytes + ids_bytes;
+    const size_t inst_bytes = scratch_round(std::max<size_t>(gw_inst.size(), 1) * sizeof(magnus::GwInst));
+    const size_t task_bytes = scratch_round(std::max<size_t>(gw_tasks.size(), 1) * sizeof(magnus::GwTask));
+    size_t gw_ws_bytes = 0;
+    for (auto& wv : p->gw) gw_ws_bytes = std::max(gw_ws_bytes, wv.bytes);
+    size_t total = tab_bytes + ids_bytes + inst_bytes + task_bytes + scratch_round(gw_ws_bytes);
     for (int b = 0; b < p->branches; ++b) total += scratch_round(need[b]);
     if (p->dmem) {
         (void)hipDeviceSynchronize();
@@ -162,11 +401,24 @@ int prepare(magnus_uneven_plan* p) {
     std::vector<uint32_t> ids_host;
     ids_host.reserve(ids_total);
     for (auto& bk : p->buckets) ids_host.insert(ids_host.end(), bk.ids.begin(), bk.ids.end());
+    for (auto& wv : p->gw)
+        for (auto& grp : wv.groups) ids_host.insert(ids_host.end(), grp.ids.begin(), grp.ids.end());
     if (ids_total && hipMemcpy(p->d_ids, ids_host.data(), ids_total * sizeof(uint32_t), hipMemcpyHostToDevice) !=
                          hipSuccess)
         return device_error(hipGetLastError(), "uneven plan: slot upload");
+    p->d_inst = reinterpret_cast<magnus::GwInst*>(p->dmem + tab_bytes + ids_bytes);
+    p->d_tasks = reinterpret_cast<magnus::GwTask*>(p->dmem + tab_bytes + ids_bytes + inst_bytes);
+    p->gw_ws = p->dmem + tab_bytes + ids_bytes + inst_bytes + task_bytes;
+    if (!gw_inst.empty()) {
+        gw_bind(p, gw_inst, gw_slot_ws);
+        if (hipMemcpy(p->d_inst, gw_inst.data(), gw_inst.size() * sizeof(magnus::GwInst), hipMemcpyHostToDevice) !=
+                hipSuccess ||
+            hipMemcpy(p->d_tasks, gw_tasks.data(), gw_tasks.size() * sizeof(magnus::GwTask), hipMemcpyHostToDevice) !=
+                hipSuccess)
+            return device_error(hipGetLastError(), "uneven plan: task upload");
+    }
     p->arenas.assign(p->branches, Arena{});
-    char* at = p->dmem + tab_bytes + ids_bytes;
+    char* at = p->gw_ws + scratch_round(gw_ws_bytes);
     for (int b = 0; b < p->branches; ++b) {
         p->arenas[b].base = at;
         p->arenas[b].size = need[b];
@@ -222,10 +474,90 @@ int enqueue_item(magnus_uneven_plan* p, const Item& it, bool vjp, hipStream_t s)
     return MAGNUS_OK;
 }
 
-// fork from `origin` onto the branch streams, every sub-batch, join back
+// the grouped pipeline, wave by wave, on stream s
+int enqueue_gw(magnus_uneven_plan* p, bool vjp, hipStream_t s) {
+    for (auto& wv : p->gw) {
+        magnus::GwArgs ga{};
+        ga.inst = p->d_inst + wv.inst_off;
+        ga.tab = p->d_tab;
+        ga.count = (long long)p->count;
+        ga.L = kGwL;
+        hipError_t e = hipSuccess;
+        if (vjp) {
+            for (auto& grp : wv.groups) {
+                magnus::SegmentArgs g{};
+                g.ids = p->d_ids + grp.ids_off;
+                g.count = (int)grp.ids.size();
+                g.tab = p->d_tab + p->count;
+                g.packed = reinterpret_cast<double*>(p->gw_ws + grp.cot);
+                g.per = (long long)(grp.pl.n_out * grp.pl.m_cot);
+                e = magnus::launch_gather(g, s);
+                if (e != hipSuccess) return device_error(e, "uneven pipeline gather");
+            }
+        }
+        for (size_t t = 0; t + 1 < std::max(wv.fwd_off[0].size(), wv.fwd_off[1].size()); ++t)
+            for (int di = 0; di < kGwDims; ++di) {
+                if (t + 1 >= wv.fwd_off[di].size()) continue;
+                ga.tasks = p->d_tasks + wv.fwd_base[di] + wv.fwd_off[di][t];
+                const int cnt = (int)(wv.fwd_off[di][t + 1] - wv.fwd_off[di][t]);
+                e = magnus::launch_gw_forward(di == 0 ? 16 : 32, ga, cnt, wv.fwd_bytes[di][t], wv.fwd_flops[di][t], s);
+                if (e != hipSuccess) return device_error(e, "uneven pipeline forward");
+            }
+        for (auto& grp : wv.groups) {
+            magnus::WbdArgs f{};
+            fill_wbd_fold(grp.pl, (long long)grp.ids.size(), f);
+            f.z = reinterpret_cast<double*>(p->gw_ws + grp.z);
+            f.z0 = reinterpret_cast<double*>(p->gw_ws + grp.z0);
+            f.totals = reinterpret_cast<double*>(p->gw_ws + grp.totals);
+            f.part = reinterpret_cast<double*>(p->gw_ws + grp.part);
+            f.work = reinterpret_cast<double*>(p->gw_ws + grp.work);
+            if (vjp) {
+                f.cot = reinterpret_cast<double*>(p->gw_ws + grp.cot);
+                f.zbar = reinterpret_cast<double*>(p->gw_ws + grp.zbar);
+                f.wbar = reinterpret_cast<double*>(p->gw_ws + grp.wbar);
+                f.part2 = reinterpret_cast<double*>(p->gw_ws + grp.part2);
+                e = magnus::launch_wbd_fold_vjp(f, s);
+                if (e != hipSuccess) return device_error(e, "uneven pipeline fold");
+            } else {
+                f.out = reinterpret_cast<double*>(p->gw_ws + grp.out);
+                e = magnus::launch_wbd_fold(f, s);
+                if (e != hipSuccess) return device_error(e, "uneven pipeline fold");
+                magnus::SegmentArgs g{};
+                g.ids = p->d_ids + grp.ids_off;
+                g.count = (int)grp.ids.size();
+                g.tab = p->d_tab + p->count;
+                g.packed = f.out;
+                g.per = (long long)(grp.pl.n_out * grp.pl.m);
+                e = magnus::launch_scatter(g, s);
+                if (e != hipSuccess) return device_error(e, "uneven pipeline scatter");
+            }
+        }
+        if (!vjp) continue;
+        for (size_t t = 0; t + 1 < std::max(wv.rev_off[0].size(), wv.rev_off[1].size()); ++t)
+            for (int di = 0; di < kGwDims; ++di) {
+                if (t + 1 >= wv.rev_off[di].size()) continue;
+                ga.tasks = p->d_tasks + wv.rev_base[di] + wv.rev_off[di][t];
+                const int cnt = (int)(wv.rev_off[di][t + 1] - wv.rev_off[di][t]);
+                e = magnus::launch_gw_reverse(di == 0 ? 16 : 32, ga, cnt, wv.rev_bytes[di][t], wv.rev_flops[di][t], s);
+                if (e != hipSuccess) return device_error(e, "uneven pipeline reverse");
+            }
+    }
+    return MAGNUS_OK;
+}
+
+// fork from `origin` onto the branch streams and the pipeline's stream, every
+// sub-batch and pipeline step, join back
 int enqueue_all(magnus_uneven_plan* p, bool vjp, hipStream_t origin) {
     hipError_t e = hipEventRecord(p->fork, origin);
     if (e != hipSuccess) return device_error(e, "uneven fork");
+    if (!p->gw.empty()) {
+        e = hipStreamWaitEvent(p->gw_stream, p->fork, 0);
+        if (e != hipSuccess) return device_error(e, "uneven fork");
+        if (int rc = enqueue_gw(p, vjp, p->gw_stream)) return rc;
+        e = hipEventRecord(p->gw_join, p->gw_stream);
+        if (e == hipSuccess) e = hipStreamWaitEvent(origin, p->gw_join, 0);
+        if (e != hipSuccess) return device_error(e, "uneven join");
+    }
     for (int b = 0; b < p->branches; ++b) {
         if (p->order[b].empty()) continue;
         e = hipStreamWaitEvent(p->streams[b], p->fork, 0);
@@ -294,6 +626,8 @@ void destroy(magnus_uneven_plan* p) {
         if (p->join[b]) (void)hipEventDestroy(p->join[b]);
     }
     if (p->cap) (void)hipStreamDestroy(p->cap);
+    if (p->gw_stream) (void)hipStreamDestroy(p->gw_stream);
+    if (p->gw_join) (void)hipEventDestroy(p->gw_join);
     if (p->fork) (void)hipEventDestroy(p->fork);
     if (p->tab_done) (void)hipEventDestroy(p->tab_done);
     if (p->h_tab) (void)hipHostFree(p->h_tab);
@@ -335,9 +669,40 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
         workspace_bytes = fr / 2;
     }
     p->budget = workspace_bytes;
-    const size_t per_branch = workspace_bytes / branches;
+    p->gw_plans.assign(count, Plan{});
+    bool others = false;
+    for (uint64_t i = 0; i < count; ++i) {
+        if (gw_eligible(plans[i])) {
+            p->gw_plans[i] = plans[i];
+            p->gw_ids.push_back((uint32_t)i);
+        } else {
+            others = true;
+        }
+    }
+    // The pipeline takes what it needs in one wave, at most half the workspace
+    // when the branches have work too, and never what the branches need for one
+    // instance of their largest bucket each (while its own largest instance fits).
+    size_t gw_need = 0, gw_min = 0, br_min = 0;
+    for (uint32_t i : p->gw_ids) {
+        gw_need += gw_inst_bytes(plans[i], kGwL);
+        gw_min = std::max(gw_min, gw_inst_bytes(plans[i], kGwL));
+    }
+    for (uint64_t i = 0; i < count; ++i)
+        if (!gw_eligible(plans[i])) br_min = std::max(br_min, item_bytes(plans[i], 1));
+    if (p->gw_ids.empty()) {
+        p->gw_budget = 0;
+    } else if (!others) {
+        p->gw_budget = workspace_bytes;
+    } else {
+        size_t gb = std::min<size_t>(gw_need, workspace_bytes / 2);
+        const size_t br_total = br_min * (size_t)branches;
+        if (workspace_bytes > br_total && workspace_bytes - br_total < gb) gb = workspace_bytes - br_total;
+        p->gw_budget = std::max(gb, std::min(gw_min, workspace_bytes));
+    }
+    const size_t per_branch = (workspace_bytes - p->gw_budget) / branches;
     std::map<std::tuple<uint64_t, uint64_t, uint64_t, int, int, int, uint64_t>, int> keys;
     for (uint64_t i = 0; i < count; ++i) {
+        if (gw_eligible(plans[i])) continue;
         const Plan& pl = plans[i];
         uint64_t dtb;
         std::memcpy(&dtb, &pl.dt, sizeof dtb);
@@ -352,10 +717,11 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
     }
     for (auto& bk : p->buckets) {
         if (item_bytes(bk.pl, 1) > per_branch) {
+            const std::string msg = "uneven plan: one instance of dim " + std::to_string(bk.pl.d) + ", " +
+                                    std::to_string(bk.pl.S) + " samples needs " +
+                                    std::to_string(item_bytes(bk.pl, 1)) + " workspace bytes per branch";
             destroy(p);
-            return fail(MAGNUS_ENOMEM, "uneven plan: one instance of dim " + std::to_string(bk.pl.d) + ", " +
-                                           std::to_string(bk.pl.S) + " samples needs " +
-                                           std::to_string(item_bytes(bk.pl, 1)) + " workspace bytes per branch");
+            return fail(MAGNUS_ENOMEM, msg);
         }
         uint64_t sub = bk.ids.size();
         while (sub > 1 && item_bytes(bk.pl, sub) > per_branch) sub = (sub + 1) / 2;
@@ -367,6 +733,8 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
         if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join[b], hipEventDisableTiming);
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->gw_stream, hipStreamNonBlocking);
+    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->gw_join, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->tab_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void**)&p->h_tab, std::max<uint64_t>(3 * count, 1) * sizeof(void*));
@@ -505,6 +873,11 @@ int magnus_uneven_set_tuning(magnus_uneven_plan* p, const char* text) {
     return prepare(p);
 }
 
+int magnus_uneven_in_pipeline(const magnus_uneven_plan* p, uint64_t i) {
+    if (!p || i >= p->count || p->gw_plans.empty()) return 0;
+    return p->gw_plans[i].n ? 1 : 0;
+}
+
 uint64_t magnus_uneven_info(const magnus_uneven_plan* p, int what) {
     if (!p) return 0;
     switch (what) {
@@ -516,6 +889,8 @@ uint64_t magnus_uneven_info(const magnus_uneven_plan* p, int what) {
             return used;
         }
         case 3: return p->dbytes;
+        case 4: return p->gw_ids.size();
+        case 5: return p->gw.size();
         default: return 0;
     }
 }

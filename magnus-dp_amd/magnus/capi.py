@@ -100,6 +100,7 @@ def lib() -> ctypes.CDLL:
         L.magnus_uneven_set_tuning.argtypes = [vp, ctypes.c_char_p]
         L.magnus_uneven_info.argtypes = [vp, ctypes.c_int]
         L.magnus_uneven_info.restype = u64
+        L.magnus_uneven_in_pipeline.argtypes = [vp, u64]
         f32p = ctypes.POINTER(ctypes.c_float)
         u32 = ctypes.c_uint32
         L.magnus_ctrl_feature_count.restype = u64

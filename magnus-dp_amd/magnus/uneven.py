@@ -108,7 +108,14 @@ class UnevenBatch:
     def set_tuning(self, text: str) -> None:
         capi.check(capi.lib().magnus_uneven_set_tuning(self._h, text.encode()))
 
+    def in_pipeline(self, i: int) -> bool:
+        """Whether instance i runs in the grouped walker pipeline (its results then
+        agree with a per-instance call to rounding) rather than in a bucket (bit
+        for bit)."""
+        return bool(capi.lib().magnus_uneven_in_pipeline(self._h, i))
+
     def info(self) -> dict:
         L = capi.lib()
         return {k: int(L.magnus_uneven_info(self._h, i))
-                for i, k in enumerate(("buckets", "sub_batches", "branches", "workspace_bytes"))}
+                for i, k in enumerate(("buckets", "sub_batches", "branches", "workspace_bytes", "pipeline_instances",
+                                       "pipeline_waves"))}

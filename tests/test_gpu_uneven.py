@@ -1,11 +1,14 @@
 """GPU: uneven batches (magnus_uneven_*) -- many matrix problems of different
 dim, samples, order, op and integrator in one enqueue or one hipGraph launch.
 
-Every instance's forward output and dA must equal a per-instance call of the
-one-problem entry points bit for bit (same kernels, same per-job arithmetic),
-in eager mode and on every replay of the captured graph; product-path
-variants chosen by the tuner are checked against the oracle (1e-12 relative
-Frobenius).
+A bucketed instance's forward output and dA must equal a per-instance call of
+the one-problem entry points bit for bit (same kernels, same per-job
+arithmetic); an instance of the grouped walker pipeline (dense dim 16 / 32,
+Boole, op many or sum) must agree with it at 1e-12 relative Frobenius (same
+recurrences, matrix-core products in another summation order) and with the
+oracle.  Every replay of the captured graph repeats the eager results bit for
+bit; product-path variants chosen by the tuner are checked against the oracle
+(1e-12).
 """
 from __future__ import annotations
 
@@ -48,7 +51,7 @@ def _mixed_problems(seed, count):
                           t0=0.0, tf=1.0, op=str(rng.choice(ops)), integrator=str(rng.choice(integs))))
     # repeated shapes so that buckets hold several instances, and one bucket of
     # instances too large to share a small workspace
-    big = dict(n=5, dim=32, samples=129, t0=0.0, tf=1.0, op="sum", integrator="Boole")
+    big = dict(n=5, dim=32, samples=129, t0=0.0, tf=1.0, op="sum", integrator="Simpson")
     return probs + [dict(p) for p in probs[: count // 2]] + [dict(big) for _ in range(3)]
 
 
@@ -75,11 +78,19 @@ def _per_instance(probs, A, cot):
     return outs, dAs
 
 
-@pytest.mark.parametrize("branches,workspace", [(1, 0), (3, 0), (4, 64 << 20)])
-def test_uneven_matches_per_instance_calls_bitwise(branches, workspace):
+def _same(plan, i, got, ref):
+    import torch
+
+    if plan.in_pipeline(i):
+        return rel_fro(got.cpu().numpy(), ref.cpu().numpy()) <= TOL
+    return torch.equal(got, ref)
+
+
+@pytest.mark.parametrize("branches,workspace", [(1, 0), (3, 0), (4, 96 << 20)])
+def test_uneven_matches_per_instance_calls(branches, workspace):
     """Eager and graph modes against per-instance calls; the small workspace
-    (16 MB per branch, about 11 MB per dim-32 instance) forces sub-batches and
-    arena reuse."""
+    (what the pipeline needs, then under 24 MB per branch against about 9 MB
+    per bucketed dim-32 instance) forces sub-batches and arena reuse."""
     import torch
 
     from magnus.uneven import UnevenBatch
@@ -90,18 +101,72 @@ def test_uneven_matches_per_instance_calls_bitwise(branches, workspace):
     plan = UnevenBatch(probs, workspace_bytes=workspace, branches=branches)
     info = plan.info()
     assert info["buckets"] <= len(probs) and info["sub_batches"] >= info["buckets"]
+    assert info["pipeline_instances"] == sum(plan.in_pipeline(i) for i in range(len(probs))) > 0
     if workspace:
         assert info["sub_batches"] > info["buckets"]
+    first = None
     for use_graph in (False, True):
         for rep in range(3 if use_graph else 1):
             outs = plan.forward(A, use_graph=use_graph)
             dAs = plan.vjp(A, cot, use_graph=use_graph)
             torch.cuda.synchronize()
             for i in range(len(probs)):
-                assert torch.equal(outs[i], ref_out[i]), (use_graph, rep, i, probs[i])
-                assert torch.equal(dAs[i], ref_dA[i]), (use_graph, rep, i, probs[i])
+                assert _same(plan, i, outs[i], ref_out[i]), (use_graph, rep, i, probs[i])
+                assert _same(plan, i, dAs[i], ref_dA[i]), (use_graph, rep, i, probs[i])
+            if first is None:
+                first = ([x.clone() for x in outs], [x.clone() for x in dAs])
+            else:
+                assert all(torch.equal(x, y) for x, y in zip(outs, first[0])), (use_graph, rep)
+                assert all(torch.equal(x, y) for x, y in zip(dAs, first[1])), (use_graph, rep)
             # eager work of the program between graph replays
             _ = sum(float(x.sum()) for x in outs)
+    plan.close()
+
+
+@pytest.mark.parametrize("workspace", [0, 80 << 20])
+def test_uneven_pipeline_matches_oracle_across_waves(workspace):
+    """The grouped walker pipeline alone: dims 16 and 32, one segment (5, 9
+    samples), a short last segment (133 samples: 128 + 4), several segments,
+    orders 2..6, ops many and sum; a workspace that splits it into waves.
+    Against per-instance calls (1e-12), the oracle (1e-12, the smaller
+    instances) and across graph replays (bit for bit)."""
+    import torch
+
+    from magnus.uneven import UnevenBatch
+
+    probs = []
+    for d in (16, 32):
+        for S in (5, 9, 133, 261, 1025):
+            for n in (2, 3, 6):
+                probs.append(dict(n=n, dim=d, samples=S, t0=0.0, tf=1.0, op="sum" if (S + n) % 2 else "many",
+                                  integrator="Boole"))
+    A, cot = _inputs(probs, 13)
+    ref_out, ref_dA = _per_instance(probs, A, cot)
+    plan = UnevenBatch(probs, workspace_bytes=workspace, branches=2)
+    info = plan.info()
+    assert info["pipeline_instances"] == len(probs) and info["buckets"] == 0
+    assert info["pipeline_waves"] >= (2 if workspace else 1)
+    outs = plan.forward(A, use_graph=False)
+    dAs = plan.vjp(A, cot, use_graph=False)
+    torch.cuda.synchronize()
+    outs = [x.clone() for x in outs]
+    dAs = [x.clone() for x in dAs]
+    for i, p in enumerate(probs):
+        assert rel_fro(outs[i].cpu().numpy(), ref_out[i].cpu().numpy()) <= TOL, (i, p)
+        assert rel_fro(dAs[i].cpu().numpy(), ref_dA[i].cpu().numpy()) <= TOL, (i, p)
+        if p["samples"] <= 261 and (p["dim"] == 16 or p["n"] <= 3):
+            a = A[i].cpu().numpy()
+            ref = O.forward(p["n"], a, 0.0, 1.0, op=p["op"], integrator="Boole")
+            ref_g = O.vjp(p["n"], a, cot[i].cpu().numpy(), 0.0, 1.0, op=p["op"], integrator="Boole")
+            assert rel_fro(outs[i].cpu().numpy(), ref) <= TOL, (i, p)
+            assert rel_fro(dAs[i].cpu().numpy(), ref_g) <= TOL, (i, p)
+    for rep in range(3):
+        g_out = plan.forward(A, use_graph=True)
+        g_dA = plan.vjp(A, cot, use_graph=True)
+        torch.cuda.synchronize()
+        assert all(torch.equal(x, y) for x, y in zip(g_out, outs)), rep
+        assert all(torch.equal(x, y) for x, y in zip(g_dA, dAs)), rep
+        _ = sum(float(x.sum()) for x in g_out)
     plan.close()
 
 
@@ -112,7 +177,7 @@ def test_uneven_product_paths_match_oracle(dim, tile):
 
     from magnus.uneven import UnevenBatch
 
-    probs = [dict(n=4, dim=dim, samples=65, t0=0.0, tf=1.0, op="many", integrator="Boole") for _ in range(3)]
+    probs = [dict(n=4, dim=dim, samples=65, t0=0.0, tf=1.0, op="many", integrator="Simpson") for _ in range(3)]
     probs.append(dict(n=3, dim=dim, samples=33, t0=0.0, tf=1.0, op="sum", integrator="Simpson"))
     A, cot = _inputs(probs, 5)
     plan = UnevenBatch(probs, branches=2)
@@ -141,7 +206,8 @@ def test_uneven_autotune_picks_measured_paths_and_keeps_parity():
     plan = UnevenBatch(probs, branches=2)
     plan.autotune(A, cot)
     lines = [ln.split() for ln in plan.tuning.splitlines()]
-    assert len(lines) == 4 and all(float(ln[4]) > 0 for ln in lines)
+    bucketed = {p["dim"] for i, p in enumerate(probs) if not plan.in_pipeline(i)}
+    assert len(lines) == len(bucketed) and all(float(ln[4]) > 0 for ln in lines)
     outs = plan.forward(A)
     dAs = plan.vjp(A, cot)
     torch.cuda.synchronize()

@@ -445,7 +445,7 @@ def _bench_uneven(args, torch, dev):
     torch.cuda.synchronize()
     capi.timing_enable(False)
     per_kernel = {}
-    for name in KERNELS + ("wbd_fold", "wbd_fold_vjp", "uneven_gather", "uneven_scatter"):
+    for name in KERNELS + ("wbd_fold", "wbd_fold_vjp", "uneven_gather", "uneven_scatter", "gw_fwd", "gw_rev"):
         cnt, tot = capi.timing_read(name)
         if cnt:
             per_kernel[name] = {"launches": cnt, "total_ms": round(tot, 3)}
@@ -771,7 +771,7 @@ def main():
     ap.add_argument("--ctrl-eval", type=int, default=10000)
     ap.add_argument("--ctrl-epochs", type=int, default=20)
     ap.add_argument("--uneven-branches", type=int, default=4)
-    ap.add_argument("--uneven-workspace-gb", type=float, default=48.0)
+    ap.add_argument("--uneven-workspace-gb", type=float, default=96.0)
     ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
     ap.add_argument("--dim", type=int, default=32, help="dense workload: matrix dimension (profiling only)")
     ap.add_argument("--sc-residual", default="checkpoint", choices=["checkpoint", "recompute"],

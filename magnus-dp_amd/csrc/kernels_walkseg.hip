@@ -436,6 +436,304 @@ __global__ void seg_reduce(WalkSegArgs a) {
 }
 
 // ---------------------------------------------------------------------------
+// d = 64: sixteen wavefronts per workgroup, wave w owning the 16 x 16 output
+// tile (w / 4, w % 4) as at d = 16, 32; a lane owns 4 entries.  A sample's
+// matrices are staged in LDS (coalesced 16-byte global loads, 16-byte LDS
+// stores) and every MFMA operand is read from there, four wavefronts sharing
+// each operand tile: operands read along rows use a row stride of 68 doubles,
+// operands read down columns 80, so that a read's 64 lanes spread evenly over
+// the banks.  The forward double-buffers its two matrices (A_i, W_{j-1}[i]:
+// 148 KB); the reverse stages three (A_i, W_{k-1}[i], barY_i: 108 KB) in one
+// buffer with the next sample's share in registers.
+// ---------------------------------------------------------------------------
+constexpr int kT64 = 1024;  // threads
+constexpr int kP_ROW = 68, kP_COL = 80;
+constexpr int kS_ROW = 64 * kP_ROW, kS_COL = 64 * kP_COL;
+constexpr int kLds64Fwd = 2 * (kS_ROW + kS_COL), kLds64Rev = kS_COL + 2 * kS_ROW;  // doubles
+
+// s_barrier after this wave's LDS traffic only: __syncthreads() would also wait
+// for every global store in flight
+__device__ __forceinline__ void lds_barrier() { asm volatile("s_waitcnt lgkmcnt(0)\n\ts_barrier" ::: "memory"); }
+
+typedef double dbl2 __attribute__((ext_vector_type(2)));
+
+struct Stage64 {
+    dbl2 v[2];  // a thread's share of one 64 x 64 matrix: 16-byte chunks tid + 1024 h
+};
+__device__ __forceinline__ void stage_load(Stage64& r, const double* src) {
+    const dbl2* p = reinterpret_cast<const dbl2*>(src);
+#pragma unroll
+    for (int h = 0; h < 2; ++h) r.v[h] = p[threadIdx.x + kT64 * h];
+}
+__device__ __forceinline__ void stage_store(const Stage64& r, double* lds, int stride) {
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int c = threadIdx.x + kT64 * h;
+        *reinterpret_cast<dbl2*>(lds + (c >> 5) * stride + ((c & 31) << 1)) = r.v[h];
+    }
+}
+
+struct Tile64 {
+    int li, lk, R0, C0, ebase;
+    __device__ Tile64() {
+        const int wave = threadIdx.x >> 6, lane = threadIdx.x & 63;
+        li = lane & 15;
+        lk = lane >> 4;
+        R0 = 16 * (wave >> 2);
+        C0 = 16 * (wave & 3);
+        ebase = (R0 + lk) * 64 + C0 + li;
+    }
+};
+// entry r of a lane: row R0 + 4 r + lk, column C0 + li
+__host__ __device__ constexpr int off64(int r) { return r * 256; }
+
+template <bool RAW>
+__device__ __forceinline__ void fwd_walk64(const FwdWalk& w, double* lds) {
+    constexpr int M = 4096;
+    const Tile64 hd;
+    const long long S = w.S, s0 = w.s0, s1 = w.s1, wb = w.wbase;
+    const double dt = w.dt;
+    const int eb = hd.ebase;
+    const double* A = w.A;
+    double* Wout = w.Wout;
+    const bool keep_sum = w.sum != nullptr;
+    PrefixState<BOOLE> st[4];
+    double sum[4] = {0.0, 0.0, 0.0, 0.0};
+    long long i0 = s0;
+    if (w.resume) {
+        i0 = s0 + 1;
+        if (Wout) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) {
+                const double P = w.ckin[hd.ebase + off64(q)], f = w.ckin[M + hd.ebase + off64(q)];
+                st[q].start(f, P);
+                if (w.restart_row) Wout[(s0 - wb) * M + (eb + off64(q))] = P;
+            }
+        }
+        if (keep_sum) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) sum[q] = w.sum[hd.ebase + off64(q)];
+        }
+    }
+    auto consume = [&](long long i, const double (&f)[4]) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            if (keep_sum) sum_acc<BOOLE>(sum[q], i, S, f[q], dt);
+            if (Wout) Wout[(i - wb) * M + (eb + off64(q))] = st[q].step(i, f[q], dt);
+        }
+    };
+    if constexpr (RAW) {
+        struct Raw {
+            double a[4];
+        };
+        auto load = [&](long long i, Raw& o) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) o.a[q] = A[i * M + (eb + off64(q))];
+        };
+        auto step = [&](long long i, const Raw& o, auto) { consume(i, o.a); };
+        Raw o[4];
+        walk_range<4, 4, true>(i0, s1, o, load, step);
+    } else if (i0 <= s1) {
+        const double* Ar = w.A;
+        const double* Win = w.Win;
+        Stage64 ra, rw;
+        stage_load(ra, Ar + i0 * M);
+        stage_load(rw, Win + (i0 - wb) * M);
+        stage_store(ra, lds, kP_ROW);
+        stage_store(rw, lds + kS_ROW, kP_COL);
+        lds_barrier();
+        int cur = 0;
+        for (long long i = i0; i <= s1; ++i) {
+            const long long nx = i < s1 ? i + 1 : i;  // past the end: a reload nobody reads
+            stage_load(ra, Ar + nx * M);
+            stage_load(rw, Win + (nx - wb) * M);
+            const double* la = lds + cur * (kS_ROW + kS_COL);
+            const double* lw = la + kS_ROW;
+            dbl4 acc = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 4
+            for (int s = 0; s < 16; ++s) {
+                const int kk = 4 * s + hd.lk;
+                acc = __builtin_amdgcn_mfma_f64_16x16x4f64(la[(hd.R0 + hd.li) * kP_ROW + kk],
+                                                           lw[kk * kP_COL + hd.C0 + hd.li], acc, 0, 0, 0);
+            }
+            const double f[4] = {acc[0], acc[1], acc[2], acc[3]};
+            consume(i, f);
+            double* ln = lds + (cur ^ 1) * (kS_ROW + kS_COL);
+            stage_store(ra, ln, kP_ROW);
+            stage_store(rw, ln + kS_ROW, kP_COL);
+            lds_barrier();
+            cur ^= 1;
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        const int e = hd.ebase + off64(q);
+        if (keep_sum) w.sum[e] = sum[q];
+        if (w.z0) w.z0[e] = sum[q];
+        if (w.ckout) {
+            w.ckout[e] = st[q].base;
+            w.ckout[M + e] = st[q].f0;
+        }
+        if (w.total) w.total[e] = st[q].base;
+    }
+}
+
+template <bool TOP>
+__device__ __forceinline__ void rev_walk64(const RevWalk& w, double* lds) {
+    constexpr int M = 4096;
+    constexpr int LA = 0, LW = kS_COL, LY = kS_COL + kS_ROW;  // A (read down columns), W, barY
+    const Tile64 hd;
+    const long long S = w.S, s0 = w.s0, itop = w.itop;
+    const int n = w.n, k = w.k;
+    const double dt = w.dt;
+    const bool last_seg = w.last_seg;
+    const int eb = hd.ebase;
+    const double* A = w.A;
+    const double* Wl = w.Wl - w.wbase * M;
+    const double* U = w.U - w.ubase * M;
+    // uniform bases, the lane's entry as a 32-bit offset (scalar base + vector offset addressing)
+    double* Uo = w.Uo - w.ubase * M;
+    double* dAs = w.dAs - w.dbase * M;
+    const bool addo = k > 2;
+    double osum[4];
+#pragma unroll
+    for (int q = 0; q < 4; ++q) osum[q] = addo ? w.zb[(k - 2) * M + eb + off64(q)] : 0.0;
+    auto fin = [&](double v, long long i, int q) { return addo ? sum_vjp_add<BOOLE>(v, i, S, osum[q], dt) : v; };
+    double g1[4], g2[4], g3[4], g4[4], pend[4];
+    if (!last_seg) {
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            g4[q] = w.rs[eb + off64(q)];
+            pend[q] = w.rs[M + eb + off64(q)];
+        }
+    }
+    // the next sample's share in registers (its loads in flight during a step)
+    Stage64 ra, rw, ru;
+    auto gload = [&](long long i) {
+        stage_load(ra, A + i * M);
+        stage_load(rw, Wl + i * M);
+        if constexpr (!TOP) stage_load(ru, U + i * M);
+    };
+    auto lstore = [&]() {
+        stage_store(ra, lds + LA, kP_COL);
+        stage_store(rw, lds + LW, kP_ROW);
+        if constexpr (!TOP) stage_store(ru, lds + LY, kP_ROW);
+    };
+    if constexpr (TOP) {  // barY^(n) = Zbar_{n-1} c_i: the matrix stays in LDS, c_i scales the operands
+        stage_load(ru, w.zb + (long long)(n - 1) * M);
+        stage_store(ru, lds + LY, kP_ROW);
+    }
+    // POS: the sample's position in its Boole block, 4 for the last sample S - 1
+    auto step_at = [&](long long i, auto posc) {
+        constexpr int POS = decltype(posc)::value;
+        const double cy = TOP ? sum_vjp_coef<BOOLE>(i, S, dt) : 1.0;
+        double da[4];  // dA so far at the lane's entries: loaded here, needed after the products
+        if constexpr (!TOP) {
+#pragma unroll
+            for (int q = 0; q < 4; ++q) da[q] = dAs[i * M + (eb + off64(q))];
+        }
+        const double* la = lds + LA;
+        const double* lw = lds + LW;
+        const double* ly = lds + LY;
+        dbl4 dav = {0.0, 0.0, 0.0, 0.0}, tmp = {0.0, 0.0, 0.0, 0.0};
+#pragma unroll 1
+        for (int s = 0; s < 16; ++s) {
+            const int kk = 4 * s + hd.lk;
+            // dA += barY W^T: barY row R0 + li, W row C0 + li
+            double y = ly[(hd.R0 + hd.li) * kP_ROW + kk];
+            // temp = A^T barY: A column R0 + li, barY column C0 + li
+            double u = ly[kk * kP_ROW + hd.C0 + hd.li];
+            if constexpr (TOP) {
+                y *= cy;
+                u *= cy;
+            }
+            dav = __builtin_amdgcn_mfma_f64_16x16x4f64(y, lw[(hd.C0 + hd.li) * kP_ROW + kk], dav, 0, 0, 0);
+            tmp = __builtin_amdgcn_mfma_f64_16x16x4f64(la[kk * kP_COL + hd.R0 + hd.li], u, tmp, 0, 0, 0);
+        }
+        const double dvv[4] = {dav[0], dav[1], dav[2], dav[3]};
+        const double tvv[4] = {tmp[0], tmp[1], tmp[2], tmp[3]};
+#pragma unroll
+        for (int q = 0; q < 4; ++q) {
+            const double tv = tvv[q];
+            dAs[i * M + (eb + off64(q))] = TOP ? dvv[q] : da[q] + dvv[q];
+            if constexpr (POS == 4) {
+                g4[q] = tv + w.wbar[eb + off64(q)];
+                pend[q] = 0.0;
+            } else if constexpr (POS == 3) {
+                g3[q] = tv;
+            } else if constexpr (POS == 2) {
+                g2[q] = tv;
+            } else if constexpr (POS == 1) {
+                g1[q] = tv;
+            } else {
+                double a0 = tv;
+                a0 += g1[q];
+                a0 += g2[q];
+                a0 += g3[q];
+                a0 += g4[q];
+                double a1 = g1[q] * (dt / 2);
+                a1 += g2[q] * (4 * dt / 3);
+                a1 += g3[q] * (9 * dt / 8);
+                a1 += g4[q] * (64 * dt / 45);
+                double a2 = g2[q] * (dt / 3);
+                a2 += g3[q] * (9 * dt / 8);
+                a2 += g4[q] * (24 * dt / 45);
+                double a3 = g3[q] * (3 * dt / 8);
+                a3 += g4[q] * (64 * dt / 45);
+                const double a4 = top_out(g4[q], pend[q], dt);
+                double pp = g1[q] * (dt / 2);
+                pp += g2[q] * (dt / 3);
+                pp += g3[q] * (3 * dt / 8);
+                pp += g4[q] * (14 * dt / 45);
+                pend[q] = pp;
+                g4[q] = a0;
+                Uo[(i + 1) * M + (eb + off64(q))] = fin(a1, i + 1, q);
+                Uo[(i + 2) * M + (eb + off64(q))] = fin(a2, i + 2, q);
+                Uo[(i + 3) * M + (eb + off64(q))] = fin(a3, i + 3, q);
+                Uo[(i + 4) * M + (eb + off64(q))] = fin(a4, i + 4, q);
+            }
+        }
+    };
+    // samples itop .. s0; the last segment's sample S - 1 first, on its own, the
+    // rest in whole Boole blocks from a block's last inner sample
+    long long top = itop;
+    gload(itop);
+    lstore();
+    lds_barrier();
+    if (last_seg) {
+        gload(itop - 1 >= s0 ? itop - 1 : itop);
+        step_at(itop, std::integral_constant<int, 4>{});
+        lds_barrier();
+        lstore();
+        lds_barrier();
+        top = itop - 1;
+    }
+    const long long span = top - s0 + 1;  // a multiple of 4
+    for (long long b0 = 0; b0 < span; b0 += 4) {
+        static_for<0, 4>([&](auto uc) {
+            constexpr int u = decltype(uc)::value;
+            const long long ii = top - b0 - u;
+            const long long nx = ii - 1 >= s0 ? ii - 1 : ii;  // past the end: a reload nobody reads
+            gload(nx);
+            step_at(ii, std::integral_constant<int, 3 - u>{});
+            lds_barrier();
+            lstore();
+            lds_barrier();
+        });
+    }
+#pragma unroll
+    for (int q = 0; q < 4; ++q) {
+        if (w.first_seg) {
+            Uo[eb + off64(q)] = fin(pend[q], 0, q);
+        } else {
+            Uo[s0 * M + (eb + off64(q))] = fin(top_out(g4[q], pend[q], dt), s0, q);
+            w.rs[eb + off64(q)] = g4[q];
+            w.rs[M + eb + off64(q)] = pend[q];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
 // Grouped walkers: one workgroup per task (instance, segment g, level) of a
 // task table, each instance with its own samples, order and full basis levels.
 // ---------------------------------------------------------------------------
@@ -524,15 +822,97 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) gw_rev(GwArgs a) {
     else rev_walk<D, false>(w);
 }
 
+__global__ void __launch_bounds__(kT64) gw_fwd64(GwArgs a) {
+    constexpr int M = 4096;
+    __shared__ alignas(16) double lds[kLds64Fwd];
+    const GwTask t = a.tasks[blockIdx.x];
+    const GwInst& in = a.inst[t.inst];
+    const long long S = in.S, L = a.L;
+    const int n = in.n, j = t.level;
+    const long long s0 = (long long)t.seg * L, s1 = gw_seg_end(s0, L, S);
+    const bool pref = j < n, last = s1 == S - 1;
+    FwdWalk w;
+    w.A = a.tab[in.id];
+    w.Wout = pref ? in.W + (long long)(j - 1) * S * M : nullptr;
+    w.Win = j >= 2 ? in.W + (long long)(j - 2) * S * M : nullptr;
+    w.wbase = 0;
+    w.ckin = pref ? in.fst + (long long)(j - 1) * 2 * M : nullptr;
+    w.ckout = (pref && !last) ? in.fst + (long long)(j - 1) * 2 * M : nullptr;
+    w.sum = in.sums + (long long)(j - 1) * M;
+    w.z0 = (j == 1 && last) ? in.z0 : nullptr;
+    w.total = (pref && last) ? in.totals + (long long)(j - 1) * M : nullptr;
+    w.S = S;
+    w.s0 = s0;
+    w.s1 = s1;
+    w.resume = t.seg > 0;
+    w.restart_row = false;
+    w.dt = in.dt;
+    if (j == 1) fwd_walk64<true>(w, lds);
+    else fwd_walk64<false>(w, lds);
+}
+
+__global__ void __launch_bounds__(kT64) gw_rev64(GwArgs a) {
+    constexpr int M = 4096;
+    __shared__ alignas(16) double lds[kLds64Rev];
+    const GwTask t = a.tasks[blockIdx.x];
+    const GwInst& in = a.inst[t.inst];
+    const long long S = in.S, L = a.L, L1 = L + 1;
+    const int n = in.n, g = t.seg;
+    const long long s0 = (long long)g * L, s1 = gw_seg_end(s0, L, S);
+    const bool last_seg = s1 == S - 1;
+    const long long itop = last_seg ? S - 1 : s1 - 1;
+    double* dA = const_cast<double*>(a.tab[2 * a.count + in.id]);
+    const int slot = g & 1;
+    if (t.kind == 2) {
+        const double* U1 = in.U + ((long long)slot * L1) * M;
+        const long long per = (itop - s0 + 1) * M;
+        for (long long x = threadIdx.x; x < per; x += kT64) {
+            const long long r = x / M;
+            const int e = (int)(x - r * M);
+            const long long i = s0 + r;
+            double v = in.zbar[e] * sum_vjp_coef<BOOLE>(i, S, in.dt);
+            double d = dA[i * M + e];
+            d += U1[r * M + e];
+            v += d;
+            dA[i * M + e] = v;
+        }
+        return;
+    }
+    const int k = t.level;
+    RevWalk w;
+    w.A = a.tab[in.id];
+    w.Wl = in.W + (long long)(k - 2) * S * M;
+    w.wbase = 0;
+    w.Uo = in.U + (((long long)(k - 2) * 2 + slot) * L1) * M;
+    w.U = k < n ? in.U + (((long long)(k - 1) * 2 + slot) * L1) * M : w.Uo;
+    w.ubase = s0;
+    w.dAs = dA;
+    w.dbase = 0;
+    w.zb = in.zbar;
+    w.wbar = in.wbar + (long long)(k - 2) * M;
+    w.rs = in.rst + (long long)(k - 2) * 2 * M;
+    w.S = S;
+    w.s0 = s0;
+    w.itop = itop;
+    w.n = n;
+    w.k = k;
+    w.last_seg = last_seg;
+    w.first_seg = g == 0;
+    w.dt = in.dt;
+    if (k == n) rev_walk64<true>(w, lds);
+    else rev_walk64<false>(w, lds);
+}
+
 }  // namespace
 
-bool gw_ok(int d, int integ, int n) { return integ == BOOLE && (d == 16 || d == 32) && n >= 2; }
+bool gw_ok(int d, int integ, int n) { return integ == BOOLE && (d == 16 || d == 32 || d == 64) && n >= 2; }
 
 hipError_t launch_gw_forward(int d, const GwArgs& a, int ntasks, double bytes, double flops, hipStream_t s) {
     if (ntasks <= 0) return hipSuccess;
     void* h = timer_begin("gw_fwd", s, bytes, flops);
     if (d == 16) hipLaunchKernelGGL((gw_fwd<16>), dim3(ntasks), dim3(SegShape<16>::THREADS), 0, s, a);
-    else hipLaunchKernelGGL((gw_fwd<32>), dim3(ntasks), dim3(SegShape<32>::THREADS), 0, s, a);
+    else if (d == 32) hipLaunchKernelGGL((gw_fwd<32>), dim3(ntasks), dim3(SegShape<32>::THREADS), 0, s, a);
+    else hipLaunchKernelGGL(gw_fwd64, dim3(ntasks), dim3(kT64), 0, s, a);
     timer_end(h);
     return hipGetLastError();
 }
@@ -541,7 +921,8 @@ hipError_t launch_gw_reverse(int d, const GwArgs& a, int ntasks, double bytes, d
     if (ntasks <= 0) return hipSuccess;
     void* h = timer_begin("gw_rev", s, bytes, flops);
     if (d == 16) hipLaunchKernelGGL((gw_rev<16>), dim3(ntasks), dim3(SegShape<16>::THREADS), 0, s, a);
-    else hipLaunchKernelGGL((gw_rev<32>), dim3(ntasks), dim3(SegShape<32>::THREADS), 0, s, a);
+    else if (d == 32) hipLaunchKernelGGL((gw_rev<32>), dim3(ntasks), dim3(SegShape<32>::THREADS), 0, s, a);
+    else hipLaunchKernelGGL(gw_rev64, dim3(ntasks), dim3(kT64), 0, s, a);
     timer_end(h);
     return hipGetLastError();
 }

@@ -1,7 +1,7 @@
 // uneven.cpp -- uneven-batch plans: many matrix problems of different dim,
 // samples and order solved by one enqueue (or one hipGraph launch).
 //
-// Schedule.  Dense W-basis problems of dim 16 or 32 under the Boole rule (op
+// Schedule.  Dense W-basis problems of dim 16, 32 or 64 under the Boole rule (op
 // many or sum, n >= 2) run as one grouped pipeline: every such instance, of
 // any samples and order, is a set of tasks (instance, segment of samples,
 // level) of the grouped walkers, and one launch per pipeline step and dim
@@ -86,10 +86,11 @@ std::vector<int> tile_candidates(const Plan& pl) {
 }
 
 // ---- grouped walker pipeline -------------------------------------------------
-constexpr int kGwDims = 2;  // dim 16, 32
+constexpr int kGwDims = 3;  // dim 16, 32, 64
+constexpr int kGwDim[kGwDims] = {16, 32, 64};
 constexpr int kGwL = 128;   // segment length in samples (a multiple of the Boole block)
 
-int gw_dim_index(uint64_t d) { return d == 16 ? 0 : 1; }
+int gw_dim_index(uint64_t d) { return d == 16 ? 0 : (d == 32 ? 1 : 2); }
 bool gw_eligible(const Plan& pl) {
     return !pl.sc && pl.op != MAGNUS_OP_ONE && magnus::gw_ok((int)pl.d, pl.integ, (int)pl.n);
 }
@@ -264,7 +265,7 @@ int gw_layout(magnus_uneven_plan* p, size_t& ids_total, std::vector<magnus::GwIn
         wv.bytes = off;
         // task tables: per dim, steps in order
         for (int di = 0; di < kGwDims; ++di) {
-            const double d = di == 0 ? 16.0 : 32.0, m = d * d;
+            const double d = kGwDim[di], m = d * d;
             std::vector<std::vector<magnus::GwTask>> fs(steps_f[di]), rs(steps_r[di]);
             std::vector<double> fb(steps_f[di], 0.0), ff(steps_f[di], 0.0), rb(steps_r[di], 0.0), rf(steps_r[di], 0.0);
             for (auto& q : pend) {
@@ -495,12 +496,17 @@ int enqueue_gw(magnus_uneven_plan* p, bool vjp, hipStream_t s) {
                 if (e != hipSuccess) return device_error(e, "uneven pipeline gather");
             }
         }
-        for (size_t t = 0; t + 1 < std::max(wv.fwd_off[0].size(), wv.fwd_off[1].size()); ++t)
+        size_t fsteps = 0, rsteps = 0;
+        for (int di = 0; di < kGwDims; ++di) {
+            fsteps = std::max(fsteps, wv.fwd_off[di].size());
+            rsteps = std::max(rsteps, wv.rev_off[di].size());
+        }
+        for (size_t t = 0; t + 1 < fsteps; ++t)
             for (int di = 0; di < kGwDims; ++di) {
                 if (t + 1 >= wv.fwd_off[di].size()) continue;
                 ga.tasks = p->d_tasks + wv.fwd_base[di] + wv.fwd_off[di][t];
                 const int cnt = (int)(wv.fwd_off[di][t + 1] - wv.fwd_off[di][t]);
-                e = magnus::launch_gw_forward(di == 0 ? 16 : 32, ga, cnt, wv.fwd_bytes[di][t], wv.fwd_flops[di][t], s);
+                e = magnus::launch_gw_forward(kGwDim[di], ga, cnt, wv.fwd_bytes[di][t], wv.fwd_flops[di][t], s);
                 if (e != hipSuccess) return device_error(e, "uneven pipeline forward");
             }
         for (auto& grp : wv.groups) {
@@ -533,12 +539,12 @@ int enqueue_gw(magnus_uneven_plan* p, bool vjp, hipStream_t s) {
             }
         }
         if (!vjp) continue;
-        for (size_t t = 0; t + 1 < std::max(wv.rev_off[0].size(), wv.rev_off[1].size()); ++t)
+        for (size_t t = 0; t + 1 < rsteps; ++t)
             for (int di = 0; di < kGwDims; ++di) {
                 if (t + 1 >= wv.rev_off[di].size()) continue;
                 ga.tasks = p->d_tasks + wv.rev_base[di] + wv.rev_off[di][t];
                 const int cnt = (int)(wv.rev_off[di][t + 1] - wv.rev_off[di][t]);
-                e = magnus::launch_gw_reverse(di == 0 ? 16 : 32, ga, cnt, wv.rev_bytes[di][t], wv.rev_flops[di][t], s);
+                e = magnus::launch_gw_reverse(kGwDim[di], ga, cnt, wv.rev_bytes[di][t], wv.rev_flops[di][t], s);
                 if (e != hipSuccess) return device_error(e, "uneven pipeline reverse");
             }
     }
@@ -669,40 +675,60 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
         workspace_bytes = fr / 2;
     }
     p->budget = workspace_bytes;
+    // The pipeline runs in one wave (its steps are efficient only when they
+    // hold many tasks): eligible instances join it in order of their chain of
+    // steps (about S + (n - 1) L samples, each costing more at a larger dim),
+    // shortest first, while their full levels fit in the workspace left after
+    // the branches' reserve (a quarter, and at least one instance of the largest
+    // remaining bucket per branch); the others, the longest chains, run in
+    // buckets on the level kernels, parallel over samples.
     p->gw_plans.assign(count, Plan{});
+    std::vector<uint32_t> cand;
+    for (uint64_t i = 0; i < count; ++i)
+        if (gw_eligible(plans[i])) cand.push_back((uint32_t)i);
+    auto chain = [&](uint32_t i) {
+        const Plan& pl = plans[i];
+        return (double)(pl.S + (pl.n - 1) * kGwL) * (double)pl.d * pl.d * pl.d;
+    };
+    std::stable_sort(cand.begin(), cand.end(), [&](uint32_t a, uint32_t b) { return chain(a) < chain(b); });
+    std::vector<char> in_gw(count, 0);
+    {
+        size_t noncand_max = 0;
+        bool noncand = false;
+        for (uint64_t k = 0; k < count; ++k)
+            if (!gw_eligible(plans[k])) {
+                noncand = true;
+                noncand_max = std::max(noncand_max, item_bytes(plans[k], 1));
+            }
+        std::vector<size_t> suffix(cand.size() + 1, 0);  // largest bucket instance among cand[c..]
+        for (size_t c = cand.size(); c-- > 0;) suffix[c] = std::max(suffix[c + 1], item_bytes(plans[cand[c]], 1));
+        size_t used = 0;
+        for (size_t c = 0; c < cand.size(); ++c) {
+            const bool left = noncand || c + 1 < cand.size();  // instances left for the branches
+            const size_t res = left ? std::max(workspace_bytes / 4,
+                                               std::max(noncand_max, suffix[c + 1]) * (size_t)branches)
+                                    : 0;
+            const size_t b = gw_inst_bytes(plans[cand[c]], kGwL);
+            if (workspace_bytes < res || used + b > workspace_bytes - res) break;
+            used += b;
+            in_gw[cand[c]] = 1;
+        }
+        p->gw_budget = used;
+    }
     bool others = false;
     for (uint64_t i = 0; i < count; ++i) {
-        if (gw_eligible(plans[i])) {
+        if (in_gw[i]) {
             p->gw_plans[i] = plans[i];
             p->gw_ids.push_back((uint32_t)i);
         } else {
             others = true;
         }
     }
-    // The pipeline takes what it needs in one wave, at most half the workspace
-    // when the branches have work too, and never what the branches need for one
-    // instance of their largest bucket each (while its own largest instance fits).
-    size_t gw_need = 0, gw_min = 0, br_min = 0;
-    for (uint32_t i : p->gw_ids) {
-        gw_need += gw_inst_bytes(plans[i], kGwL);
-        gw_min = std::max(gw_min, gw_inst_bytes(plans[i], kGwL));
-    }
-    for (uint64_t i = 0; i < count; ++i)
-        if (!gw_eligible(plans[i])) br_min = std::max(br_min, item_bytes(plans[i], 1));
-    if (p->gw_ids.empty()) {
-        p->gw_budget = 0;
-    } else if (!others) {
-        p->gw_budget = workspace_bytes;
-    } else {
-        size_t gb = std::min<size_t>(gw_need, workspace_bytes / 2);
-        const size_t br_total = br_min * (size_t)branches;
-        if (workspace_bytes > br_total && workspace_bytes - br_total < gb) gb = workspace_bytes - br_total;
-        p->gw_budget = std::max(gb, std::min(gw_min, workspace_bytes));
-    }
+    if (!others) p->gw_budget = workspace_bytes;
     const size_t per_branch = (workspace_bytes - p->gw_budget) / branches;
     std::map<std::tuple<uint64_t, uint64_t, uint64_t, int, int, int, uint64_t>, int> keys;
     for (uint64_t i = 0; i < count; ++i) {
-        if (gw_eligible(plans[i])) continue;
+        if (in_gw[i]) continue;
         const Plan& pl = plans[i];
         uint64_t dtb;
         std::memcpy(&dtb, &pl.dt, sizeof dtb);

@@ -186,7 +186,7 @@ int magnus_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* 
  *      reference; north-star "checkpointed prefix carries in place of full
  *      vjp_data when memory requires it").  The VJP of magnus_vjp_f64 with
  *      carry == NULL (VJP::none, src/grad.hpp:95-110 with
- *      src/composer/backends.hpp:222-251) for dense problems of dim 16 or 32
+ *      src/composer/backends.hpp:222-251) for dense problems of dim 16, 32 or 64
  *      under the Boole rule with op "many" or "sum" and n >= 2 (EINVAL
  *      otherwise), keeping the basis levels for one segment of samples at a
  *      time: the first pass keeps each level's prefix state at every segment
@@ -244,13 +244,16 @@ int magnus_sc_expm_f64(uint64_t count, const double* in3, double* out, void* str
 
 /* ---- uneven batches (no counterpart in the reference, which solves one problem
  *      per call).  A plan holds `count` matrix problems of any dim, samples,
- *      order, op and integrator.  Dense problems of dim 16 or 32 under the
+ *      order, op and integrator.  Dense problems of dim 16, 32 or 64 under the
  *      Boole rule with op "many" or "sum" and n >= 2 run as one grouped
  *      pipeline: each is cut into segments of samples, and one launch per
  *      pipeline step and dim runs one level of one segment of every such
  *      instance that the step reaches (grouped walkers; the GL fold once per
- *      (dim, order, op) group); instances that do not fit the pipeline's share
- *      of the workspace together run in consecutive waves.  The other problems
+ *      (dim, order, op) group).  They join it in order of their chain of steps
+ *      (samples + (n - 1) 128, weighted by dim^3), shortest first, while their
+ *      basis levels fit in the workspace left after a reserve for the branches
+ *      (a quarter, and at least one instance per branch of the largest bucket
+ *      left); the rest, the longest chains, run in buckets.  The other problems
  *      are grouped into buckets of equal shape, each bucket split into
  *      sub-batches that fit its workspace and spread over `branches` HIP
  *      streams (longest first), each sub-batch a gather of the instances'

@@ -1340,8 +1340,8 @@ int magnus_vjp_segmented_f64(const magnus_problem* p, uint64_t count, const doub
     Plan pl;
     if (int rc = make_plan(p, false, false, &pl)) return rc;
     if (!use_walkseg(pl))
-        return fail(MAGNUS_EINVAL, "segmented VJP needs a dense W-basis problem (op many or sum, n >= 2) with dim 16 "
-                                   "or 32 under the Boole rule");
+        return fail(MAGNUS_EINVAL, "segmented VJP needs a dense W-basis problem (op many or sum, n >= 2) with dim 16, "
+                                   "32 or 64 under the Boole rule");
     if (count == 0) return MAGNUS_OK;
     if (!in || !cot || !dA) return fail(MAGNUS_EINVAL, "null argument");
     return run_wbd_vjp_seg(pl, count, in, cot, dA, segments, (hipStream_t)stream);

@@ -1,5 +1,5 @@
-// kernels_walkseg.hip -- the dense W-basis VJP with time-segment checkpoints,
-// d in {16, 32}, Boole.
+// kernels_walkseg.hip -- the dense W-basis VJP with time-segment checkpoints
+// and the grouped walker pipeline of uneven batches, d in {16, 32, 64}, Boole.
 //
 // The W-basis VJP (one basis sweep W_j = Prefix(A W_{j-1}) for all GL nodes,
 // the fold's adjoint, then a reverse level loop) keeps the K = n - 1 basis
@@ -175,13 +175,10 @@ __device__ __forceinline__ void fwd_walk(const FwdWalk& w) {
     }
 }
 
-// Level j of segment g of every instance (one workgroup each): mode 0 is the
-// first pass (sums, checkpoints, totals), mode 1 rebuilds the segment's levels.
-template <int D, bool RAW>
-__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
-    constexpr int M = SegShape<D>::M;
-    const long long b = blockIdx.x;
-    const long long S = a.S, L1 = a.L + 1;
+// Level j of segment g of instance b: mode 0 is the first pass (sums,
+// checkpoints, totals), mode 1 rebuilds the segment's levels.
+__device__ __forceinline__ FwdWalk seg_fwd_walk(const WalkSegArgs& a, long long b) {
+    const long long S = a.S, L1 = a.L + 1, M = a.m;
     const int n = a.n, K = n - 1, j = a.level, g = a.g, G = a.G;
     const bool pref = j < n, first = a.mode == 0, last = a.s1 == S - 1;
     FwdWalk w;
@@ -200,7 +197,13 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
     w.resume = g > 0;
     w.restart_row = true;
     w.dt = a.dt;
-    fwd_walk<D, RAW>(w);
+    return w;
+}
+
+// one workgroup per instance
+template <int D, bool RAW>
+__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_fwd(WalkSegArgs a) {
+    fwd_walk<D, RAW>(seg_fwd_walk(a, blockIdx.x));
 }
 
 // The top output of a reverse Boole block, g4 14 dt / 45 + pending: the block
@@ -387,11 +390,9 @@ __device__ __forceinline__ void rev_walk(const RevWalk& w) {
     }
 }
 
-template <int D, bool TOP>
-__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
-    constexpr int M = SegShape<D>::M;
-    const long long b = blockIdx.x;
-    const long long S = a.S, s0 = a.s0, L1 = a.L + 1;
+// Reverse level k of segment g of instance b
+__device__ __forceinline__ RevWalk seg_rev_walk(const WalkSegArgs& a, long long b) {
+    const long long S = a.S, s0 = a.s0, L1 = a.L + 1, M = a.m;
     const int n = a.n, K = n - 1, k = a.level, g = a.g, G = a.G;
     RevWalk w;
     w.A = a.in + b * S * M;
@@ -413,7 +414,12 @@ __global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
     w.n = n;
     w.k = k;
     w.dt = a.dt;
-    rev_walk<D, TOP>(w);
+    return w;
+}
+
+template <int D, bool TOP>
+__global__ void __launch_bounds__(SegShape<D>::THREADS) seg_rev(WalkSegArgs a) {
+    rev_walk<D, TOP>(seg_rev_walk(a, blockIdx.x));
 }
 
 // dA on the segment's own samples: sum_vjp(Zbar_0, dt) + (dA_job + barY^(1))
@@ -594,11 +600,12 @@ __device__ __forceinline__ void rev_walk64(const RevWalk& w, double* lds) {
     // uniform bases, the lane's entry as a 32-bit offset (scalar base + vector offset addressing)
     double* Uo = w.Uo - w.ubase * M;
     double* dAs = w.dAs - w.dbase * M;
+    // Zbar_{k-2} at the lane's entries, read where a block is finished (kept in
+    // registers, its products with the rule's weights would be hoisted out of
+    // the walk and outgrow the register budget)
     const bool addo = k > 2;
-    double osum[4];
-#pragma unroll
-    for (int q = 0; q < 4; ++q) osum[q] = addo ? w.zb[(k - 2) * M + eb + off64(q)] : 0.0;
-    auto fin = [&](double v, long long i, int q) { return addo ? sum_vjp_add<BOOLE>(v, i, S, osum[q], dt) : v; };
+    const double* zrow = w.zb + (long long)(k - 2) * M + eb;
+    auto fin = [&](double v, long long i, double os) { return addo ? sum_vjp_add<BOOLE>(v, i, S, os, dt) : v; };
     double g1[4], g2[4], g3[4], g4[4], pend[4];
     if (!last_seg) {
 #pragma unroll
@@ -687,10 +694,11 @@ __device__ __forceinline__ void rev_walk64(const RevWalk& w, double* lds) {
                 pp += g4[q] * (14 * dt / 45);
                 pend[q] = pp;
                 g4[q] = a0;
-                Uo[(i + 1) * M + (eb + off64(q))] = fin(a1, i + 1, q);
-                Uo[(i + 2) * M + (eb + off64(q))] = fin(a2, i + 2, q);
-                Uo[(i + 3) * M + (eb + off64(q))] = fin(a3, i + 3, q);
-                Uo[(i + 4) * M + (eb + off64(q))] = fin(a4, i + 4, q);
+                const double os = addo ? zrow[off64(q)] : 0.0;
+                Uo[(i + 1) * M + (eb + off64(q))] = fin(a1, i + 1, os);
+                Uo[(i + 2) * M + (eb + off64(q))] = fin(a2, i + 2, os);
+                Uo[(i + 3) * M + (eb + off64(q))] = fin(a3, i + 3, os);
+                Uo[(i + 4) * M + (eb + off64(q))] = fin(a4, i + 4, os);
             }
         }
     };
@@ -723,14 +731,27 @@ __device__ __forceinline__ void rev_walk64(const RevWalk& w, double* lds) {
     }
 #pragma unroll
     for (int q = 0; q < 4; ++q) {
+        const double os = addo ? zrow[off64(q)] : 0.0;
         if (w.first_seg) {
-            Uo[eb + off64(q)] = fin(pend[q], 0, q);
+            Uo[eb + off64(q)] = fin(pend[q], 0, os);
         } else {
-            Uo[s0 * M + (eb + off64(q))] = fin(top_out(g4[q], pend[q], dt), s0, q);
+            Uo[s0 * M + (eb + off64(q))] = fin(top_out(g4[q], pend[q], dt), s0, os);
             w.rs[eb + off64(q)] = g4[q];
             w.rs[M + eb + off64(q)] = pend[q];
         }
     }
+}
+
+template <bool RAW>
+__global__ void __launch_bounds__(kT64) seg_fwd64(WalkSegArgs a) {
+    __shared__ alignas(16) double lds[kLds64Fwd];
+    fwd_walk64<RAW>(seg_fwd_walk(a, blockIdx.x), lds);
+}
+
+template <bool TOP>
+__global__ void __launch_bounds__(kT64) seg_rev64(WalkSegArgs a) {
+    __shared__ alignas(16) double lds[kLds64Rev];
+    rev_walk64<TOP>(seg_rev_walk(a, blockIdx.x), lds);
 }
 
 // ---------------------------------------------------------------------------
@@ -927,7 +948,7 @@ hipError_t launch_gw_reverse(int d, const GwArgs& a, int ntasks, double bytes, d
     return hipGetLastError();
 }
 
-bool walkseg_ok(int d, int integ, int n) { return integ == BOOLE && (d == 16 || d == 32) && n >= 2; }
+bool walkseg_ok(int d, int integ, int n) { return integ == BOOLE && (d == 16 || d == 32 || d == 64) && n >= 2; }
 
 hipError_t launch_walkseg_forward(const WalkSegArgs& a, hipStream_t s) {
     if (a.B == 0) return hipSuccess;
@@ -937,7 +958,10 @@ hipError_t launch_walkseg_forward(const WalkSegArgs& a, hipStream_t s) {
     const double flops = (double)a.B * len * ((j >= 2 ? 2.0 * a.d * a.d * a.d : 0.0) + 8.0 * a.m);
     void* h = timer_begin("walkseg_fwd", s, bytes, flops);
     const dim3 grid((unsigned)a.B);
-    if (a.d == 16) {
+    if (a.d == 64) {
+        if (j == 1) hipLaunchKernelGGL(seg_fwd64<true>, grid, dim3(kT64), 0, s, a);
+        else hipLaunchKernelGGL(seg_fwd64<false>, grid, dim3(kT64), 0, s, a);
+    } else if (a.d == 16) {
         if (j == 1) hipLaunchKernelGGL((seg_fwd<16, true>), grid, dim3(SegShape<16>::THREADS), 0, s, a);
         else hipLaunchKernelGGL((seg_fwd<16, false>), grid, dim3(SegShape<16>::THREADS), 0, s, a);
     } else {
@@ -956,7 +980,10 @@ hipError_t launch_walkseg_reverse(const WalkSegArgs& a, hipStream_t s) {
     const double flops = (double)a.B * len * (4.0 * a.d * a.d * a.d + 16.0 * a.m);
     void* h = timer_begin("walkseg_rev", s, bytes, flops);
     const dim3 grid((unsigned)a.B);
-    if (a.d == 16) {
+    if (a.d == 64) {
+        if (top) hipLaunchKernelGGL(seg_rev64<true>, grid, dim3(kT64), 0, s, a);
+        else hipLaunchKernelGGL(seg_rev64<false>, grid, dim3(kT64), 0, s, a);
+    } else if (a.d == 16) {
         if (top) hipLaunchKernelGGL((seg_rev<16, true>), grid, dim3(SegShape<16>::THREADS), 0, s, a);
         else hipLaunchKernelGGL((seg_rev<16, false>), grid, dim3(SegShape<16>::THREADS), 0, s, a);
     } else {

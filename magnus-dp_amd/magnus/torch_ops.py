@@ -163,7 +163,7 @@ def vjp_basis(n: int, A: torch.Tensor, cot: torch.Tensor, state: torch.Tensor, t
 def vjp_segmented(n: int, A: torch.Tensor, cot: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
                   matrix_backend: str = "Auto", integrator: str = "Auto", segments: int = 0):
     """vjp() without a carry that keeps the basis levels for one time segment at
-    a time (dense dim 16 or 32, Boole, op many or sum, n >= 2); segments = 0
+    a time (dense dim 16, 32 or 64, Boole, op many or sum, n >= 2); segments = 0
     picks the segment length.  The result does not depend on the segments."""
     _check_input(A, 4, "A")
     A = A.contiguous()

@@ -126,7 +126,7 @@ def test_segmented_vjp_sizes_and_scratch_limit():
     assert L.magnus_vjp_segment_length(ctypes.byref(p), 1000) == 4
     assert L.magnus_vjp_segmented_scratch_bytes(ctypes.byref(p), 2, 0) > 0
     for kw in (dict(d=8, op="sum", integ="Boole"), dict(d=16, op="one", integ="Boole"),
-               dict(d=16, op="sum", integ="Simpson"), dict(d=64, op="sum", integ="Boole")):
+               dict(d=16, op="sum", integ="Simpson"), dict(d=48, op="sum", integ="Boole")):
         q = capi.problem(4, kw["d"], 65, 0.0, 1.0, op=kw["op"], integrator=kw["integ"])
         assert L.magnus_vjp_segmented_scratch_bytes(ctypes.byref(q), 2, 0) == 0, kw
     assert capi.scratch_limit() == 0

@@ -2,7 +2,7 @@
 
 magnus_vjp_segmented_f64 keeps the basis levels for one segment of samples at
 a time.  Its dA is checked against the oracle at 1e-12 (relative Frobenius)
-for dims 16 and 32, ops many and sum, orders 2..8, one segment up to one
+for dims 16, 32 and 64, ops many and sum, orders 2..8, one segment up to one
 segment per Boole block; any segment count gives the same dA bit for bit; and
 its scratch at S = 4097 and 8193 is a few percent of the n - 1 full levels the
 unsegmented VJP keeps."""
@@ -35,7 +35,7 @@ def numpy_gl_table():
     O.gl_reset()
 
 
-@pytest.mark.parametrize("d", [16, 32])
+@pytest.mark.parametrize("d", [16, 32, 64])
 @pytest.mark.parametrize("op", ["many", "sum"])
 @pytest.mark.parametrize("n,S,B", [(2, 9, 2), (4, 65, 3), (6, 33, 1), (8, 21, 2)])
 def test_segmented_vjp_matches_oracle_for_every_segment_count(d, op, n, S, B):
@@ -62,7 +62,7 @@ def test_segmented_vjp_matches_oracle_for_every_segment_count(d, op, n, S, B):
         assert rel_fro(got[b], ref) <= TOL, b
 
 
-@pytest.mark.parametrize("d,S,n", [(32, 4097, 6), (16, 8193, 6)])
+@pytest.mark.parametrize("d,S,n", [(32, 4097, 6), (16, 8193, 6), (64, 8193, 6)])
 def test_segmented_vjp_full_length_matches_the_unsegmented_vjp(d, S, n):
     """At the dense configurations' lengths: the segmented dA against the
     W-basis VJP that keeps every level (1e-12), and its scratch under 7 % of

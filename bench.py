@@ -771,7 +771,7 @@ def main():
     ap.add_argument("--ctrl-eval", type=int, default=10000)
     ap.add_argument("--ctrl-epochs", type=int, default=20)
     ap.add_argument("--uneven-branches", type=int, default=4)
-    ap.add_argument("--uneven-workspace-gb", type=float, default=96.0)
+    ap.add_argument("--uneven-workspace-gb", type=float, default=128.0)
     ap.add_argument("--batch", type=int, default=None, help="override the batch size (testing only)")
     ap.add_argument("--dim", type=int, default=32, help="dense workload: matrix dimension (profiling only)")
     ap.add_argument("--sc-residual", default="checkpoint", choices=["checkpoint", "recompute"],

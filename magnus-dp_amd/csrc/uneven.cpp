@@ -16,7 +16,8 @@
 // sized to the per-branch workspace; sub-batches are assigned longest-first to
 // the branch (HIP stream) with the least estimated work, and each branch owns
 // one arena sized for its largest sub-batch, reused in stream order.  The
-// pipeline runs on a stream of its own beside the branches.  All device memory
+// pipeline of each dim runs on a stream of its own beside the branches (the
+// dims' instances share nothing).  All device memory
 // (pointer tables, slot lists, arenas, task tables, the pipeline's workspace)
 // is allocated when the plan is prepared, so the enqueue makes no allocation
 // call and can be captured.
@@ -153,8 +154,8 @@ struct magnus_uneven_plan {
     magnus::GwInst* d_inst = nullptr;
     magnus::GwTask* d_tasks = nullptr;
     char* gw_ws = nullptr;
-    hipStream_t gw_stream = nullptr;
-    hipEvent_t gw_join = nullptr;
+    hipStream_t gw_stream[kGwDims] = {};  // one pipeline per dim: the dims' instances are independent
+    hipEvent_t gw_join[kGwDims] = {};
 };
 
 namespace {
@@ -475,78 +476,89 @@ int enqueue_item(magnus_uneven_plan* p, const Item& it, bool vjp, hipStream_t s)
     return MAGNUS_OK;
 }
 
-// the grouped pipeline, wave by wave, on stream s
-int enqueue_gw(magnus_uneven_plan* p, bool vjp, hipStream_t s) {
-    for (auto& wv : p->gw) {
+// the grouped pipeline, wave by wave; each dim's tasks, folds and copies on
+// that dim's stream (instances of different dims share nothing), the dims
+// joined between waves (a wave reuses the workspace of the one before)
+int enqueue_gw(magnus_uneven_plan* p, bool vjp) {
+    for (size_t wi = 0; wi < p->gw.size(); ++wi) {
+        GwWave& wv = p->gw[wi];
+        if (wi > 0) {
+            for (int di = 0; di < kGwDims; ++di) {
+                hipError_t e = hipEventRecord(p->gw_join[di], p->gw_stream[di]);
+                if (e != hipSuccess) return device_error(e, "uneven pipeline join");
+            }
+            for (int di = 0; di < kGwDims; ++di)
+                for (int dj = 0; dj < kGwDims; ++dj) {
+                    if (di == dj) continue;
+                    hipError_t e = hipStreamWaitEvent(p->gw_stream[di], p->gw_join[dj], 0);
+                    if (e != hipSuccess) return device_error(e, "uneven pipeline join");
+                }
+        }
         magnus::GwArgs ga{};
         ga.inst = p->d_inst + wv.inst_off;
         ga.tab = p->d_tab;
         ga.count = (long long)p->count;
         ga.L = kGwL;
         hipError_t e = hipSuccess;
-        if (vjp) {
-            for (auto& grp : wv.groups) {
-                magnus::SegmentArgs g{};
-                g.ids = p->d_ids + grp.ids_off;
-                g.count = (int)grp.ids.size();
-                g.tab = p->d_tab + p->count;
-                g.packed = reinterpret_cast<double*>(p->gw_ws + grp.cot);
-                g.per = (long long)(grp.pl.n_out * grp.pl.m_cot);
-                e = magnus::launch_gather(g, s);
-                if (e != hipSuccess) return device_error(e, "uneven pipeline gather");
-            }
-        }
-        size_t fsteps = 0, rsteps = 0;
         for (int di = 0; di < kGwDims; ++di) {
-            fsteps = std::max(fsteps, wv.fwd_off[di].size());
-            rsteps = std::max(rsteps, wv.rev_off[di].size());
-        }
-        for (size_t t = 0; t + 1 < fsteps; ++t)
-            for (int di = 0; di < kGwDims; ++di) {
-                if (t + 1 >= wv.fwd_off[di].size()) continue;
+            hipStream_t s = p->gw_stream[di];
+            if (vjp) {
+                for (auto& grp : wv.groups) {
+                    if ((int)grp.pl.d != kGwDim[di]) continue;
+                    magnus::SegmentArgs g{};
+                    g.ids = p->d_ids + grp.ids_off;
+                    g.count = (int)grp.ids.size();
+                    g.tab = p->d_tab + p->count;
+                    g.packed = reinterpret_cast<double*>(p->gw_ws + grp.cot);
+                    g.per = (long long)(grp.pl.n_out * grp.pl.m_cot);
+                    e = magnus::launch_gather(g, s);
+                    if (e != hipSuccess) return device_error(e, "uneven pipeline gather");
+                }
+            }
+            for (size_t t = 0; t + 1 < wv.fwd_off[di].size(); ++t) {
                 ga.tasks = p->d_tasks + wv.fwd_base[di] + wv.fwd_off[di][t];
                 const int cnt = (int)(wv.fwd_off[di][t + 1] - wv.fwd_off[di][t]);
                 e = magnus::launch_gw_forward(kGwDim[di], ga, cnt, wv.fwd_bytes[di][t], wv.fwd_flops[di][t], s);
                 if (e != hipSuccess) return device_error(e, "uneven pipeline forward");
             }
-        for (auto& grp : wv.groups) {
-            magnus::WbdArgs f{};
-            fill_wbd_fold(grp.pl, (long long)grp.ids.size(), f);
-            f.z = reinterpret_cast<double*>(p->gw_ws + grp.z);
-            f.z0 = reinterpret_cast<double*>(p->gw_ws + grp.z0);
-            f.totals = reinterpret_cast<double*>(p->gw_ws + grp.totals);
-            f.part = reinterpret_cast<double*>(p->gw_ws + grp.part);
-            f.work = reinterpret_cast<double*>(p->gw_ws + grp.work);
-            if (vjp) {
-                f.cot = reinterpret_cast<double*>(p->gw_ws + grp.cot);
-                f.zbar = reinterpret_cast<double*>(p->gw_ws + grp.zbar);
-                f.wbar = reinterpret_cast<double*>(p->gw_ws + grp.wbar);
-                f.part2 = reinterpret_cast<double*>(p->gw_ws + grp.part2);
-                e = magnus::launch_wbd_fold_vjp(f, s);
-                if (e != hipSuccess) return device_error(e, "uneven pipeline fold");
-            } else {
-                f.out = reinterpret_cast<double*>(p->gw_ws + grp.out);
-                e = magnus::launch_wbd_fold(f, s);
-                if (e != hipSuccess) return device_error(e, "uneven pipeline fold");
-                magnus::SegmentArgs g{};
-                g.ids = p->d_ids + grp.ids_off;
-                g.count = (int)grp.ids.size();
-                g.tab = p->d_tab + p->count;
-                g.packed = f.out;
-                g.per = (long long)(grp.pl.n_out * grp.pl.m);
-                e = magnus::launch_scatter(g, s);
-                if (e != hipSuccess) return device_error(e, "uneven pipeline scatter");
+            for (auto& grp : wv.groups) {
+                if ((int)grp.pl.d != kGwDim[di]) continue;
+                magnus::WbdArgs f{};
+                fill_wbd_fold(grp.pl, (long long)grp.ids.size(), f);
+                f.z = reinterpret_cast<double*>(p->gw_ws + grp.z);
+                f.z0 = reinterpret_cast<double*>(p->gw_ws + grp.z0);
+                f.totals = reinterpret_cast<double*>(p->gw_ws + grp.totals);
+                f.part = reinterpret_cast<double*>(p->gw_ws + grp.part);
+                f.work = reinterpret_cast<double*>(p->gw_ws + grp.work);
+                if (vjp) {
+                    f.cot = reinterpret_cast<double*>(p->gw_ws + grp.cot);
+                    f.zbar = reinterpret_cast<double*>(p->gw_ws + grp.zbar);
+                    f.wbar = reinterpret_cast<double*>(p->gw_ws + grp.wbar);
+                    f.part2 = reinterpret_cast<double*>(p->gw_ws + grp.part2);
+                    e = magnus::launch_wbd_fold_vjp(f, s);
+                    if (e != hipSuccess) return device_error(e, "uneven pipeline fold");
+                } else {
+                    f.out = reinterpret_cast<double*>(p->gw_ws + grp.out);
+                    e = magnus::launch_wbd_fold(f, s);
+                    if (e != hipSuccess) return device_error(e, "uneven pipeline fold");
+                    magnus::SegmentArgs g{};
+                    g.ids = p->d_ids + grp.ids_off;
+                    g.count = (int)grp.ids.size();
+                    g.tab = p->d_tab + p->count;
+                    g.packed = f.out;
+                    g.per = (long long)(grp.pl.n_out * grp.pl.m);
+                    e = magnus::launch_scatter(g, s);
+                    if (e != hipSuccess) return device_error(e, "uneven pipeline scatter");
+                }
             }
-        }
-        if (!vjp) continue;
-        for (size_t t = 0; t + 1 < rsteps; ++t)
-            for (int di = 0; di < kGwDims; ++di) {
-                if (t + 1 >= wv.rev_off[di].size()) continue;
+            if (!vjp) continue;
+            for (size_t t = 0; t + 1 < wv.rev_off[di].size(); ++t) {
                 ga.tasks = p->d_tasks + wv.rev_base[di] + wv.rev_off[di][t];
                 const int cnt = (int)(wv.rev_off[di][t + 1] - wv.rev_off[di][t]);
                 e = magnus::launch_gw_reverse(kGwDim[di], ga, cnt, wv.rev_bytes[di][t], wv.rev_flops[di][t], s);
                 if (e != hipSuccess) return device_error(e, "uneven pipeline reverse");
             }
+        }
     }
     return MAGNUS_OK;
 }
@@ -557,11 +569,13 @@ int enqueue_all(magnus_uneven_plan* p, bool vjp, hipStream_t origin) {
     hipError_t e = hipEventRecord(p->fork, origin);
     if (e != hipSuccess) return device_error(e, "uneven fork");
     if (!p->gw.empty()) {
-        e = hipStreamWaitEvent(p->gw_stream, p->fork, 0);
+        for (int di = 0; di < kGwDims && e == hipSuccess; ++di) e = hipStreamWaitEvent(p->gw_stream[di], p->fork, 0);
         if (e != hipSuccess) return device_error(e, "uneven fork");
-        if (int rc = enqueue_gw(p, vjp, p->gw_stream)) return rc;
-        e = hipEventRecord(p->gw_join, p->gw_stream);
-        if (e == hipSuccess) e = hipStreamWaitEvent(origin, p->gw_join, 0);
+        if (int rc = enqueue_gw(p, vjp)) return rc;
+        for (int di = 0; di < kGwDims && e == hipSuccess; ++di) {
+            e = hipEventRecord(p->gw_join[di], p->gw_stream[di]);
+            if (e == hipSuccess) e = hipStreamWaitEvent(origin, p->gw_join[di], 0);
+        }
         if (e != hipSuccess) return device_error(e, "uneven join");
     }
     for (int b = 0; b < p->branches; ++b) {
@@ -632,8 +646,10 @@ void destroy(magnus_uneven_plan* p) {
         if (p->join[b]) (void)hipEventDestroy(p->join[b]);
     }
     if (p->cap) (void)hipStreamDestroy(p->cap);
-    if (p->gw_stream) (void)hipStreamDestroy(p->gw_stream);
-    if (p->gw_join) (void)hipEventDestroy(p->gw_join);
+    for (int di = 0; di < kGwDims; ++di) {
+        if (p->gw_stream[di]) (void)hipStreamDestroy(p->gw_stream[di]);
+        if (p->gw_join[di]) (void)hipEventDestroy(p->gw_join[di]);
+    }
     if (p->fork) (void)hipEventDestroy(p->fork);
     if (p->tab_done) (void)hipEventDestroy(p->tab_done);
     if (p->h_tab) (void)hipHostFree(p->h_tab);
@@ -759,8 +775,10 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
         if (e == hipSuccess) e = hipEventCreateWithFlags(&p->join[b], hipEventDisableTiming);
     }
     if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->cap, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipStreamCreateWithFlags(&p->gw_stream, hipStreamNonBlocking);
-    if (e == hipSuccess) e = hipEventCreateWithFlags(&p->gw_join, hipEventDisableTiming);
+    for (int di = 0; di < kGwDims && e == hipSuccess; ++di) {
+        e = hipStreamCreateWithFlags(&p->gw_stream[di], hipStreamNonBlocking);
+        if (e == hipSuccess) e = hipEventCreateWithFlags(&p->gw_join[di], hipEventDisableTiming);
+    }
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->fork, hipEventDisableTiming);
     if (e == hipSuccess) e = hipEventCreateWithFlags(&p->tab_done, hipEventDisableTiming);
     if (e == hipSuccess) e = hipHostMalloc((void**)&p->h_tab, std::max<uint64_t>(3 * count, 1) * sizeof(void*));

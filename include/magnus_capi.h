@@ -284,15 +284,20 @@ int magnus_uneven_vjp_f64(magnus_uneven_plan* plan, const double* const* in, con
  * sweep) of its first sub-batch under each product path worth trying for its
  * dim (dim 16: FP64 MFMA or VALU row products; dim 32: MFMA with 1, 2 or 4
  * wavefronts per product) on the given buffers, keeps the fastest, and
- * rebalances the branches with the measured times.  Synchronizes the stream. */
+ * rebalances the branches with the measured times; for the grouped pipeline,
+ * times its whole VJP with segments of 64, 128 and 256 samples and keeps the
+ * fastest.  Synchronizes the stream. */
 int magnus_uneven_autotune(magnus_uneven_plan* plan, const double* const* in, const double* const* cot,
                            double* const* dA, void* stream);
 
-/* Tuning table as text, one bucket per line: "dim samples n tile ms_per_instance".
+/* Tuning table as text: "pipeline L ms" (the grouped pipeline's segment length in
+ * samples and its measured VJP time) when the plan has a pipeline, then one bucket
+ * per line: "dim samples n tile ms_per_instance".
  * magnus_uneven_tuning writes at most `len` bytes (NUL-terminated) and returns
  * the full length; magnus_uneven_set_tuning applies the lines whose
  * (dim, samples, n) match a bucket (tile: 0 default, 1 VALU, 2 / 3 MFMA with
- * 2 / 4 wavefronts per product at dim 32). */
+ * 2 / 4 wavefronts per product at dim 32) and a "pipeline L" line (L a multiple
+ * of 4 up to 256). */
 uint64_t magnus_uneven_tuning(const magnus_uneven_plan* plan, char* buf, uint64_t len);
 int magnus_uneven_set_tuning(magnus_uneven_plan* plan, const char* text);
 

@@ -89,7 +89,9 @@ std::vector<int> tile_candidates(const Plan& pl) {
 // ---- grouped walker pipeline -------------------------------------------------
 constexpr int kGwDims = 3;  // dim 16, 32, 64
 constexpr int kGwDim[kGwDims] = {16, 32, 64};
-constexpr int kGwL = 128;   // segment length in samples (a multiple of the Boole block)
+constexpr int kGwL = 128;   // default segment length in samples (a multiple of the Boole block)
+constexpr int kGwLCandidates[3] = {64, 128, 256};  // the autotuner's choices; sizes are planned for the largest
+constexpr int kGwLMax = 256;
 
 int gw_dim_index(uint64_t d) { return d == 16 ? 0 : (d == 32 ? 1 : 2); }
 bool gw_eligible(const Plan& pl) {
@@ -151,6 +153,8 @@ struct magnus_uneven_plan {
     std::vector<uint32_t> gw_ids;
     std::vector<GwWave> gw;
     size_t gw_budget = 0;
+    int gw_L = kGwL;            // segment length of the pipeline's tasks
+    double gw_ms = -1.0;        // measured pipeline VJP time (< 0: not measured)
     magnus::GwInst* d_inst = nullptr;
     magnus::GwTask* d_tasks = nullptr;
     char* gw_ws = nullptr;
@@ -176,7 +180,7 @@ int gw_layout(magnus_uneven_plan* p, size_t& ids_total, std::vector<magnus::GwIn
               std::vector<magnus::GwTask>& tasks, std::vector<size_t>& slot_ws) {
     p->gw.clear();
     if (p->gw_ids.empty()) return MAGNUS_OK;
-    const int L = kGwL;
+    const int L = p->gw_L;
     std::vector<uint32_t> order = p->gw_ids;
     auto work = [&](uint32_t i) {
         const Plan& pl = p->gw_plans[i];
@@ -332,7 +336,7 @@ void gw_bind(magnus_uneven_plan* p, std::vector<magnus::GwInst>& inst, const std
                 gi.fst = reinterpret_cast<double*>(ws + at);
                 at += scratch_round(K * 2 * m * 8);
                 gi.U = reinterpret_cast<double*>(ws + at);
-                at += scratch_round(K * 2 * (kGwL + 1) * m * 8);
+                at += scratch_round(K * 2 * (p->gw_L + 1) * m * 8);
                 gi.rst = reinterpret_cast<double*>(ws + at);
                 gi.sums = reinterpret_cast<double*>(ws + grp.z) + b * n * m;
                 gi.z0 = reinterpret_cast<double*>(ws + grp.z0) + b * m;
@@ -498,7 +502,7 @@ int enqueue_gw(magnus_uneven_plan* p, bool vjp) {
         ga.inst = p->d_inst + wv.inst_off;
         ga.tab = p->d_tab;
         ga.count = (long long)p->count;
-        ga.L = kGwL;
+        ga.L = p->gw_L;
         hipError_t e = hipSuccess;
         for (int di = 0; di < kGwDims; ++di) {
             hipStream_t s = p->gw_stream[di];
@@ -724,7 +728,7 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
             const size_t res = left ? std::max(workspace_bytes / 4,
                                                std::max(noncand_max, suffix[c + 1]) * (size_t)branches)
                                     : 0;
-            const size_t b = gw_inst_bytes(plans[cand[c]], kGwL);
+            const size_t b = gw_inst_bytes(plans[cand[c]], kGwLMax);
             if (workspace_bytes < res || used + b > workspace_bytes - res) break;
             used += b;
             in_gw[cand[c]] = 1;
@@ -867,6 +871,45 @@ int magnus_uneven_autotune(magnus_uneven_plan* p, const double* const* in, const
         bk.pl.tile = best_tile;
         if (best_ms >= 0) bk.ms_per_inst = best_ms;
     }
+    // the pipeline's segment length: the whole pipeline's VJP under each candidate
+    if (rc == MAGNUS_OK && !p->gw_ids.empty()) {
+        auto once = [&]() -> int {
+            hipError_t e = hipEventRecord(p->fork, s);
+            for (int di = 0; di < kGwDims && e == hipSuccess; ++di) e = hipStreamWaitEvent(p->gw_stream[di], p->fork, 0);
+            if (e != hipSuccess) return device_error(e, "uneven autotune");
+            if (int r = enqueue_gw(p, true)) return r;
+            for (int di = 0; di < kGwDims && e == hipSuccess; ++di) {
+                e = hipEventRecord(p->gw_join[di], p->gw_stream[di]);
+                if (e == hipSuccess) e = hipStreamWaitEvent(s, p->gw_join[di], 0);
+            }
+            return e == hipSuccess ? MAGNUS_OK : device_error(e, "uneven autotune");
+        };
+        double best = -1.0;
+        int best_L = p->gw_L;
+        for (int L : kGwLCandidates) {
+            p->gw_L = L;
+            if ((rc = prepare(p))) break;  // reallocates the tables: upload them again
+            if ((rc = upload_tables(p, in, cot, const_cast<const double* const*>(dA), s))) break;
+            if ((rc = once())) break;  // warm-up
+            if (hipEventRecord(t0, s) != hipSuccess) {
+                rc = device_error(hipGetLastError(), "event");
+                break;
+            }
+            for (int r = 0; r < 2 && !rc; ++r) rc = once();
+            if (rc) break;
+            float ms = 0.f;
+            if (hipEventRecord(t1, s) == hipSuccess && hipEventSynchronize(t1) == hipSuccess &&
+                hipEventElapsedTime(&ms, t0, t1) == hipSuccess) {
+                const double per = ms / 2.0;
+                if (best < 0 || per < best) {
+                    best = per;
+                    best_L = L;
+                }
+            }
+        }
+        p->gw_L = best_L;
+        if (best >= 0) p->gw_ms = best;
+    }
     timing_suspend(prev);
     (void)hipEventDestroy(t0);
     (void)hipEventDestroy(t1);
@@ -878,6 +921,11 @@ int magnus_uneven_autotune(magnus_uneven_plan* p, const double* const* in, const
 uint64_t magnus_uneven_tuning(const magnus_uneven_plan* p, char* buf, uint64_t len) {
     if (!p) return 0;
     std::ostringstream os;
+    if (!p->gw_ids.empty()) {
+        char line[128];
+        std::snprintf(line, sizeof line, "pipeline %d %.6g\n", p->gw_L, p->gw_ms);
+        os << line;
+    }
     for (auto& bk : p->buckets) {
         char line[128];
         std::snprintf(line, sizeof line, "%llu %llu %llu %d %.6g\n", (unsigned long long)bk.pl.d,
@@ -899,6 +947,18 @@ int magnus_uneven_set_tuning(magnus_uneven_plan* p, const char* text) {
     std::string line;
     bool changed = false;
     while (std::getline(is, line)) {
+        int L = 0;
+        double pms = -1.0;
+        const int pg = std::sscanf(line.c_str(), "pipeline %d %lf", &L, &pms);
+        if (pg >= 1) {
+            if (L < 4 || L > kGwLMax || L % 4) return fail(MAGNUS_EINVAL, "pipeline segment length must be a multiple of 4 in 4..256");
+            if (!p->gw_ids.empty()) {
+                p->gw_L = L;
+                if (pg == 2) p->gw_ms = pms;
+                changed = true;
+            }
+            continue;
+        }
         unsigned long long d, S, n;
         int tile;
         double ms = -1.0;

@@ -214,9 +214,12 @@ def test_uneven_autotune_picks_measured_paths_and_keeps_parity():
     A, cot = _inputs(probs, 7)
     plan = UnevenBatch(probs, branches=2)
     plan.autotune(A, cot)
-    lines = [ln.split() for ln in plan.tuning.splitlines()]
+    lines = [ln.split() for ln in plan.tuning.splitlines() if not ln.startswith("pipeline")]
+    pipe = [ln.split() for ln in plan.tuning.splitlines() if ln.startswith("pipeline")]
     bucketed = {p["dim"] for i, p in enumerate(probs) if not plan.in_pipeline(i)}
     assert len(lines) == len(bucketed) and all(float(ln[4]) > 0 for ln in lines)
+    # the pipeline's segment length is chosen among 64, 128, 256 by its measured VJP
+    assert len(pipe) == 1 and int(pipe[0][1]) in (64, 128, 256) and float(pipe[0][2]) > 0
     outs = plan.forward(A)
     dAs = plan.vjp(A, cot)
     torch.cuda.synchronize()
@@ -228,7 +231,12 @@ def test_uneven_autotune_picks_measured_paths_and_keeps_parity():
     # a second plan takes the table as text
     plan2 = UnevenBatch(probs, branches=1)
     plan2.set_tuning(plan.tuning)
-    assert [ln.split()[3] for ln in plan2.tuning.splitlines()] == [ln[3] for ln in lines]
+    lines2 = [ln.split() for ln in plan2.tuning.splitlines()]
+    assert [ln[3] for ln in lines2 if ln[0] != "pipeline"] == [ln[3] for ln in lines]
+    assert [ln[1] for ln in lines2 if ln[0] == "pipeline"] == [pipe[0][1]]
+    outs2 = plan2.forward(A)
+    torch.cuda.synchronize()
+    assert all(rel_fro(x.cpu().numpy(), y.cpu().numpy()) <= TOL for x, y in zip(outs2, outs))
     plan.close()
     plan2.close()
 

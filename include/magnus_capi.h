@@ -252,7 +252,7 @@ int magnus_sc_expm_f64(uint64_t count, const double* in3, double* out, void* str
  *      (dim, order, op) group).  They join it in order of their chain of steps
  *      (samples + (n - 1) 128, weighted by dim^3), shortest first, while their
  *      basis levels fit in the workspace left after a reserve for the branches
- *      (a quarter, and at least one instance per branch of the largest bucket
+ *      (an eighth, and at least one instance per branch of the largest bucket
  *      left); the rest, the longest chains, run in buckets.  The other problems
  *      are grouped into buckets of equal shape, each bucket split into
  *      sub-batches that fit its workspace and spread over `branches` HIP

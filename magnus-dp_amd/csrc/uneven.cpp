@@ -699,7 +699,7 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
     // hold many tasks): eligible instances join it in order of their chain of
     // steps (about S + (n - 1) L samples, each costing more at a larger dim),
     // shortest first, while their full levels fit in the workspace left after
-    // the branches' reserve (a quarter, and at least one instance of the largest
+    // the branches' reserve (an eighth, and at least one instance of the largest
     // remaining bucket per branch); the others, the longest chains, run in
     // buckets on the level kernels, parallel over samples.
     p->gw_plans.assign(count, Plan{});
@@ -725,7 +725,7 @@ int magnus_uneven_create(const magnus_problem* probs, uint64_t count, uint64_t w
         size_t used = 0;
         for (size_t c = 0; c < cand.size(); ++c) {
             const bool left = noncand || c + 1 < cand.size();  // instances left for the branches
-            const size_t res = left ? std::max(workspace_bytes / 4,
+            const size_t res = left ? std::max(workspace_bytes / 8,
                                                std::max(noncand_max, suffix[c + 1]) * (size_t)branches)
                                     : 0;
             const size_t b = gw_inst_bytes(plans[cand[c]], kGwLMax);

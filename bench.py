@@ -20,7 +20,8 @@ Other workloads (--workload):
   plain-d4          one d = 4 instance, S = 1025, Boole, n = 4, sum, forward only
   uneven            2048 instances of d 2..64, S 257..8193, n 2..6 in one graph launch
   wbasis-d8         one resolution group of the controller's solve: 1000 d = 8 instances,
-                    S = 2049, Boole, n = 8, sum, forward + VJP without a carry
+                    S = 2049, Boole, n = 8, sum, forward keeping the basis state + VJP from
+                    it, samples formed in the sweeps from the Fourier coefficients
 
   python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload NAME] [--no-cpu]
 """
@@ -852,20 +853,27 @@ def main():
                   "op": "sum", "carry": "recorded"}
         do_vjp = True
     elif wl == "wbasis-d8":
-        # one resolution group of the controller's solve: forward (op 'sum') + VJP
-        # without a carry on the dense W-basis path, 1000 d = 8 instances at S = 2049
+        # one resolution group of the controller's solve: forward (op 'sum')
+        # keeping the basis state + VJP from it on the d = 8 single-pass sweeps,
+        # 1000 d = 8 instances at S = 2049 given by their Fourier coefficients
+        # (the sweeps form the samples, as the controller runs them)
+        import numpy as np
+
         B, S, d, n, modes = args.batch or 1000, 2049, 8, 8, 5
-        a, cot = _dense_inputs(torch, B, S, d, modes, dev, seed=5)
+        coeffs = torch.from_numpy(np.stack([np.random.default_rng([5, b]).standard_normal((2 * modes + 1, d, d))
+                                            / np.sqrt(d) for b in range(B)])).to(dev)
+        cot = torch.from_numpy(np.stack([np.random.default_rng([5, b, 1]).standard_normal((d, d))
+                                         for b in range(B)])).to(dev)
 
         def step():
-            out = torch_ops.forward(n, a, 0.0, 1.0, op="sum", integrator="Boole")
-            dA = torch_ops.vjp(n, a, cot, None, 0.0, 1.0, op="sum", integrator="Boole")
+            out, state = torch_ops.fourier_forward_basis(n, coeffs, S, 0.0, 1.0, op="sum", integrator="Boole")
+            dA = torch_ops.fourier_vjp_basis(n, coeffs, S, cot, state, 0.0, 1.0, op="sum", integrator="Boole")
             return out, dA
 
         # one job per instance on this path (the W-basis holds all GL nodes)
         Q, K, m, m_in, P, n_out = 1, n - 1, d * d, d * d, 2 * d ** 3, 1
         config = {"workload": "wbasis-d8", "instances": B, "samples": S, "dim": d, "n": n, "integrator": "Boole",
-                  "op": "sum", "carry": "none (dense W-basis sweep)"}
+                  "op": "sum", "carry": "none (basis state kept by the forward)", "input": "Fourier coefficients"}
         do_vjp = True
     else:
         B, S, d, n, modes = 1, 1025, 4, 4, 3

@@ -202,6 +202,25 @@ uint64_t magnus_vjp_segment_length(const magnus_problem* p, uint64_t segments);
 int magnus_vjp_segmented_f64(const magnus_problem* p, uint64_t count, const double* in, const double* cot, double* dA,
                              uint64_t segments, void* stream);
 
+/* ---- Fourier-form input (no counterpart in the reference, whose samples come
+ *      from a Python callable evaluated on the grid, src/api/magnus/__init__.py:
+ *      58-85).  The basis-state forward, the VJP from that state and the
+ *      all-orders forward of dim-8 problems under the Boole rule (2 <= n <= 8)
+ *      for A(t) = C_0 + sum_{k=1..modes} (sin(2 pi k t) C_{2k-1} + cos(2 pi k t)
+ *      C_{2k}), coeffs [count][2 modes + 1][8][8], modes <= 8 (ERANGE above):
+ *      the single-pass sweeps form each sample from the coefficients instead of
+ *      reading a sample array, so nothing is sampled into HBM.  Same sin/cos
+ *      table and fused multiply-adds as magnus_sample_fourier_f64, so the
+ *      results are those of sampling first and calling magnus_forward_basis_f64 /
+ *      magnus_vjp_basis_f64 / magnus_forward_orders_from_f64, bit for bit.
+ *      Other problems: EINVAL. ---- */
+int magnus_fourier_forward_basis_f64(const magnus_problem* p, uint64_t count, const double* coeffs, uint64_t modes,
+                                     double* out, double* state, void* stream);
+int magnus_fourier_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* coeffs, uint64_t modes,
+                                 const double* cot, const double* state, double* dA, void* stream);
+int magnus_fourier_forward_orders_from_f64(const magnus_problem* p, uint64_t count, uint64_t first,
+                                           const double* coeffs, uint64_t modes, double* out, void* stream);
+
 /* ---- SpaceCurve VJP that allocates nothing.  No counterpart in the reference:
  *      for callers that cannot allocate device memory (code under HIP stream
  *      capture, tight serving loops).  Same arguments, kernels and results as
@@ -217,8 +236,10 @@ int magnus_sc_vjp_noalloc_f64(const magnus_problem* p, uint64_t count, const dou
 /* ---- sampling A(t) on the device, on the reference's grid
  *      t_i = np.linspace(t0, tf, samples) (replaces evaluating a Python callable
  *      there, src/api/magnus/__init__.py:58-85, for these two families).
- *      Arithmetic follows numpy's order; samples differ from a numpy
- *      evaluation only through sin/cos (a few ulp).  modes <= 32. ---- */
+ *      The grid and the sin/cos arguments follow numpy's order and rounding;
+ *      the matrix form adds its terms in k order with one fused multiply-add
+ *      each, so samples differ from a numpy evaluation through sin/cos and
+ *      those roundings (a few ulp).  modes <= 32. ---- */
 
 /* out[b][i][e] = C[b][0][e] + sum_{k=1..modes} ( sin(2 pi k t_i) C[b][2k-1][e]
  *                                               + cos(2 pi k t_i) C[b][2k][e] ),

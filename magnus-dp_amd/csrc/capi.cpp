@@ -639,9 +639,18 @@ WbdState wbd_state_view(const Plan& pl, uint64_t count, double* base) {
     return st;
 }
 
+// Samples formed inside the d = 8 single-pass sweeps from Fourier coefficients
+// ([count][2 modes + 1][64]) and the grid's sin/cos table ([S][modes][2])
+// instead of read from an input array.
+struct FourierSrc {
+    const double* coef;
+    const double* tab;
+    int modes;
+};
+
 // The basis sweep into a state: levels (when st.W is set), sums and totals.
 int wbd_sweep_state(const Plan& pl, uint64_t count, const double* in, const WbdState& st, hipStream_t stream,
-                    Arena* arena) {
+                    Arena* arena, const FourierSrc* fs = nullptr) {
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const size_t S = pl.S, m = pl.m;
@@ -653,6 +662,11 @@ int wbd_sweep_state(const Plan& pl, uint64_t count, const double* in, const WbdS
     fill_wbd_lm(pl, B, a);
     if (levels && !alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
+    if (fs) {
+        a.fcoef = fs->coef;
+        a.ftab = fs->tab;
+        a.fmodes = fs->modes;
+    }
     a.carry = st.W;
     a.pingpong = pingpong;
     a.Y = Y;
@@ -665,7 +679,7 @@ int wbd_sweep_state(const Plan& pl, uint64_t count, const double* in, const WbdS
 
 // forward (op sum / many); `state` (optional) keeps the basis state for a later VJP
 int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* out, hipStream_t stream, Arena* arena,
-                    double* state = nullptr) {
+                    double* state = nullptr, const FourierSrc* fs = nullptr) {
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const size_t m = pl.m, n = pl.n;
@@ -681,7 +695,7 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
     double* part = scratch.alloc<double>(B * Q * n * m);
     double* work = scratch.alloc<double>(B * Q * (2 * (n - 1) + 1) * m);
     if (!st.z || !st.z0 || !st.totals || !part || !work) return fail(MAGNUS_ENOMEM, "device allocation failed");
-    if (int rc = wbd_sweep_state(pl, count, in, st, stream, arena)) return rc;
+    if (int rc = wbd_sweep_state(pl, count, in, st, stream, arena, fs)) return rc;
     magnus::WbdArgs f{};
     fill_wbd_fold(pl, B, f);
     f.z = st.z;
@@ -699,7 +713,7 @@ int run_wbd_forward(const Plan& pl, uint64_t count, const double* in, double* ou
 // VJP; `state` (optional) is the basis state a forward kept for the same input,
 // otherwise the sweep is rerun into scratch (the reference's VJP::none)
 int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* cot, double* dA, hipStream_t stream,
-                Arena* arena, const double* state = nullptr) {
+                Arena* arena, const double* state = nullptr, const FourierSrc* fs = nullptr) {
     DeviceScratch scratch(stream, arena);
     const long long B = (long long)count;
     const size_t S = pl.S, m = pl.m, n = pl.n;
@@ -714,7 +728,7 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
         st.z0 = scratch.alloc<double>(B * m);
         st.totals = scratch.alloc<double>(B * n * m);
         if ((we && !st.W) || !st.z || !st.z0 || !st.totals) return fail(MAGNUS_ENOMEM, "device allocation failed");
-        if (int rc = wbd_sweep_state(pl, count, in, st, stream, arena)) return rc;
+        if (int rc = wbd_sweep_state(pl, count, in, st, stream, arena, fs)) return rc;
     }
     double* part = scratch.alloc<double>(B * Q * n * m);
     double* part2 = scratch.alloc<double>(B * Q * n * m);
@@ -742,6 +756,11 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
     if (rev8) {
         magnus::Wbd8RevArgs r{};
         r.in = in;
+        if (fs) {
+            r.fcoef = fs->coef;
+            r.ftab = fs->tab;
+            r.fmodes = fs->modes;
+        }
         r.W = st.W;
         r.totals = st.totals;
         r.zbar = zbar;
@@ -799,7 +818,8 @@ int run_wbd_vjp(const Plan& pl, uint64_t count, const double* in, const double* 
 // op="sum" output of order k (its own Q(k) GL nodes).
 // Magnus sums of orders first..n from one basis sweep: row k - first of out is
 // the op="sum" output of order k (its own Q(k) GL nodes).
-int run_orders(const Plan& pl, uint64_t count, uint64_t first, const double* in, double* out, hipStream_t stream) {
+int run_orders(const Plan& pl, uint64_t count, uint64_t first, const double* in, double* out, hipStream_t stream,
+               const FourierSrc* fs = nullptr) {
     DeviceScratch scratch(stream);
     const long long B = (long long)count;
     const size_t S = pl.S, m = pl.m, n = pl.n, Q = pl.Q;
@@ -817,6 +837,11 @@ int run_orders(const Plan& pl, uint64_t count, uint64_t first, const double* in,
     fill_wbd_lm(pl, B, a);
     if (levels && !alloc_scan_chunks(scratch, pl, B, a)) return fail(MAGNUS_ENOMEM, "device allocation failed");
     a.in = in;
+    if (fs) {
+        a.fcoef = fs->coef;
+        a.ftab = fs->tab;
+        a.fmodes = fs->modes;
+    }
     a.pingpong = pingpong;
     a.Y = Y;
     a.partial = z;
@@ -1357,6 +1382,87 @@ int magnus_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* 
     if (count == 0) return MAGNUS_OK;
     if (!in || !cot || !state || !dA) return fail(MAGNUS_EINVAL, "null argument");
     return run_wbd_vjp(pl, count, in, cot, dA, (hipStream_t)stream, nullptr, state);
+}
+
+// ---- Fourier-form input: the d = 8 single-pass sweeps form the samples
+// themselves.  The entry checks the problem, builds the grid's sin/cos table
+// in scratch and runs the sample-array entry's work with the samples formed
+// in the sweeps' loader wavefront (the same products and sums as
+// magnus_sample_fourier_f64, so the results equal sampling first, bit for bit).
+namespace {
+int fourier_check(const Plan& pl, uint64_t modes, const char* what) {
+    if (pl.sc || !use_wbd8(pl))
+        return fail(MAGNUS_EINVAL, std::string(what) + " with Fourier-form input needs a dim-8 matrix problem under the "
+                                                       "Boole rule with 2 <= n <= 8");
+    if (modes > (uint64_t)magnus::kWbd8MaxModes) return fail(MAGNUS_ERANGE, "too many Fourier modes (max 8)");
+    return MAGNUS_OK;
+}
+double* fourier_table(DeviceScratch& scratch, const magnus_problem* p, uint64_t modes, hipStream_t stream, int* rc) {
+    *rc = MAGNUS_OK;
+    if (modes == 0) return nullptr;
+    double* tab = scratch.alloc<double>(p->samples * modes * 2);
+    if (!tab) {
+        *rc = fail(MAGNUS_ENOMEM, "device allocation failed");
+        return nullptr;
+    }
+    magnus::SampleArgs a{};
+    a.S = (long long)p->samples;
+    a.modes = (int)modes;
+    a.t0 = p->t0;
+    a.tf = p->tf;
+    a.step = (p->tf - p->t0) / (double)(p->samples - 1);
+    hipError_t e = magnus::launch_fourier_table(a, tab, stream);
+    if (e != hipSuccess) *rc = device_error(e, "table kernel launch");
+    return tab;
+}
+}  // namespace
+
+int magnus_fourier_forward_basis_f64(const magnus_problem* p, uint64_t count, const double* coeffs, uint64_t modes,
+                                     double* out, double* state, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, true, &pl)) return rc;
+    if (int rc = fourier_check(pl, modes, "basis-state forward")) return rc;
+    if (pl.op == MAGNUS_OP_ONE) return fail(MAGNUS_EINVAL, "basis-state forward needs op many or sum");
+    if (count == 0) return MAGNUS_OK;
+    if (!coeffs || !out || !state) return fail(MAGNUS_EINVAL, "null argument");
+    DeviceScratch scratch((hipStream_t)stream);
+    int rc;
+    double* tab = fourier_table(scratch, p, modes, (hipStream_t)stream, &rc);
+    if (rc) return rc;
+    const FourierSrc fs{coeffs, tab, (int)modes};
+    return run_wbd_forward(pl, count, nullptr, out, (hipStream_t)stream, nullptr, state, &fs);
+}
+
+int magnus_fourier_vjp_basis_f64(const magnus_problem* p, uint64_t count, const double* coeffs, uint64_t modes,
+                                 const double* cot, const double* state, double* dA, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, false, &pl)) return rc;
+    if (int rc = fourier_check(pl, modes, "basis-state VJP")) return rc;
+    if (pl.op == MAGNUS_OP_ONE) return fail(MAGNUS_EINVAL, "basis-state VJP needs op many or sum");
+    if (count == 0) return MAGNUS_OK;
+    if (!coeffs || !cot || !state || !dA) return fail(MAGNUS_EINVAL, "null argument");
+    DeviceScratch scratch((hipStream_t)stream);
+    int rc;
+    double* tab = fourier_table(scratch, p, modes, (hipStream_t)stream, &rc);
+    if (rc) return rc;
+    const FourierSrc fs{coeffs, tab, (int)modes};
+    return run_wbd_vjp(pl, count, nullptr, cot, dA, (hipStream_t)stream, nullptr, state, &fs);
+}
+
+int magnus_fourier_forward_orders_from_f64(const magnus_problem* p, uint64_t count, uint64_t first,
+                                           const double* coeffs, uint64_t modes, double* out, void* stream) {
+    Plan pl;
+    if (int rc = make_plan(p, false, true, &pl)) return rc;
+    if (int rc = fourier_check(pl, modes, "all-orders forward")) return rc;
+    if (first < 1 || first > pl.n) return fail(MAGNUS_EINVAL, "first order must lie in 1..n");
+    if (count == 0) return MAGNUS_OK;
+    if (!coeffs || !out) return fail(MAGNUS_EINVAL, "null argument");
+    DeviceScratch scratch((hipStream_t)stream);
+    int rc;
+    double* tab = fourier_table(scratch, p, modes, (hipStream_t)stream, &rc);
+    if (rc) return rc;
+    const FourierSrc fs{coeffs, tab, (int)modes};
+    return run_orders(pl, count, first, nullptr, out, (hipStream_t)stream, &fs);
 }
 
 uint64_t magnus_sc_checkpoint_elems(const magnus_problem* p, uint64_t count) {

@@ -43,6 +43,10 @@ struct LmArgs {
     const double* wbar;  // [jobs][n][m] cotangents of the level totals, or nullptr (VJP)
     double* cscan;       // [3][nchunks][jobs*m] chunk scratch of the sample-parallel scans, or nullptr
     double* dbar;        // [jobs][2][m] deferred cotangents of the level totals (level walkers' VJP)
+    // d = 8 single-pass sweeps: samples formed from Fourier coefficients instead of read from `in`
+    const double* fcoef; // [jobs][2 fmodes + 1][64] or nullptr
+    const double* ftab;  // [S][fmodes][2]: sin, cos of 2 pi k t_i (fourier_table)
+    int fmodes;
     int clen;            // samples per chunk of the sample-parallel scan (multiple of 4)
     long long S;
     long long jobs;
@@ -173,6 +177,7 @@ hipError_t launch_gw_reverse(int d, const GwArgs& a, int ntasks, double bytes, d
 // ([B][K][S][64]) for the reverse sweep.  Uses in, carry, partial, omega1,
 // totals, S, jobs (= instances), n, d, dt of LmArgs.
 constexpr int kWbd8MaxK = 7;
+constexpr int kWbd8MaxModes = 8;  // Fourier-formed samples: at most this many modes
 bool wbd8_sweep_ok(int d, int integ, int n);
 hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s);
 
@@ -182,7 +187,10 @@ hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s);
 // samples from the last to the first and writes the whole dA (no per-level
 // arrays).
 struct Wbd8RevArgs {
-    const double* in;    // [B][S][64]
+    const double* in;    // [B][S][64], or nullptr with fcoef set
+    const double* fcoef; // [B][2 fmodes + 1][64]: the samples formed from Fourier coefficients, or nullptr
+    const double* ftab;  // [S][fmodes][2]
+    int fmodes;
     const double* W;       // [B][K / 2][S][64]: the even levels W_2, W_4, ...
     const double* totals;  // [B][n][64]: rows 0..K-1 are What_1..What_K, from which the reverse rebuilds the odd levels
     const double* zbar;  // [B][n][64]

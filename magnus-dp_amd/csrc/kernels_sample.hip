@@ -5,10 +5,18 @@
 //   matrix form:      A(t) = C_0 + sum_{k=1..M} ( sin(2 pi k t) C_{2k-1} + cos(2 pi k t) C_{2k} )
 //   SpaceCurve form:  a_c(t) = sum_{k=1..M} amp_{k,c} sin(2 pi k t + phase_{k,c})
 //
-// The grid and every product and sum are rounded separately in numpy's order
-// (t_i = i*step + t0, the last point set to tf; ((2 pi) k) t; terms added in k
-// order), so the samples differ from a numpy evaluation only by the sin/cos
-// implementations (a few ulp).
+// The grid and the sin/cos arguments are rounded separately in numpy's order
+// (t_i = i*step + t0, the last point set to tf; ((2 pi) k) t), and the
+// matrix form's terms are accumulated in k order, each with one fused
+// multiply-add (C_0, then + sin_k C_{2k-1}, then + cos_k C_{2k}), so the
+// samples differ from a numpy evaluation by the sin/cos implementations and
+// the fused roundings (a few ulp).  The d = 8 sweeps that form samples from
+// the coefficients (kernels_wbd8.hip) use this table and accumulation order.
+//
+// No other contraction: the pragma precedes the includes so that it also
+// covers the inline arithmetic helpers the headers define.
+#pragma clang fp contract(off)
+
 #include "kernels_sample.hpp"
 
 namespace magnus {
@@ -75,8 +83,8 @@ __global__ void __launch_bounds__(kSampleThreads) sample_fourier_kernel(SampleAr
             double v = C[e];
             for (int k = 1; k <= modes; ++k) {
                 const double* tk = tab + 2 * (si * modes + k - 1);
-                v = __dadd_rn(v, __dmul_rn(tk[0], C[(2 * k - 1) * m + e]));
-                v = __dadd_rn(v, __dmul_rn(tk[1], C[(2 * k) * m + e]));
+                v = __fma_rn(tk[0], C[(2 * k - 1) * m + e], v);
+                v = __fma_rn(tk[1], C[(2 * k) * m + e], v);
             }
             a.out[(b * a.S + i) * m + e] = v;
         }
@@ -127,12 +135,24 @@ __global__ void __launch_bounds__(kSampleThreads) sample_fourier_reg(SampleArgs 
             for (int k = 1; k <= kRegModes; ++k) {
                 if (k > modes) break;
                 const double* tk = tab + 2 * (si * modes + k - 1);
-                v = __dadd_rn(v, __dmul_rn(tk[0], c[2 * k - 1]));
-                v = __dadd_rn(v, __dmul_rn(tk[1], c[2 * k]));
+                v = __fma_rn(tk[0], c[2 * k - 1], v);
+                v = __fma_rn(tk[1], c[2 * k], v);
             }
             a.out[(b * a.S + i) * m + e] = v;
         }
     }
+}
+
+// one thread per (sample, mode)
+__global__ void __launch_bounds__(256) fourier_table_kernel(SampleArgs a, double* out) {
+    const long long x = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+    if (x >= a.S * a.modes) return;
+    const long long i = x / a.modes;
+    const int k = (int)(x - i * a.modes) + 1;
+    const double t = grid_t(i, a.S, a.t0, a.tf, a.step);
+    const double arg = __dmul_rn(__dmul_rn(kTwoPi, (double)k), t);
+    out[2 * x] = sin(arg);
+    out[2 * x + 1] = cos(arg);
 }
 
 // one thread per (instance, sample, component)
@@ -180,6 +200,13 @@ hipError_t launch_sample_fourier(SampleArgs a, hipStream_t s) {
     else
         hipLaunchKernelGGL(sample_fourier_kernel<false>, dim3(blocks), dim3(kSampleThreads), 0, s, a);
     timer_end(h);
+    return hipGetLastError();
+}
+
+hipError_t launch_fourier_table(const SampleArgs& a, double* out, hipStream_t s) {
+    const long long total = a.S * a.modes;
+    if (total == 0) return hipSuccess;
+    hipLaunchKernelGGL(fourier_table_kernel, dim3((unsigned)((total + 255) / 256)), dim3(256), 0, s, a, out);
     return hipGetLastError();
 }
 

@@ -21,6 +21,9 @@ struct SampleArgs {
 };
 
 int sample_max_modes();
+// sin, cos of (2 pi k) t_i on the grid of a (S, t0, tf, step, modes): out[i][k-1][0..1],
+// the values the sampling kernels use
+hipError_t launch_fourier_table(const SampleArgs& a, double* out, hipStream_t s);
 hipError_t launch_sample_fourier(SampleArgs a, hipStream_t s);
 hipError_t launch_sample_sc_fourier(const SampleArgs& a, hipStream_t s);
 

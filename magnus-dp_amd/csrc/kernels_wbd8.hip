@@ -34,6 +34,7 @@
 // regrouped), so results differ from the reference's sequential order by
 // rounding only.  Each instance's results depend on its own data alone.
 #include "kernels.hpp"
+#include "walk.hpp"
 #include "rules.hpp"
 
 namespace magnus {
@@ -177,6 +178,122 @@ __device__ __forceinline__ void ring_loader(RingSlot* ring, const double* A, lon
     }
 }
 
+// The sampled matrix A(t_i) = C_0 + sum_k (sin_k C_{2k-1} + cos_k C_{2k}) at one
+// entry, from that entry's coefficients c[0..2M] and the grid's sin/cos row
+// tk[0..2M-1]: one fused multiply-add per term in k order, as the sampling
+// kernels (kernels_sample.hip) accumulate, so the values are theirs bit for bit.
+__device__ __forceinline__ double fourier_value(const double* c, const double* tk, int modes) {
+    double v = c[0];
+#pragma unroll
+    for (int k = 1; k <= kWbd8MaxModes; ++k) {
+        if (k > modes) break;
+        v = __fma_rn(tk[2 * k - 2], c[2 * k - 1], v);
+        v = __fma_rn(tk[2 * k - 1], c[2 * k], v);
+    }
+    return v;
+}
+// entry e of sample i of an instance, its coefficients read from global memory
+__device__ __forceinline__ double fourier_entry(const double* coef, const double* tab, int modes, long long i, int e) {
+    double c[2 * kWbd8MaxModes + 1];
+#pragma unroll
+    for (int q = 0; q < 2 * kWbd8MaxModes + 1; ++q) c[q] = q <= 2 * modes ? coef[q * kM + e] : 0.0;
+    return fourier_value(c, tab + i * 2 * modes, modes);
+}
+
+// lane `src`'s value of v, in every lane
+__device__ __forceinline__ double lane_bcast(double v, int src) {
+    const unsigned long long u = __builtin_bit_cast(unsigned long long, v);
+    const unsigned lo = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)u, src);
+    const unsigned hi = (unsigned)__builtin_amdgcn_readlane((int)(unsigned)(u >> 32), src);
+    return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
+}
+
+// The loader of a pipeline whose samples are formed from M Fourier modes:
+// lane l keeps entry l's 2M + 1 coefficients in registers and, for each ring
+// index, writes that entry of the block's four samples (4 blk + 1 .. 4 blk + 4)
+// to the slot, the four as independent fused multiply-add chains.  The
+// block's rows of the sin/cos table (4 x 2M <= 64 contiguous doubles) are
+// loaded one per lane, the next index's while this one is formed, stored to
+// the wave's 64-double LDS buffer `tb` and read back as (sin, cos) pairs
+// that every lane reads at one address (broadcast reads, 16 bytes each, the
+// next mode's in flight while this mode's multiply-adds run).  LDS accesses
+// of one wave complete in order, so the reads see this index's store and the
+// previous index's reads precede it.
+//
+// The step protocol is ring_loader's: indices 0 .. kAhead - 2, the barrier
+// that opens the pipeline, then at step t index t + kAhead - 1 and the step's
+// barrier (which publishes the writes, complete before it).  Every index up to
+// the last step's is formed, those past the last block from the last block's
+// rows: their slots are free by the protocol and no wave reads them, and the
+// loop then has no branch around a load (walk_range), so its waits count only
+// the loads issued since.
+template <int R, int M, typename BlkOf>
+__device__ __forceinline__ void ring_fourier_m(RingSlot* ring, double* tb, const double* coef, const double* tab,
+                                               long long nb, long long steps, int lane, BlkOf blk_of) {
+    const int word = ring_word(lane >> 3, lane & 7);
+    double c[2 * M + 1];
+#pragma unroll
+    for (int q = 0; q <= 2 * M; ++q) c[q] = coef[q * kM + lane];
+    const int tl = lane < 8 * M ? lane : 0;
+    auto load = [&](long long ri, double& tv) {
+        if constexpr (M > 0) tv = tab[(4 * blk_of(ri < nb ? ri : nb - 1) + 1) * (2 * M) + tl];
+    };
+    auto pair = [&](int p, int k) { return *reinterpret_cast<const double2*>(tb + p * 2 * M + 2 * k - 2); };
+    auto step = [&](long long ri, double& tv, auto) {
+        RingSlot& slot = ring[ri % R];
+        double v[4];
+#pragma unroll
+        for (int p = 0; p < 4; ++p) v[p] = c[0];
+        if constexpr (M > 0) {
+            tb[lane] = tv;
+            // groups (k, h): samples 2h, 2h + 1 of mode k, in k-major order,
+            // the next group's pairs read while this group's multiply-adds run
+            double2 cur[2] = {pair(0, 1), pair(1, 1)};
+#pragma unroll
+            for (int g = 0; g < 2 * M; ++g) {
+                const int k = g / 2 + 1, h = g % 2;
+                double2 nxt[2];
+                if (g + 1 < 2 * M) {
+                    const int kn = (g + 1) / 2 + 1, hn = (g + 1) % 2;
+                    nxt[0] = pair(2 * hn, kn);
+                    nxt[1] = pair(2 * hn + 1, kn);
+                }
+#pragma unroll
+                for (int q = 0; q < 2; ++q) v[2 * h + q] = __fma_rn(cur[q].x, c[2 * k - 1], v[2 * h + q]);
+#pragma unroll
+                for (int q = 0; q < 2; ++q) v[2 * h + q] = __fma_rn(cur[q].y, c[2 * k], v[2 * h + q]);
+                if (g + 1 < 2 * M) {
+                    cur[0] = nxt[0];
+                    cur[1] = nxt[1];
+                }
+                __builtin_amdgcn_sched_barrier(0);  // one group's reads ahead, no more (registers)
+            }
+        }
+#pragma unroll
+        for (int p = 0; p < 4; ++p) slot.v[p][word] = v[p];
+        if (ri >= kAhead - 2) step_barrier();
+    };
+    double tv[2] = {0.0, 0.0};
+    walk_range<2, 2, true>(0, steps + kAhead - 2, tv, load, step);
+}
+
+template <int R, typename BlkOf>
+__device__ __forceinline__ void ring_fourier(RingSlot* ring, double* tb, const double* coef, const double* tab,
+                                             int modes, long long nb, long long steps, int lane, BlkOf blk_of) {
+    switch (modes) {
+        case 0: ring_fourier_m<R, 0>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 1: ring_fourier_m<R, 1>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 2: ring_fourier_m<R, 2>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 3: ring_fourier_m<R, 3>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 4: ring_fourier_m<R, 4>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 5: ring_fourier_m<R, 5>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 6: ring_fourier_m<R, 6>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 7: ring_fourier_m<R, 7>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        default: ring_fourier_m<R, kWbd8MaxModes>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+    }
+    static_assert(kWbd8MaxModes == 8, "one case per mode count");
+}
+
 // The B pair (V[2k][c], V[2k + 1][c]) of lane L from its own entry
 // V[2k + I][c]: a DPP row rotation by 8 brings lane L ^ 8's entry, and the
 // bank mask keeps the own value in the lanes (I = 0 for the first, I = 1 for
@@ -258,7 +375,8 @@ __device__ __forceinline__ void sweep_wave(const LmArgs& a, int b, int w, const 
 
     // sample 0: every prefix is 0 there, so the products of levels 2..K+1 are 0
     if constexpr (FIRST) {
-        const double y0 = A[own];
+        const double y0 = a.fcoef ? fourier_entry(a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, 0, own)
+                                  : A[own];
         s1 = Acc{0.0, y0, 0.0};
         z1 = bw.b14 * y0;
     }
@@ -331,12 +449,17 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     __shared__ RingSlot ring[R];
     // level blocks handed from wave w to wave w + 1: [step parity][w][4 samples][64 words]
     __shared__ alignas(16) double hand[2][K > 1 ? K - 1 : 1][4][64];
+    __shared__ alignas(16) double ftb[64];  // the Fourier loader's table rows
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int b = blockIdx.x;
     const long long nb = (a.S - 1) / 4;
     const Lane L(lane);
     if (w == K) {
-        ring_loader<R, 1>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane, [](long long r) { return r; });
+        if (a.fcoef)
+            ring_fourier<R>(ring, ftb, a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, nb, nb + K - 1,
+                            lane, [](long long r) { return r; });
+        else
+            ring_loader<R, 1>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane, [](long long r) { return r; });
     } else if (w == 0) {
         sweep_wave<K, LEVELS, true, (K > 1)>(a, b, w, L, ring, hand);
     } else if (w < K - 1) {
@@ -438,7 +561,9 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     const int rc = 2 * chunk_pos(c, k);      // MODE 2: A's row chunk A[c][2k .. 2k + 1]
     double wend = 0.0, a0t = 0.0;  // MODE 1, 2: W_j[c][2k + I] at the current block's end; MODE 1: A_0 there
     if constexpr (MODE != 0) wend = a.totals[((long long)b * n + (j - 1)) * kM + c * 8 + 2 * k + I];
-    if constexpr (MODE == 1) a0t = a.in[(long long)b * S * kM + c * 8 + 2 * k + I];
+    if constexpr (MODE == 1)
+        a0t = a.fcoef ? fourier_entry(a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, 0, c * 8 + 2 * k + I)
+                      : a.in[(long long)b * S * kM + c * 8 + 2 * k + I];
     double* __restrict__ dA = a.dA + (long long)b * S * kM;
     const BooleW bw(a.dt);
 
@@ -612,6 +737,7 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     // Ybar_j blocks (ybar_word) and dA sums (by lane) handed from wave w to wave w + 1
     __shared__ alignas(16) double ybh[2][H][4][64];
     __shared__ double dah[2][H][4][64];
+    __shared__ alignas(16) double ftb[64];  // the Fourier loader's table rows
     constexpr RevRoles<K> roles;
     const int hw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int w = roles.role[hw];
@@ -619,8 +745,12 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     const long long nb = (a.S - 1) / 4;
     const Lane L(lane);
     if (w == K) {  // ring index r holds block nb - 1 - r (the blocks are walked top-down)
-        ring_loader<R, 2>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane,
-                          [nb](long long r) { return nb - 1 - r; });
+        if (a.fcoef)
+            ring_fourier<R>(ring, ftb, a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, nb, nb + K - 1,
+                            lane, [nb](long long r) { return nb - 1 - r; });
+        else
+            ring_loader<R, 2>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane,
+                              [nb](long long r) { return nb - 1 - r; });
     } else if (K == 1) {
         rev_wave<K, true, true, 1>(a, b, w, lane, L, ring, ybh, dah);
     } else if (w == 0) {
@@ -653,7 +783,7 @@ hipError_t launch_wbd8_sweep(const LmArgs& a, hipStream_t s) {
     // written for a VJP); per sample K products of 2 d^3 flops plus a prefix step and a sum
     // of 8 flops per entry for each of the K + 1 levels
     const double K = a.n - 1, S = (double)a.S, B = (double)a.jobs;
-    const double bytes = 8.0 * B * S * kM * (1.0 + (a.carry ? kept_levels(a.n - 1) : 0.0));
+    const double bytes = 8.0 * B * S * kM * ((a.fcoef ? 0.0 : 1.0) + (a.carry ? kept_levels(a.n - 1) : 0.0));
     const double flops = B * S * (K * (2.0 * 512.0 + 8.0 * kM) + 8.0 * kM);
     void* h = timer_begin("wbd8_sweep", s, bytes, flops);
     hipError_t e = a.carry ? launch_k<true>(a, s) : launch_k<false>(a, s);
@@ -670,7 +800,7 @@ hipError_t launch_wbd8_rev(const Wbd8RevArgs& a, hipStream_t s) {
     // per sample and level two products (A^T Ybar and Ybar W^T) and the
     // transposed prefix step and sum cotangent, 10 flops per entry
     const double K = a.n - 1, S = (double)a.S, B = (double)a.B;
-    const double bytes = 8.0 * B * S * kM * (kept_levels(a.n - 1) + 2.0);
+    const double bytes = 8.0 * B * S * kM * (kept_levels(a.n - 1) + (a.fcoef ? 1.0 : 2.0));
     const double flops = B * S * K * (2.0 * 2.0 * 512.0 + 10.0 * kM);
     void* h = timer_begin("wbd8_rev", s, bytes, flops);
     hipError_t e;

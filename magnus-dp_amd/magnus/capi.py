@@ -62,6 +62,9 @@ def lib() -> ctypes.CDLL:
         L.magnus_basis_state_elems.restype = u64
         L.magnus_forward_basis_f64.argtypes = [P, u64, vp, vp, vp, vp]
         L.magnus_vjp_basis_f64.argtypes = [P, u64, vp, vp, vp, vp, vp]
+        L.magnus_fourier_forward_basis_f64.argtypes = [P, u64, vp, u64, vp, vp, vp]
+        L.magnus_fourier_vjp_basis_f64.argtypes = [P, u64, vp, u64, vp, vp, vp, vp]
+        L.magnus_fourier_forward_orders_from_f64.argtypes = [P, u64, u64, vp, u64, vp, vp]
         L.magnus_vjp_segmented_f64.argtypes = [P, u64, vp, vp, vp, u64, vp]
         L.magnus_vjp_segmented_scratch_bytes.argtypes = [P, u64, u64]
         L.magnus_vjp_segmented_scratch_bytes.restype = u64

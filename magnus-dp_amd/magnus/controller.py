@@ -23,7 +23,9 @@ Pieces, all on the device:
     and/or raises n, and repeats.  The learned path starts the judge at the
     network's class; a round keeps the basis state of its (S, n) sweep
     (torch_ops.forward_basis) and runs the VJP from it (torch_ops.vjp_basis),
-    and checks with a sweep at 2S - 1 that keeps no levels, so an accepted
+    and checks with a sweep at 2S - 1 that keeps no levels (the d = 8 sweeps
+    form their samples from the Fourier coefficients, torch_ops.fourier_*, so
+    no sample array is written or read), so an accepted
     instance costs one sweep keeping its levels and one reverse sweep at the
     delivered resolution plus one level-free sweep at twice the samples; the
     same judge started at (65, 2) is the method without the network (judge
@@ -48,6 +50,7 @@ REF_S, REF_N = 32769, 8
 PLAIN = (16385, 8)
 CLASSES = tuple((S, n) for S in S_GRID for n in N_GRID)
 FEATURE_GRID = 65
+FOURIER_MAX_MODES = 8  # the d = 8 sweeps form samples from at most this many modes
 MAX_MODES = 8
 JUDGE_MARGIN = 0.25  # next-order check: |sum(2S - 1, n + 1) - sum(2S - 1, n)| <= JUDGE_MARGIN tol
 JUDGE_FINE = 0.75    # doubling check: |sum(2S - 1, n) - sum(S, n)| <= JUDGE_FINE tol (sum(S, n) is delivered)
@@ -93,10 +96,32 @@ def features(A_coarse: torch.Tensor, t0: float, tf: float) -> torch.Tensor:
     return out
 
 
+def _fourier(coeffs: torch.Tensor, n: int, integrator: str = "Boole") -> bool:
+    """The d = 8 single-pass sweeps form the samples from the coefficients
+    themselves (Boole, n <= 8, at most 8 modes): the same results as sampling
+    first, bit for bit, without a sample array."""
+    return (coeffs.shape[-1] == 8 and integrator == "Boole" and 2 <= n <= 8
+            and (coeffs.shape[1] - 1) // 2 <= FOURIER_MAX_MODES)
+
+
 def sums(coeffs: torch.Tensor, S: int, n: int, integrator: str = "Boole", first: int = 1) -> torch.Tensor:
     """Magnus sums of orders first..n at S samples: [B][n - first + 1][d][d]."""
+    if _fourier(coeffs, n, integrator):
+        return torch_ops.fourier_forward_orders(n, coeffs, S, 0.0, 1.0, integrator=integrator, first=first)
     return torch_ops.forward_orders(n, torch_ops.sample_fourier(coeffs, S), 0.0, 1.0, integrator=integrator,
                                     first=first)
+
+
+def _forward_basis(c: torch.Tensor, S: int, n: int):
+    if _fourier(c, n):
+        return torch_ops.fourier_forward_basis(n, c, S, 0.0, 1.0, op="sum", integrator="Boole")
+    return torch_ops.forward_basis(n, torch_ops.sample_fourier(c, S), 0.0, 1.0, op="sum", integrator="Boole")
+
+
+def _vjp_basis(c: torch.Tensor, S: int, n: int, cot: torch.Tensor, state: torch.Tensor):
+    if _fourier(c, n):
+        return torch_ops.fourier_vjp_basis(n, c, S, cot, state, 0.0, 1.0, op="sum", integrator="Boole")
+    return torch_ops.vjp_basis(n, torch_ops.sample_fourier(c, S), cot, state, 0.0, 1.0, op="sum", integrator="Boole")
 
 
 def rel_err(x: torch.Tensor, ref: torch.Tensor) -> torch.Tensor:
@@ -346,17 +371,15 @@ def accept(coeffs: torch.Tensor, start, tol: float, max_rounds: int = 24, cot: t
                 with torch.cuda.stream(st):
                     idx = _index(ids, coeffs.device)
                     c = coeffs[idx]
-                    A = torch_ops.sample_fourier(c, S)
                     if cot is not None:
-                        cur, state = torch_ops.forward_basis(n, A, 0.0, 1.0, op="sum", integrator="Boole")
-                        g = torch_ops.vjp_basis(n, A, cot[idx], state, 0.0, 1.0, op="sum", integrator="Boole")
+                        cur, state = _forward_basis(c, S, n)
+                        g = _vjp_basis(c, S, n, cot[idx], state)
                         del state
                         work += len(ids) * 3 * cost(S, n)  # sweep + reverse sweep
                     else:
-                        cur = torch_ops.forward(n, A, 0.0, 1.0, op="sum", integrator="Boole")
+                        cur = sums(c, S, n, first=n)[:, 0]
                         g = None
                         work += len(ids) * cost(S, n)
-                    del A
                     nm = min(n + 1, REF_N)
                     hi = sums(c, S2, nm, first=n)  # orders n..nm at 2S - 1 samples, no levels kept
                     work += len(ids) * cost(S2, nm)
@@ -424,16 +447,13 @@ def solve(coeffs: torch.Tensor, res, cot: torch.Tensor, keep_dA: bool = False, f
             st.wait_stream(main)
             with torch.cuda.stream(st):
                 idx = _index(ids, coeffs.device)
-                A = torch_ops.sample_fourier(coeffs[idx], S)
+                c = coeffs[idx]
+                o, state = _forward_basis(c, S, n)
                 if forward_out is None:
-                    o, state = torch_ops.forward_basis(n, A, 0.0, 1.0, op="sum", integrator="Boole")
                     out[idx] = o
-                    g = torch_ops.vjp_basis(n, A, cot[idx], state, 0.0, 1.0, op="sum", integrator="Boole")
-                    del state
-                    work += len(ids) * 3 * cost(S, n)  # sweep + reverse sweep
-                else:
-                    g = torch_ops.vjp(n, A, cot[idx], None, 0.0, 1.0, op="sum", integrator="Boole")
-                    work += len(ids) * 3 * cost(S, n)  # sweep rerun + reverse sweep
+                g = _vjp_basis(c, S, n, cot[idx], state)  # with forward_out: the sweep rerun, then the reverse
+                del state
+                work += len(ids) * 3 * cost(S, n)  # sweep + reverse sweep
                 if keep_dA:
                     for kk, i in enumerate(ids.tolist()):
                         dA[i] = g[kk]

@@ -160,6 +160,64 @@ def vjp_basis(n: int, A: torch.Tensor, cot: torch.Tensor, state: torch.Tensor, t
     return dA
 
 
+def _fourier_shape(coeffs: torch.Tensor):
+    _check_input(coeffs, 4, "coeffs")
+    B, terms, d, d2 = coeffs.shape
+    if d != d2 or terms % 2 != 1:
+        raise ValueError("coeffs must have shape (B, 2*modes+1, d, d)")
+    return B, (terms - 1) // 2, d
+
+
+def fourier_forward_basis(n: int, coeffs: torch.Tensor, samples: int, t0: float = 0.0, tf: float = 1.0, *,
+                          op: str = "sum", matrix_backend: str = "Auto", integrator: str = "Auto"):
+    """forward_basis(n, sample_fourier(coeffs, samples)) with the samples formed
+    inside the sweep (dim 8, Boole, n <= 8, at most 8 modes): the same results
+    bit for bit, and no sample array in HBM."""
+    coeffs = coeffs.contiguous()
+    B, modes, d = _fourier_shape(coeffs)
+    p = capi.problem(n, d, samples, t0, tf, op, matrix_backend, integrator)
+    elems = int(capi.lib().magnus_basis_state_elems(ctypes.byref(p), B))
+    if elems == 0:
+        raise ValueError("this problem has no basis state (it does not take the dense W-basis path)")
+    out = torch.empty((B, n, d, d) if op == "many" else (B, d, d), dtype=coeffs.dtype, device=coeffs.device)
+    state = torch.empty((elems,), dtype=coeffs.dtype, device=coeffs.device)
+    capi.check(capi.lib().magnus_fourier_forward_basis_f64(ctypes.byref(p), B, _ptr(coeffs), modes, _ptr(out),
+                                                           _ptr(state), _stream()))
+    return out, state
+
+
+def fourier_vjp_basis(n: int, coeffs: torch.Tensor, samples: int, cot: torch.Tensor, state: torch.Tensor,
+                      t0: float = 0.0, tf: float = 1.0, *, op: str = "sum", matrix_backend: str = "Auto",
+                      integrator: str = "Auto"):
+    """vjp_basis(n, sample_fourier(coeffs, samples), cot, state) with the samples
+    formed inside the reverse sweep; dA has shape (B, samples, d, d)."""
+    coeffs = coeffs.contiguous()
+    B, modes, d = _fourier_shape(coeffs)
+    p = capi.problem(n, d, samples, t0, tf, op, matrix_backend, integrator)
+    cot = _check_operand(cot, coeffs, (B, n, d, d) if op == "many" else (B, d, d), "cotangent")
+    elems = int(capi.lib().magnus_basis_state_elems(ctypes.byref(p), B))
+    if elems == 0:
+        raise ValueError("this problem has no basis state (it does not take the dense W-basis path)")
+    state = _check_operand(state, coeffs, (elems,), "basis state")
+    dA = torch.empty((B, samples, d, d), dtype=coeffs.dtype, device=coeffs.device)
+    capi.check(capi.lib().magnus_fourier_vjp_basis_f64(ctypes.byref(p), B, _ptr(coeffs), modes, _ptr(cot),
+                                                       _ptr(state), _ptr(dA), _stream()))
+    return dA
+
+
+def fourier_forward_orders(n: int, coeffs: torch.Tensor, samples: int, t0: float = 0.0, tf: float = 1.0, *,
+                           matrix_backend: str = "Auto", integrator: str = "Auto", first: int = 1):
+    """forward_orders(n, sample_fourier(coeffs, samples), first=first) with the
+    samples formed inside the sweep."""
+    coeffs = coeffs.contiguous()
+    B, modes, d = _fourier_shape(coeffs)
+    p = capi.problem(n, d, samples, t0, tf, "sum", matrix_backend, integrator)
+    out = torch.empty((B, n - first + 1, d, d), dtype=coeffs.dtype, device=coeffs.device)
+    capi.check(capi.lib().magnus_fourier_forward_orders_from_f64(ctypes.byref(p), B, first, _ptr(coeffs), modes,
+                                                                 _ptr(out), _stream()))
+    return out
+
+
 def vjp_segmented(n: int, A: torch.Tensor, cot: torch.Tensor, t0: float = 0.0, tf: float = 1.0, *, op: str = "sum",
                   matrix_backend: str = "Auto", integrator: str = "Auto", segments: int = 0):
     """vjp() without a carry that keeps the basis levels for one time segment at

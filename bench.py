@@ -738,7 +738,8 @@ def _bench_controller(args, torch, dev):
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "f64",
-        "data": "synthetic (seeded Fourier-form A(t), d=8, 3-8 modes, ||A||_2 T ~ U[0.1, 4]; device-sampled)",
+        "data": "synthetic (seeded Fourier-form A(t), d=8, 3-8 modes, ||A||_2 T ~ U[0.1, 4]; samples formed on the "
+                "device from the coefficients)",
         "config": {"workload": "controller", "train": n_tr, "eval": n_ev, "dim": 8, "tol": tol,
                    "samples": list(C.S_GRID), "orders": list(C.N_GRID), "reference": [C.REF_S, C.REF_N],
                    "plain": list(C.PLAIN), "mlp": "24-128-128-63, bf16 weights, fp32 accumulation, Adam"},

@@ -208,17 +208,28 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
     return __builtin_bit_cast(double, ((unsigned long long)hi << 32) | lo);
 }
 
+// acc + (lane J of this lane's 16-lane row of t) * c, one fused multiply-add:
+// v_fmac_f64 with the row_newbcast DPP source modifier (FP64 DPP broadcasts a
+// row lane to the whole row).
+template <int J>
+__device__ __forceinline__ double fma_row_bcast(double t, double c, double acc) {
+    static_assert(J >= 0 && J < 16, "a lane of the row");
+    asm("v_fmac_f64_dpp %0, %1, %2 row_newbcast:%3 row_mask:0xf bank_mask:0xf bound_ctrl:1"
+        : "+v"(acc)
+        : "v"(t), "v"(c), "n"(J));
+    return acc;
+}
+
 // The loader of a pipeline whose samples are formed from M Fourier modes:
 // lane l keeps entry l's 2M + 1 coefficients in registers and, for each ring
 // index, writes that entry of the block's four samples (4 blk + 1 .. 4 blk + 4)
 // to the slot, the four as independent fused multiply-add chains.  The
-// block's rows of the sin/cos table (4 x 2M <= 64 contiguous doubles) are
-// loaded one per lane, the next index's while this one is formed, stored to
-// the wave's 64-double LDS buffer `tb` and read back as (sin, cos) pairs
-// that every lane reads at one address (broadcast reads, 16 bytes each, the
-// next mode's in flight while this mode's multiply-adds run).  LDS accesses
-// of one wave complete in order, so the reads see this index's store and the
-// previous index's reads precede it.
+// block's rows of the sin/cos table (4 x 2M <= 64 contiguous doubles, value x
+// = 2M p + 2(k - 1) + {0: sin, 1: cos}) sit in ceil(M / 2) registers, value x
+// in register x / 16 at lane x % 16 of every 16-lane row (each row loads the
+// same 128 bytes), so each multiply-add takes its table value straight from
+// the register by a row broadcast: no LDS, no scalar registers, one
+// instruction per term.  The next index's rows load while this one is formed.
 //
 // The step protocol is ring_loader's: indices 0 .. kAhead - 2, the barrier
 // that opens the pipeline, then at step t index t + kAhead - 1 and the step's
@@ -227,69 +238,66 @@ __device__ __forceinline__ double lane_bcast(double v, int src) {
 // rows: their slots are free by the protocol and no wave reads them, and the
 // loop then has no branch around a load (walk_range), so its waits count only
 // the loads issued since.
+template <int M>
+struct TabRows {
+    double r[M > 0 ? (M + 1) / 2 : 1];
+};
+
 template <int R, int M, typename BlkOf>
-__device__ __forceinline__ void ring_fourier_m(RingSlot* ring, double* tb, const double* coef, const double* tab,
-                                               long long nb, long long steps, int lane, BlkOf blk_of) {
+__device__ __forceinline__ void ring_fourier_m(RingSlot* ring, const double* coef, const double* tab, long long nb,
+                                               long long steps, int lane, BlkOf blk_of) {
+    constexpr int NR = (M + 1) / 2;  // table registers per block
     const int word = ring_word(lane >> 3, lane & 7);
     double c[2 * M + 1];
 #pragma unroll
     for (int q = 0; q <= 2 * M; ++q) c[q] = coef[q * kM + lane];
-    const int tl = lane < 8 * M ? lane : 0;
-    auto load = [&](long long ri, double& tv) {
-        if constexpr (M > 0) tv = tab[(4 * blk_of(ri < nb ? ri : nb - 1) + 1) * (2 * M) + tl];
+    auto load = [&](long long ri, TabRows<M>& t) {
+        if constexpr (M > 0) {
+            const double* rows = tab + (4 * blk_of(ri < nb ? ri : nb - 1) + 1) * (2 * M);
+#pragma unroll
+            for (int q = 0; q < NR; ++q) {
+                const int x = 16 * q + (lane & 15);
+                t.r[q] = rows[x < 8 * M ? x : 0];
+            }
+        }
     };
-    auto pair = [&](int p, int k) { return *reinterpret_cast<const double2*>(tb + p * 2 * M + 2 * k - 2); };
-    auto step = [&](long long ri, double& tv, auto) {
+    auto step = [&](long long ri, TabRows<M>& t, auto) {
         RingSlot& slot = ring[ri % R];
         double v[4];
 #pragma unroll
         for (int p = 0; p < 4; ++p) v[p] = c[0];
-        if constexpr (M > 0) {
-            tb[lane] = tv;
-            // groups (k, h): samples 2h, 2h + 1 of mode k, in k-major order,
-            // the next group's pairs read while this group's multiply-adds run
-            double2 cur[2] = {pair(0, 1), pair(1, 1)};
-#pragma unroll
-            for (int g = 0; g < 2 * M; ++g) {
-                const int k = g / 2 + 1, h = g % 2;
-                double2 nxt[2];
-                if (g + 1 < 2 * M) {
-                    const int kn = (g + 1) / 2 + 1, hn = (g + 1) % 2;
-                    nxt[0] = pair(2 * hn, kn);
-                    nxt[1] = pair(2 * hn + 1, kn);
-                }
-#pragma unroll
-                for (int q = 0; q < 2; ++q) v[2 * h + q] = __fma_rn(cur[q].x, c[2 * k - 1], v[2 * h + q]);
-#pragma unroll
-                for (int q = 0; q < 2; ++q) v[2 * h + q] = __fma_rn(cur[q].y, c[2 * k], v[2 * h + q]);
-                if (g + 1 < 2 * M) {
-                    cur[0] = nxt[0];
-                    cur[1] = nxt[1];
-                }
-                __builtin_amdgcn_sched_barrier(0);  // one group's reads ahead, no more (registers)
-            }
-        }
+        static_for<1, M + 1>([&](auto kc) {
+            constexpr int k = decltype(kc)::value;
+            static_for<0, 4>([&](auto pc) {
+                constexpr int p = decltype(pc)::value, x = 2 * M * p + 2 * (k - 1);
+                v[p] = fma_row_bcast<x % 16>(t.r[x / 16], c[2 * k - 1], v[p]);
+            });
+            static_for<0, 4>([&](auto pc) {
+                constexpr int p = decltype(pc)::value, x = 2 * M * p + 2 * (k - 1) + 1;
+                v[p] = fma_row_bcast<x % 16>(t.r[x / 16], c[2 * k], v[p]);
+            });
+        });
 #pragma unroll
         for (int p = 0; p < 4; ++p) slot.v[p][word] = v[p];
         if (ri >= kAhead - 2) step_barrier();
     };
-    double tv[2] = {0.0, 0.0};
-    walk_range<2, 2, true>(0, steps + kAhead - 2, tv, load, step);
+    TabRows<M> tr[2] = {};
+    walk_range<2, 2, true>(0, steps + kAhead - 2, tr, load, step);
 }
 
 template <int R, typename BlkOf>
-__device__ __forceinline__ void ring_fourier(RingSlot* ring, double* tb, const double* coef, const double* tab,
-                                             int modes, long long nb, long long steps, int lane, BlkOf blk_of) {
+__device__ __forceinline__ void ring_fourier(RingSlot* ring, const double* coef, const double* tab, int modes,
+                                             long long nb, long long steps, int lane, BlkOf blk_of) {
     switch (modes) {
-        case 0: ring_fourier_m<R, 0>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        case 1: ring_fourier_m<R, 1>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        case 2: ring_fourier_m<R, 2>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        case 3: ring_fourier_m<R, 3>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        case 4: ring_fourier_m<R, 4>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        case 5: ring_fourier_m<R, 5>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        case 6: ring_fourier_m<R, 6>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        case 7: ring_fourier_m<R, 7>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
-        default: ring_fourier_m<R, kWbd8MaxModes>(ring, tb, coef, tab, nb, steps, lane, blk_of); break;
+        case 0: ring_fourier_m<R, 0>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        case 1: ring_fourier_m<R, 1>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        case 2: ring_fourier_m<R, 2>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        case 3: ring_fourier_m<R, 3>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        case 4: ring_fourier_m<R, 4>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        case 5: ring_fourier_m<R, 5>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        case 6: ring_fourier_m<R, 6>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        case 7: ring_fourier_m<R, 7>(ring, coef, tab, nb, steps, lane, blk_of); break;
+        default: ring_fourier_m<R, kWbd8MaxModes>(ring, coef, tab, nb, steps, lane, blk_of); break;
     }
     static_assert(kWbd8MaxModes == 8, "one case per mode count");
 }
@@ -449,14 +457,13 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     __shared__ RingSlot ring[R];
     // level blocks handed from wave w to wave w + 1: [step parity][w][4 samples][64 words]
     __shared__ alignas(16) double hand[2][K > 1 ? K - 1 : 1][4][64];
-    __shared__ alignas(16) double ftb[64];  // the Fourier loader's table rows
     const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int b = blockIdx.x;
     const long long nb = (a.S - 1) / 4;
     const Lane L(lane);
     if (w == K) {
         if (a.fcoef)
-            ring_fourier<R>(ring, ftb, a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, nb, nb + K - 1,
+            ring_fourier<R>(ring, a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, nb, nb + K - 1,
                             lane, [](long long r) { return r; });
         else
             ring_loader<R, 1>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane, [](long long r) { return r; });
@@ -737,7 +744,6 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     // Ybar_j blocks (ybar_word) and dA sums (by lane) handed from wave w to wave w + 1
     __shared__ alignas(16) double ybh[2][H][4][64];
     __shared__ double dah[2][H][4][64];
-    __shared__ alignas(16) double ftb[64];  // the Fourier loader's table rows
     constexpr RevRoles<K> roles;
     const int hw = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
     const int w = roles.role[hw];
@@ -746,7 +752,7 @@ __global__ void __launch_bounds__(64 * (K + 1)) __attribute__((amdgpu_waves_per_
     const Lane L(lane);
     if (w == K) {  // ring index r holds block nb - 1 - r (the blocks are walked top-down)
         if (a.fcoef)
-            ring_fourier<R>(ring, ftb, a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, nb, nb + K - 1,
+            ring_fourier<R>(ring, a.fcoef + (long long)b * (2 * a.fmodes + 1) * kM, a.ftab, a.fmodes, nb, nb + K - 1,
                             lane, [nb](long long r) { return nb - 1 - r; });
         else
             ring_loader<R, 2>(ring, a.in + (long long)b * a.S * kM, nb, nb + K - 1, lane,

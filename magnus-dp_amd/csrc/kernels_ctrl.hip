@@ -59,8 +59,9 @@ __device__ double spectral_norm_d(const double* __restrict__ Ag) {
 #pragma unroll
         for (int i = 0; i < D; ++i) wn = fma(w[i], w[i], wn);
         lam = sqrt(wn);
+        const double inv = 1.0 / nv;  // one division per iteration, not D
 #pragma unroll
-        for (int k = 0; k < D; ++k) v[k] /= nv;
+        for (int k = 0; k < D; ++k) v[k] *= inv;
     }
     return lam;
 }
@@ -88,7 +89,8 @@ __device__ double spectral_norm(const double* A, int d) {
         double wn = 0.0;
         for (int i = 0; i < d; ++i) wn = fma(w[i], w[i], wn);
         lam = sqrt(wn);  // ||A v_prev|| with ||v_prev|| = 1 after the first step
-        for (int k = 0; k < d; ++k) v[k] /= nv;
+        const double inv = 1.0 / nv;
+        for (int k = 0; k < d; ++k) v[k] *= inv;
     }
     return lam;
 }
@@ -124,7 +126,7 @@ __global__ void __launch_bounds__(kThreads) ctrl_features(CtrlArgs a) {
         const int k = 1 + p / m, e = p - (k - 1) * m;
         double re = 0.0, im = 0.0;
         for (int i = 0; i < N; ++i) {
-            const int j = (int)((long long)k * i % N);
+            const int j = (k * i) & (N - 1);  // k i mod N, N a power of two
             const double sn = s_sn[j], cs = s_cs[j];
             const double x = A[(long long)i * m + e];
             re = fma(x, cs, re);

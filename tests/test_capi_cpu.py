@@ -152,6 +152,38 @@ def test_sampling_argument_checks_before_device_work():
     assert L.magnus_sample_fourier_f64(0, 9, 0.0, 1.0, 4, 2, dummy, dummy, None) == capi.OK  # empty batch
 
 
+def test_fourier_input_argument_checks_before_device_work():
+    """The Fourier-form entry points refuse, before any device call, problems
+    off the d = 8 single-pass sweeps (other dims, Simpson, orders above 8, op
+    one), more than 8 modes, an order range outside 1..n and null pointers;
+    an empty batch is a no-op."""
+    L = capi.lib()
+    dummy = ctypes.c_void_p(16)  # never dereferenced: every call below fails validation or is empty
+    ok = capi.problem(4, 8, 65, 0.0, 1.0, op="sum", integrator="Boole")
+    fwd, vjp, orders = (L.magnus_fourier_forward_basis_f64, L.magnus_fourier_vjp_basis_f64,
+                        L.magnus_fourier_forward_orders_from_f64)
+    for kw in (dict(n=4, d=16, integ="Boole", op="sum"), dict(n=4, d=8, integ="Simpson", op="sum"),
+               dict(n=9, d=8, integ="Boole", op="sum")):
+        q = capi.problem(kw["n"], kw["d"], 65, 0.0, 1.0, op=kw["op"], integrator=kw["integ"])
+        assert fwd(ctypes.byref(q), 1, dummy, 3, dummy, dummy, None) == capi.EINVAL, kw
+        assert vjp(ctypes.byref(q), 1, dummy, 3, dummy, dummy, dummy, None) == capi.EINVAL, kw
+        assert orders(ctypes.byref(q), 1, 1, dummy, 3, dummy, None) == capi.EINVAL, kw
+    assert fwd(ctypes.byref(ok), 1, dummy, 9, dummy, dummy, None) == capi.ERANGE
+    assert "modes" in L.magnus_last_error().decode()
+    assert vjp(ctypes.byref(ok), 1, dummy, 9, dummy, dummy, dummy, None) == capi.ERANGE
+    assert orders(ctypes.byref(ok), 1, 1, dummy, 9, dummy, None) == capi.ERANGE
+    one = capi.problem(4, 8, 65, 0.0, 1.0, op="one", integrator="Boole")
+    assert fwd(ctypes.byref(one), 1, dummy, 3, dummy, dummy, None) == capi.EINVAL
+    assert vjp(ctypes.byref(one), 1, dummy, 3, dummy, dummy, dummy, None) == capi.EINVAL
+    assert orders(ctypes.byref(ok), 1, 0, dummy, 3, dummy, None) == capi.EINVAL
+    assert orders(ctypes.byref(ok), 1, 5, dummy, 3, dummy, None) == capi.EINVAL
+    assert fwd(ctypes.byref(ok), 1, None, 3, dummy, dummy, None) == capi.EINVAL
+    assert vjp(ctypes.byref(ok), 1, dummy, 3, dummy, None, dummy, None) == capi.EINVAL
+    assert orders(ctypes.byref(ok), 1, 1, dummy, 3, None, None) == capi.EINVAL
+    assert fwd(ctypes.byref(ok), 0, dummy, 3, dummy, dummy, None) == capi.OK  # empty batch
+    assert orders(ctypes.byref(ok), 0, 2, dummy, 3, dummy, None) == capi.OK
+
+
 def test_spacecurve_checkpoint_size_and_argument_checks():
     """Compact checkpoint: 2(n-1)+1 4-vectors per chunk of samples on the
     single-pass path; none (size 0, EINVAL) elsewhere.  Host-only calls."""

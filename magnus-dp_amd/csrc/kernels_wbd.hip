@@ -323,13 +323,16 @@ __global__ void __launch_bounds__(256) wbd_fold_vjp_reduce(WbdArgs a) {
 }
 
 // ---------------------------------------------------------------------------
-// Small problems (d <= 8, n <= NM): one wavefront per (instance, node).  Lane e
-// owns entry e (row e / d, column e % d) of every matrix; the instance's totals
-// and sums, the node's T_j and (VJP) its cotangents sit in LDS, so a product is
-// 2d LDS reads and d FMAs per lane with no global-memory round trip between the
-// dependent products.  Every entry is formed by the operations of mat_set /
-// mat_axpy / mat_acc above (their d <= 8 branch), in the same order.  The
-// workgroup is one wavefront, so its barriers cost nothing.
+// Small problems (d <= 8, n <= NM): one wavefront per (instance, node).  Each
+// lane owns one entry of every matrix; the instance's totals and sums, the
+// node's T_j and (VJP) its cotangents sit in LDS (row-major), so there is no
+// global-memory round trip between the dependent products.  For d < 8 lane e
+// owns entry e and a product is 2d LDS reads and d FMAs per lane, the
+// operations of mat_acc's d <= 8 branch in the same order.  For d = 8 a
+// product is two v_mfma_f64_4x4x4f64 (four 4 x 4 output blocks over two
+// halves of the contraction) from 4 LDS reads per lane, and lanes own the
+// entries in the instruction's output layout, so each result lands in its
+// owner.  The workgroup is one wavefront, so its barriers cost nothing.
 // ---------------------------------------------------------------------------
 constexpr int kSmallD = 8;
 constexpr int kSmallM = kSmallD * kSmallD;
@@ -352,6 +355,33 @@ __device__ __forceinline__ double sdot(const double* X, const double* Y, int d, 
     return acc;
 }
 
+// The entry lane L owns: e = L for d < 8; for d = 8 the output layout of
+// v_mfma_f64_4x4x4f64 (block b's D[i][j] in lane 16i + 4b + j) with block
+// b = 2I + J holding rows 4I.., columns 4J.. of the product
+__device__ __forceinline__ int small_owned(int lane, int d) {
+    if (d != kSmallD) return lane;
+    return 8 * (4 * ((lane >> 3) & 1) + (lane >> 4)) + 4 * ((lane >> 2) & 1) + (lane & 3);
+}
+
+// this lane's entry of X' Y' (X' = X or X^T, Y' = Y or Y^T), entry (r, c) =
+// small_owned(lane, d) as (row, column)
+template <bool TX, bool TY>
+__device__ __forceinline__ double sprod(const double* X, const double* Y, int d, int r, int c) {
+    if (d != kSmallD) return sdot<TX, TY>(X, Y, d, r, c);
+    // block b = 2I + J of instruction h: A[i][k] = X'[4I + i][4h + k] in lane
+    // 16k + 4b + i, B[k][j] = Y'[4h + k][4J + j] in lane 16k + 4b + j
+    const int lane = (int)threadIdx.x, k = lane >> 4, q = lane & 3, I = (lane >> 3) & 1, J = (lane >> 2) & 1;
+    double acc = 0.0;
+#pragma unroll
+    for (int h = 0; h < 2; ++h) {
+        const int ar = 4 * I + q, ac = 4 * h + k, br = 4 * h + k, bc = 4 * J + q;
+        const double x = TX ? X[ac * kSmallD + ar] : X[ar * kSmallD + ac];
+        const double y = TY ? Y[bc * kSmallD + br] : Y[br * kSmallD + bc];
+        acc = __builtin_amdgcn_mfma_f64_4x4x4f64(x, y, acc, 0, 0, 0);
+    }
+    return acc;
+}
+
 template <int NM>
 struct SmallFold {
     double wh[NM][kSmallM];  // What_1..What_K at rows 0..K-1
@@ -362,7 +392,7 @@ struct SmallFold {
 // load What_1..What_K and Z_0..Z_{n-1} of instance b (orders up to n), then T_1..T_K
 template <int NM>
 __device__ void small_totals(SmallFold<NM>& L, const WbdArgs& a, long long b, int n, double s) {
-    const int K = n - 1, d = a.d, e = threadIdx.x, r = e / d, c = e - r * d;
+    const int K = n - 1, d = a.d, e = small_owned((int)threadIdx.x, d), r = e / d, c = e - r * d;
     const long long M = a.m;
     const bool live = e < a.m;
     const double* What = a.totals + b * (long long)row_stride(a) * M;
@@ -374,7 +404,7 @@ __device__ void small_totals(SmallFold<NM>& L, const WbdArgs& a, long long b, in
     for (int j = 1; j <= K; ++j) {
         if (live) {
             double v = 1.0 * L.wh[j - 1][e];
-            for (int l = 1; l < j; ++l) v = fma(s, sdot<false, false>(L.wh[l - 1], L.t[j - l], d, r, c), v);
+            for (int l = 1; l < j; ++l) v = fma(s, sprod<false, false>(L.wh[l - 1], L.t[j - l], d, r, c), v);
             L.t[j][e] = v;
         }
         __syncthreads();
@@ -384,12 +414,12 @@ __device__ void small_totals(SmallFold<NM>& L, const WbdArgs& a, long long b, in
 // node (weight wq, shift s) shares of Omega_2..Omega_n into part rows Om + (j-1) M
 template <int NM>
 __device__ void small_omegas(SmallFold<NM>& L, const WbdArgs& a, int n, double s, double wq, double* Om) {
-    const int d = a.d, e = threadIdx.x, r = e / d, c = e - r * d;
+    const int d = a.d, e = small_owned((int)threadIdx.x, d), r = e / d, c = e - r * d;
     if (e >= a.m) return;
     const long long M = a.m;
     for (int j = 2; j <= n; ++j) {
         double v = wq * L.z[j - 1][e];
-        for (int l = 0; l <= j - 2; ++l) v = fma(wq * s, sdot<false, false>(L.z[l], L.t[j - 1 - l], d, r, c), v);
+        for (int l = 0; l <= j - 2; ++l) v = fma(wq * s, sprod<false, false>(L.z[l], L.t[j - 1 - l], d, r, c), v);
         Om[(j - 1) * M + e] = v;
     }
 }
@@ -429,7 +459,7 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
     __shared__ SmallFoldVjp<NM> L;
     const long long b = blockIdx.x / a.Q;
     const int q = (int)(blockIdx.x - b * a.Q);
-    const int n = a.n, K = n - 1, d = a.d, e = threadIdx.x, r = e / d, c = e - r * d;
+    const int n = a.n, K = n - 1, d = a.d, e = small_owned((int)threadIdx.x, d), r = e / d, c = e - r * d;
     const long long M = a.m;
     const bool live = e < a.m;
     const double s = a.x[q] - 1.0, wq = a.w[q];
@@ -453,8 +483,8 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
             const double* Ob = L.ob[j - 1];
             L.zb[j - 1][e] = fma(wq, Ob[e], L.zb[j - 1][e]);
             for (int l = 0; l <= j - 2; ++l) {
-                L.zb[l][e] = fma(wq * s, sdot<false, true>(Ob, T[j - 1 - l], d, r, c), L.zb[l][e]);
-                L.tb[j - 1 - l][e] = fma(wq * s, sdot<true, false>(Z[l], Ob, d, r, c), L.tb[j - 1 - l][e]);
+                L.zb[l][e] = fma(wq * s, sprod<false, true>(Ob, T[j - 1 - l], d, r, c), L.zb[l][e]);
+                L.tb[j - 1 - l][e] = fma(wq * s, sprod<true, false>(Z[l], Ob, d, r, c), L.tb[j - 1 - l][e]);
             }
         }
     }
@@ -463,8 +493,8 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
         if (live) {
             L.wb[j - 1][e] = fma(1.0, L.tb[j][e], L.wb[j - 1][e]);
             for (int l = 1; l < j; ++l) {
-                L.wb[l - 1][e] = fma(s, sdot<false, true>(L.tb[j], T[j - l], d, r, c), L.wb[l - 1][e]);
-                L.tb[j - l][e] = fma(s, sdot<true, false>(L.f.wh[l - 1], L.tb[j], d, r, c), L.tb[j - l][e]);
+                L.wb[l - 1][e] = fma(s, sprod<false, true>(L.tb[j], T[j - l], d, r, c), L.wb[l - 1][e]);
+                L.tb[j - l][e] = fma(s, sprod<true, false>(L.f.wh[l - 1], L.tb[j], d, r, c), L.tb[j - l][e]);
             }
         }
         __syncthreads();

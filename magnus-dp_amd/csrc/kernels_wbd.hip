@@ -449,11 +449,13 @@ template <int NM>
 struct SmallFoldVjp {
     SmallFold<NM> f;
     double tb[NM][kSmallM];  // T_bar_1..T_bar_K at rows 1..K
-    double zb[NM][kSmallM];  // Zbar_0..Zbar_{n-1}
-    double wb[NM][kSmallM];  // What_bar_1..What_bar_n at rows 0..n-1
     double ob[NM][kSmallM];  // cotangent of Omega_j at row j-1 (rows with obar(j) set)
 };
 
+// Zbar_l and What_bar_j are only ever accumulated entry by entry, so each lane
+// keeps its entries of them in registers (loops unrolled to NM, so that the
+// row indices are compile-time); the LDS holds only matrices read as product
+// operands, which lets more of these one-wavefront workgroups share a CU.
 template <int NM>
 __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
     __shared__ SmallFoldVjp<NM> L;
@@ -463,11 +465,13 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
     const long long M = a.m;
     const bool live = e < a.m;
     const double s = a.x[q] - 1.0, wq = a.w[q];
+    double zb[NM], wb[NM];  // Zbar_0..Zbar_{n-1}, What_bar_1..What_bar_n (rows 0..n-1)
+#pragma unroll
+    for (int l = 0; l < NM; ++l) {
+        zb[l] = 0.0;
+        wb[l] = 0.0;
+    }
     if (live) {
-        for (int l = 0; l < n; ++l) {
-            L.zb[l][e] = 0.0;
-            L.wb[l][e] = 0.0;
-        }
         for (int j = 1; j <= K; ++j) L.tb[j][e] = 0.0;
         for (int j = 2; j <= n; ++j) {
             const double* Ob = obar(a, b, j);
@@ -478,22 +482,28 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
     auto& T = L.f.t;
     auto& Z = L.f.z;
     if (live) {
-        for (int j = 2; j <= n; ++j) {
+#pragma unroll
+        for (int j = 2; j <= NM; ++j) {
+            if (j > n) break;
             if (!obar(a, b, j)) continue;
             const double* Ob = L.ob[j - 1];
-            L.zb[j - 1][e] = fma(wq, Ob[e], L.zb[j - 1][e]);
+            zb[j - 1] = fma(wq, Ob[e], zb[j - 1]);
+#pragma unroll
             for (int l = 0; l <= j - 2; ++l) {
-                L.zb[l][e] = fma(wq * s, sprod<false, true>(Ob, T[j - 1 - l], d, r, c), L.zb[l][e]);
+                zb[l] = fma(wq * s, sprod<false, true>(Ob, T[j - 1 - l], d, r, c), zb[l]);
                 L.tb[j - 1 - l][e] = fma(wq * s, sprod<true, false>(Z[l], Ob, d, r, c), L.tb[j - 1 - l][e]);
             }
         }
     }
     __syncthreads();
-    for (int j = K; j >= 1; --j) {
+#pragma unroll
+    for (int j = NM - 1; j >= 1; --j) {
+        if (j > K) continue;
         if (live) {
-            L.wb[j - 1][e] = fma(1.0, L.tb[j][e], L.wb[j - 1][e]);
+            wb[j - 1] = fma(1.0, L.tb[j][e], wb[j - 1]);
+#pragma unroll
             for (int l = 1; l < j; ++l) {
-                L.wb[l - 1][e] = fma(s, sprod<false, true>(L.tb[j], T[j - l], d, r, c), L.wb[l - 1][e]);
+                wb[l - 1] = fma(s, sprod<false, true>(L.tb[j], T[j - l], d, r, c), wb[l - 1]);
                 L.tb[j - l][e] = fma(s, sprod<true, false>(L.f.wh[l - 1], L.tb[j], d, r, c), L.tb[j - l][e]);
             }
         }
@@ -502,9 +512,11 @@ __global__ void __launch_bounds__(64) wbd_fold_vjp_small(WbdArgs a) {
     if (live) {
         double* Zb = a.part + (long long)blockIdx.x * n * M;
         double* Wb = a.part2 + (long long)blockIdx.x * n * M;
-        for (int l = 0; l < n; ++l) {
-            Zb[l * M + e] = L.zb[l][e];
-            Wb[l * M + e] = L.wb[l][e];
+#pragma unroll
+        for (int l = 0; l < NM; ++l) {
+            if (l >= n) break;
+            Zb[l * M + e] = zb[l];
+            Wb[l * M + e] = wb[l];
         }
     }
 }

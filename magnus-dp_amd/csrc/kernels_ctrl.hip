@@ -26,42 +26,44 @@ constexpr int kThreads = 256;
 constexpr int kMaxG = 129;
 constexpr int kHarm = 16;
 
-// The same iteration with the dimension known at compile time: the matrix and
-// both vectors stay in registers (a runtime dimension indexes them in scratch).
+// The same iteration with the dimension known at compile time, on the Gram
+// matrix B = A^T A formed once (its upper triangle from the sample in memory,
+// mirrored): one D x D matrix-vector product per iteration instead of two,
+// with ||A v|| = sqrt(v . B v).  B and the vectors stay in registers (a
+// runtime dimension would index them in scratch).
 template <int D>
 __device__ double spectral_norm_d(const double* __restrict__ Ag) {
-    double A[D * D], v[D], w[D];
+    double B[D * D], v[D], u[D];
 #pragma unroll
-    for (int e = 0; e < D * D; ++e) A[e] = Ag[e];
+    for (int i = 0; i < D; ++i)
+#pragma unroll
+        for (int k = i; k < D; ++k) {
+            double acc = 0.0;
+#pragma unroll
+            for (int r = 0; r < D; ++r) acc = fma(Ag[r * D + i], Ag[r * D + k], acc);
+            B[i * D + k] = acc;
+            B[k * D + i] = acc;
+        }
 #pragma unroll
     for (int i = 0; i < D; ++i) v[i] = 1.0 + 0.01 * i;
     double lam = 0.0;
     for (int it = 0; it < 60; ++it) {
-        double nv = 0.0;
+        double nv = 0.0, wn = 0.0;
 #pragma unroll
         for (int i = 0; i < D; ++i) {
             double acc = 0.0;
 #pragma unroll
-            for (int k = 0; k < D; ++k) acc = fma(A[i * D + k], v[k], acc);
-            w[i] = acc;
-        }
-#pragma unroll
-        for (int k = 0; k < D; ++k) {
-            double acc = 0.0;
-#pragma unroll
-            for (int i = 0; i < D; ++i) acc = fma(A[i * D + k], w[i], acc);
-            v[k] = acc;
+            for (int k = 0; k < D; ++k) acc = fma(B[i * D + k], v[k], acc);
+            u[i] = acc;  // u = A^T A v
             nv = fma(acc, acc, nv);
+            wn = fma(v[i], acc, wn);  // v . A^T A v = ||A v||^2
         }
         nv = sqrt(nv);
         if (nv == 0.0) return 0.0;
-        double wn = 0.0;
-#pragma unroll
-        for (int i = 0; i < D; ++i) wn = fma(w[i], w[i], wn);
-        lam = sqrt(wn);
+        lam = sqrt(fmax(wn, 0.0));  // ||A v_prev|| with ||v_prev|| = 1 after the first step
         const double inv = 1.0 / nv;  // one division per iteration, not D
 #pragma unroll
-        for (int k = 0; k < D; ++k) v[k] *= inv;
+        for (int k = 0; k < D; ++k) v[k] = u[k] * inv;
     }
     return lam;
 }

@@ -597,6 +597,7 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
     loadw(wa, nb - 1);
     if (nb > 1) loadw(wbuf, nb - 2);
     int slot = 0;  // ring slot of this wave's next block
+    double ftop = 0.0;  // MODE 2: Y_j at the top sample of the next block (this block's bottom)
 
     // one pipeline step; r = this wave's reverse position (block nb - 1 - r;
     // out of range: the wave only takes part in the barrier), wt: the loaded
@@ -617,12 +618,16 @@ __device__ __forceinline__ void rev_wave(const Wbd8RevArgs& a, int b, int w, int
 #pragma unroll
                     for (int p = 0; p < 4; ++p) f[p + 1] = sl.v[p][rq];
                 } else {
-                    // Y_j[0] = A_0 W_{j-1}[0] = 0; otherwise the block below's last sample
+                    // Y_j[0] = A_0 W_{j-1}[0] = 0; otherwise the block below's last sample,
+                    // which is also that block's f[4] in the next step (the wave's first
+                    // block forms its own)
+#pragma unroll
+                    for (int p = 0; p < 3; ++p)
+                        f[p + 1] = prod(pair4_dpp(wt[p]), *reinterpret_cast<const double2*>(&sl.v[p][rc]));
+                    f[4] = r == 0 ? prod(pair4_dpp(wt[3]), *reinterpret_cast<const double2*>(&sl.v[3][rc])) : ftop;
                     f[0] = blk > 0 ? prod(pair4_dpp(wn[3]), *reinterpret_cast<const double2*>(&ring[slot].v[3][rc]))
                                    : 0.0;
-#pragma unroll
-                    for (int p = 0; p < 4; ++p)
-                        f[p + 1] = prod(pair4_dpp(wt[p]), *reinterpret_cast<const double2*>(&sl.v[p][rc]));
+                    ftop = f[0];
                 }
                 const double base =
                     wend - fma(bw.b14, f[4], fma(bw.b64, f[3], fma(bw.b24, f[2], fma(bw.b64, f[1], bw.b14 * f[0]))));
